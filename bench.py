@@ -1,0 +1,173 @@
+"""Benchmark: the resident FP64 LES step (advection + Smagorinsky diffusion) on 512 x 512 x 128 cells.
+
+One step is one call of the C-ABI entry cpp_fort_shared_les_step (F_new = F + dt * tendency, halos
+included) on fields resident in HBM, submitted as one hipGraph replay; the fields alternate between two
+buffer sets. Prints one JSON line:
+
+  value      ms per LES step (lower is better), timed over --steps steps between two device syncs
+  roofline   algorithmic HBM bytes of the step kernel, 8*(4P + 4I) with P padded and I interior cells,
+             divided by its average duration from HIP events recorded on the context stream
+  cpu_baseline  the oracle's OpenMP C restatement of the same step on the host cores, on a bounded sample
+             (one warm-up and --cpu-steps steps of the full grid), also bit-compared with the first
+             device step
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import __graft_entry__ as graft  # noqa: E402
+
+METRIC = "ms per LES step + HBM GB/s; mixed vs FP64 resident at 1e-10 rel err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(name: str):
+    """HBM bytes per launch of `name` measured by rocprofv3 PMC runs and recorded under profiles/."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    data = json.loads(path.read_text())
+    entry = data.get(name)
+    return None if entry is None else entry.get("bytes_per_launch")
+
+
+def main() -> int:
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--nx", type=int, default=512)
+    p.add_argument("--ny", type=int, default=512)
+    p.add_argument("--nz", type=int, default=128)
+    p.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps for the CPU baseline (0: skip)")
+    args = p.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the LES domain would shard by "
+              "x-y slabs with halo exchange; see the design notes)", file=sys.stderr)
+        return 2
+
+    ftc = graft.package()
+    scope = graft._module("scope")
+    fields = graft._module("fields")
+    lib = ftc.load()
+    if lib.ftc_device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+
+    nx, ny, nz = args.nx, args.ny, args.nz
+    dx = dy = dz = 10.0
+    dt, cs = 0.2, 0.17
+    log(f"bench: generating fields {nx}x{ny}x{nz}")
+    U, V, W, S = fields.les_fields(nx, ny, nz, dx, dy, dz)
+    initial = [U.copy(order="F"), V.copy(order="F"), W.copy(order="F"), S.copy(order="F")] if args.cpu_steps else None
+    b = [np.zeros_like(U, order="F") for _ in range(4)]
+    a = [U, V, W, S]
+
+    dom = scope.Domain(lib)
+    ha = [dom.bind(x) for x in a]
+    hb = [dom.bind(x) for x in b]
+    scalars = [ftc.dp(dx), ftc.dp(dy), ftc.dp(dz), ftc.dp(dt), ftc.dp(cs), ftc.ip(nx), ftc.ip(ny), ftc.ip(nz),
+               ftc.ip(1)]
+
+    def step(n: int) -> None:
+        src, dst = (ha, hb) if n % 2 == 0 else (hb, ha)
+        scope.check(lib, lib.cpp_fort_shared_les_step(dom.ctx, 1, *dst, *src, *scalars))
+        dom.flush()
+
+    log("bench: first step (upload + direct launch)")
+    step(0)
+    dom.observe(*hb)
+    first = [x.copy(order="F") for x in b] if args.cpu_steps else None
+    for n in range(1, args.warmup):
+        step(n)
+    dom.wait()
+
+    stream = ctypes.c_void_p(dom.stream())
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    lib.ftc_event_create(ctypes.byref(ev0))
+    lib.ftc_event_create(ctypes.byref(ev1))
+    log(f"bench: timing {args.steps} steps")
+    dom.wait()
+    t0 = time.perf_counter()
+    lib.ftc_event_record(ev0, stream)
+    for n in range(args.warmup, args.warmup + args.steps):
+        step(n)
+    lib.ftc_event_record(ev1, stream)
+    dom.wait()
+    wall = time.perf_counter() - t0
+    ms = ctypes.c_float()
+    lib.ftc_event_elapsed_ms(ev0, ev1, ctypes.byref(ms))
+    gpu_ms_per_step = ms.value / args.steps
+    wall_ms_per_step = wall * 1e3 / args.steps
+    gstats = dom.graph_stats()
+    dom.close()
+
+    P = (nx + 2) * (ny + 2) * (nz + 2)
+    I = nx * ny * nz
+    bytes_alg = 8 * (4 * P + 4 * I)
+    achieved = bytes_alg / (gpu_ms_per_step * 1e-3) / 1e9
+    traffic = traffic_from_profiles("les_kernel<15,true,true>")
+
+    cpu = None
+    parity = None
+    if args.cpu_steps:
+        from oracle import binding as oracle
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+        log(f"bench: CPU baseline with {threads} threads")
+        want = oracle.les_step(*initial, dx, dy, dz, dt, cs)  # warm-up, and the parity reference
+        parity = all(np.array_equal(g.view(np.uint64), w.view(np.uint64)) for g, w in zip(first, want))
+        t0 = time.perf_counter()
+        cur = want
+        for _ in range(args.cpu_steps):
+            cur = oracle.les_step(*cur, dx, dy, dz, dt, cs)
+        cpu_ms = (time.perf_counter() - t0) * 1e3 / args.cpu_steps
+        cpu = {"value": round(cpu_ms, 3), "unit": "ms/step", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} full {nx}x{ny}x{nz} steps after one warm-up step, OpenMP C oracle "
+                         f"(gcc -O2 -ffp-contract=off)"}
+
+    line = {
+        "metric": METRIC,
+        "value": round(gpu_ms_per_step, 4),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(gpu_ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"resident LES step: advection + Smagorinsky diffusion (Cs={cs}) + halo fill, "
+                               f"FP64, {nx}x{ny}x{nz} cells, one hipGraph replay per step",
+                   "nx": nx, "ny": ny, "nz": nz, "dx": dx, "dt": dt},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+        "wall_ms_per_step": round(wall_ms_per_step, 4),
+        "parity_first_step_bit_exact": parity,
+        "graph": {"instantiations": gstats.instantiations, "replays": gstats.replays,
+                  "parameter_updates": gstats.parameter_updates},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

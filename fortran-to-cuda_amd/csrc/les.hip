@@ -1,0 +1,540 @@
+// Staggered-grid LES stencils for gfx950: momentum advection (CDU, CDV, CDW), flux-form scalar advection,
+// Smagorinsky viscosity and eddy-viscosity diffusion, fused into one k-marching kernel, and the C-ABI
+// entries that submit it through the fort_scope queue.
+//
+// Numerical contract: this file is compiled with -ffp-contract=off and without fast-math, and every
+// expression below keeps the operand order of the Fortran nest it implements, so each output is the same
+// IEEE-754 binary64 result as the native build of those nests
+// (benchmarks/cdu/src/cdu.f90:48-107, compiler/tests/fixtures/stencils/{cvd,cdw}.f90:44-98).
+//
+// Data layout in HBM: every field is one column-major (nx+2, ny+2, nz+2) array of doubles with one ghost
+// layer, exactly the host layout the Fortran caller registered; i is the fastest index.
+//
+// Kernel shape: a workgroup owns a TX x TY tile of columns and marches a chunk of k planes. Each lane
+// computes one (i, j) point per plane, so every wavefront reads 64 consecutive doubles of a row
+// (coalesced 512-byte rows). The Smagorinsky viscosity of the planes k-1, k, k+1 over the tile plus a
+// one-cell ring lives in an LDS ring buffer of three planes: each plane is computed once per chunk
+// instead of once per consuming stencil point.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
+#include "entry.hpp"
+
+#pragma clang fp contract(off)
+
+namespace ftc {
+
+constexpr int TX = 64;
+constexpr int TY = 4;
+
+struct LesArgs {
+    const double *U, *V, *W, *S;
+    double *OU, *OV, *OW, *OS;
+    int nx, ny, nz;  // i loop 2..nx+1, j loop 2..ny+1, U/V/S k loop 2..nz+1
+    int kw;          // W k loop 2..kw
+    int kchunk;
+    int reserved;
+    long long sj, sk;                   // strides of j and k in elements
+    double c25[3], c50[3], c125[3];     // 0.25/d, 0.5/d, 0.125/d for d = dx, dy, dz
+    double sgs[10];                     // 1/d, 0.25/d, 1/d^2 (x, y, z each), (Cs*Delta)^2
+    double dt;
+};
+
+enum : int { kU = 1, kV = 2, kW = 4, kS = 8 };
+
+__device__ __forceinline__ double cdu_point(const LesArgs &a, long long p)
+{
+    const double *U = a.U, *V = a.V, *W = a.W;
+    const long long sj = a.sj, sk = a.sk;
+    const double u0 = U[p], uxp = U[p + 1], uxm = U[p - 1], uyp = U[p + sj], uym = U[p - sj];
+    const double uzp = U[p + sk], uzm = U[p - sk];
+    const double v0 = V[p], vxp = V[p + 1], vym = V[p - sj], vxpym = V[p + 1 - sj];
+    const double w0 = W[p], wxp = W[p + 1], wzm = W[p - sk], wxpzm = W[p + 1 - sk];
+    const double Ax = a.c25[0], Ay = a.c25[1], Az = a.c25[2];
+    double x = Ax * (uxp + u0) * (uxp + u0) - Ax * (u0 + uxm) * (u0 + uxm);
+    double y = Ay * (uyp + u0) * (vxp + v0) - Ay * (u0 + uym) * (vxpym + vym);
+    double z = Az * (uzp + u0) * (wxp + w0) - Az * (u0 + uzm) * (wxpzm + wzm);
+    double t = -((x + y) + z);
+    double Vadv = ((v0 + vxp) + vym) + vxpym;
+    double Wadv = ((w0 + wxp) + wzm) + wxpzm;
+    t = t - ((a.c50[0] * (uxp - uxm) * u0 + a.c125[1] * (uyp - uym) * Vadv) + a.c125[2] * (uzp - uzm) * Wadv);
+    return t * 0.5;
+}
+
+__device__ __forceinline__ double cdv_point(const LesArgs &a, long long p)
+{
+    const double *U = a.U, *V = a.V, *W = a.W;
+    const long long sj = a.sj, sk = a.sk;
+    const double v0 = V[p], vxp = V[p + 1], vxm = V[p - 1], vyp = V[p + sj], vym = V[p - sj];
+    const double vzp = V[p + sk], vzm = V[p - sk];
+    const double u0 = U[p], uyp = U[p + sj], uxm = U[p - 1], uxmyp = U[p - 1 + sj];
+    const double w0 = W[p], wyp = W[p + sj], wzm = W[p - sk], wypzm = W[p + sj - sk];
+    const double Ax = a.c25[0], Ay = a.c25[1], Az = a.c25[2];
+    double y = Ay * (vyp + v0) * (vyp + v0) - Ay * (v0 + vym) * (v0 + vym);
+    double x = Ax * (vxp + v0) * (uyp + u0) - Ax * (v0 + vxm) * (uxmyp + uxm);
+    double z = Az * (vzp + v0) * (wyp + w0) - Az * (v0 + vzm) * (wypzm + wzm);
+    double t = -((y + x) + z);
+    double Uadv = ((u0 + uyp) + uxm) + uxmyp;
+    double Wadv = ((w0 + wyp) + wzm) + wypzm;
+    t = t - ((a.c125[0] * (vxp - vxm) * Uadv + a.c50[1] * (vyp - vym) * v0) + a.c125[2] * (vzp - vzm) * Wadv);
+    return t * 0.5;
+}
+
+__device__ __forceinline__ double cdw_point(const LesArgs &a, long long p)
+{
+    const double *U = a.U, *V = a.V, *W = a.W;
+    const long long sj = a.sj, sk = a.sk;
+    const double w0 = W[p], wxp = W[p + 1], wxm = W[p - 1], wyp = W[p + sj], wym = W[p - sj];
+    const double wzp = W[p + sk], wzm = W[p - sk];
+    const double u0 = U[p], uzp = U[p + sk], uxm = U[p - 1], uxmzp = U[p - 1 + sk];
+    const double v0 = V[p], vzp = V[p + sk], vym = V[p - sj], vymzp = V[p - sj + sk];
+    const double Ax = a.c25[0], Ay = a.c25[1], Az = a.c25[2];
+    double z = Az * (wzp + w0) * (wzp + w0) - Az * (w0 + wzm) * (w0 + wzm);
+    double y = Ay * (wyp + w0) * (vzp + v0) - Ay * (w0 + wym) * (vym + vymzp);
+    double x = Ax * (wxp + w0) * (uzp + u0) - Ax * (w0 + wxm) * (uxmzp + uxm);
+    double t = -((z + y) + x);
+    double Uadv = ((u0 + uzp) + uxm) + uxmzp;
+    double Vadv = ((v0 + vzp) + vym) + vymzp;
+    t = t - ((a.c125[0] * (wxp - wxm) * Uadv + a.c125[1] * (wyp - wym) * Vadv) + a.c50[2] * (wzp - wzm) * w0);
+    return t * 0.5;
+}
+
+__device__ __forceinline__ double sadv_point(const LesArgs &a, long long p)
+{
+    const double *U = a.U, *V = a.V, *W = a.W, *S = a.S;
+    const long long sj = a.sj, sk = a.sk;
+    const double s0 = S[p];
+    const double Cx = a.c50[0], Cy = a.c50[1], Cz = a.c50[2];
+    double x = Cx * U[p] * (S[p + 1] + s0) - Cx * U[p - 1] * (s0 + S[p - 1]);
+    double y = Cy * V[p] * (S[p + sj] + s0) - Cy * V[p - sj] * (s0 + S[p - sj]);
+    double z = Cz * W[p] * (S[p + sk] + s0) - Cz * W[p - sk] * (s0 + S[p - sk]);
+    return -((x + y) + z);
+}
+
+// Smagorinsky viscosity at an interior cell whose linear index is p.
+__device__ __forceinline__ double nu_point(const LesArgs &a, long long p)
+{
+    const double *U = a.U, *V = a.V, *W = a.W;
+    const long long sj = a.sj, sk = a.sk;
+    const double *q = a.sgs;
+    double s11 = (U[p] - U[p - 1]) * q[0];
+    double s22 = (V[p] - V[p - sj]) * q[1];
+    double s33 = (W[p] - W[p - sk]) * q[2];
+    double dudy = ((U[p + sj] + U[p - 1 + sj]) - (U[p - sj] + U[p - 1 - sj])) * q[4];
+    double dvdx = ((V[p + 1] + V[p + 1 - sj]) - (V[p - 1] + V[p - 1 - sj])) * q[3];
+    double dudz = ((U[p + sk] + U[p - 1 + sk]) - (U[p - sk] + U[p - 1 - sk])) * q[5];
+    double dwdx = ((W[p + 1] + W[p + 1 - sk]) - (W[p - 1] + W[p - 1 - sk])) * q[3];
+    double dvdz = ((V[p + sk] + V[p - sj + sk]) - (V[p - sk] + V[p - sj - sk])) * q[5];
+    double dwdy = ((W[p + sj] + W[p + sj - sk]) - (W[p - sj] + W[p - sj - sk])) * q[4];
+    double s12 = 0.5 * (dudy + dvdx);
+    double s13 = 0.5 * (dudz + dwdx);
+    double s23 = 0.5 * (dvdz + dwdy);
+    double ss = 2.0 * ((s11 * s11 + s22 * s22) + s33 * s33) + 4.0 * ((s12 * s12 + s13 * s13) + s23 * s23);
+    return q[9] * sqrt(ss);
+}
+
+template <int R>
+using NuRing = double[R][TY + 2][TX + 2];
+
+// Fills ring slot kp % 3 with nu at plane kp over the tile and its one-cell ring. A ghost cell takes the
+// value of its image: periodic in x and y, zero-gradient in z.
+__device__ __forceinline__ void nu_plane(const LesArgs &a, NuRing<3> &ring, int kp, int i0, int j0, int tid)
+{
+    const int kk = kp < 2 ? 2 : (kp > a.nz + 1 ? a.nz + 1 : kp);
+    double(*plane)[TX + 2] = ring[kp % 3];
+    for (int c = tid; c < (TX + 2) * (TY + 2); c += TX * TY) {
+        const int li = c % (TX + 2), lj = c / (TX + 2);
+        const int gi = i0 - 1 + li, gj = j0 - 1 + lj;
+        double value = 0.0;
+        if (gi <= a.nx + 2 && gj <= a.ny + 2) {
+            const int ii = gi == 1 ? a.nx + 1 : (gi == a.nx + 2 ? 2 : gi);
+            const int jj = gj == 1 ? a.ny + 1 : (gj == a.ny + 2 ? 2 : gj);
+            value = nu_point(a, (long long)(ii - 1) + (long long)(jj - 1) * a.sj + (long long)(kk - 1) * a.sk);
+        }
+        plane[lj][li] = value;
+    }
+}
+
+// Writes a new interior value and every ghost whose image it is (the result of the x, then y, then z
+// halo passes). zmode: 0 tangential velocity (sign change at the walls), 1 wall-normal velocity (no z
+// ghost), 2 scalar (zero gradient).
+__device__ __forceinline__ void put_with_images(const LesArgs &a, double *F, long long p, double val, int i, int j,
+                                                int k, int zmode)
+{
+    long long ox[3] = {0, 0, 0}, oy[3] = {0, 0, 0}, oz[2] = {0, 0};
+    int nxo = 1, nyo = 1, nzo = 0;
+    if (i == 2) ox[nxo++] = a.nx;
+    if (i == a.nx + 1) ox[nxo++] = -a.nx;
+    if (j == 2) oy[nyo++] = (long long)a.ny * a.sj;
+    if (j == a.ny + 1) oy[nyo++] = -(long long)a.ny * a.sj;
+    if (zmode != 1) {
+        if (k == 2) oz[nzo++] = -a.sk;
+        if (k == a.nz + 1) oz[nzo++] = a.sk;
+    }
+    const double zval = zmode == 0 ? -val : val;
+    for (int x = 0; x < nxo; ++x)
+        for (int y = 0; y < nyo; ++y) {
+            const long long q = p + ox[x] + oy[y];
+            F[q] = val;
+            for (int z = 0; z < nzo; ++z) F[q + oz[z]] = zval;
+        }
+}
+
+template <int MASK, bool SGS, bool STEP>
+__global__ __launch_bounds__(TX *TY) void les_kernel(const LesArgs a)
+{
+    __shared__ NuRing<SGS ? 3 : 1> ring;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const int i0 = 2 + blockIdx.x * TX, j0 = 2 + blockIdx.y * TY;
+    const int i = i0 + tx, j = j0 + ty;
+    int klast = (MASK & (kU | kV | kS)) ? a.nz + 1 : 0;
+    if ((MASK & kW) && a.kw > klast) klast = a.kw;
+    const int k0 = 2 + (int)blockIdx.z * a.kchunk;
+    if (k0 > klast) return;  // uniform over the workgroup
+    const int k1 = min(k0 + a.kchunk - 1, klast);
+    const bool inside = i <= a.nx + 1 && j <= a.ny + 1;
+    if constexpr (SGS) {
+        nu_plane(a, ring, k0 - 1, i0, j0, tid);
+        nu_plane(a, ring, k0, i0, j0, tid);
+    }
+    const double dt = a.dt;
+    const double *q = a.sgs;
+    for (int k = k0; k <= k1; ++k) {
+        if constexpr (SGS) {
+            nu_plane(a, ring, k + 1, i0, j0, tid);
+            __syncthreads();
+        }
+        if (inside) {
+            const long long p = (long long)(i - 1) + (long long)(j - 1) * a.sj + (long long)(k - 1) * a.sk;
+            const int lx = tx + 1, ly = ty + 1;
+            const int slm = (k - 1) % 3, slz = k % 3, slp = (k + 1) % 3;
+#define NU(di, dj, dk) ring[(dk) < 0 ? slm : ((dk) > 0 ? slp : slz)][ly + (dj)][lx + (di)]
+            const long long sj = a.sj, sk = a.sk;
+            if ((MASK & kU) && k <= a.nz + 1) {
+                double t = cdu_point(a, p);
+                if constexpr (SGS) {
+                    const double *U = a.U;
+                    const double u0 = U[p];
+                    double fxp = NU(1, 0, 0) * (U[p + 1] - u0);
+                    double fxm = NU(0, 0, 0) * (u0 - U[p - 1]);
+                    double exy0 = 0.25 * (((NU(0, 0, 0) + NU(1, 0, 0)) + NU(0, 1, 0)) + NU(1, 1, 0));
+                    double exym = 0.25 * (((NU(0, -1, 0) + NU(1, -1, 0)) + NU(0, 0, 0)) + NU(1, 0, 0));
+                    double fyp = exy0 * (U[p + sj] - u0);
+                    double fym = exym * (u0 - U[p - sj]);
+                    double exz0 = 0.25 * (((NU(0, 0, 0) + NU(1, 0, 0)) + NU(0, 0, 1)) + NU(1, 0, 1));
+                    double exzm = 0.25 * (((NU(0, 0, -1) + NU(1, 0, -1)) + NU(0, 0, 0)) + NU(1, 0, 0));
+                    double fzp = exz0 * (U[p + sk] - u0);
+                    double fzm = exzm * (u0 - U[p - sk]);
+                    t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                }
+                if constexpr (STEP) put_with_images(a, a.OU, p, a.U[p] + dt * t, i, j, k, 0);
+                else a.OU[p] = t;
+            }
+            if ((MASK & kV) && k <= a.nz + 1) {
+                double t = cdv_point(a, p);
+                if constexpr (SGS) {
+                    const double *V = a.V;
+                    const double v0 = V[p];
+                    double exy0 = 0.25 * (((NU(0, 0, 0) + NU(1, 0, 0)) + NU(0, 1, 0)) + NU(1, 1, 0));
+                    double exyxm = 0.25 * (((NU(-1, 0, 0) + NU(0, 0, 0)) + NU(-1, 1, 0)) + NU(0, 1, 0));
+                    double fxp = exy0 * (V[p + 1] - v0);
+                    double fxm = exyxm * (v0 - V[p - 1]);
+                    double fyp = NU(0, 1, 0) * (V[p + sj] - v0);
+                    double fym = NU(0, 0, 0) * (v0 - V[p - sj]);
+                    double eyz0 = 0.25 * (((NU(0, 0, 0) + NU(0, 1, 0)) + NU(0, 0, 1)) + NU(0, 1, 1));
+                    double eyzm = 0.25 * (((NU(0, 0, -1) + NU(0, 1, -1)) + NU(0, 0, 0)) + NU(0, 1, 0));
+                    double fzp = eyz0 * (V[p + sk] - v0);
+                    double fzm = eyzm * (v0 - V[p - sk]);
+                    t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                }
+                if constexpr (STEP) put_with_images(a, a.OV, p, a.V[p] + dt * t, i, j, k, 0);
+                else a.OV[p] = t;
+            }
+            if ((MASK & kW) && k <= a.kw) {
+                double t = cdw_point(a, p);
+                if constexpr (SGS) {
+                    const double *W = a.W;
+                    const double w0 = W[p];
+                    double exz0 = 0.25 * (((NU(0, 0, 0) + NU(1, 0, 0)) + NU(0, 0, 1)) + NU(1, 0, 1));
+                    double exzxm = 0.25 * (((NU(-1, 0, 0) + NU(0, 0, 0)) + NU(-1, 0, 1)) + NU(0, 0, 1));
+                    double fxp = exz0 * (W[p + 1] - w0);
+                    double fxm = exzxm * (w0 - W[p - 1]);
+                    double eyz0 = 0.25 * (((NU(0, 0, 0) + NU(0, 1, 0)) + NU(0, 0, 1)) + NU(0, 1, 1));
+                    double eyzym = 0.25 * (((NU(0, -1, 0) + NU(0, 0, 0)) + NU(0, -1, 1)) + NU(0, 0, 1));
+                    double fyp = eyz0 * (W[p + sj] - w0);
+                    double fym = eyzym * (w0 - W[p - sj]);
+                    double fzp = NU(0, 0, 1) * (W[p + sk] - w0);
+                    double fzm = NU(0, 0, 0) * (w0 - W[p - sk]);
+                    t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                }
+                if constexpr (STEP) put_with_images(a, a.OW, p, a.W[p] + dt * t, i, j, k, 1);
+                else a.OW[p] = t;
+            }
+            if ((MASK & kS) && k <= a.nz + 1) {
+                double t = sadv_point(a, p);
+                if constexpr (SGS) {
+                    const double *S = a.S;
+                    const double s0 = S[p];
+                    double fxp = (0.5 * (NU(0, 0, 0) + NU(1, 0, 0))) * (S[p + 1] - s0);
+                    double fxm = (0.5 * (NU(-1, 0, 0) + NU(0, 0, 0))) * (s0 - S[p - 1]);
+                    double fyp = (0.5 * (NU(0, 0, 0) + NU(0, 1, 0))) * (S[p + sj] - s0);
+                    double fym = (0.5 * (NU(0, -1, 0) + NU(0, 0, 0))) * (s0 - S[p - sj]);
+                    double fzp = (0.5 * (NU(0, 0, 0) + NU(0, 0, 1))) * (S[p + sk] - s0);
+                    double fzm = (0.5 * (NU(0, 0, -1) + NU(0, 0, 0))) * (s0 - S[p - sk]);
+                    t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                }
+                if constexpr (STEP) put_with_images(a, a.OS, p, a.S[p] + dt * t, i, j, k, 2);
+                else a.OS[p] = t;
+            }
+#undef NU
+        }
+        if constexpr (SGS) __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ host
+static void fill_coefficients(LesArgs &a, double dx, double dy, double dz, double cs)
+{
+    const double d[3] = {dx, dy, dz};
+    for (int c = 0; c < 3; ++c) {
+        a.c25[c] = 0.25 / d[c];
+        a.c50[c] = 0.5 / d[c];
+        a.c125[c] = 0.125 / d[c];
+    }
+    const double delta = std::cbrt(dx * dy * dz);
+    a.sgs[0] = 1.0 / dx;
+    a.sgs[1] = 1.0 / dy;
+    a.sgs[2] = 1.0 / dz;
+    a.sgs[3] = 0.25 / dx;
+    a.sgs[4] = 0.25 / dy;
+    a.sgs[5] = 0.25 / dz;
+    a.sgs[6] = 1.0 / (dx * dx);
+    a.sgs[7] = 1.0 / (dy * dy);
+    a.sgs[8] = 1.0 / (dz * dz);
+    a.sgs[9] = (cs * delta) * (cs * delta);
+}
+
+// Launch geometry: enough k chunks that the grid holds at least ~8 workgroups per CU of a 256-CU
+// device, without chunks shorter than 4 planes (each chunk recomputes two viscosity planes).
+static int choose_kchunk(int nx, int ny, int kplanes)
+{
+    const long long tiles = (long long)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY);
+    const long long target = 2048;
+    int chunks = (int)std::max<long long>(1, (target + tiles - 1) / tiles);
+    int kchunk = (kplanes + chunks - 1) / chunks;
+    if (const char *v = std::getenv("FTC_LES_KCHUNK")) kchunk = std::atoi(v);
+    return std::max(std::min(kchunk, kplanes), std::min(4, kplanes));
+}
+
+template <int MASK, bool SGS, bool STEP>
+static const void *kernel_of()
+{
+    return reinterpret_cast<const void *>(&les_kernel<MASK, SGS, STEP>);
+}
+
+static dim3 grid_of(const LesArgs &a, int kplanes)
+{
+    return dim3((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY, (kplanes + a.kchunk - 1) / a.kchunk);
+}
+
+static Sec3 box(std::size_t i0, std::size_t i1, std::size_t j0, std::size_t j1, std::size_t k0, std::size_t k1)
+{
+    return Sec3{{i0, j0, k0}, {i1, j1, k1}};
+}
+
+// One reference advection component (CDU, CDV or CDW) on arrays of any extents >= its loop extents + 2.
+template <int MASK>
+static int advection_component(fort_scope_t ctx, int mode, fort_buffer_t out, fort_buffer_t u, fort_buffer_t v,
+                               fort_buffer_t w, const double *dxmin, const double *dymin, const double *dzmin,
+                               const int *pnx, const int *pny, const int *pnz)
+{
+    FTC_CHECK(entry_preamble(mode));
+    if (!dxmin || !dymin || !dzmin || !pnx || !pny || !pnz)
+        return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "null shared scalar argument");
+    const int nx = *pnx, ny = *pny, nz = *pnz;
+    std::size_t eo[3], eu[3], ev[3], ew[3];
+    FTC_CHECK(layout3(ctx, out, FORT_SCOPE_REAL64, 8, eo));
+    FTC_CHECK(layout3(ctx, u, FORT_SCOPE_REAL64, 8, eu));
+    FTC_CHECK(layout3(ctx, v, FORT_SCOPE_REAL64, 8, ev));
+    FTC_CHECK(layout3(ctx, w, FORT_SCOPE_REAL64, 8, ew));
+    for (int d = 0; d < 3; ++d)
+        if (eo[d] != eu[d] || ev[d] != eu[d] || ew[d] != eu[d])
+            return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "advection arrays of different shapes");
+    FTC_CHECK(distinct_outputs({out}, {u, v, w}));
+    if (nx <= 0 || ny <= 0 || nz <= 0) return FORT_SCOPE_OK;  // empty domains do not launch
+    if ((std::size_t)nx + 2 > eu[0] || (std::size_t)ny + 2 > eu[1] || (std::size_t)nz + 2 > eu[2])
+        return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "loop extents exceed the array extents");
+    FTC_CHECK(fort_scope_forget_definition(ctx, out));
+    std::vector<void *> dev;
+    FTC_CHECK(reserve(ctx, {out, u, v, w}, dev));
+    LesArgs a{};
+    a.U = (const double *)dev[1];
+    a.V = (const double *)dev[2];
+    a.W = (const double *)dev[3];
+    a.S = nullptr;
+    a.OU = (double *)dev[0];
+    a.OV = (double *)dev[0];
+    a.OW = (double *)dev[0];
+    a.OS = nullptr;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.kw = nz + 1;
+    a.sj = (long long)eu[0];
+    a.sk = (long long)eu[0] * (long long)eu[1];
+    fill_coefficients(a, *dxmin, *dymin, *dzmin, 0.0);
+    a.kchunk = choose_kchunk(nx, ny, nz);
+    const std::size_t X = nx, Y = ny, Z = nz;
+    std::vector<Binding3> b(4);
+    b[0] = {out, {}, {}, {box(1, X + 1, 1, Y + 1, 1, Z + 1)}};
+    if (MASK == kU) {
+        b[1] = {u, {box(0, X + 2, 0, Y + 2, 0, Z + 2)}, {}, {}};
+        b[2] = {v, {box(1, X + 2, 0, Y + 1, 1, Z + 1)}, {}, {}};
+        b[3] = {w, {box(1, X + 2, 1, Y + 1, 0, Z + 1)}, {}, {}};
+    } else if (MASK == kV) {
+        b[1] = {u, {box(0, X + 1, 1, Y + 2, 1, Z + 1)}, {}, {}};
+        b[2] = {v, {box(0, X + 2, 0, Y + 2, 0, Z + 2)}, {}, {}};
+        b[3] = {w, {box(1, X + 1, 1, Y + 2, 0, Z + 1)}, {}, {}};
+    } else {
+        b[1] = {u, {box(0, X + 1, 1, Y + 1, 1, Z + 2)}, {}, {}};
+        b[2] = {v, {box(1, X + 1, 0, Y + 1, 1, Z + 2)}, {}, {}};
+        b[3] = {w, {box(0, X + 2, 0, Y + 2, 0, Z + 2)}, {}, {}};
+    }
+    const uint64_t key = 0xadc0000000000000ull | MASK;
+    return enqueue_kernel(ctx, key, kernel_of<MASK, false, false>(), grid_of(a, nz), dim3(TX, TY, 1), 0, &a,
+                          sizeof a, b);
+}
+
+// The LES tendency (advection, optionally with Smagorinsky diffusion) or the explicit step built on it,
+// over nx x ny x nz cells with one ghost layer: U, V and S on all interior cells, W on the interior
+// faces k = 2..nz.
+static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fort_buffer_t ov, fort_buffer_t ow,
+                     fort_buffer_t os, fort_buffer_t u, fort_buffer_t v, fort_buffer_t w, fort_buffer_t s,
+                     const double *dx, const double *dy, const double *dz, const double *dt, const double *cs,
+                     const int *pnx, const int *pny, const int *pnz, const int *psgs)
+{
+    FTC_CHECK(entry_preamble(mode));
+    if (!dx || !dy || !dz || !cs || !pnx || !pny || !pnz || !psgs || (step && !dt))
+        return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "null shared scalar argument");
+    const int nx = *pnx, ny = *pny, nz = *pnz;
+    const bool sgs = *psgs != 0;
+    const fort_buffer_t all[8] = {ou, ov, ow, os, u, v, w, s};
+    std::size_t e[8][3];
+    for (int n = 0; n < 8; ++n) FTC_CHECK(layout3(ctx, all[n], FORT_SCOPE_REAL64, 8, e[n]));
+    for (int n = 1; n < 8; ++n)
+        for (int d = 0; d < 3; ++d)
+            if (e[n][d] != e[0][d])
+                return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "LES arrays of different shapes");
+    FTC_CHECK(distinct_outputs({ou, ov, ow, os}, {u, v, w, s}));
+    if (nx <= 0 || ny <= 0 || nz <= 0) return FORT_SCOPE_OK;
+    if (e[0][0] != (std::size_t)nx + 2 || e[0][1] != (std::size_t)ny + 2 || e[0][2] != (std::size_t)nz + 2)
+        return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "LES arrays must have one ghost layer");
+    if (!step)
+        for (int n = 0; n < 4; ++n) FTC_CHECK(fort_scope_forget_definition(ctx, all[n]));
+    std::vector<void *> dev;
+    FTC_CHECK(reserve(ctx, {ou, ov, ow, os, u, v, w, s}, dev));
+    LesArgs a{};
+    a.OU = (double *)dev[0];
+    a.OV = (double *)dev[1];
+    a.OW = (double *)dev[2];
+    a.OS = (double *)dev[3];
+    a.U = (const double *)dev[4];
+    a.V = (const double *)dev[5];
+    a.W = (const double *)dev[6];
+    a.S = (const double *)dev[7];
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.kw = nz;
+    a.sj = nx + 2;
+    a.sk = (long long)(nx + 2) * (ny + 2);
+    a.dt = step ? *dt : 0.0;
+    fill_coefficients(a, *dx, *dy, *dz, *cs);
+    a.kchunk = choose_kchunk(nx, ny, nz);
+    const std::size_t X = nx, Y = ny, Z = nz;
+    const Sec3 full = box(0, X + 2, 0, Y + 2, 0, Z + 2);
+    std::vector<Binding3> b(8);
+    if (step) {
+        b[0] = {ou, {}, {}, {full}};
+        b[1] = {ov, {}, {}, {full}};
+        b[2] = {ow, {}, {}, {box(0, X + 2, 0, Y + 2, 1, Z)}};
+        b[3] = {os, {}, {}, {full}};
+    } else {
+        const Sec3 interior = box(1, X + 1, 1, Y + 1, 1, Z + 1);
+        b[0] = {ou, {}, {}, {interior}};
+        b[1] = {ov, {}, {}, {interior}};
+        b[2] = {ow, {}, {}, {box(1, X + 1, 1, Y + 1, 1, Z)}};
+        b[3] = {os, {}, {}, {interior}};
+    }
+    b[4] = {u, {full}, {}, {}};
+    b[5] = {v, {full}, {}, {}};
+    b[6] = {w, {box(0, X + 2, 0, Y + 2, 0, Z + 1)}, {}, {}};
+    b[7] = {s, {full}, {}, {}};
+    const void *fn;
+    if (step) fn = sgs ? kernel_of<15, true, true>() : kernel_of<15, false, true>();
+    else fn = sgs ? kernel_of<15, true, false>() : kernel_of<15, false, false>();
+    const uint64_t key = 0x1e5000000000000full ^ ((uint64_t)step << 8) ^ ((uint64_t)sgs << 9);
+    return enqueue_kernel(ctx, key, fn, grid_of(a, nz), dim3(TX, TY, 1), 0, &a, sizeof a, b);
+}
+
+}  // namespace ftc
+
+using namespace ftc;
+
+extern "C" {
+
+int cpp_fort_shared_momentumadvection_cdu(fort_scope_t ctx, int mode, fort_buffer_t u2, fort_buffer_t u,
+                                          fort_buffer_t v, fort_buffer_t w, const double *dxmin,
+                                          const double *dymin, const double *dzmin, const int *unx,
+                                          const int *uny, const int *unz)
+{
+    return advection_component<kU>(ctx, mode, u2, u, v, w, dxmin, dymin, dzmin, unx, uny, unz);
+}
+
+int cpp_fort_shared_momentumadvection_cdv(fort_scope_t ctx, int mode, fort_buffer_t v2, fort_buffer_t u,
+                                          fort_buffer_t v, fort_buffer_t w, const double *dxmin,
+                                          const double *dymin, const double *dzmin, const int *vnx,
+                                          const int *vny, const int *vnz)
+{
+    return advection_component<kV>(ctx, mode, v2, u, v, w, dxmin, dymin, dzmin, vnx, vny, vnz);
+}
+
+int cpp_fort_shared_momentumadvection_cdw(fort_scope_t ctx, int mode, fort_buffer_t w2, fort_buffer_t u,
+                                          fort_buffer_t v, fort_buffer_t w, const double *dxmin,
+                                          const double *dymin, const double *dzmin, const int *wnx,
+                                          const int *wny, const int *wnz)
+{
+    return advection_component<kW>(ctx, mode, w2, u, v, w, dxmin, dymin, dzmin, wnx, wny, wnz);
+}
+
+int cpp_fort_shared_les_tendency(fort_scope_t ctx, int mode, fort_buffer_t tu, fort_buffer_t tv, fort_buffer_t tw,
+                                 fort_buffer_t ts, fort_buffer_t u, fort_buffer_t v, fort_buffer_t w,
+                                 fort_buffer_t s, const double *dx, const double *dy, const double *dz,
+                                 const double *cs, const int *nx, const int *ny, const int *nz, const int *sgs)
+{
+    return les_entry(false, ctx, mode, tu, tv, tw, ts, u, v, w, s, dx, dy, dz, nullptr, cs, nx, ny, nz, sgs);
+}
+
+int cpp_fort_shared_les_step(fort_scope_t ctx, int mode, fort_buffer_t un, fort_buffer_t vn, fort_buffer_t wn,
+                             fort_buffer_t sn, fort_buffer_t u, fort_buffer_t v, fort_buffer_t w, fort_buffer_t s,
+                             const double *dx, const double *dy, const double *dz, const double *dt,
+                             const double *cs, const int *nx, const int *ny, const int *nz, const int *sgs)
+{
+    return les_entry(true, ctx, mode, un, vn, wn, sn, u, v, w, s, dx, dy, dz, dt, cs, nx, ny, nz, sgs);
+}
+
+// The host-side coefficients every LES entry passes to its kernel, for callers that restate the step.
+void ftc_les_coefficients(double dx, double dy, double dz, double cs, double *out19)
+{
+    LesArgs a{};
+    fill_coefficients(a, dx, dy, dz, cs);
+    for (int c = 0; c < 3; ++c) {
+        out19[c] = a.c25[c];
+        out19[3 + c] = a.c50[c];
+        out19[6 + c] = a.c125[c];
+    }
+    for (int c = 0; c < 10; ++c) out19[9 + c] = a.sgs[c];
+}
+
+}  // extern "C"

@@ -1,0 +1,1637 @@
+// fort_scope runtime for MI355X: residency domains, HBM mirrors of borrowed host arrays with exact section
+// coherence, a task queue that submits each batch as one hipGraph, scratch leases and the external-provider
+// transaction. The ABI it implements is declared in include/fort_scope.h.
+//
+// Reference behaviour followed (compiler/runtime/scoped_runtime.cu): registration and layouts (:1834-1900),
+// prepared device/host accesses with uploads of only the missing sections (:899-954, :1010-1036), downloads
+// on observation or host access (:734-777, :873-885), LRU eviction under a byte budget (:1434-1529), the
+// command queue with graph replay (:1038-1149, :1227-1401), poisoning after execution errors (:363-384).
+//
+// Built twice: with hipcc for the device backend, and with g++ -DFTC_CPU_BACKEND as a reference backend
+// whose "device" memory is host memory and whose graph submission returns RESOURCE, so that coherence,
+// eviction and queue bookkeeping are testable without a GPU.
+#include "../../include/fort_scope.h"
+#include "regions.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cfenv>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#ifndef FTC_CPU_BACKEND
+#include <hip/hip_runtime_api.h>
+#endif
+
+namespace ftc {
+
+// ------------------------------------------------------------------------------------------------ errors
+struct Error {
+    int status;
+    std::string message;
+};
+static thread_local std::string g_error;
+
+[[noreturn]] static void fail(int status, const std::string &message) { throw Error{status, message}; }
+static void require(bool ok, int status, const char *message)
+{
+    if (!ok) fail(status, message);
+}
+
+// ------------------------------------------------------------------------------------------------ device
+namespace dev {
+#ifdef FTC_CPU_BACKEND
+using Stream = void *;
+static int set(int) { return 0; }
+static int current() { return 0; }
+static void *alloc(std::size_t n) { return std::malloc(n ? n : 1); }
+static void release(void *p) { std::free(p); }
+static void copy2d(void *dst, std::size_t dpitch, const void *src, std::size_t spitch, std::size_t width,
+                   std::size_t height, bool, Stream)
+{
+    for (std::size_t r = 0; r < height; ++r)
+        std::memcpy(static_cast<char *>(dst) + r * dpitch, static_cast<const char *>(src) + r * spitch, width);
+}
+static void sync(Stream) {}
+static Stream make_stream() { return nullptr; }
+static void destroy_stream(Stream) {}
+static std::size_t free_bytes() { return std::size_t(64) << 30; }
+static std::size_t total_bytes() { return std::size_t(64) << 30; }
+#else
+using Stream = hipStream_t;
+static void check(hipError_t e, int status, const char *what)
+{
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(status, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+static int set(int d)
+{
+    check(hipSetDevice(d), FORT_SCOPE_RESOURCE, "hipSetDevice");
+    return 0;
+}
+static int current()
+{
+    int d = 0;
+    check(hipGetDevice(&d), FORT_SCOPE_RESOURCE, "hipGetDevice");
+    return d;
+}
+static void *alloc(std::size_t n)
+{
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, n ? n : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+static void release(void *p) { (void)hipFree(p); }
+static void copy2d(void *dst, std::size_t dpitch, const void *src, std::size_t spitch, std::size_t width,
+                   std::size_t height, bool h2d, Stream s)
+{
+    check(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height,
+                           h2d ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, s),
+          FORT_SCOPE_EXECUTION, "hipMemcpy2DAsync");
+}
+static void sync(Stream s) { check(hipStreamSynchronize(s), FORT_SCOPE_EXECUTION, "hipStreamSynchronize"); }
+static Stream make_stream()
+{
+    hipStream_t s = nullptr;
+    check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), FORT_SCOPE_RESOURCE, "hipStreamCreate");
+    return s;
+}
+static void destroy_stream(Stream s)
+{
+    if (s) (void)hipStreamDestroy(s);
+}
+static std::size_t free_bytes()
+{
+    std::size_t f = 0, t = 0;
+    check(hipMemGetInfo(&f, &t), FORT_SCOPE_RESOURCE, "hipMemGetInfo");
+    return f;
+}
+static std::size_t total_bytes()
+{
+    std::size_t f = 0, t = 0;
+    check(hipMemGetInfo(&f, &t), FORT_SCOPE_RESOURCE, "hipMemGetInfo");
+    return t;
+}
+#endif
+}  // namespace dev
+
+// ------------------------------------------------------------------------------------------------ state
+struct Domain;
+struct Context;
+
+struct Buffer {
+    fort_buffer_t handle = 0;
+    Domain *domain = nullptr;
+    uint64_t identity = 0, generation = 0;
+    uint32_t rank = 0, type = 0;
+    std::size_t elem = 0;
+    void *host = nullptr;
+    std::size_t extents[kMaxRank] = {};
+    int64_t lower_bounds[kMaxRank] = {};
+    std::size_t count = 0, bytes = 0;
+    void *device = nullptr;
+    Region init, host_cur, dev_cur;
+    // A prepared access: 0 none, 1 device, 2 host; with the sections it may change.
+    int prepared = 0;
+    std::vector<Box> prep_writes, prep_overwrites;
+    uint64_t last_use = 0;
+    int queued = 0;  // accepted tasks not yet submitted that use this buffer
+
+    Box full() const
+    {
+        Box b;
+        b.rank = static_cast<int>(rank);
+        for (uint32_t d = 0; d < rank; ++d) b.hi[d] = extents[d];
+        return b;
+    }
+};
+
+struct Domain {
+    fort_scope_domain_t handle = 0;
+    int device = 0;
+    std::size_t budget = 0;
+    bool budget_known = false;
+    std::map<fort_buffer_t, std::unique_ptr<Buffer>> buffers;
+    std::map<uint64_t, fort_buffer_t> by_identity;
+    Context *borrower = nullptr;
+    bool is_private = false;
+    bool failed = false;
+    std::size_t device_bytes = 0, workspace_bytes = 0;
+    std::map<uint64_t, std::size_t> workspace;
+    uint64_t next_lease = 1;
+    uint64_t clock = 0;
+    fort_scope_stats stats{};
+    fort_scope_residency_stats_v1 residency{};
+    std::mutex mu;
+};
+
+struct Task {
+    uint64_t code_key = 0;
+    const void *function = nullptr;
+    uint32_t grid[3] = {}, block[3] = {};
+    std::size_t shared = 0;
+    std::vector<unsigned char> args;
+    std::vector<std::size_t> offsets, sizes;
+    std::vector<Buffer *> buffers;
+    uint64_t threads = 0;
+};
+
+#ifndef FTC_CPU_BACKEND
+// One instantiated graph of a topology, with the argument values and launch shapes it holds.
+struct GraphEntry {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraphNode_t> nodes;
+    std::vector<std::vector<unsigned char>> args;
+    std::vector<std::vector<uint32_t>> dims;
+    uint64_t last_use = 0;
+};
+// A topology (the ordered kernels and their argument sizes) and up to kGraphVariants instantiations of
+// it, so that a step alternating between buffer pairs replays without parameter updates.
+constexpr std::size_t kGraphVariants = 4;
+struct GraphFamily {
+    uint64_t seen = 0, clock = 0;
+    std::vector<GraphEntry> variants;
+};
+#endif
+
+struct Context {
+    fort_scope_t handle = 0;
+    Domain *domain = nullptr;
+    bool owns_domain = false;
+    dev::Stream stream = nullptr;
+    bool stream_ready = false;
+    bool poisoned = false;
+    std::vector<Task> queue;
+    uint64_t queued_threads = 0;
+    bool queue_enabled = true, graphs_enabled = true;
+    uint64_t queue_work = uint64_t(1) << 18;
+    fort_scope_graph_stats_v1 gstats{};
+    fort_scope_graph_queue_stats_v1 qstats{};
+#ifndef FTC_CPU_BACKEND
+    std::unordered_map<uint64_t, GraphFamily> graphs;
+#endif
+    void *scratch = nullptr;
+    std::size_t scratch_capacity = 0;
+    uint64_t scratch_token = 0, next_token = 1;
+    bool scratch_active = false;
+};
+
+static std::mutex g_registry;
+static std::map<fort_scope_t, std::unique_ptr<Context>> g_contexts;
+static std::map<fort_scope_domain_t, std::unique_ptr<Domain>> g_domains;
+static std::atomic<uint64_t> g_next_handle{0x5c0e000000000001ull};
+
+static uint64_t fresh_handle() { return g_next_handle.fetch_add(0x9e37ull); }
+
+static uint64_t env_u64(const char *name, uint64_t fallback)
+{
+    const char *v = std::getenv(name);
+    if (!v || !*v) return fallback;
+    char *end = nullptr;
+    unsigned long long x = std::strtoull(v, &end, 10);
+    return end && *end == 0 ? x : fallback;
+}
+
+// Parses "123", "64M", "2G" (binary multiples).
+static bool parse_bytes(const char *v, std::size_t *out)
+{
+    if (!v || !*v) return false;
+    char *end = nullptr;
+    unsigned long long x = std::strtoull(v, &end, 10);
+    if (end == v) return false;
+    unsigned shift = 0;
+    if (*end == 'K' || *end == 'k') shift = 10, ++end;
+    else if (*end == 'M' || *end == 'm') shift = 20, ++end;
+    else if (*end == 'G' || *end == 'g') shift = 30, ++end;
+    if (*end) return false;
+    *out = static_cast<std::size_t>(x) << shift;
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------------ lookup
+static Context &context(fort_scope_t h)
+{
+    auto it = g_contexts.find(h);
+    require(it != g_contexts.end(), FORT_SCOPE_STALE, "unknown or closed scoped context");
+    return *it->second;
+}
+
+static Domain &domain_of(fort_scope_domain_t h)
+{
+    auto it = g_domains.find(h);
+    require(it != g_domains.end(), FORT_SCOPE_STALE, "unknown or closed residency domain");
+    return *it->second;
+}
+
+static Buffer &buffer(Context &c, fort_buffer_t h)
+{
+    auto it = c.domain->buffers.find(h);
+    require(it != c.domain->buffers.end(), FORT_SCOPE_STALE, "unknown or retired buffer handle");
+    return *it->second;
+}
+
+// ------------------------------------------------------------------------------------------------ copies
+// Column-major strides of a buffer in elements.
+static void strides(const Buffer &b, std::size_t *s)
+{
+    s[0] = 1;
+    for (uint32_t d = 1; d < b.rank; ++d) s[d] = s[d - 1] * b.extents[d - 1];
+}
+
+// Copies one box between the host array and its mirror: the leading dimensions the box spans completely
+// form one contiguous run, the next dimension is the row count of a pitched 2D copy, and the remaining
+// dimensions are iterated (compiler/runtime/section_copy.hpp:14-80 plans the same decomposition).
+static void copy_box(Context &c, Buffer &b, const Box &box, bool h2d)
+{
+    if (box.empty()) return;
+    std::size_t s[kMaxRank];
+    strides(b, s);
+    int r = static_cast<int>(b.rank);
+    int m = 0;  // dims [0, m) are spanned completely
+    while (m < r - 1 && box.lo[m] == 0 && box.hi[m] == b.extents[m]) ++m;
+    std::size_t run = (box.hi[m] - box.lo[m]) * s[m] * b.elem;
+    int row_dim = m + 1;
+    std::size_t rows = row_dim < r ? box.hi[row_dim] - box.lo[row_dim] : 1;
+    std::size_t pitch = row_dim < r ? s[row_dim] * b.elem : run;
+    std::size_t idx[kMaxRank];
+    for (int d = 0; d < r; ++d) idx[d] = box.lo[d];
+    for (;;) {
+        std::size_t off = 0;
+        for (int d = 0; d < r; ++d) off += idx[d] * s[d];
+        off *= b.elem;
+        char *h = static_cast<char *>(b.host) + off;
+        char *g = static_cast<char *>(b.device) + off;
+        if (h2d) dev::copy2d(g, pitch, h, pitch, run, rows, true, c.stream);
+        else dev::copy2d(h, pitch, g, pitch, run, rows, false, c.stream);
+        int d = row_dim + 1;
+        while (d < r) {
+            if (++idx[d] < box.hi[d]) break;
+            idx[d] = box.lo[d];
+            ++d;
+        }
+        if (d >= r) break;
+    }
+    std::size_t bytes = box.volume() * b.elem;
+    if (h2d) {
+        c.domain->stats.uploads++;
+        c.domain->stats.upload_bytes += bytes;
+    } else {
+        c.domain->stats.downloads++;
+        c.domain->stats.download_bytes += bytes;
+    }
+}
+
+static void ensure_stream(Context &c)
+{
+    if (c.stream_ready) return;
+    dev::set(c.domain->device);
+    c.stream = dev::make_stream();
+    c.stream_ready = true;
+}
+
+static void wait(Context &c)
+{
+    if (!c.stream_ready) return;
+    dev::sync(c.stream);
+    c.domain->stats.waits++;
+}
+
+// ------------------------------------------------------------------------------------------------ queue
+static void flush(Context &c);
+
+static void drain_buffer(Context &c, const Buffer &b)
+{
+    if (b.queued > 0) {
+        c.qstats.conflict_flushes++;
+        flush(c);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ memory
+static void evict_one(Context &c, Buffer &victim)
+{
+    // Publish the values only the device holds, then drop the mirror.
+    std::vector<Box> dirty;
+    for (const Box &box : victim.dev_cur.boxes)
+        for (const Box &m : victim.host_cur.missing(box)) dirty.push_back(m);
+    for (const Box &box : dirty) {
+        copy_box(c, victim, box, false);
+        c.domain->residency.dirty_download_bytes += box.volume() * victim.elem;
+    }
+    if (!dirty.empty()) {
+        wait(c);
+        c.domain->residency.dirty_evictions++;
+        for (const Box &box : dirty) require(victim.host_cur.add(box), FORT_SCOPE_RESOURCE, "region limit");
+    }
+    dev::release(victim.device);
+    victim.device = nullptr;
+    victim.dev_cur.clear();
+    c.domain->device_bytes -= victim.bytes;
+    c.domain->residency.evictions++;
+    c.domain->residency.evicted_bytes += victim.bytes;
+}
+
+static std::size_t budget_of(Domain &d)
+{
+    if (!d.budget_known) {
+        int device = d.device;
+        std::size_t bytes = 0;
+        if (fort_scope_device_budget_default_v1(device, &bytes) != FORT_SCOPE_OK)
+            fail(FORT_SCOPE_RESOURCE, g_error);
+        d.budget = bytes;
+        d.budget_known = true;
+    }
+    return d.budget;
+}
+
+// Gives every requested buffer a device allocation, evicting least-recently-used unrequested mirrors
+// that no prepared access or queued task holds.
+static void reserve(Context &c, Buffer *const *list, std::size_t n)
+{
+    Domain &d = *c.domain;
+    d.residency.reserve_batches++;
+    d.residency.requested_resources += n;
+    std::size_t need = 0;
+    for (std::size_t i = 0; i < n; ++i) {
+        bool dup = false;
+        for (std::size_t j = 0; j < i; ++j) dup |= list[j] == list[i];
+        if (!dup && !list[i]->device) need += list[i]->bytes;
+        if (list[i]->device) d.residency.reuse_hits++;
+    }
+    std::size_t budget = budget_of(d);
+    std::size_t pinned = d.workspace_bytes + c.scratch_capacity;
+    require(need <= budget, FORT_SCOPE_RESOURCE, "task footprint exceeds the device budget");
+    if (d.device_bytes + pinned + need > budget && !c.queue.empty()) {
+        d.residency.pressure_flushes++;
+        flush(c);
+    }
+    while (d.device_bytes + pinned + need > budget) {
+        Buffer *victim = nullptr;
+        for (auto &kv : d.buffers) {
+            Buffer *b = kv.second.get();
+            if (!b->device || b->prepared || b->queued) continue;
+            bool requested = false;
+            for (std::size_t i = 0; i < n; ++i) requested |= list[i] == b;
+            if (requested) continue;
+            if (!victim || b->last_use < victim->last_use) victim = b;
+        }
+        require(victim != nullptr, FORT_SCOPE_RESOURCE, "device budget exhausted by pinned buffers");
+        evict_one(c, *victim);
+    }
+    for (std::size_t i = 0; i < n; ++i) {
+        Buffer &b = *list[i];
+        b.last_use = ++d.clock;
+        if (b.device) continue;
+        dev::set(d.device);
+        b.device = dev::alloc(b.bytes);
+        require(b.device != nullptr, FORT_SCOPE_RESOURCE, "device allocation failed");
+        d.device_bytes += b.bytes;
+        d.stats.allocations++;
+        d.stats.allocated_bytes += b.bytes;
+        std::size_t total = d.device_bytes + d.workspace_bytes + c.scratch_capacity;
+        if (total > d.stats.peak_device_bytes) d.stats.peak_device_bytes = total;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ access
+struct Effects {
+    std::vector<Box> reads, writes, overwrites;
+};
+
+static Box section_box(const Buffer &b, const fort_scope_section &s)
+{
+    require(s.lower && s.upper, FORT_SCOPE_ARGUMENT, "section without bounds");
+    Box box;
+    box.rank = static_cast<int>(b.rank);
+    for (uint32_t d = 0; d < b.rank; ++d) {
+        require(s.lower[d] <= s.upper[d] && s.upper[d] <= b.extents[d], FORT_SCOPE_ARGUMENT,
+                "section outside the registered layout");
+        box.lo[d] = s.lower[d];
+        box.hi[d] = s.upper[d];
+    }
+    return box;
+}
+
+static Effects effects(const Buffer &b, const fort_scope_access *a)
+{
+    Effects e;
+    require(a != nullptr, FORT_SCOPE_ARGUMENT, "missing access description");
+    require((a->flags & ~7u) == 0, FORT_SCOPE_ARGUMENT, "unknown access flags");
+    if (a->flags & FORT_SCOPE_READ_ALL) e.reads.push_back(b.full());
+    if (a->flags & FORT_SCOPE_WRITE_ALL) e.writes.push_back(b.full());
+    if (a->flags & FORT_SCOPE_OVERWRITE_ALL) e.overwrites.push_back(b.full());
+    require(!a->read_count || a->reads, FORT_SCOPE_ARGUMENT, "missing read sections");
+    require(!a->write_count || a->writes, FORT_SCOPE_ARGUMENT, "missing write sections");
+    require(!a->overwrite_count || a->overwrites, FORT_SCOPE_ARGUMENT, "missing overwrite sections");
+    for (std::size_t i = 0; i < a->read_count; ++i) e.reads.push_back(section_box(b, a->reads[i]));
+    for (std::size_t i = 0; i < a->write_count; ++i) e.writes.push_back(section_box(b, a->writes[i]));
+    for (std::size_t i = 0; i < a->overwrite_count; ++i) e.overwrites.push_back(section_box(b, a->overwrites[i]));
+    for (Box &x : e.reads) require(!x.empty() || true, FORT_SCOPE_ARGUMENT, "");
+    return e;
+}
+
+// Checks that every read is defined; returns the boxes to upload for a device access (reads, and the
+// defined parts of partial writes, that the device does not hold) or to download for a host access.
+static std::vector<Box> plan_transfer(const Buffer &b, const Effects &e, bool device)
+{
+    for (const Box &r : e.reads)
+        require(b.init.covers(r), FORT_SCOPE_UNINITIALIZED, "task reads values that were never defined");
+    const Region &have = device ? b.dev_cur : b.host_cur;
+    std::vector<Box> need;
+    Region acc;
+    auto want = [&](const Box &box) {
+        for (const Box &defined : b.init.within(box))
+            for (const Box &m : have.missing(defined)) {
+                for (const Box &fresh : acc.missing(m)) {
+                    need.push_back(fresh);
+                    require(acc.add(fresh), FORT_SCOPE_RESOURCE, "region limit exceeded");
+                }
+            }
+    };
+    for (const Box &r : e.reads) want(r);
+    for (const Box &w : e.writes) want(w);
+    return need;
+}
+
+// Commits the effects of a finished (or accepted) access to the coherence state.
+static void commit(Buffer &b, const std::vector<Box> &writes, const std::vector<Box> &overwrites, bool device)
+{
+    Region &mine = device ? b.dev_cur : b.host_cur;
+    Region &other = device ? b.host_cur : b.dev_cur;
+    for (const Box &o : overwrites) require(b.init.add(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
+    for (const Box &w : writes) {
+        // A partial write changes only defined values or values it defines itself.
+        require(mine.add(w), FORT_SCOPE_RESOURCE, "region limit exceeded");
+        require(other.remove(w), FORT_SCOPE_RESOURCE, "region limit exceeded");
+    }
+    for (const Box &o : overwrites) {
+        require(mine.add(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
+        require(other.remove(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
+    }
+    // What is current where must stay within what is defined.
+    Region keep;
+    for (const Box &i : b.init.boxes)
+        for (const Box &x : mine.within(i)) require(keep.add(x), FORT_SCOPE_RESOURCE, "region limit exceeded");
+    mine = keep;
+}
+
+static void prepare_device(Context &c, const fort_scope_plan_binding *bindings, std::size_t count, void **devices,
+                           bool commit_now)
+{
+    require(!count || bindings, FORT_SCOPE_ARGUMENT, "missing task access bindings");
+    require(!count || devices, FORT_SCOPE_ARGUMENT, "missing device pointer outputs");
+    std::vector<Buffer *> list;
+    std::vector<Effects> eff;
+    for (std::size_t i = 0; i < count; ++i) {
+        Buffer &b = buffer(c, bindings[i].buffer);
+        for (Buffer *o : list) require(o != &b, FORT_SCOPE_ALIAS, "task access bindings repeat a canonical buffer");
+        require(!b.prepared, FORT_SCOPE_STATE, "buffer already has a prepared access");
+        list.push_back(&b);
+        eff.push_back(effects(b, &bindings[i].access));
+    }
+    std::vector<std::vector<Box>> uploads;
+    for (std::size_t i = 0; i < count; ++i) uploads.push_back(plan_transfer(*list[i], eff[i], true));
+    if (!count) return;
+    ensure_stream(c);
+    reserve(c, list.data(), list.size());
+    for (std::size_t i = 0; i < count; ++i) {
+        Buffer &b = *list[i];
+        for (const Box &box : uploads[i]) {
+            copy_box(c, b, box, true);
+            require(b.dev_cur.add(box), FORT_SCOPE_RESOURCE, "region limit exceeded");
+        }
+        devices[i] = b.device;
+        if (commit_now) {
+            commit(b, eff[i].writes, eff[i].overwrites, true);
+        } else {
+            b.prepared = 1;
+            b.prep_writes = eff[i].writes;
+            b.prep_overwrites = eff[i].overwrites;
+        }
+    }
+}
+
+static void end_access(Context &c, Buffer &b, bool device, bool cancel)
+{
+    require(b.prepared == (device ? 1 : 2), FORT_SCOPE_STATE, "no prepared access of this kind on the buffer");
+    if (!cancel) commit(b, b.prep_writes, b.prep_overwrites, device);
+    if (!cancel && device) c.domain->stats.launches++;
+    b.prepared = 0;
+    b.prep_writes.clear();
+    b.prep_overwrites.clear();
+}
+
+static void download(Context &c, Buffer &b, const std::vector<Box> &boxes)
+{
+    if (boxes.empty()) return;
+    require(b.device != nullptr, FORT_SCOPE_STATE, "defined values are on no copy");
+    for (const Box &box : boxes) copy_box(c, b, box, false);
+    wait(c);
+    for (const Box &box : boxes) require(b.host_cur.add(box), FORT_SCOPE_RESOURCE, "region limit exceeded");
+}
+
+static void observe(Context &c, Buffer &b)
+{
+    drain_buffer(c, b);
+    std::vector<Box> need;
+    for (const Box &i : b.init.boxes)
+        for (const Box &m : b.host_cur.missing(i)) need.push_back(m);
+    download(c, b, need);
+}
+
+// ------------------------------------------------------------------------------------------------ graphs
+static uint64_t fnv(uint64_t h, const void *p, std::size_t n)
+{
+    const unsigned char *s = static_cast<const unsigned char *>(p);
+    for (std::size_t i = 0; i < n; ++i) {
+        h ^= s[i];
+        h *= 0x100000001b3ull;
+    }
+    return h;
+}
+
+static void launch_direct(Context &c, Task &t)
+{
+#ifdef FTC_CPU_BACKEND
+    (void)c;
+    (void)t;
+    fail(FORT_SCOPE_RESOURCE, "the CPU reference backend executes no device kernels");
+#else
+    std::vector<void *> params(t.offsets.size());
+    for (std::size_t i = 0; i < t.offsets.size(); ++i) params[i] = t.args.data() + t.offsets[i];
+    hipError_t e = hipLaunchKernel(t.function, dim3(t.grid[0], t.grid[1], t.grid[2]),
+                                   dim3(t.block[0], t.block[1], t.block[2]), params.data(), t.shared, c.stream);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c.poisoned = true;
+        fail(FORT_SCOPE_EXECUTION, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    }
+#endif
+}
+
+#ifndef FTC_CPU_BACKEND
+static hipKernelNodeParams node_params(Task &t, std::vector<void *> &params)
+{
+    params.resize(t.offsets.size());
+    for (std::size_t i = 0; i < t.offsets.size(); ++i) params[i] = t.args.data() + t.offsets[i];
+    hipKernelNodeParams p{};
+    p.blockDim = dim3(t.block[0], t.block[1], t.block[2]);
+    p.gridDim = dim3(t.grid[0], t.grid[1], t.grid[2]);
+    p.func = const_cast<void *>(t.function);
+    p.kernelParams = params.data();
+    p.sharedMemBytes = static_cast<unsigned>(t.shared);
+    p.extra = nullptr;
+    return p;
+}
+
+static void destroy_graph(Context &c, GraphEntry &g)
+{
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    if (g.exec) c.gstats.destructions++;
+    g.exec = nullptr;
+    g.graph = nullptr;
+    g.nodes.clear();
+}
+
+// Builds a chain graph (each node depends on its predecessor, so kernel boundaries stay ordered) and
+// instantiates it. Returns false if a graph resource could not be created; nothing was launched then.
+static bool build_graph(Context &c, GraphEntry &g, std::vector<Task> &tasks)
+{
+    destroy_graph(c, g);
+    if (hipGraphCreate(&g.graph, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    hipGraphNode_t prev = nullptr;
+    for (Task &t : tasks) {
+        std::vector<void *> params;
+        hipKernelNodeParams p = node_params(t, params);
+        hipGraphNode_t node = nullptr;
+        if (hipGraphAddKernelNode(&node, g.graph, prev ? &prev : nullptr, prev ? 1 : 0, &p) != hipSuccess) {
+            (void)hipGetLastError();
+            destroy_graph(c, g);
+            return false;
+        }
+        g.nodes.push_back(node);
+        prev = node;
+    }
+    if (hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        destroy_graph(c, g);
+        return false;
+    }
+    c.gstats.creations++;
+    c.gstats.instantiations++;
+    g.args.clear();
+    g.dims.clear();
+    for (Task &t : tasks) {
+        g.args.push_back(t.args);
+        g.dims.push_back({t.grid[0], t.grid[1], t.grid[2], t.block[0], t.block[1], t.block[2],
+                          static_cast<uint32_t>(t.shared)});
+    }
+    return true;
+}
+#endif
+
+// Submits the queued tasks in order: the first time a topology is met its kernels launch directly; from
+// the second time on it runs as one instantiated chain graph. A topology keeps up to kGraphVariants
+// instantiations with different argument values (a step alternating between two buffer sets replays two
+// graphs with no update); past that the least recently used one has its changed node parameters updated
+// in place. (compiler/runtime/scoped_graphs.hpp:134-247 warms up, builds and replays in the same order.)
+static void flush(Context &c)
+{
+    if (c.queue.empty()) return;
+    std::vector<Task> tasks;
+    tasks.swap(c.queue);
+    c.queued_threads = 0;
+    for (Task &t : tasks)
+        for (Buffer *b : t.buffers) b->queued--;
+    c.qstats.pending_tasks = 0;
+    c.qstats.submissions++;
+    c.gstats.submissions++;
+    c.gstats.logical_kernels += tasks.size();
+#ifdef FTC_CPU_BACKEND
+    launch_direct(c, tasks.front());
+#else
+    dev::set(c.domain->device);
+    if (!c.graphs_enabled) {
+        for (Task &t : tasks) launch_direct(c, t);
+        c.qstats.direct_submissions++;
+        c.qstats.direct_kernels += tasks.size();
+        c.domain->stats.launches += tasks.size();
+        return;
+    }
+    uint64_t key = 0xcbf29ce484222325ull;
+    for (Task &t : tasks) {
+        key = fnv(key, &t.code_key, sizeof t.code_key);
+        key = fnv(key, &t.function, sizeof t.function);
+        key = fnv(key, t.sizes.data(), t.sizes.size() * sizeof(std::size_t));
+    }
+    GraphFamily &family = c.graphs[key];
+    family.seen++;
+    if (family.seen == 1) {
+        for (Task &t : tasks) launch_direct(c, t);
+        c.qstats.direct_submissions++;
+        c.qstats.direct_kernels += tasks.size();
+        c.domain->stats.launches += tasks.size();
+        return;
+    }
+    auto dims_of = [](const Task &t) {
+        return std::vector<uint32_t>{t.grid[0], t.grid[1], t.grid[2], t.block[0], t.block[1], t.block[2],
+                                     static_cast<uint32_t>(t.shared)};
+    };
+    GraphEntry *g = nullptr;
+    for (GraphEntry &v : family.variants) {
+        bool same = true;
+        for (std::size_t i = 0; same && i < tasks.size(); ++i)
+            same = v.args[i] == tasks[i].args && v.dims[i] == dims_of(tasks[i]);
+        if (same) {
+            g = &v;
+            break;
+        }
+    }
+    if (g) {
+        c.gstats.replays++;
+    } else if (family.variants.size() < kGraphVariants) {
+        family.variants.emplace_back();
+        g = &family.variants.back();
+        if (!build_graph(c, *g, tasks)) {
+            family.variants.pop_back();
+            c.qstats.graph_resource_fallbacks++;
+            for (Task &t : tasks) launch_direct(c, t);
+            c.domain->stats.launches += tasks.size();
+            return;
+        }
+        c.gstats.live_graphs++;
+    } else {
+        // Every variant slot is taken: update the least recently used one in place.
+        g = &family.variants.front();
+        for (GraphEntry &v : family.variants)
+            if (v.last_use < g->last_use) g = &v;
+        for (std::size_t i = 0; i < tasks.size(); ++i) {
+            Task &t = tasks[i];
+            std::vector<uint32_t> dims = dims_of(t);
+            if (g->args[i] == t.args && g->dims[i] == dims) continue;
+            std::vector<void *> params;
+            hipKernelNodeParams p = node_params(t, params);
+            hipError_t e = hipGraphExecKernelNodeSetParams(g->exec, g->nodes[i], &p);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                c.poisoned = true;
+                fail(FORT_SCOPE_EXECUTION, std::string("graph parameter update failed: ") + hipGetErrorString(e));
+            }
+            g->args[i] = t.args;
+            g->dims[i] = dims;
+            c.gstats.parameter_updates++;
+        }
+        c.gstats.replays++;
+    }
+    g->last_use = ++family.clock;
+    hipError_t e = hipGraphLaunch(g->exec, c.stream);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c.poisoned = true;
+        fail(FORT_SCOPE_EXECUTION, std::string("graph launch failed: ") + hipGetErrorString(e));
+    }
+    c.domain->stats.launches += tasks.size();
+#endif
+}
+
+// ------------------------------------------------------------------------------------------------ guard
+enum class Mode { Change, Query };
+
+// Runs one ABI call on a live context: saves and restores the caller's floating-point environment,
+// converts errors to status codes and refuses work on a poisoned context.
+template <class F>
+static int with(fort_scope_t h, F &&body, Mode mode = Mode::Change)
+{
+    std::fenv_t env;
+    std::fegetenv(&env);
+    int status = FORT_SCOPE_OK;
+    try {
+        std::lock_guard<std::mutex> lock(g_registry);
+        Context &c = context(h);
+        if (mode == Mode::Change && c.poisoned) fail(FORT_SCOPE_EXECUTION, "context failed; only abandon is allowed");
+        std::lock_guard<std::mutex> dl(c.domain->mu);
+        try {
+            body(c);
+        } catch (const Error &e) {
+            if (e.status == FORT_SCOPE_EXECUTION) c.poisoned = true;
+            throw;
+        }
+    } catch (const Error &e) {
+        g_error = e.message;
+        status = e.status;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        status = FORT_SCOPE_RESOURCE;
+    }
+    std::fesetenv(&env);
+    return status;
+}
+
+template <class F>
+static int global(F &&body)
+{
+    std::fenv_t env;
+    std::fegetenv(&env);
+    int status = FORT_SCOPE_OK;
+    try {
+        std::lock_guard<std::mutex> lock(g_registry);
+        body();
+    } catch (const Error &e) {
+        g_error = e.message;
+        status = e.status;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        status = FORT_SCOPE_RESOURCE;
+    }
+    std::fesetenv(&env);
+    return status;
+}
+
+static void close_domain_storage(Domain &d)
+{
+    for (auto &kv : d.buffers)
+        if (kv.second->device) dev::release(kv.second->device);
+    d.buffers.clear();
+    d.by_identity.clear();
+    d.device_bytes = 0;
+}
+
+static void release_context(Context &c)
+{
+#ifndef FTC_CPU_BACKEND
+    for (auto &kv : c.graphs)
+        for (GraphEntry &g : kv.second.variants) destroy_graph(c, g);
+    c.graphs.clear();
+#endif
+    if (c.scratch) dev::release(c.scratch);
+    c.scratch = nullptr;
+    if (c.stream_ready) dev::destroy_stream(c.stream);
+    c.stream_ready = false;
+}
+
+static Context &new_context(Domain &d, bool owns)
+{
+    require(d.borrower == nullptr, FORT_SCOPE_STATE, "the domain is already borrowed by a context");
+    auto c = std::make_unique<Context>();
+    c->handle = fresh_handle();
+    c->domain = &d;
+    c->owns_domain = owns;
+    c->queue_enabled = env_u64("FORT_SCOPE_GRAPH_QUEUE", 1) != 0;
+    c->graphs_enabled = env_u64("FORT_SCOPE_HIP_GRAPHS", env_u64("FORT_SCOPE_CUDA_GRAPHS", 1)) != 0;
+    c->queue_work = env_u64("FORT_SCOPE_GRAPH_QUEUE_WORK", uint64_t(1) << 18);
+    c->qstats.version = 1;
+    c->qstats.queue_enabled = c->queue_enabled;
+#ifdef FTC_CPU_BACKEND
+    c->qstats.graphs_enabled = 0;
+#else
+    c->qstats.graphs_enabled = c->graphs_enabled;
+#endif
+    c->gstats.version = 1;
+    d.borrower = c.get();
+    Context &ref = *c;
+    g_contexts[c->handle] = std::move(c);
+    return ref;
+}
+
+static Domain &new_domain(int device)
+{
+    require(device >= 0, FORT_SCOPE_ARGUMENT, "negative device ordinal");
+    auto d = std::make_unique<Domain>();
+    d->handle = fresh_handle();
+    d->device = device;
+    d->residency.version = 1;
+    std::size_t bytes = 0;
+    if (parse_bytes(std::getenv("FORT_RESIDENT_DEVICE_BUDGET"), &bytes)) {
+        d->budget = bytes;
+        d->budget_known = true;
+    }
+    Domain &ref = *d;
+    g_domains[d->handle] = std::move(d);
+    return ref;
+}
+
+}  // namespace ftc
+
+using namespace ftc;
+
+// ================================================================================================ C ABI
+extern "C" {
+
+uint32_t fort_scope_abi_version(void) { return FORT_SCOPE_ABI_VERSION; }
+
+const char *fort_scope_error(void) { return g_error.c_str(); }
+
+int fort_scope_report_error(int status, const char *message)
+{
+    g_error = message ? message : "";
+    return status;
+}
+
+int fort_scope_device_budget_default_v1(int device, std::size_t *bytes)
+{
+    return global([&] {
+        require(bytes != nullptr, FORT_SCOPE_ARGUMENT, "missing budget output");
+        std::size_t parsed = 0;
+        if (parse_bytes(std::getenv("FORT_RESIDENT_DEVICE_BUDGET"), &parsed)) {
+            *bytes = parsed;
+            return;
+        }
+        int previous = dev::current();
+        dev::set(device);
+        std::size_t free = dev::free_bytes(), total = dev::total_bytes();
+        dev::set(previous);
+        std::size_t reserve = std::max(total / 16, std::size_t(512) << 20);
+        *bytes = free > reserve ? free - reserve : 0;
+    });
+}
+
+int fort_scope_create(int device, fort_scope_t *out)
+{
+    return global([&] {
+        require(out != nullptr, FORT_SCOPE_ARGUMENT, "missing context output");
+        Domain &d = new_domain(device);
+        d.is_private = true;
+        Context &c = new_context(d, true);
+        *out = c.handle;
+    });
+}
+
+int fort_scope_domain_create_v1(int device, fort_scope_domain_t *out)
+{
+    return global([&] {
+        require(out != nullptr, FORT_SCOPE_ARGUMENT, "missing domain output");
+        *out = new_domain(device).handle;
+    });
+}
+
+int fort_scope_create_in_domain_v1(fort_scope_domain_t domain, fort_scope_t *out)
+{
+    return global([&] {
+        require(out != nullptr, FORT_SCOPE_ARGUMENT, "missing context output");
+        Domain &d = domain_of(domain);
+        require(!d.failed, FORT_SCOPE_STATE, "the domain failed");
+        *out = new_context(d, false).handle;
+    });
+}
+
+int fort_scope_domain_get_v1(fort_scope_t h, fort_scope_domain_t *out)
+{
+    return with(h, [&](Context &c) {
+        require(out != nullptr, FORT_SCOPE_ARGUMENT, "missing domain output");
+        *out = c.owns_domain ? 0 : c.domain->handle;
+    }, Mode::Query);
+}
+
+int fort_scope_domain_device_get_v1(fort_scope_domain_t domain, int *device)
+{
+    return global([&] {
+        require(device != nullptr, FORT_SCOPE_ARGUMENT, "missing device output");
+        *device = domain_of(domain).device;
+    });
+}
+
+int fort_scope_domain_stats_get_v1(fort_scope_domain_t domain, fort_scope_stats *stats)
+{
+    return global([&] {
+        require(stats != nullptr, FORT_SCOPE_ARGUMENT, "missing statistics output");
+        *stats = domain_of(domain).stats;
+    });
+}
+
+int fort_scope_domain_close_v1(fort_scope_domain_t domain)
+{
+    return global([&] {
+        Domain &d = domain_of(domain);
+        require(d.borrower == nullptr, FORT_SCOPE_STATE, "a context still borrows the domain");
+        require(d.workspace.empty(), FORT_SCOPE_STATE, "workspace leases are outstanding");
+        // Publication frontier: every defined value reaches the host before storage is released.
+        if (!d.failed && !d.buffers.empty()) {
+            Context tmp;
+            tmp.domain = &d;
+            for (auto &kv : d.buffers) {
+                Buffer &b = *kv.second;
+                std::vector<Box> need;
+                for (const Box &i : b.init.boxes)
+                    for (const Box &m : b.host_cur.missing(i)) need.push_back(m);
+                if (!need.empty()) {
+                    ensure_stream(tmp);
+                    download(tmp, b, need);
+                }
+            }
+            release_context(tmp);
+        }
+        close_domain_storage(d);
+        g_domains.erase(domain);
+    });
+}
+
+int fort_scope_domain_abandon_v1(fort_scope_domain_t domain)
+{
+    return global([&] {
+        Domain &d = domain_of(domain);
+        require(d.borrower == nullptr, FORT_SCOPE_STATE, "a context still borrows the domain");
+        require(d.workspace.empty(), FORT_SCOPE_STATE, "workspace leases are outstanding");
+        close_domain_storage(d);
+        g_domains.erase(domain);
+    });
+}
+
+int fort_scope_domain_workspace_reserve_v1(fort_scope_domain_t domain, std::size_t bytes, uint64_t *lease)
+{
+    return global([&] {
+        require(lease != nullptr, FORT_SCOPE_ARGUMENT, "missing lease output");
+        Domain &d = domain_of(domain);
+        require(d.workspace_bytes + d.device_bytes + bytes <= budget_of(d), FORT_SCOPE_RESOURCE,
+                "workspace exceeds the unclaimed budget");
+        uint64_t id = d.next_lease++;
+        d.workspace[id] = bytes;
+        d.workspace_bytes += bytes;
+        *lease = id;
+    });
+}
+
+int fort_scope_domain_workspace_resize_v1(fort_scope_domain_t domain, uint64_t lease, std::size_t bytes)
+{
+    return global([&] {
+        Domain &d = domain_of(domain);
+        auto it = d.workspace.find(lease);
+        require(it != d.workspace.end(), FORT_SCOPE_STALE, "unknown workspace lease");
+        std::size_t others = d.workspace_bytes - it->second;
+        require(others + d.device_bytes + bytes <= budget_of(d), FORT_SCOPE_RESOURCE,
+                "workspace exceeds the unclaimed budget");
+        d.workspace_bytes = others + bytes;
+        it->second = bytes;
+    });
+}
+
+int fort_scope_domain_workspace_release_v1(fort_scope_domain_t domain, uint64_t lease)
+{
+    return global([&] {
+        Domain &d = domain_of(domain);
+        auto it = d.workspace.find(lease);
+        require(it != d.workspace.end(), FORT_SCOPE_STALE, "unknown workspace lease");
+        d.workspace_bytes -= it->second;
+        d.workspace.erase(it);
+    });
+}
+
+int fort_scope_domain_budget_get_v1(fort_scope_domain_t domain, std::size_t *budget, std::size_t *unclaimed)
+{
+    return global([&] {
+        require(budget && unclaimed, FORT_SCOPE_ARGUMENT, "missing budget outputs");
+        Domain &d = domain_of(domain);
+        std::size_t b = budget_of(d);
+        std::size_t claimed = d.device_bytes + d.workspace_bytes;
+        if (d.borrower) claimed += d.borrower->scratch_capacity;
+        *budget = b;
+        *unclaimed = b > claimed ? b - claimed : 0;
+    });
+}
+
+int fort_scope_set_device_budget(fort_scope_t h, std::size_t bytes)
+{
+    return with(h, [&](Context &c) {
+        require(c.domain->device_bytes <= bytes, FORT_SCOPE_STATE, "budget below the current allocations");
+        c.domain->budget = bytes;
+        c.domain->budget_known = true;
+    });
+}
+
+int fort_scope_scratch_acquire_v1(fort_scope_t h, std::size_t bytes, fort_scope_scratch_lease_v1 *lease)
+{
+    return with(h, [&](Context &c) {
+        require(lease != nullptr, FORT_SCOPE_ARGUMENT, "missing lease output");
+        require(!c.scratch_active, FORT_SCOPE_STATE, "a scratch lease is already active");
+        if (bytes > c.scratch_capacity) {
+            ensure_stream(c);
+            flush(c);
+            wait(c);
+            Domain &d = *c.domain;
+            require(d.device_bytes + d.workspace_bytes + bytes <= budget_of(d), FORT_SCOPE_RESOURCE,
+                    "scratch exceeds the device budget");
+            if (c.scratch) dev::release(c.scratch);
+            c.scratch = nullptr;
+            c.scratch_capacity = 0;
+            dev::set(d.device);
+            c.scratch = dev::alloc(bytes);
+            require(c.scratch != nullptr, FORT_SCOPE_RESOURCE, "scratch allocation failed");
+            c.scratch_capacity = bytes;
+        }
+        c.scratch_active = true;
+        c.scratch_token = c.next_token++;
+        lease->version = 1;
+        lease->reserved = 0;
+        lease->token = c.scratch_token;
+        lease->device = bytes ? c.scratch : nullptr;
+        lease->bytes = bytes;
+        lease->capacity = c.scratch_capacity;
+    });
+}
+
+int fort_scope_scratch_release_v1(fort_scope_t h, uint64_t token)
+{
+    return with(h, [&](Context &c) {
+        require(c.scratch_active && token == c.scratch_token, FORT_SCOPE_STATE, "no such active scratch lease");
+        c.scratch_active = false;
+    });
+}
+
+static int register_impl(fort_scope_t h, uint64_t identity, uint64_t generation, const fort_scope_layout *layout,
+                         const fort_scope_section *init, std::size_t init_count, bool all, fort_buffer_t *out)
+{
+    return with(h, [&](Context &c) {
+        require(layout && out, FORT_SCOPE_ARGUMENT, "missing layout or handle output");
+        require(layout->rank >= 1 && layout->rank <= static_cast<uint32_t>(kMaxRank), FORT_SCOPE_ARGUMENT,
+                "unsupported rank");
+        require(layout->element_bytes > 0 && layout->extents && layout->host, FORT_SCOPE_ARGUMENT,
+                "incomplete layout");
+        Domain &d = *c.domain;
+        auto found = d.by_identity.find(identity);
+        if (found != d.by_identity.end()) {
+            Buffer &b = *d.buffers.at(found->second);
+            bool same = b.rank == layout->rank && b.elem == layout->element_bytes && b.host == layout->host &&
+                        b.generation == generation;
+            for (uint32_t k = 0; same && k < b.rank; ++k) same = b.extents[k] == layout->extents[k];
+            require(same, FORT_SCOPE_STALE, "identity re-registered with another allocation");
+            *out = b.handle;
+            return;
+        }
+        auto b = std::make_unique<Buffer>();
+        b->handle = fresh_handle();
+        b->domain = &d;
+        b->identity = identity;
+        b->generation = generation;
+        b->rank = layout->rank;
+        b->type = layout->type;
+        b->elem = layout->element_bytes;
+        b->host = layout->host;
+        b->count = 1;
+        for (uint32_t k = 0; k < b->rank; ++k) {
+            b->extents[k] = layout->extents[k];
+            b->lower_bounds[k] = layout->lower_bounds ? layout->lower_bounds[k] : 1;
+            b->count *= b->extents[k];
+        }
+        b->bytes = b->count * b->elem;
+        if (all && b->count) {
+            b->init.add(b->full());
+            b->host_cur.add(b->full());
+        }
+        for (std::size_t i = 0; i < init_count; ++i) {
+            Box box = section_box(*b, init[i]);
+            require(b->init.add(box) && b->host_cur.add(box), FORT_SCOPE_RESOURCE, "region limit exceeded");
+        }
+        *out = b->handle;
+        d.by_identity[identity] = b->handle;
+        d.buffers[b->handle] = std::move(b);
+    });
+}
+
+int fort_scope_register(fort_scope_t h, uint64_t identity, uint64_t generation, const fort_scope_layout *layout,
+                        int host_initialized, fort_buffer_t *out)
+{
+    return register_impl(h, identity, generation, layout, nullptr, 0, host_initialized != 0, out);
+}
+
+int fort_scope_register_sections(fort_scope_t h, uint64_t identity, uint64_t generation,
+                                 const fort_scope_layout *layout, const fort_scope_section *initialized,
+                                 std::size_t count, fort_buffer_t *out)
+{
+    if (count && !initialized) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "missing sections");
+    return register_impl(h, identity, generation, layout, initialized, count, false, out);
+}
+
+int fort_scope_forget_definition(fort_scope_t h, fort_buffer_t handle)
+{
+    return with(h, [&](Context &c) {
+        Buffer &b = buffer(c, handle);
+        require(!b.prepared, FORT_SCOPE_STATE, "buffer has a prepared access");
+        // Semantic definitions go; queued tasks that write the buffer keep their effects because the
+        // coherence state already reflects them at acceptance.
+        b.init.clear();
+        b.host_cur.clear();
+        b.dev_cur.clear();
+    });
+}
+
+int fort_scope_layout_get(fort_scope_t h, fort_buffer_t handle, fort_scope_layout *layout)
+{
+    return with(h, [&](Context &c) {
+        require(layout != nullptr, FORT_SCOPE_ARGUMENT, "missing layout output");
+        Buffer &b = buffer(c, handle);
+        layout->rank = b.rank;
+        layout->type = b.type;
+        layout->element_bytes = b.elem;
+        layout->host = b.host;
+        layout->extents = b.extents;
+        layout->lower_bounds = b.lower_bounds;
+        layout->generation = b.generation;
+    }, Mode::Query);
+}
+
+int fort_scope_access_begin_v1(fort_scope_t h, const fort_scope_plan_binding *bindings, std::size_t count,
+                               int device, void **devices)
+{
+    return with(h, [&](Context &c) {
+        require(device == 0 || device == 1, FORT_SCOPE_ARGUMENT, "invalid task access execution mode");
+        for (std::size_t i = 0; i < count; ++i) drain_buffer(c, buffer(c, bindings[i].buffer));
+        if (device) {
+            prepare_device(c, bindings, count, devices, false);
+            return;
+        }
+        std::vector<Buffer *> list;
+        std::vector<Effects> eff;
+        for (std::size_t i = 0; i < count; ++i) {
+            Buffer &b = buffer(c, bindings[i].buffer);
+            for (Buffer *o : list) require(o != &b, FORT_SCOPE_ALIAS, "task access bindings repeat a buffer");
+            require(!b.prepared, FORT_SCOPE_STATE, "buffer already has a prepared access");
+            list.push_back(&b);
+            eff.push_back(effects(b, &bindings[i].access));
+        }
+        std::vector<std::vector<Box>> downloads;
+        for (std::size_t i = 0; i < count; ++i) downloads.push_back(plan_transfer(*list[i], eff[i], false));
+        for (std::size_t i = 0; i < count; ++i) {
+            if (!downloads[i].empty()) ensure_stream(c);
+            download(c, *list[i], downloads[i]);
+            list[i]->prepared = 2;
+            list[i]->prep_writes = eff[i].writes;
+            list[i]->prep_overwrites = eff[i].overwrites;
+        }
+    });
+}
+
+int fort_scope_device_begin(fort_scope_t h, fort_buffer_t handle, const fort_scope_access *access, void **device)
+{
+    fort_scope_plan_binding b{};
+    b.buffer = handle;
+    if (!access) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "missing access description");
+    b.access = *access;
+    return fort_scope_access_begin_v1(h, &b, 1, 1, device);
+}
+
+int fort_scope_device_end(fort_scope_t h, fort_buffer_t handle)
+{
+    return with(h, [&](Context &c) { end_access(c, buffer(c, handle), true, false); });
+}
+
+int fort_scope_host_begin(fort_scope_t h, fort_buffer_t handle, const fort_scope_access *access)
+{
+    fort_scope_plan_binding b{};
+    b.buffer = handle;
+    if (!access) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "missing access description");
+    b.access = *access;
+    return fort_scope_access_begin_v1(h, &b, 1, 0, nullptr);
+}
+
+int fort_scope_host_end(fort_scope_t h, fort_buffer_t handle)
+{
+    return with(h, [&](Context &c) { end_access(c, buffer(c, handle), false, false); });
+}
+
+int fort_scope_cancel_access(fort_scope_t h, fort_buffer_t handle)
+{
+    return with(h, [&](Context &c) {
+        Buffer &b = buffer(c, handle);
+        end_access(c, b, b.prepared == 1, true);
+    });
+}
+
+int fort_scope_device_reserve_v1(fort_scope_t h, fort_buffer_t handle, void **device)
+{
+    return fort_scope_device_reserve_many_v1(h, &handle, 1, device);
+}
+
+int fort_scope_device_reserve_many_v1(fort_scope_t h, const fort_buffer_t *handles, std::size_t count,
+                                      void **devices)
+{
+    return with(h, [&](Context &c) {
+        require(!count || (handles && devices), FORT_SCOPE_ARGUMENT, "missing reserve arguments");
+        std::vector<Buffer *> list;
+        for (std::size_t i = 0; i < count; ++i) list.push_back(&buffer(c, handles[i]));
+        if (!count) return;
+        ensure_stream(c);
+        reserve(c, list.data(), list.size());
+        for (std::size_t i = 0; i < count; ++i) devices[i] = list[i]->device;
+    });
+}
+
+int fort_scope_buffer_valid_v1(fort_scope_t h, fort_buffer_t handle, int *valid)
+{
+    return with(h, [&](Context &c) {
+        require(valid != nullptr, FORT_SCOPE_ARGUMENT, "missing output");
+        *valid = c.domain->buffers.count(handle) ? 1 : 0;
+    }, Mode::Query);
+}
+
+int fort_scope_residency_stats_get_v1(fort_scope_t h, fort_scope_residency_stats_v1 *stats)
+{
+    return with(h, [&](Context &c) {
+        require(stats != nullptr, FORT_SCOPE_ARGUMENT, "missing statistics output");
+        *stats = c.domain->residency;
+    }, Mode::Query);
+}
+
+int fort_scope_observe_v1(fort_scope_t h, const fort_buffer_t *handles, std::size_t count)
+{
+    return with(h, [&](Context &c) {
+        require(!count || handles, FORT_SCOPE_ARGUMENT, "missing buffer list");
+        for (std::size_t i = 0; i < count; ++i) {
+            Buffer &b = buffer(c, handles[i]);
+            require(!b.prepared, FORT_SCOPE_STATE, "buffer has a prepared access");
+            observe(c, b);
+        }
+    });
+}
+
+int fort_scope_observe_all_v1(fort_scope_t h)
+{
+    return with(h, [&](Context &c) {
+        flush(c);
+        for (auto &kv : c.domain->buffers) {
+            require(!kv.second->prepared, FORT_SCOPE_STATE, "buffer has a prepared access");
+            observe(c, *kv.second);
+        }
+    });
+}
+
+int fort_scope_discard_v1(fort_scope_t h, fort_buffer_t handle)
+{
+    return with(h, [&](Context &c) {
+        Buffer &b = buffer(c, handle);
+        require(!b.prepared, FORT_SCOPE_STATE, "buffer has a prepared access");
+        drain_buffer(c, b);
+        b.init.clear();
+        b.host_cur.clear();
+        b.dev_cur.clear();
+    });
+}
+
+int fort_scope_unregister(fort_scope_t h, fort_buffer_t handle)
+{
+    return with(h, [&](Context &c) {
+        Buffer &b = buffer(c, handle);
+        require(!b.prepared, FORT_SCOPE_STATE, "buffer has a prepared access");
+        observe(c, b);
+        wait(c);
+        if (b.device) {
+            dev::release(b.device);
+            c.domain->device_bytes -= b.bytes;
+        }
+        c.domain->by_identity.erase(b.identity);
+        c.domain->buffers.erase(handle);
+    });
+}
+
+int fort_scope_gpu_enter(fort_scope_t h, int *previous, void **stream)
+{
+    return with(h, [&](Context &c) {
+        require(previous && stream, FORT_SCOPE_ARGUMENT, "missing outputs");
+        *previous = dev::current();
+        ensure_stream(c);
+        dev::set(c.domain->device);
+        *stream = reinterpret_cast<void *>(c.stream);
+    });
+}
+
+int fort_scope_gpu_leave(fort_scope_t h, int previous)
+{
+    return with(h, [&](Context &) { dev::set(previous); }, Mode::Query);
+}
+
+int fort_scope_note_launch(fort_scope_t h)
+{
+    return with(h, [&](Context &c) { c.domain->stats.launches++; });
+}
+
+int fort_scope_graph_enqueue_v1(fort_scope_t h, uint64_t code_key, const fort_scope_graph_task_v1 *task,
+                                const std::size_t *argument_sizes)
+{
+    return with(h, [&](Context &c) {
+        require(task != nullptr, FORT_SCOPE_ARGUMENT, "missing task");
+        const fort_scope_kernel_v1 &k = task->kernel;
+        require(k.function != nullptr, FORT_SCOPE_ARGUMENT, "missing kernel function");
+        require(!k.argument_count || (k.arguments && argument_sizes), FORT_SCOPE_ARGUMENT, "missing arguments");
+        uint64_t threads = uint64_t(k.grid[0]) * k.grid[1] * k.grid[2] * k.block[0] * k.block[1] * k.block[2];
+        if (!threads) return;  // empty launches initialise nothing
+#ifdef FTC_CPU_BACKEND
+        fail(FORT_SCOPE_RESOURCE, "the CPU reference backend executes no device kernels");
+#endif
+        // No drain: tasks on one context run in acceptance order on its stream, and uploads only touch
+        // cells no accepted task holds on the device.
+        std::vector<void *> devices(task->binding_count);
+        prepare_device(c, task->bindings, task->binding_count, devices.data(), true);
+        Task t;
+        t.code_key = code_key;
+        t.function = k.function;
+        for (int d = 0; d < 3; ++d) {
+            t.grid[d] = k.grid[d];
+            t.block[d] = k.block[d];
+        }
+        t.shared = k.shared_bytes;
+        t.threads = threads;
+        std::size_t off = 0;
+        for (std::size_t i = 0; i < k.argument_count; ++i) {
+            std::size_t sz = argument_sizes[i];
+            std::size_t align = sz >= 8 ? 8 : (sz >= 4 ? 4 : (sz >= 2 ? 2 : 1));
+            off = (off + align - 1) / align * align;
+            t.offsets.push_back(off);
+            t.sizes.push_back(sz);
+            off += sz;
+        }
+        t.args.resize(off);
+        for (std::size_t i = 0; i < k.argument_count; ++i)
+            std::memcpy(t.args.data() + t.offsets[i], k.arguments[i], t.sizes[i]);
+        for (std::size_t i = 0; i < task->binding_count; ++i) {
+            Buffer &b = buffer(c, task->bindings[i].buffer);
+            b.queued++;
+            t.buffers.push_back(&b);
+        }
+        c.qstats.copied_argument_bytes += off;
+        c.qstats.accepted_tasks++;
+        c.queue.push_back(std::move(t));
+        c.queued_threads += threads;
+        c.qstats.pending_tasks = c.queue.size();
+        if (c.queue.size() > c.qstats.peak_pending_tasks) c.qstats.peak_pending_tasks = c.queue.size();
+        if (!c.queue_enabled || (c.queue_work && c.queued_threads >= c.queue_work)) flush(c);
+    });
+}
+
+int fort_scope_graph_flush_v1(fort_scope_t h)
+{
+    return with(h, [&](Context &c) {
+        if (!c.queue.empty()) c.qstats.explicit_flushes++;
+        flush(c);
+    });
+}
+
+int fort_scope_graph_queue_cancel_v1(fort_scope_t h)
+{
+    return with(h, [&](Context &c) {
+        // Accepted effects were committed at acceptance; cancelling unsubmitted work forgets what it
+        // would have written.
+        for (Task &t : c.queue)
+            for (Buffer *b : t.buffers) {
+                b->queued--;
+                b->init.clear();
+                b->host_cur.clear();
+                b->dev_cur.clear();
+            }
+        c.queue.clear();
+        c.queued_threads = 0;
+        c.qstats.pending_tasks = 0;
+    });
+}
+
+int fort_scope_graph_queue_stats_get_v1(fort_scope_t h, fort_scope_graph_queue_stats_v1 *stats)
+{
+    return with(h, [&](Context &c) {
+        require(stats != nullptr, FORT_SCOPE_ARGUMENT, "missing statistics output");
+        *stats = c.qstats;
+    }, Mode::Query);
+}
+
+int fort_scope_graph_stats_get_v1(fort_scope_t h, fort_scope_graph_stats_v1 *stats)
+{
+    return with(h, [&](Context &c) {
+        require(stats != nullptr, FORT_SCOPE_ARGUMENT, "missing statistics output");
+        *stats = c.gstats;
+    }, Mode::Query);
+}
+
+int fort_scope_external_execute_v1(fort_scope_t h, const fort_scope_external_call_v1 *call,
+                                   fort_scope_external_worker_v1 worker, void *user,
+                                   fort_scope_external_report_v1 *report)
+{
+    if (report) {
+        report->version = 1;
+        report->status = FORT_SCOPE_OK;
+        report->started = 0;
+        report->committed = 0;
+    }
+    int status = with(h, [&](Context &c) {
+        require(call && worker, FORT_SCOPE_ARGUMENT, "missing external call or worker");
+        require(call->version == 1, FORT_SCOPE_ARGUMENT, "unsupported external call version");
+        require(!call->binding_count || call->bindings, FORT_SCOPE_ARGUMENT, "missing external bindings");
+        if (!call->binding_count) return;
+        std::vector<fort_scope_plan_binding> plan(call->binding_count);
+        for (std::size_t i = 0; i < call->binding_count; ++i) {
+            const fort_scope_external_binding_v1 &eb = call->bindings[i];
+            Buffer &b = buffer(c, eb.buffer);
+            require(b.type == eb.type && b.rank == eb.rank && b.elem == eb.element_bytes, FORT_SCOPE_ARGUMENT,
+                    "external binding type, rank or element size mismatch");
+            plan[i].buffer = eb.buffer;
+            plan[i].access = eb.access;
+            drain_buffer(c, b);
+        }
+        std::vector<void *> devices(plan.size());
+        prepare_device(c, plan.data(), plan.size(), devices.data(), false);
+        std::vector<fort_scope_external_view_v1> views(plan.size());
+        std::vector<std::vector<std::size_t>> strides_b(plan.size());
+        for (std::size_t i = 0; i < plan.size(); ++i) {
+            Buffer &b = buffer(c, plan[i].buffer);
+            std::size_t s[kMaxRank];
+            strides(b, s);
+            for (uint32_t d = 0; d < b.rank; ++d) strides_b[i].push_back(s[d] * b.elem);
+            views[i] = fort_scope_external_view_v1{b.handle, b.generation, b.type, b.rank, b.elem, b.bytes,
+                                                   b.extents, strides_b[i].data(), b.lower_bounds, b.device};
+        }
+        int started = -1;
+        int rc = FORT_SCOPE_EXECUTION;
+        try {
+            rc = worker(user, views.data(), views.size(), reinterpret_cast<void *>(c.stream), &started);
+        } catch (...) {
+            rc = FORT_SCOPE_EXECUTION;
+        }
+        if (report) report->started = started == 1;
+        if (rc == FORT_SCOPE_OK && started == 1) {
+            for (auto &p : plan) end_access(c, buffer(c, p.buffer), true, false);
+            if (report) report->committed = 1;
+            return;
+        }
+        if (started == 0 && rc != FORT_SCOPE_OK) {
+            for (auto &p : plan) end_access(c, buffer(c, p.buffer), true, true);
+            fail(rc == FORT_SCOPE_EXECUTION ? FORT_SCOPE_BOUNDARY : rc, "external provider declined before enqueue");
+        }
+        for (auto &p : plan) end_access(c, buffer(c, p.buffer), true, true);
+        c.poisoned = true;
+        fail(FORT_SCOPE_EXECUTION, "external provider failed after it started work");
+    });
+    if (report) report->status = status;
+    return status;
+}
+
+int fort_scope_execution_error(fort_scope_t h, const char *message)
+{
+    std::string m = message ? message : "execution error";
+    int status = with(h, [&](Context &c) {
+        c.poisoned = true;
+        fail(FORT_SCOPE_EXECUTION, m);
+    }, Mode::Query);
+    return status == FORT_SCOPE_STALE ? status : FORT_SCOPE_EXECUTION;
+}
+
+int fort_scope_wait(fort_scope_t h)
+{
+    return with(h, [&](Context &c) {
+        flush(c);
+        wait(c);
+    });
+}
+
+int fort_scope_stats_get(fort_scope_t h, fort_scope_stats *stats)
+{
+    return with(h, [&](Context &c) {
+        require(stats != nullptr, FORT_SCOPE_ARGUMENT, "missing statistics output");
+        *stats = c.domain->stats;
+    }, Mode::Query);
+}
+
+int fort_scope_close(fort_scope_t h)
+{
+    fort_scope_domain_t owned = 0;
+    int status = with(h, [&](Context &c) {
+        require(!c.scratch_active, FORT_SCOPE_STATE, "a scratch lease is still active");
+        for (auto &kv : c.domain->buffers)
+            require(!kv.second->prepared, FORT_SCOPE_STATE, "an access is still prepared");
+        flush(c);
+        wait(c);
+        release_context(c);
+        c.domain->borrower = nullptr;
+        if (c.owns_domain) owned = c.domain->handle;
+    });
+    if (status != FORT_SCOPE_OK) return status;
+    {
+        std::lock_guard<std::mutex> lock(g_registry);
+        g_contexts.erase(h);
+    }
+    if (owned) return fort_scope_domain_close_v1(owned);
+    return FORT_SCOPE_OK;
+}
+
+int fort_scope_abandon(fort_scope_t h)
+{
+    return global([&] {
+        Context &c = context(h);
+        require(c.poisoned, FORT_SCOPE_STATE, "abandon is only valid after an execution failure");
+        c.queue.clear();
+        for (auto &kv : c.domain->buffers) {
+            kv.second->prepared = 0;
+            kv.second->queued = 0;
+        }
+        if (c.stream_ready) {
+#ifndef FTC_CPU_BACKEND
+            (void)hipStreamSynchronize(c.stream);
+            (void)hipGetLastError();
+#endif
+        }
+        release_context(c);
+        Domain *d = c.domain;
+        d->borrower = nullptr;
+        d->failed = true;
+        bool owns = c.owns_domain;
+        g_contexts.erase(h);
+        if (owns) {
+            close_domain_storage(*d);
+            g_domains.erase(d->handle);
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+"""ORACLE — test infrastructure only.
+
+ctypes bindings of the oracle's C restatement (oracle/liboracle.so) and of the reference's own Fortran
+built into oracle/_ref/. Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline load this
+module, and only as the checker or the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB = ORACLE_DIR / "liboracle.so"
+REF = ORACLE_DIR / "_ref"
+
+_V = c_void_p
+_D = c_double
+_I = c_int
+
+SIGNATURES = {
+    "orc_cdu": [_V, _V, _V, _V, _D, _D, _D, _I, _I, _I, _I, _I, _I],
+    "orc_cdv": [_V, _V, _V, _V, _D, _D, _D, _I, _I, _I, _I, _I, _I],
+    "orc_cdw": [_V, _V, _V, _V, _D, _D, _D, _I, _I, _I, _I, _I, _I],
+    "orc_scalar_adv": [_V, _V, _V, _V, _V, _D, _D, _D, _I, _I, _I, _I, _I, _I],
+    "orc_sgs_coefficients": [_D, _D, _D, _D, _V],
+    "orc_sgs_nu": [_V, _V, _V, _V, _V, _I, _I, _I, _I, _I, _I],
+    "orc_sgs_diffusion": [_V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _I, _I, _I, _I, _I, _I],
+    "orc_halo_les": [_V, _I, _I, _I, _I, _I, _I, _I],
+    "orc_box_periodic": [_V, _I, _I, _I],
+    "orc_box_diffuse_store": [_V, _V, _I, _I, _I, _D, _D, _D],
+    "orc_les_tendency": [_V, _V, _V, _V, _V, _V, _V, _V, _V, _D, _D, _D, _V, _I, _I, _I],
+    "orc_les_step": [_V, _V, _V, _V, _V, _V, _V, _V, _D, _D, _D, _D, _V, _I, _I, _I],
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} is missing: run `make -C oracle`")
+        _lib = ctypes.CDLL(str(LIB))
+        for name, args in SIGNATURES.items():
+            fn = getattr(_lib, name, None)
+            if fn is not None:
+                fn.restype = None
+                fn.argtypes = args
+    return _lib
+
+
+def ptr(a: np.ndarray | None) -> int | None:
+    return None if a is None else a.ctypes.data
+
+
+def sgs_coefficients(dx, dy, dz, cs) -> np.ndarray:
+    out = np.zeros(10)
+    lib().orc_sgs_coefficients(dx, dy, dz, cs, ptr(out))
+    return out
+
+
+def les_tendency(U, V, W, S, dx, dy, dz, cs=None):
+    """Tendencies (TU, TV, TW, TS) of the LES fields; cs None means advection alone."""
+    nx, ny, nz = (d - 2 for d in U.shape)
+    outs = [np.zeros_like(U, order="F") for _ in range(5)]
+    coef = None if cs is None else sgs_coefficients(dx, dy, dz, cs)
+    lib().orc_les_tendency(*(ptr(o) for o in outs), ptr(U), ptr(V), ptr(W), ptr(S), dx, dy, dz, ptr(coef),
+                           nx, ny, nz)
+    return outs[:4]
+
+
+def les_step(U, V, W, S, dx, dy, dz, dt, cs=None):
+    nx, ny, nz = (d - 2 for d in U.shape)
+    outs = [np.zeros_like(U, order="F") for _ in range(4)]
+    coef = None if cs is None else sgs_coefficients(dx, dy, dz, cs)
+    lib().orc_les_step(*(ptr(o) for o in outs), ptr(U), ptr(V), ptr(W), ptr(S), dx, dy, dz, dt, ptr(coef),
+                       nx, ny, nz)
+    return outs
+
+
+def component(name: str, U, V, W, dx, dy, dz, n1, n2, n3, out=None):
+    """One reference advection component (cdu, cdv, cdw) over loop extents n1, n2, n3."""
+    out = np.zeros_like(U, order="F") if out is None else out
+    e1, e2, e3 = U.shape
+    getattr(lib(), "orc_" + name)(ptr(out), ptr(U), ptr(V), ptr(W), dx, dy, dz, n1, n2, n3, e1, e2, e3)
+    return out
+
+
+def sgs_nu(U, V, W, dx, dy, dz, cs):
+    nx, ny, nz = (d - 2 for d in U.shape)
+    NU = np.zeros_like(U, order="F")
+    coef = sgs_coefficients(dx, dy, dz, cs)
+    lib().orc_sgs_nu(ptr(NU), ptr(U), ptr(V), ptr(W), ptr(coef), nx, ny, nz, *U.shape)
+    return NU
+
+
+def halo_les(F, kind: int):
+    nx, ny, nz = (d - 2 for d in F.shape)
+    lib().orc_halo_les(ptr(F), kind, nx, ny, nz, *F.shape)
+
+
+# ---------------------------------------------------------------------------------- reference builds
+_refs: dict[str, ctypes.CDLL] = {}
+
+
+def reference_component(name: str):
+    """The reference's own CDU/CDV/CDW built by `make -C oracle ref`, or None if it was not built."""
+    path = REF / f"libref_{name}.so"
+    if not path.exists():
+        return None
+    if name not in _refs:
+        l = ctypes.CDLL(str(path))
+        fn = getattr(l, "ref_" + name)
+        fn.restype = None
+        fn.argtypes = [_V, _V, _V, _V, _D, _D, _D, _I, _I, _I, _I, _I, _I]
+        _refs[name] = l
+    fn = getattr(_refs[name], "ref_" + name)
+
+    def call(U, V, W, dx, dy, dz, n1, n2, n3, out=None):
+        out = np.zeros_like(U, order="F") if out is None else out
+        fn(ptr(out), ptr(U), ptr(V), ptr(W), dx, dy, dz, n1, n2, n3, *U.shape)
+        return out
+
+    return call

@@ -385,14 +385,25 @@ static void evict_one(Context &c, Buffer &victim)
     c.domain->residency.evicted_bytes += victim.bytes;
 }
 
+// FORT_RESIDENT_DEVICE_BUDGET if set, else the free device memory less a reserve of one sixteenth of
+// the device, at least 512 MiB (compiler/runtime/scoped_runtime.h:169-174). Called with the registry
+// lock held or from the public wrapper; it takes no lock itself.
+static std::size_t default_budget(int device)
+{
+    std::size_t parsed = 0;
+    if (parse_bytes(std::getenv("FORT_RESIDENT_DEVICE_BUDGET"), &parsed)) return parsed;
+    int previous = dev::current();
+    dev::set(device);
+    std::size_t free = dev::free_bytes(), total = dev::total_bytes();
+    dev::set(previous);
+    std::size_t reserve = std::max(total / 16, std::size_t(512) << 20);
+    return free > reserve ? free - reserve : 0;
+}
+
 static std::size_t budget_of(Domain &d)
 {
     if (!d.budget_known) {
-        int device = d.device;
-        std::size_t bytes = 0;
-        if (fort_scope_device_budget_default_v1(device, &bytes) != FORT_SCOPE_OK)
-            fail(FORT_SCOPE_RESOURCE, g_error);
-        d.budget = bytes;
+        d.budget = default_budget(d.device);
         d.budget_known = true;
     }
     return d.budget;
@@ -929,17 +940,7 @@ int fort_scope_device_budget_default_v1(int device, std::size_t *bytes)
 {
     return global([&] {
         require(bytes != nullptr, FORT_SCOPE_ARGUMENT, "missing budget output");
-        std::size_t parsed = 0;
-        if (parse_bytes(std::getenv("FORT_RESIDENT_DEVICE_BUDGET"), &parsed)) {
-            *bytes = parsed;
-            return;
-        }
-        int previous = dev::current();
-        dev::set(device);
-        std::size_t free = dev::free_bytes(), total = dev::total_bytes();
-        dev::set(previous);
-        std::size_t reserve = std::max(total / 16, std::size_t(512) << 20);
-        *bytes = free > reserve ? free - reserve : 0;
+        *bytes = default_budget(device);
     });
 }
 

@@ -1,0 +1,315 @@
+"""The fort_scope runtime on its CPU reference backend (libftc_cpu.so: "device" memory is host memory,
+device kernels are refused with RESOURCE), as the reference tests its runtime without a GPU
+(compiler/tests/test_scoped_runtime.py, test_scoped_domains.py, test_scoped_scratch.py).
+
+Covers registration and layouts, exact section coherence (only missing sections move), definition
+checks, aliasing, LRU eviction with publication of dirty mirrors, scratch leases, the external-provider
+transaction, poisoning and abandonment.
+"""
+import ctypes
+from ctypes import byref, c_int, c_size_t, c_uint64, c_void_p
+
+import numpy as np
+import pytest
+
+
+def new_domain(scope, lib, budget=None):
+    return scope.Domain(lib, budget=budget)
+
+
+def binding(scope, handle, flags=0, reads=(), writes=(), overwrites=()):
+    access, keep = scope.make_access(flags, reads, writes, overwrites)
+    return scope.PlanBinding(handle, access), keep
+
+
+def begin_device(scope, lib, dom, bindings):
+    arr = (scope.PlanBinding * len(bindings))(*[b for b, _ in bindings])
+    devs = (c_void_p * len(bindings))()
+    status = lib.fort_scope_access_begin_v1(dom.ctx, arr, len(bindings), 1, devs)
+    return status, [devs[i] for i in range(len(bindings))]
+
+
+def dev_view(ptr, shape):
+    n = int(np.prod(shape))
+    buf = (ctypes.c_double * n).from_address(ptr)
+    return np.frombuffer(buf, dtype=np.float64).reshape(shape, order="F")
+
+
+def test_register_and_layout(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.zeros((5, 4, 3), order="F")
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        layout = scope.Layout()
+        scope.check(lib, lib.fort_scope_layout_get(dom.ctx, h, byref(layout)))
+        assert (layout.rank, layout.type, layout.element_bytes) == (3, scope.REAL64, 8)
+        assert [layout.extents[i] for i in range(3)] == [5, 4, 3]
+        assert layout.host == a.ctypes.data
+        # The same identity registered again returns the same handle.
+        layout2 = scope.Layout(3, scope.REAL64, 8, a.ctypes.data, layout.extents, layout.lower_bounds, 0)
+        again = c_uint64()
+        identity = 1
+        scope.check(lib, lib.fort_scope_register(dom.ctx, identity, 0, byref(layout2), 1, byref(again)))
+        assert again.value == h
+
+
+def test_device_writes_reach_the_host_only_where_written(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.arange(6 * 5 * 4, dtype=np.float64).reshape((6, 5, 4), order="F").copy(order="F")
+    original = a.copy(order="F")
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        box = ((1, 1, 1), (4, 3, 2))
+        b = binding(scope, h, reads=[box], overwrites=[box])
+        st, devs = begin_device(scope, lib, dom, [b])
+        scope.check(lib, st)
+        s = dom.stats()
+        # Only the read box went up.
+        assert s.upload_bytes == 3 * 2 * 1 * 8
+        view = dev_view(devs[0], a.shape)
+        view[1:4, 1:3, 1:2] = -view[1:4, 1:3, 1:2]
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        assert np.array_equal(a, original)  # the host copy is untouched until observation
+        dom.observe(h)
+        assert dom.stats().download_bytes == 3 * 2 * 1 * 8
+    expect = original.copy(order="F")
+    expect[1:4, 1:3, 1:2] *= -1
+    assert np.array_equal(a, expect)
+
+
+def test_a_second_read_of_a_current_section_moves_nothing(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.ones((8, 8), order="F")
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        for _ in range(2):
+            b = binding(scope, h, flags=scope.READ_ALL)
+            st, _ = begin_device(scope, lib, dom, [b])
+            scope.check(lib, st)
+            scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        assert dom.stats().uploads >= 1
+        assert dom.stats().upload_bytes == 64 * 8
+
+
+def test_reading_undefined_values_is_refused(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.zeros((4, 4), order="F")
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a, initialized=False)
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL)])
+        assert st == scope.UNINITIALIZED
+        # An overwrite defines the section; a later read of it is accepted.
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, overwrites=[((0, 0), (4, 2))])])
+        scope.check(lib, st)
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, reads=[((0, 0), (4, 2))])])
+        scope.check(lib, st)
+        scope.check(lib, lib.fort_scope_cancel_access(dom.ctx, h))
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, reads=[((0, 0), (4, 3))])])
+        assert st == scope.UNINITIALIZED
+
+
+def test_repeated_buffers_and_bad_sections_are_rejected(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.zeros((4, 4), order="F")
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL),
+                                               binding(scope, h, flags=scope.READ_ALL)])
+        assert st == scope.ALIAS
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, reads=[((0, 0), (5, 1))])])
+        assert st == scope.ARGUMENT
+
+
+def test_budget_pressure_evicts_the_least_recently_used_and_publishes_it(scope, cpu_runtime):
+    lib = cpu_runtime
+    n = 1000
+    a, b, c = (np.full((n,), float(v), order="F") for v in (1, 2, 3))
+    with new_domain(scope, lib, budget=2 * n * 8 + 16) as dom:
+        ha, hb, hc = dom.bind(a), dom.bind(b), dom.bind(c)
+        for h in (ha, hb):
+            st, devs = begin_device(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL | scope.OVERWRITE_ALL)])
+            scope.check(lib, st)
+            dev_view(devs[0], (n,))[:] += 10.0
+            scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        st, _ = begin_device(scope, lib, dom, [binding(scope, hc, flags=scope.READ_ALL)])
+        scope.check(lib, st)
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, hc))
+        res = scope.ResidencyStats()
+        scope.check(lib, lib.fort_scope_residency_stats_get_v1(dom.ctx, byref(res)))
+        assert res.evictions == 1 and res.dirty_evictions == 1
+        # The evicted buffer (a, least recently used) was published before its mirror went away.
+        assert np.all(a == 11.0)
+        assert np.all(b == 2.0)
+    assert np.all(b == 12.0) and np.all(c == 3.0)
+
+
+def test_a_footprint_larger_than_the_budget_fails_before_any_change(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.zeros(100, order="F")
+    with new_domain(scope, lib, budget=400) as dom:
+        h = dom.bind(a)
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL)])
+        assert st == scope.RESOURCE
+        assert dom.stats().allocations == 0
+
+
+def test_device_kernels_are_refused_by_the_cpu_backend(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.zeros(16, order="F")
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        access, keep = scope.make_access(scope.READ_ALL)
+        bindings = (scope.PlanBinding * 1)(scope.PlanBinding(h, access))
+
+        class Kernel(ctypes.Structure):
+            _fields_ = [("version", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("function", c_void_p),
+                        ("grid", ctypes.c_uint32 * 3), ("block", ctypes.c_uint32 * 3), ("shared_bytes", c_size_t),
+                        ("arguments", c_void_p), ("argument_count", c_size_t)]
+
+        class Task(ctypes.Structure):
+            _fields_ = [("kernel", Kernel), ("bindings", c_void_p), ("binding_count", c_size_t)]
+
+        arg = ctypes.c_double(1.0)
+        argv = (c_void_p * 1)(ctypes.addressof(arg))
+        sizes = (c_size_t * 1)(8)
+        k = Kernel(1, 0, c_void_p(1), (ctypes.c_uint32 * 3)(1, 1, 1), (ctypes.c_uint32 * 3)(64, 1, 1), 0,
+                   ctypes.cast(argv, c_void_p), 1)
+        t = Task(k, ctypes.cast(bindings, c_void_p), 1)
+        st = lib.fort_scope_graph_enqueue_v1(dom.ctx, 1, ctypes.addressof(t), sizes)
+        assert st == scope.RESOURCE
+        assert dom.stats().allocations == 0
+
+
+def test_scratch_lease(scope, cpu_runtime):
+    lib = cpu_runtime
+    with new_domain(scope, lib) as dom:
+        lease = scope.ScratchLease()
+        scope.check(lib, lib.fort_scope_scratch_acquire_v1(dom.ctx, 4096, byref(lease)))
+        assert lease.device and lease.capacity >= 4096
+        again = scope.ScratchLease()
+        assert lib.fort_scope_scratch_acquire_v1(dom.ctx, 16, byref(again)) == scope.STATE
+        scope.check(lib, lib.fort_scope_scratch_release_v1(dom.ctx, lease.token))
+        scope.check(lib, lib.fort_scope_scratch_acquire_v1(dom.ctx, 1024, byref(again)))
+        assert again.device == lease.device  # reused arena
+        scope.check(lib, lib.fort_scope_scratch_release_v1(dom.ctx, again.token))
+
+
+def test_external_provider_transaction(scope, cpu_runtime):
+    lib = cpu_runtime
+    rhs = np.arange(12, dtype=np.float64).reshape((3, 4), order="F").copy(order="F")
+    phi = np.zeros((3, 4), order="F")
+    seen = {}
+
+    @scope.EXTERNAL_WORKER
+    def worker(user, views, count, stream, started):
+        started[0] = 1
+        seen["count"] = count
+        seen["extents"] = [views[0].extents[d] for d in range(2)]
+        seen["strides"] = [views[0].strides[d] for d in range(2)]
+        src = dev_view(views[1].device, (3, 4))
+        dst = dev_view(views[0].device, (3, 4))
+        dst[:] = 2.0 * src
+        return 0
+
+    with new_domain(scope, lib) as dom:
+        hp, hr = dom.bind(phi, initialized=False), dom.bind(rhs)
+        a0, k0 = scope.make_access(scope.OVERWRITE_ALL)
+        a1, k1 = scope.make_access(scope.READ_ALL)
+        eb = (scope.ExternalBinding * 2)(scope.ExternalBinding(hp, scope.REAL64, 2, 8, a0),
+                                         scope.ExternalBinding(hr, scope.REAL64, 2, 8, a1))
+        call = scope.ExternalCall(1, 0, eb, 2, 0, 0)
+        report = scope.ExternalReport()
+        scope.check(lib, lib.fort_scope_external_execute_v1(dom.ctx, byref(call), worker, None, byref(report)))
+        assert report.started == 1 and report.committed == 1
+        dom.observe(hp)
+    assert seen == {"count": 2, "extents": [3, 4], "strides": [8, 24]}
+    assert np.array_equal(phi, 2.0 * rhs)
+
+
+def test_a_provider_that_fails_after_starting_poisons_the_context(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.zeros(4, order="F")
+
+    @scope.EXTERNAL_WORKER
+    def worker(user, views, count, stream, started):
+        started[0] = 1
+        return scope.EXECUTION
+
+    dom = new_domain(scope, lib)
+    h = dom.bind(a)
+    acc, keep = scope.make_access(scope.READ_ALL | scope.WRITE_ALL)
+    eb = (scope.ExternalBinding * 1)(scope.ExternalBinding(h, scope.REAL64, 1, 8, acc))
+    call = scope.ExternalCall(1, 0, eb, 1, 0, 0)
+    assert lib.fort_scope_external_execute_v1(dom.ctx, byref(call), worker, None, None) == scope.EXECUTION
+    assert lib.fort_scope_wait(dom.ctx) == scope.EXECUTION
+    scope.check(lib, lib.fort_scope_abandon(dom.ctx))
+    scope.check(lib, lib.fort_scope_domain_abandon_v1(dom.domain))
+
+
+def test_a_provider_that_declines_before_enqueue_changes_nothing(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.ones(4, order="F")
+
+    @scope.EXTERNAL_WORKER
+    def worker(user, views, count, stream, started):
+        started[0] = 0
+        return scope.BOUNDARY
+
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        acc, keep = scope.make_access(scope.READ_ALL | scope.OVERWRITE_ALL)
+        eb = (scope.ExternalBinding * 1)(scope.ExternalBinding(h, scope.REAL64, 1, 8, acc))
+        call = scope.ExternalCall(1, 0, eb, 1, 0, 0)
+        assert lib.fort_scope_external_execute_v1(dom.ctx, byref(call), worker, None, None) == scope.BOUNDARY
+        scope.check(lib, lib.fort_scope_wait(dom.ctx))  # the context is still usable
+
+
+def test_execution_error_poisons_until_abandon(scope, cpu_runtime):
+    lib = cpu_runtime
+    dom = new_domain(scope, lib)
+    assert lib.fort_scope_execution_error(dom.ctx, b"kernel failed") == scope.EXECUTION
+    assert lib.fort_scope_wait(dom.ctx) == scope.EXECUTION
+    assert b"only abandon" in lib.fort_scope_error()
+    scope.check(lib, lib.fort_scope_abandon(dom.ctx))
+    assert lib.fort_scope_wait(dom.ctx) == scope.STALE
+    scope.check(lib, lib.fort_scope_domain_abandon_v1(dom.domain))
+
+
+def test_one_context_per_domain(scope, cpu_runtime):
+    lib = cpu_runtime
+    with new_domain(scope, lib) as dom:
+        other = c_uint64()
+        assert lib.fort_scope_create_in_domain_v1(dom.domain, byref(other)) == scope.STATE
+
+
+def test_default_budget_honours_the_environment(scope, cpu_runtime, monkeypatch):
+    lib = cpu_runtime
+    monkeypatch.setenv("FORT_RESIDENT_DEVICE_BUDGET", "3M")
+    out = c_size_t()
+    scope.check(lib, lib.fort_scope_device_budget_default_v1(0, byref(out)))
+    assert out.value == 3 << 20
+    # A domain created afterwards takes it, and reservation runs without deadlock.
+    a = np.zeros(16, order="F")
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        devs = (c_void_p * 1)()
+        hs = (c_uint64 * 1)(h)
+        scope.check(lib, lib.fort_scope_device_reserve_many_v1(dom.ctx, hs, 1, devs))
+        budget, unclaimed = c_size_t(), c_size_t()
+        scope.check(lib, lib.fort_scope_domain_budget_get_v1(dom.domain, byref(budget), byref(unclaimed)))
+        assert budget.value == 3 << 20 and unclaimed.value == (3 << 20) - 128
+
+
+def test_reservation_without_a_stated_budget(scope, cpu_runtime, monkeypatch):
+    # The first reservation of a domain with no stated budget queries the device (regression: it once
+    # re-entered the registry lock).
+    monkeypatch.delenv("FORT_RESIDENT_DEVICE_BUDGET", raising=False)
+    lib = cpu_runtime
+    a = np.zeros(16, order="F")
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        devs = (c_void_p * 1)()
+        hs = (c_uint64 * 1)(h)
+        scope.check(lib, lib.fort_scope_device_reserve_many_v1(dom.ctx, hs, 1, devs))
+        assert devs[0]

@@ -55,6 +55,8 @@ def main() -> int:
     p.add_argument("--ny", type=int, default=512)
     p.add_argument("--nz", type=int, default=128)
     p.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps for the CPU baseline (0: skip)")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="only time a plain 2 GiB device copy (HBM ceiling and byte-counter calibration)")
     args = p.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the LES domain would shard by "
@@ -68,6 +70,14 @@ def main() -> int:
     if lib.ftc_device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
+
+    if args.copy_probe:
+        n = 1 << 28
+        ms = ctypes.c_float()
+        scope.check(lib, lib.ftc_copy_probe(n, 20, ctypes.byref(ms)))
+        print(json.dumps({"copy_probe": {"bytes_per_launch": 16 * n, "ms_per_launch": round(ms.value, 4),
+                                         "GB_per_s": round(16 * n / (ms.value * 1e-3) / 1e9, 1)}}), flush=True)
+        return 0
 
     nx, ny, nz = args.nx, args.ny, args.nz
     dx = dy = dz = 10.0

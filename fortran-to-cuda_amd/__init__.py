@@ -38,6 +38,7 @@ HELPER_SIGNATURES = {
     "ftc_event_elapsed_ms": (c_int, [c_void_p, c_void_p, POINTER(c_float)]),
     "ftc_event_destroy": (c_int, [c_void_p]),
     "ftc_device_count": (c_int, []),
+    "ftc_copy_probe": (c_int, [ctypes.c_size_t, c_int, POINTER(c_float)]),
 }
 
 _loaded: dict[str, ctypes.CDLL] = {}

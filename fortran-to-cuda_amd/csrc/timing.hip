@@ -4,7 +4,50 @@
 
 #include "../../include/fort_scope.h"
 
+__global__ void ftc_copy_kernel(const double *__restrict__ src, double *__restrict__ dst, size_t n)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) dst[e] = src[e];
+}
+
 extern "C" {
+
+// Measures a plain device-to-device copy of n doubles (reps launches of a grid-stride copy kernel on a
+// private stream): the practical HBM ceiling, and the calibration kernel for the FETCH_SIZE/WRITE_SIZE
+// counters (each launch reads and writes exactly 8*n bytes). Writes the average milliseconds per launch.
+int ftc_copy_probe(size_t n, int reps, float *ms_per_launch)
+{
+    if (!ms_per_launch || n == 0 || reps <= 0) return FORT_SCOPE_ARGUMENT;
+    double *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int status = FORT_SCOPE_RESOURCE;
+    if (hipMalloc(&a, n * sizeof(double)) == hipSuccess && hipMalloc(&b, n * sizeof(double)) == hipSuccess &&
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+        hipEventCreate(&e1) == hipSuccess) {
+        (void)hipMemsetAsync(a, 0, n * sizeof(double), s);
+        const unsigned blocks = 256 * 32;
+        hipLaunchKernelGGL(ftc_copy_kernel, dim3(blocks), dim3(256), 0, s, a, b, n);
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(ftc_copy_kernel, dim3(blocks), dim3(256), 0, s, a, b, n);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) == hipSuccess) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            *ms_per_launch = ms / reps;
+            status = FORT_SCOPE_OK;
+        } else {
+            status = FORT_SCOPE_EXECUTION;
+        }
+    }
+    (void)hipGetLastError();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return status;
+}
 
 int ftc_event_create(void **event)
 {

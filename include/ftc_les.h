@@ -65,6 +65,9 @@ int ftc_event_record(void *event, void *stream);
 int ftc_event_elapsed_ms(void *start, void *stop, float *ms);
 int ftc_event_destroy(void *event);
 int ftc_device_count(void);
+/* A plain device-to-device copy of n doubles, launched reps times: the measured HBM ceiling and the
+ * calibration kernel of the HBM byte counters. No reference counterpart. */
+int ftc_copy_probe(size_t n, int reps, float *ms_per_launch);
 
 #ifdef __cplusplus
 }

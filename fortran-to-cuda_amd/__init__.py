@@ -30,7 +30,18 @@ ENTRY_SIGNATURES = {
                                      _IP, _IP, _IP, _IP],
     "cpp_fort_shared_les_step": [c_uint64, c_int, _H, _H, _H, _H, _H, _H, _H, _H, _DP, _DP, _DP, _DP, _DP,
                                  _IP, _IP, _IP, _IP],
+    "cpp_fort_shared_chain_contraction": [c_uint64, c_int, _H, _H, _H, _H, _IP, _IP],
+    "cpp_fort_shared_fixed_order_reduction": [c_uint64, c_int, _H, _IP, POINTER(ctypes.c_longlong),
+                                              POINTER(ctypes.c_longlong), _DP],
 }
+
+
+class GemmDesc(ctypes.Structure):
+    _fields_ = [("a", c_void_p), ("b", c_void_p), ("c", c_void_p), ("type_a", c_int), ("type_b", c_int),
+                ("type_c", c_int), ("m", c_int), ("n", c_int), ("k", c_int), ("batch", c_int),
+                ("lda", ctypes.c_longlong), ("ldb", ctypes.c_longlong), ("ldc", ctypes.c_longlong),
+                ("stride_a", ctypes.c_longlong), ("stride_b", ctypes.c_longlong), ("stride_c", ctypes.c_longlong),
+                ("alpha", c_double), ("beta", c_double)]
 HELPER_SIGNATURES = {
     "ftc_les_coefficients": (None, [c_double, c_double, c_double, c_double, _DP]),
     "ftc_event_create": (c_int, [POINTER(c_void_p)]),
@@ -39,6 +50,9 @@ HELPER_SIGNATURES = {
     "ftc_event_destroy": (c_int, [c_void_p]),
     "ftc_device_count": (c_int, []),
     "ftc_copy_probe": (c_int, [ctypes.c_size_t, c_int, POINTER(c_float)]),
+    "ftc_gemm_device": (c_int, [POINTER(GemmDesc), c_int, c_int, c_void_p]),
+    "ftc_gemm_tile_count": (c_int, []),
+    "ftc_mfma_layout_probe": (c_int, [c_int, c_int, c_int, c_int, c_int, _DP]),
 }
 
 _loaded: dict[str, ctypes.CDLL] = {}

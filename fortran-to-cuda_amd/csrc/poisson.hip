@@ -1,0 +1,651 @@
+// Pressure Poisson solve on gfx950 (the provider declared in include/ftc_poisson.h) and the ELMM adapter
+// that runs it inside a fort_scope external transaction (benchmarks/elmm/sources/gpu_pressure.cpp).
+//
+// Solve A phi = b, A the FD2 7-point Laplacian. Write Qx (nx x nx) and Qy (ny x ny) for the orthonormal
+// real eigenvector bases of the 1D second differences in x and y (periodic: 1, cos, sin and the
+// alternating vector; Neumann on the staggered faces: the DCT-II basis), with eigenvalues lx, ly. Then
+//   B1 = Qx^T b          (per z plane: an nx x nx by nx x ny contraction)
+//   B2 = B1 Qy           (per z plane: nx x ny by ny x ny)
+//   (T_z + (lx_p + ly_q) I) Psi_pq = B2_pq   for every mode (p, q): FP64 Thomas along z
+//   B3 = Psi Qy^T,  phi = Qx B3.
+// The four contractions run on the MFMA engine (gemm.hpp): FP64 MFMA for the FP64 solve, split-bf16
+// products with FP32 accumulation on FP32 workspaces for the mixed solve, whose FP64 residual
+// r = b - A phi (fixed-order norms) drives FP64 iterative refinement phi += M^-1 r.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ftc_poisson.h"
+#include "entry.hpp"
+#include "gemm.hpp"
+#include "reduce.hpp"
+
+#pragma clang fp contract(off)
+
+namespace ftc {
+
+static thread_local std::string g_poisson_error;
+
+static int perr(int status, const std::string &message)
+{
+    g_poisson_error = message;
+    return status;
+}
+
+struct View3 {
+    double *x;  // interior origin
+    long long ld1, ld2;
+};
+
+// r = b - A phi over the interior (phi == nullptr: r = b). phi's neighbours wrap periodically or mirror
+// (Neumann on the staggered face) instead of reading its ghost layers.
+__global__ void poisson_residual_kernel(const double *__restrict__ phi, long long p1, long long p2,
+                                        const double *__restrict__ b, long long b1, long long b2,
+                                        double *__restrict__ r, int nx, int ny, int nz, double rx, double ry,
+                                        double rz, int perx, int pery)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    const double bv = b[i + j * b1 + k * b2];
+    const long long out = (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
+    if (!phi) {
+        r[out] = bv;
+        return;
+    }
+    const int im = i == 0 ? (perx ? nx - 1 : 0) : i - 1, ip = i == nx - 1 ? (perx ? 0 : nx - 1) : i + 1;
+    const int jm = j == 0 ? (pery ? ny - 1 : 0) : j - 1, jp = j == ny - 1 ? (pery ? 0 : ny - 1) : j + 1;
+    const int km = k == 0 ? 0 : k - 1, kp = k == nz - 1 ? nz - 1 : k + 1;
+#define P(a, c, e) phi[(a) + (c) * p1 + (long long)(e) * p2]
+    const double c0 = P(i, j, k);
+    const double lap = (((P(im, j, k) + P(ip, j, k)) - 2.0 * c0) * rx + ((P(i, jm, k) + P(i, jp, k)) - 2.0 * c0) * ry) +
+                       ((P(i, j, km) + P(i, j, kp)) - 2.0 * c0) * rz;
+#undef P
+    r[out] = bv - lap;
+}
+
+// One FP64 Thomas solve per (p, q) column of the transformed right-hand side (p fastest, so a wavefront
+// reads 64 consecutive values of each z plane). The Thomas coefficients of each column live in LDS.
+template <class T>
+__global__ void tridiag_z_kernel(const T *__restrict__ in, T *__restrict__ out, int nx, int ny, int nz,
+                                 const double *__restrict__ lx, const double *__restrict__ ly, double rz, int cols)
+{
+    extern __shared__ double lds[];
+    double *cp = lds, *dp = lds + (long long)nz * cols;
+    const int c = threadIdx.x;
+    const long long col = (long long)blockIdx.x * cols + c;
+    const long long plane = (long long)nx * ny;
+    if (col >= plane) return;
+    const double lam = lx[col % nx] + ly[col / nx];
+#define IN(k) ((double)in[col + (long long)(k) * plane])
+#define OUT(k, v) (out[col + (long long)(k) * plane] = (T)(v))
+    if (lam == 0.0) {
+        // The constant null mode: the solution of zero mean. psi_0 = 0 fixes the constant, the first
+        // nz-1 equations give the rest, then the mean is removed (the last equation holds when the
+        // right-hand side has zero mean).
+        double prev = 0.0, cur = 0.0, sum = 0.0;
+        dp[c] = 0.0;
+        if (nz > 1) {
+            cur = prev + IN(0) / rz;
+            dp[(long long)cols + c] = cur;
+            sum = cur;
+            for (int k = 1; k < nz - 1; ++k) {
+                const double next = (IN(k) / rz - prev) + 2.0 * cur;
+                prev = cur;
+                cur = next;
+                dp[(long long)(k + 1) * cols + c] = cur;
+                sum = sum + cur;
+            }
+        }
+        const double mean = sum / nz;
+        for (int k = 0; k < nz; ++k) OUT(k, dp[(long long)k * cols + c] - mean);
+        return;
+    }
+    double cprev = 0.0, dprev = 0.0;
+    for (int k = 0; k < nz; ++k) {
+        double diag = -2.0 * rz + lam;
+        if (k == 0) diag = diag + rz;
+        if (k == nz - 1) diag = diag + rz;
+        const double m = 1.0 / (diag - rz * cprev);
+        cprev = rz * m;
+        dprev = (IN(k) - rz * dprev) * m;
+        cp[(long long)k * cols + c] = cprev;
+        dp[(long long)k * cols + c] = dprev;
+    }
+    double psi = dprev;
+    OUT(nz - 1, psi);
+    for (int k = nz - 2; k >= 0; --k) {
+        psi = dp[(long long)k * cols + c] - cp[(long long)k * cols + c] * psi;
+        OUT(k, psi);
+    }
+#undef IN
+#undef OUT
+}
+
+// Orthonormal eigenvector basis of the 1D second difference on n cells of width h, column m = mode m,
+// and its eigenvalues. periodic: 1/sqrt(n); sqrt(2/n) cos / sin(2 pi f i / n); (-1)^i / sqrt(n) for even n.
+// Neumann (staggered): DCT-II, c_m cos(pi m (i + 1/2) / n).
+static void basis(int n, double h, bool periodic, std::vector<double> &Q, std::vector<double> &lam)
+{
+    Q.assign((std::size_t)n * n, 0.0);
+    lam.assign(n, 0.0);
+    const double pi = 3.14159265358979323846;
+    const double r2 = std::sqrt(2.0 / n), r1 = 1.0 / std::sqrt((double)n);
+    if (periodic) {
+        for (int i = 0; i < n; ++i) Q[i] = r1;
+        lam[0] = 0.0;
+        for (int f = 1; 2 * f < n; ++f) {
+            const double l = (2.0 * std::cos(2.0 * pi * f / n) - 2.0) / (h * h);
+            for (int i = 0; i < n; ++i) {
+                Q[(std::size_t)(2 * f - 1) * n + i] = r2 * std::cos(2.0 * pi * f * i / n);
+                Q[(std::size_t)(2 * f) * n + i] = r2 * std::sin(2.0 * pi * f * i / n);
+            }
+            lam[2 * f - 1] = l;
+            lam[2 * f] = l;
+        }
+        if (n % 2 == 0) {
+            for (int i = 0; i < n; ++i) Q[(std::size_t)(n - 1) * n + i] = (i % 2 ? -r1 : r1);
+            lam[n - 1] = -4.0 / (h * h);
+        }
+    } else {
+        for (int m = 0; m < n; ++m) {
+            const double c = m == 0 ? r1 : r2;
+            for (int i = 0; i < n; ++i) Q[(std::size_t)m * n + i] = c * std::cos(pi * m * (i + 0.5) / n);
+            lam[m] = (2.0 * std::cos(pi * m / n) - 2.0) / (h * h);
+        }
+    }
+}
+
+static std::vector<double> transpose(const std::vector<double> &Q, int n)
+{
+    std::vector<double> T(Q.size());
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) T[(std::size_t)b * n + a] = Q[(std::size_t)a * n + b];
+    return T;
+}
+
+}  // namespace ftc
+
+using namespace ftc;
+
+struct poisson_solver_handle_v1 {
+    poisson_solver_config_v1 config{};
+    int nx = 0, ny = 0, nz = 0;
+    double hx = 0, hy = 0, hz = 0;
+    bool perx = true, pery = true, mixed = false;
+    double tol = 1e-12;
+    int max_solves = 6, measure = 1;
+    int device = 0;
+    int cols = 64;
+    // device storage (one allocation, carved)
+    void *arena = nullptr;
+    std::size_t arena_bytes = 0;
+    double *Qx = nullptr, *QxT = nullptr, *Qy = nullptr, *QyT = nullptr, *lx = nullptr, *ly = nullptr;
+    float *Qxf = nullptr, *QxTf = nullptr, *Qyf = nullptr, *QyTf = nullptr;
+    void *w0 = nullptr, *w1 = nullptr;
+    double *res = nullptr, *slots = nullptr, *final256 = nullptr;
+    double *host_final = nullptr;  // pinned, 256 doubles
+    poisson_solver_report_v1 report{};
+    std::mutex mutex;
+};
+
+namespace ftc {
+
+static int map_status(int scope_status)
+{
+    switch (scope_status) {
+    case FORT_SCOPE_OK: return POISSON_SOLVER_OK;
+    case FORT_SCOPE_ARGUMENT: return POISSON_SOLVER_ARGUMENT;
+    case FORT_SCOPE_RESOURCE: return POISSON_SOLVER_RESOURCE;
+    default: return POISSON_SOLVER_EXECUTION;
+    }
+}
+
+// The contraction calls of one application of the approximate (mixed) or exact (FP64) inverse:
+// phi_out = beta * phi_out + Qx [T^-1 ((Qx^T in) Qy)] Qy^T.
+static int apply_inverse(poisson_solver_handle_v1 &s, const double *in, long long in1, long long in2, View3 out,
+                         double beta, hipStream_t stream)
+{
+    const int nx = s.nx, ny = s.ny, nz = s.nz;
+    const long long plane = (long long)nx * ny;
+    const int prec = s.mixed ? kBf16x3 : kF64;
+    const int wt = s.mixed ? kFloat : kDouble;
+    const void *QxT = s.mixed ? (const void *)s.QxTf : (const void *)s.QxT;
+    const void *Qy = s.mixed ? (const void *)s.Qyf : (const void *)s.Qy;
+    const void *QyT = s.mixed ? (const void *)s.QyTf : (const void *)s.QyT;
+    const void *Qx = s.mixed ? (const void *)s.Qxf : (const void *)s.Qx;
+    GemmCall g1{QxT, in, s.w0, wt, kDouble, wt, nx, ny, nx, nz, nx, in1, nx, 0, in2, plane, 1.0, 0.0};
+    FTC_CHECK(gemm_launch(g1, prec, stream));
+    GemmCall g2{s.w0, Qy, s.w1, wt, wt, wt, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
+    FTC_CHECK(gemm_launch(g2, prec, stream));
+    const unsigned blocks = (unsigned)((plane + s.cols - 1) / s.cols);
+    const std::size_t lds = 2 * (std::size_t)nz * s.cols * sizeof(double);
+    const double rz = 1.0 / (s.hz * s.hz);
+    if (s.mixed)
+        hipLaunchKernelGGL(tridiag_z_kernel<float>, dim3(blocks), dim3(s.cols), lds, stream,
+                           static_cast<const float *>(s.w1), static_cast<float *>(s.w0), nx, ny, nz, s.lx, s.ly, rz,
+                           s.cols);
+    else
+        hipLaunchKernelGGL(tridiag_z_kernel<double>, dim3(blocks), dim3(s.cols), lds, stream,
+                           static_cast<const double *>(s.w1), static_cast<double *>(s.w0), nx, ny, nz, s.lx, s.ly,
+                           rz, s.cols);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "tridiagonal launch failed");
+    GemmCall g3{s.w0, QyT, s.w1, wt, wt, wt, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
+    FTC_CHECK(gemm_launch(g3, prec, stream));
+    GemmCall g4{Qx, s.w1, out.x, wt, wt, kDouble, nx, ny, nx, nz, nx, nx, out.ld1, 0, plane, out.ld2, 1.0, beta};
+    return gemm_launch(g4, prec, stream);
+}
+
+static int residual(poisson_solver_handle_v1 &s, const double *phi, View3 phiv, View3 b, hipStream_t stream)
+{
+    const dim3 grid((s.nx + 63) / 64, (s.ny + 3) / 4, s.nz);
+    hipLaunchKernelGGL(poisson_residual_kernel, grid, dim3(64, 4), 0, stream, phi, phiv.ld1, phiv.ld2, b.x, b.ld1,
+                       b.ld2, s.res, s.nx, s.ny, s.nz, 1.0 / (s.hx * s.hx), 1.0 / (s.hy * s.hy), 1.0 / (s.hz * s.hz),
+                       s.perx ? 1 : 0, s.pery ? 1 : 0);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "residual launch failed");
+    return FORT_SCOPE_OK;
+}
+
+// Fixed-order 2-norm of a box; synchronises the stream.
+static int norm2(poisson_solver_handle_v1 &s, const ReduceBox &box, hipStream_t stream, double *out)
+{
+    FTC_CHECK(reduce_launch(kReduceSumSquares, box, s.slots, s.final256, stream));
+    if (hipMemcpyAsync(s.host_final, s.final256, kReduceFinal * sizeof(double), hipMemcpyDeviceToHost, stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return fort_scope_report_error(FORT_SCOPE_EXECUTION, "norm readback failed");
+    *out = std::sqrt(reduce_host_fold(kReduceSumSquares, 0.0, s.host_final));
+    return FORT_SCOPE_OK;
+}
+
+static std::size_t align_up(std::size_t x) { return (x + 255) & ~std::size_t(255); }
+
+static std::size_t arena_size(const poisson_solver_handle_v1 &s)
+{
+    const std::size_t N = (std::size_t)s.nx * s.ny * s.nz, wbytes = s.mixed ? 4 : 8;
+    std::size_t b = 0;
+    b += 2 * align_up((std::size_t)s.nx * s.nx * 8) + 2 * align_up((std::size_t)s.ny * s.ny * 8);
+    b += 2 * align_up((std::size_t)s.nx * s.nx * 4) + 2 * align_up((std::size_t)s.ny * s.ny * 4);
+    b += align_up((std::size_t)s.nx * 8) + align_up((std::size_t)s.ny * 8);
+    b += 2 * align_up(N * wbytes) + align_up(N * 8);
+    b += align_up((kReduceSlots + kReduceFinal) * 8);
+    return b;
+}
+
+}  // namespace ftc
+
+extern "C" {
+
+const char *poisson_solver_last_error_v1(void) { return g_poisson_error.c_str(); }
+
+int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_handle_v1 **out)
+{
+    if (!c || !out) return perr(POISSON_SOLVER_ARGUMENT, "missing configuration or output");
+    *out = nullptr;
+    if (c->version != 1 || (c->flags & ~uint32_t(POISSON_SOLVER_REAL32 | POISSON_SOLVER_MIXED)))
+        return perr(POISSON_SOLVER_ARGUMENT, "unknown solver configuration");
+    if (c->flags & POISSON_SOLVER_REAL32)
+        return perr(POISSON_SOLVER_UNSUPPORTED, "REAL32 fields are not supported (use POISSON_SOLVER_MIXED)");
+    if (c->dimensions != 3) return perr(POISSON_SOLVER_UNSUPPORTED, "only three dimensions are supported");
+    if (c->approximation != 2) return perr(POISSON_SOLVER_UNSUPPORTED, "only the FD2 7-point operator is supported");
+    for (int d = 0; d < 3; ++d) {
+        if (c->sizes[d] < 1 || c->sizes[d] > (1 << 20)) return perr(POISSON_SOLVER_ARGUMENT, "invalid size");
+        if (!(c->lengths[d] > 0) || !std::isfinite(c->lengths[d]))
+            return perr(POISSON_SOLVER_ARGUMENT, "invalid length");
+        const int a = c->boundary_conditions[2 * d], b = c->boundary_conditions[2 * d + 1];
+        if (a != b || (a != 0 && a != 4)) return perr(POISSON_SOLVER_UNSUPPORTED, "only periodic/Neumann face pairs");
+    }
+    if (c->boundary_conditions[4] != 4) return perr(POISSON_SOLVER_UNSUPPORTED, "z must be Neumann (staggered)");
+    auto s = std::make_unique<poisson_solver_handle_v1>();
+    s->config = *c;
+    s->nx = c->sizes[0];
+    s->ny = c->sizes[1];
+    s->nz = c->sizes[2];
+    s->hx = c->lengths[0] / s->nx;
+    s->hy = c->lengths[1] / s->ny;
+    s->hz = c->lengths[2] / s->nz;
+    s->perx = c->boundary_conditions[0] == 0;
+    s->pery = c->boundary_conditions[2] == 0;
+    s->mixed = (c->flags & POISSON_SOLVER_MIXED) != 0;
+    s->max_solves = s->mixed ? 6 : 1;
+    s->cols = 2 * (std::size_t)s->nz * 64 * 8 <= 160 * 1024 ? 64 : 32;
+    if (2 * (std::size_t)s->nz * s->cols * 8 > 160 * 1024)
+        return perr(POISSON_SOLVER_UNSUPPORTED, "nz above 320 exceeds the LDS of the z solve");
+    s->arena_bytes = arena_size(*s);
+    if (c->device_budget_bytes && s->arena_bytes > c->device_budget_bytes)
+        return perr(POISSON_SOLVER_RESOURCE, "workspace exceeds device_budget_bytes");
+    if (hipGetDevice(&s->device) != hipSuccess) return perr(POISSON_SOLVER_RESOURCE, "no device");
+    if (hipMalloc(&s->arena, s->arena_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return perr(POISSON_SOLVER_RESOURCE, "workspace allocation failed");
+    }
+    if (hipHostMalloc(reinterpret_cast<void **>(&s->host_final), kReduceFinal * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(s->arena);
+        return perr(POISSON_SOLVER_RESOURCE, "pinned allocation failed");
+    }
+    char *p = static_cast<char *>(s->arena);
+    auto carve = [&](std::size_t bytes) {
+        char *r = p;
+        p += align_up(bytes);
+        return r;
+    };
+    const int nx = s->nx, ny = s->ny;
+    const std::size_t N = (std::size_t)nx * ny * s->nz, wbytes = s->mixed ? 4 : 8;
+    s->Qx = (double *)carve((std::size_t)nx * nx * 8);
+    s->QxT = (double *)carve((std::size_t)nx * nx * 8);
+    s->Qy = (double *)carve((std::size_t)ny * ny * 8);
+    s->QyT = (double *)carve((std::size_t)ny * ny * 8);
+    s->Qxf = (float *)carve((std::size_t)nx * nx * 4);
+    s->QxTf = (float *)carve((std::size_t)nx * nx * 4);
+    s->Qyf = (float *)carve((std::size_t)ny * ny * 4);
+    s->QyTf = (float *)carve((std::size_t)ny * ny * 4);
+    s->lx = (double *)carve((std::size_t)nx * 8);
+    s->ly = (double *)carve((std::size_t)ny * 8);
+    s->w0 = carve(N * wbytes);
+    s->w1 = carve(N * wbytes);
+    s->res = (double *)carve(N * 8);
+    s->slots = (double *)carve((kReduceSlots + kReduceFinal) * 8);
+    s->final256 = s->slots + kReduceSlots;
+    std::vector<double> qx, lx, qy, ly;
+    basis(nx, s->hx, s->perx, qx, lx);
+    basis(ny, s->hy, s->pery, qy, ly);
+    std::vector<double> qxt = transpose(qx, nx), qyt = transpose(qy, ny);
+    auto tofloat = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
+    std::vector<float> qxf = tofloat(qx), qxtf = tofloat(qxt), qyf = tofloat(qy), qytf = tofloat(qyt);
+    bool ok = hipMemcpy(s->Qx, qx.data(), qx.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->QxT, qxt.data(), qxt.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->Qy, qy.data(), qy.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->QyT, qyt.data(), qyt.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->Qxf, qxf.data(), qxf.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->QxTf, qxtf.data(), qxtf.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->Qyf, qyf.data(), qyf.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->QyTf, qytf.data(), qytf.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->lx, lx.data(), lx.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->ly, ly.data(), ly.size() * 8, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        (void)hipFree(s->arena);
+        (void)hipHostFree(s->host_final);
+        return perr(POISSON_SOLVER_RESOURCE, "basis upload failed");
+    }
+    s->report.version = 1;
+    s->report.relative_residual = -1.0;
+    *out = s.release();
+    return POISSON_SOLVER_OK;
+}
+
+int poisson_solver_workspace_query_v1(poisson_solver_handle_v1 *s, poisson_solver_workspace_v1 *bytes)
+{
+    if (!s || !bytes) return perr(POISSON_SOLVER_ARGUMENT, "missing solver or output");
+    bytes->version = 1;
+    bytes->flags = POISSON_SOLVER_DEVICE_BUFFERS_EXACT | POISSON_SOLVER_DRIVER_OVERHEAD_EXCLUDED;
+    bytes->persistent_device_bytes = s->arena_bytes;
+    bytes->peak_device_bytes = s->arena_bytes;
+    bytes->execution_extra_bytes = 0;
+    return POISSON_SOLVER_OK;
+}
+
+int poisson_solver_set_refinement_v1(poisson_solver_handle_v1 *s, double tol, int32_t max_solves, int32_t measure)
+{
+    if (!s || !(tol >= 0) || max_solves < 1) return perr(POISSON_SOLVER_ARGUMENT, "invalid refinement controls");
+    std::lock_guard<std::mutex> lock(s->mutex);
+    s->tol = tol;
+    s->max_solves = s->mixed ? max_solves : 1;
+    s->measure = measure != 0;
+    return POISSON_SOLVER_OK;
+}
+
+int poisson_solver_last_report_v1(poisson_solver_handle_v1 *s, poisson_solver_report_v1 *report)
+{
+    if (!s || !report) return perr(POISSON_SOLVER_ARGUMENT, "missing solver or output");
+    *report = s->report;
+    return POISSON_SOLVER_OK;
+}
+
+int poisson_solver_execute_device_v1(poisson_solver_handle_v1 *s, void *phi, size_t phi_bytes, const void *rhs,
+                                     size_t rhs_bytes, const int32_t phi_ghosts[3], const int32_t rhs_ghosts[3],
+                                     void *stream_ptr, int *started)
+{
+    if (started) *started = 0;
+    if (!s || !phi || !rhs || !phi_ghosts || !rhs_ghosts || !started)
+        return perr(POISSON_SOLVER_ARGUMENT, "missing execute arguments");
+    std::lock_guard<std::mutex> lock(s->mutex);
+    const int n[3] = {s->nx, s->ny, s->nz};
+    long long pe[3], re[3];
+    std::size_t pcount = 1, rcount = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (phi_ghosts[d] < 0 || rhs_ghosts[d] < 0) return perr(POISSON_SOLVER_ARGUMENT, "negative ghost width");
+        pe[d] = n[d] + 2LL * phi_ghosts[d];
+        re[d] = n[d] + 2LL * rhs_ghosts[d];
+        pcount *= (std::size_t)pe[d];
+        rcount *= (std::size_t)re[d];
+    }
+    if (phi_bytes != pcount * 8 || rhs_bytes != rcount * 8)
+        return perr(POISSON_SOLVER_ARGUMENT, "phi/rhs bytes differ from the configured layouts");
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+    View3 pv{static_cast<double *>(phi) + phi_ghosts[0] + phi_ghosts[1] * pe[0] + phi_ghosts[2] * pe[0] * pe[1], pe[0],
+             pe[0] * pe[1]};
+    View3 bv{const_cast<double *>(static_cast<const double *>(rhs)) + rhs_ghosts[0] + rhs_ghosts[1] * re[0] +
+                 rhs_ghosts[2] * re[0] * re[1],
+             re[0], re[0] * re[1]};
+    *started = 1;
+    const long long plane = (long long)s->nx * s->ny;
+    s->report.iterations = 0;
+    s->report.converged = 0;
+    s->report.relative_residual = -1.0;
+    s->report.rhs_norm = -1.0;
+    int st = FORT_SCOPE_OK;
+    double bnorm = 0.0;
+    if (s->measure) {
+        ReduceBox box{bv.x, s->nx, s->ny, s->nz, bv.ld1, bv.ld2};
+        st = norm2(*s, box, stream, &bnorm);
+        s->report.rhs_norm = bnorm;
+    }
+    if (st == FORT_SCOPE_OK) st = apply_inverse(*s, bv.x, bv.ld1, bv.ld2, pv, 0.0, stream);
+    s->report.iterations = 1;
+    while (st == FORT_SCOPE_OK) {
+        const bool last = s->report.iterations >= s->max_solves;
+        if (!s->measure && last) break;
+        st = residual(*s, pv.x, pv, bv, stream);
+        if (st) break;
+        if (s->measure) {
+            double rn = 0.0;
+            ReduceBox box{s->res, s->nx, s->ny, s->nz, s->nx, plane};
+            st = norm2(*s, box, stream, &rn);
+            if (st) break;
+            s->report.relative_residual = bnorm > 0 ? rn / bnorm : rn;
+            if (s->report.relative_residual <= s->tol) {
+                s->report.converged = 1;
+                break;
+            }
+            if (last) break;
+        }
+        st = apply_inverse(*s, s->res, s->nx, plane, pv, 1.0, stream);
+        s->report.iterations++;
+    }
+    if (st) return perr(map_status(st), fort_scope_error());
+    return POISSON_SOLVER_OK;
+}
+
+int poisson_solver_destroy_v1(poisson_solver_handle_v1 **h)
+{
+    if (!h) return perr(POISSON_SOLVER_ARGUMENT, "missing solver pointer");
+    poisson_solver_handle_v1 *s = *h;
+    if (!s) return POISSON_SOLVER_OK;
+    (void)hipDeviceSynchronize();
+    if (s->arena) (void)hipFree(s->arena);
+    if (s->host_final) (void)hipHostFree(s->host_final);
+    delete s;
+    *h = nullptr;
+    return POISSON_SOLVER_OK;
+}
+
+}  // extern "C"
+
+// ================================================================================================ ELMM adapter
+struct elmm_gpu_pressure_handle_v1 {
+    fort_scope_domain_t domain = 0;
+    uint64_t lease = 0;
+    std::size_t workspace = 0;
+    int device = -1;
+    poisson_solver_handle_v1 *solver = nullptr;
+    elmm_gpu_pressure_config_v1 config{};
+    bool ready = false;
+    std::mutex mutex;
+};
+
+namespace ftc {
+static thread_local std::string g_adapter_error;
+
+static int adapter_error(int status, const std::string &m)
+{
+    g_adapter_error = m;
+    return status;
+}
+
+static int provider_status(int status)
+{
+    if (!status) return FORT_SCOPE_OK;
+    g_adapter_error = poisson_solver_last_error_v1();
+    if (status == POISSON_SOLVER_ARGUMENT || status == POISSON_SOLVER_UNSUPPORTED) return FORT_SCOPE_ARGUMENT;
+    if (status == POISSON_SOLVER_RESOURCE) return FORT_SCOPE_RESOURCE;
+    return FORT_SCOPE_EXECUTION;
+}
+
+// The views the transaction hands over must be the dense REAL64 layouts the configuration promises.
+static bool view_ok(const fort_scope_external_view_v1 &v, const elmm_gpu_pressure_handle_v1 &h, bool phi)
+{
+    if (v.type != FORT_SCOPE_REAL64 || v.rank != 3 || v.element_bytes != 8 || !v.extents || !v.strides ||
+        !v.lower_bounds || !v.device)
+        return false;
+    const int g = phi ? 2 : 1;
+    const int64_t *lower = phi ? h.config.phi_lower_bounds : h.config.rhs_lower_bounds;
+    std::size_t bytes = 8;
+    for (int d = 0; d < 3; ++d) {
+        if (v.extents[d] != (std::size_t)h.config.solver.sizes[d] + 2 * g || v.strides[d] != bytes ||
+            v.lower_bounds[d] != lower[d])
+            return false;
+        bytes *= v.extents[d];
+    }
+    return v.bytes == bytes;
+}
+
+static int solve_worker(void *opaque, const fort_scope_external_view_v1 *views, std::size_t count, void *stream,
+                        int *started)
+{
+    *started = 0;
+    auto &h = *static_cast<elmm_gpu_pressure_handle_v1 *>(opaque);
+    if (count != 2 || !views) return adapter_error(FORT_SCOPE_ARGUMENT, "pressure transaction requires Phi and RHS");
+    if (!view_ok(views[0], h, true) || !view_ok(views[1], h, false))
+        return adapter_error(FORT_SCOPE_ARGUMENT, "pressure view differs from the checked dense layout");
+    const int32_t pg[3] = {2, 2, 2}, rg[3] = {1, 1, 1};
+    return provider_status(poisson_solver_execute_device_v1(h.solver, views[0].device, views[0].bytes, views[1].device,
+                                                            views[1].bytes, pg, rg, stream, started));
+}
+}  // namespace ftc
+
+extern "C" {
+
+const char *elmm_gpu_pressure_error_v1(void) { return g_adapter_error.c_str(); }
+
+poisson_solver_handle_v1 *elmm_gpu_pressure_provider_v1(elmm_gpu_pressure_handle_v1 *h) { return h ? h->solver : nullptr; }
+
+int elmm_gpu_pressure_create_v1(fort_scope_domain_t domain, const elmm_gpu_pressure_config_v1 *c,
+                                elmm_gpu_pressure_handle_v1 **out)
+{
+    if (!out) return adapter_error(FORT_SCOPE_ARGUMENT, "missing pressure provider output");
+    *out = nullptr;
+    if (!c || !domain) return adapter_error(FORT_SCOPE_ARGUMENT, "missing pressure config/domain");
+    if (c->version != 1 || c->reserved || c->solver.version != 1)
+        return adapter_error(FORT_SCOPE_ARGUMENT, "unknown pressure adapter configuration");
+    if (!c->solver.device_budget_bytes) return adapter_error(FORT_SCOPE_ARGUMENT, "invalid provider workspace cap");
+    for (int d = 0; d < 3; ++d)
+        if (c->solver.sizes[d] < 2)
+            return adapter_error(FORT_SCOPE_ARGUMENT, "two-layer pressure faces require extents >= 2");
+    int device = -1;
+    int st = fort_scope_domain_device_get_v1(domain, &device);
+    if (st) return adapter_error(st, fort_scope_error());
+    auto h = std::make_unique<elmm_gpu_pressure_handle_v1>();
+    h->domain = domain;
+    h->config = *c;
+    h->device = device;
+    st = fort_scope_domain_workspace_reserve_v1(domain, (std::size_t)c->solver.device_budget_bytes, &h->lease);
+    if (st) return adapter_error(st, fort_scope_error());
+    int previous = 0;
+    (void)hipGetDevice(&previous);
+    (void)hipSetDevice(device);
+    st = provider_status(poisson_solver_create_v1(&c->solver, &h->solver));
+    poisson_solver_workspace_v1 bytes{};
+    if (!st) st = provider_status(poisson_solver_workspace_query_v1(h->solver, &bytes));
+    if (!st) {
+        h->workspace = (std::size_t)(bytes.persistent_device_bytes + bytes.execution_extra_bytes);
+        st = fort_scope_domain_workspace_resize_v1(domain, h->lease, h->workspace);
+        if (st) g_adapter_error = fort_scope_error();
+    }
+    (void)hipSetDevice(previous);
+    if (st) {
+        if (h->solver) (void)poisson_solver_destroy_v1(&h->solver);
+        (void)fort_scope_domain_workspace_release_v1(domain, h->lease);
+        return st;
+    }
+    h->ready = true;
+    *out = h.release();
+    return FORT_SCOPE_OK;
+}
+
+int elmm_gpu_pressure_solve_v1(fort_scope_t context, elmm_gpu_pressure_handle_v1 *h, fort_buffer_t phi,
+                               fort_buffer_t rhs, fort_scope_external_report_v1 *report)
+{
+    if (report) *report = fort_scope_external_report_v1{1, FORT_SCOPE_ARGUMENT, 0, 0};
+    if (!h || !report || !phi || !rhs || phi == rhs)
+        return adapter_error(FORT_SCOPE_ARGUMENT, "missing/aliased pressure roots or provider");
+    std::lock_guard<std::mutex> lock(h->mutex);
+    fort_scope_domain_t domain = 0;
+    int st = fort_scope_domain_get_v1(context, &domain);
+    if (st) return adapter_error(st, fort_scope_error());
+    if (domain != h->domain || !domain || !h->ready)
+        return adapter_error(FORT_SCOPE_ARGUMENT, "pressure provider belongs to another domain or is not ready");
+    std::size_t pl[3] = {2, 2, 2}, pu[3], rl[3] = {1, 1, 1}, ru[3];
+    for (int d = 0; d < 3; ++d) {
+        pu[d] = (std::size_t)h->config.solver.sizes[d] + 2;
+        ru[d] = (std::size_t)h->config.solver.sizes[d] + 1;
+    }
+    fort_scope_section interior{pl, pu}, input{rl, ru};
+    fort_scope_access write{0, 0, nullptr, 1, &interior, 1, &interior}, read{0, 1, &input, 0, nullptr, 0, nullptr};
+    fort_scope_external_binding_v1 bindings[2] = {{phi, FORT_SCOPE_REAL64, 3, 8, write},
+                                                  {rhs, FORT_SCOPE_REAL64, 3, 8, read}};
+    fort_scope_external_call_v1 call{1, 0, bindings, 2, h->lease, h->workspace};
+    st = fort_scope_external_execute_v1(context, &call, solve_worker, h, report);
+    report->status = st;
+    if (st) g_adapter_error = g_adapter_error.empty() ? fort_scope_error() : g_adapter_error;
+    return st;
+}
+
+int elmm_gpu_pressure_destroy_v1(elmm_gpu_pressure_handle_v1 **out)
+{
+    if (!out) return adapter_error(FORT_SCOPE_ARGUMENT, "missing pressure provider pointer");
+    elmm_gpu_pressure_handle_v1 *h = *out;
+    if (!h) return FORT_SCOPE_OK;
+    {
+        std::lock_guard<std::mutex> lock(h->mutex);
+        if (h->solver) {
+            int st = provider_status(poisson_solver_destroy_v1(&h->solver));
+            if (st) return st;
+        }
+        if (h->lease) {
+            int st = fort_scope_domain_workspace_release_v1(h->domain, h->lease);
+            if (st) return adapter_error(st, fort_scope_error());
+            h->lease = 0;
+        }
+    }
+    delete h;
+    *out = nullptr;
+    return FORT_SCOPE_OK;
+}
+
+}  // extern "C"

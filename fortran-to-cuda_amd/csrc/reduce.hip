@@ -1,0 +1,175 @@
+// Fixed-order reductions (compiler/resident/reductions.py:1-45, SLOTS/FINAL/DIRECT at :66-72).
+//
+// A nest of n iterations (Fortran order over its loop box) with n > 65536 is split over 65536 slots, slot s
+// taking iterations s, s + 65536, s + 2*65536, ... in order; 256 fold threads then take slot results
+// f, f + 256, ... in order; the host folds the 256 values in index order into the caller's variable.
+// A nest of at most 65536 iterations goes straight to 256 partial results (thread f takes iterations
+// f, f + 256, ...). The order never depends on the launch, so real sums repeat exactly from run to run,
+// and max/min/maxabs equal the serial result.
+#include <cmath>
+
+#include "entry.hpp"
+#include "reduce.hpp"
+
+namespace ftc {
+
+// max and min fold as the reference's ordered maximum/minimum (compiler/runtime/numeric.hpp:48-58):
+// max(acc, x) = acc > x ? acc : x, so a NaN operand decides by position exactly as on the host.
+template <int OP>
+__device__ __forceinline__ double combine(double acc, double x)
+{
+    if constexpr (OP == kReduceSum) return acc + x;
+    else if constexpr (OP == kReduceSumSquares) return acc + x * x;
+    else if constexpr (OP == kReduceMax) return acc > x ? acc : x;
+    else if constexpr (OP == kReduceMin) return acc < x ? acc : x;
+    else {
+        const double m = x < 0.0 ? -x : x;
+        return acc > m ? acc : m;
+    }
+}
+
+template <int OP>
+__device__ __forceinline__ double identity()
+{
+    if constexpr (OP == kReduceSum || OP == kReduceSumSquares || OP == kReduceMaxAbs) return 0.0;
+    else if constexpr (OP == kReduceMax) return -HUGE_VAL;
+    else return HUGE_VAL;
+}
+
+// Element of iteration t of the loop box (column-major over the box).
+__device__ __forceinline__ double at(const ReduceBox &b, long long t)
+{
+    const long long i = t % b.n1, r = t / b.n1;
+    const long long j = r % b.n2, k = r / b.n2;
+    return b.x[i + j * b.ld1 + k * b.ld2];
+}
+
+template <int OP>
+__global__ void reduce_slots(const ReduceBox b, long long n, int slots, double *out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slots) return;
+    double acc = identity<OP>();
+    for (long long t = s; t < n; t += slots) acc = combine<OP>(acc, at(b, t));
+    out[s] = acc;
+}
+
+// Folds `count` partial results into 256 (thread f takes f, f + 256, ... in order). For sums of squares
+// the partial results are already squared, so the fold is a plain sum.
+template <int OP>
+__global__ void reduce_fold(const double *partial, int count, double *out)
+{
+    const int f = threadIdx.x;
+    constexpr int FOP = OP == kReduceSumSquares ? kReduceSum : (OP == kReduceMaxAbs ? kReduceMax : OP);
+    double acc = identity<FOP>();
+    for (int t = f; t < count; t += kReduceFinal) acc = combine<FOP>(acc, partial[t]);
+    out[f] = acc;
+}
+
+template <int OP>
+static int launch(const ReduceBox &b, double *slots, double *final256, hipStream_t stream)
+{
+    const long long n = b.n1 * b.n2 * b.n3;
+    if (n <= kReduceDirect) {
+        hipLaunchKernelGGL(reduce_slots<OP>, dim3(1), dim3(kReduceFinal), 0, stream, b, n, kReduceFinal, final256);
+    } else {
+        hipLaunchKernelGGL(reduce_slots<OP>, dim3(kReduceSlots / 256), dim3(256), 0, stream, b, n, kReduceSlots,
+                           slots);
+        hipLaunchKernelGGL(reduce_fold<OP>, dim3(1), dim3(kReduceFinal), 0, stream, slots, kReduceSlots, final256);
+    }
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "reduction launch failed");
+    return FORT_SCOPE_OK;
+}
+
+int reduce_launch(int op, const ReduceBox &b, double *slots, double *final256, hipStream_t stream)
+{
+    if (b.n1 < 0 || b.n2 < 0 || b.n3 < 0) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "negative extent");
+    switch (op) {
+    case kReduceSum: return launch<kReduceSum>(b, slots, final256, stream);
+    case kReduceSumSquares: return launch<kReduceSumSquares>(b, slots, final256, stream);
+    case kReduceMax: return launch<kReduceMax>(b, slots, final256, stream);
+    case kReduceMin: return launch<kReduceMin>(b, slots, final256, stream);
+    case kReduceMaxAbs: return launch<kReduceMaxAbs>(b, slots, final256, stream);
+    default: return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unknown reduction");
+    }
+}
+
+double reduce_host_fold(int op, double initial, const double *final256)
+{
+    double acc = initial;
+    for (int f = 0; f < kReduceFinal; ++f) {
+        const double x = final256[f];
+        switch (op) {
+        case kReduceSum:
+        case kReduceSumSquares: acc = acc + x; break;
+        case kReduceMax:
+        case kReduceMaxAbs: acc = acc > x ? acc : x; break;
+        default: acc = acc < x ? acc : x; break;
+        }
+    }
+    return acc;
+}
+
+}  // namespace ftc
+
+using namespace ftc;
+
+extern "C" {
+
+// Reduces the section [lo, hi) (zero-based, rank 3) of a registered REAL64 array in the fixed order and
+// folds it into *value (in: the variable's value before the nest, out: after). op: 0 sum, 1 sum of
+// squares, 2 max, 3 min, 4 max of magnitudes. The result is back on the host when the call returns.
+int cpp_fort_shared_fixed_order_reduction(fort_scope_t ctx, int mode, fort_buffer_t x, const int *op,
+                                          const long long *lo3, const long long *hi3, double *value)
+{
+    FTC_CHECK(entry_preamble(mode));
+    if (!op || !lo3 || !hi3 || !value) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "null scalar argument");
+    std::size_t e[3];
+    FTC_CHECK(layout3(ctx, x, FORT_SCOPE_REAL64, 8, e));
+    for (int d = 0; d < 3; ++d)
+        if (lo3[d] < 0 || hi3[d] < lo3[d] || (std::size_t)hi3[d] > e[d])
+            return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "reduction box outside the array");
+    const long long n1 = hi3[0] - lo3[0], n2 = hi3[1] - lo3[1], n3 = hi3[2] - lo3[2];
+    if (!n1 || !n2 || !n3) return FORT_SCOPE_OK;
+    std::vector<void *> dev;
+    FTC_CHECK(reserve(ctx, {x}, dev));
+    fort_scope_scratch_lease_v1 lease{};
+    FTC_CHECK(fort_scope_scratch_acquire_v1(ctx, (kReduceSlots + kReduceFinal) * sizeof(double), &lease));
+    double *slots = static_cast<double *>(lease.device), *final256 = slots + kReduceSlots;
+    // The reduction reads the device copy: make it current through a device access of the box, run on
+    // the context stream, then bring the 256 values back.
+    Sec3 box{{(std::size_t)lo3[0], (std::size_t)lo3[1], (std::size_t)lo3[2]},
+             {(std::size_t)hi3[0], (std::size_t)hi3[1], (std::size_t)hi3[2]}};
+    fort_scope_section sec{box.lo, box.hi};
+    fort_scope_plan_binding bind{};
+    bind.buffer = x;
+    bind.access.read_count = 1;
+    bind.access.reads = &sec;
+    void *device = nullptr;
+    int status = fort_scope_access_begin_v1(ctx, &bind, 1, 1, &device);
+    int prev = 0;
+    void *stream = nullptr;
+    if (status == FORT_SCOPE_OK) status = fort_scope_gpu_enter(ctx, &prev, &stream);
+    double host[kReduceFinal];
+    if (status == FORT_SCOPE_OK) {
+        ReduceBox b{static_cast<const double *>(device) + lo3[0] + lo3[1] * (long long)e[0] +
+                        lo3[2] * (long long)e[0] * (long long)e[1],
+                    n1, n2, n3, (long long)e[0], (long long)e[0] * (long long)e[1]};
+        status = reduce_launch(*op, b, slots, final256, static_cast<hipStream_t>(stream));
+        if (status == FORT_SCOPE_OK &&
+            hipMemcpyAsync(host, final256, sizeof host, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)) !=
+                hipSuccess)
+            status = FORT_SCOPE_EXECUTION;
+        if (status == FORT_SCOPE_OK && hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess)
+            status = FORT_SCOPE_EXECUTION;
+        (void)fort_scope_gpu_leave(ctx, prev);
+        int ended = fort_scope_device_end(ctx, x);
+        if (status == FORT_SCOPE_OK) status = ended;
+        if (status == FORT_SCOPE_OK) *value = reduce_host_fold(*op, *value, host);
+        else if (status == FORT_SCOPE_EXECUTION) (void)fort_scope_execution_error(ctx, "reduction failed after start");
+    }
+    int released = fort_scope_scratch_release_v1(ctx, lease.token);
+    return status ? status : released;
+}
+
+}  // extern "C"

@@ -217,13 +217,14 @@ class Domain:
     def ctx(self) -> int:
         return self.context.value
 
-    def bind(self, array: np.ndarray, initialized: bool = True, identity: int | None = None) -> int:
-        """Registers a Fortran-ordered host array (borrowed until close) and returns its handle."""
+    def bind(self, array: np.ndarray, initialized: bool = True, identity: int | None = None, lower=None) -> int:
+        """Registers a Fortran-ordered host array (borrowed until close) and returns its handle. `lower`
+        gives the Fortran lower bounds (default 1 in every dimension)."""
         if not array.flags.f_contiguous:
             raise ValueError("arrays must be Fortran-contiguous")
         kind = _TYPES.get(array.dtype, BYTES)
         ext = _sizes(array.shape)
-        lower = (c_int64 * array.ndim)(*([1] * array.ndim))
+        lower = (c_int64 * array.ndim)(*(lower if lower is not None else [1] * array.ndim))
         layout = Layout(array.ndim, kind, array.itemsize, array.ctypes.data, ext, lower, 0)
         handle = c_uint64()
         if identity is None:

@@ -1,0 +1,28 @@
+/* Fixed-order reductions of resident arrays.
+ *
+ * The reference lowers an accumulating nest into up to 65536 slots (slot s takes iterations s, s+65536, ...
+ * in order), a fold into 256 values (value f takes slots f, f+256, ...) and a host fold of the 256 in index
+ * order; nests of at most 65536 iterations go straight to 256 values (compiler/resident/reductions.py:1-45,
+ * constants :66-72). Max and min then equal the serial result and real sums repeat exactly run to run.
+ */
+#ifndef FTC_REDUCE_H
+#define FTC_REDUCE_H
+#include "fort_scope.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ftc_reduction { FTC_REDUCE_SUM = 0, FTC_REDUCE_SUM_SQUARES = 1, FTC_REDUCE_MAX = 2, FTC_REDUCE_MIN = 3,
+                     FTC_REDUCE_MAX_ABS = 4 };
+
+/* Reduces the zero-based half-open box [lo3, hi3) of a registered rank-3 REAL64 array (iterations in
+ * Fortran order over the box) and folds the result into *value (in: the variable before the nest; out:
+ * after). max/min fold as `value > x ? value : x` (compiler/runtime/numeric.hpp:48-58). The result is on
+ * the host when the call returns. */
+int cpp_fort_shared_fixed_order_reduction(fort_scope_t ctx, int mode, fort_buffer_t x, const int *op,
+                                          const long long *lo3, const long long *hi3, double *value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

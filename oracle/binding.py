@@ -52,6 +52,18 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
+def fixed_order_reduce(x: np.ndarray, lo, hi, op: int, initial: float) -> float:
+    """The fixed-order reduction of the box [lo, hi) of a rank-3 Fortran array (oracle/reduce.c)."""
+    l = lib()
+    fn = l.orc_fixed_order_reduce
+    fn.restype = c_double
+    fn.argtypes = [_V, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
+                   ctypes.c_longlong, _I, _D]
+    e1, e2 = x.shape[0], x.shape[1]
+    base = x.ctypes.data + 8 * (lo[0] + lo[1] * e1 + lo[2] * e1 * e2)
+    return fn(base, hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], e1, e1 * e2, op, initial)
+
+
 def ptr(a: np.ndarray | None) -> int | None:
     return None if a is None else a.ctypes.data
 

@@ -1,0 +1,82 @@
+"""The device pressure solve through the ELMM adapter and a fort_scope external transaction, against the
+oracle (oracle/poisson.py, pinned in test_poisson_oracle.py).
+
+Tolerances: the FP64 solve is direct (dense FP64 transforms + FP64 Thomas), so its field error is a few
+ulps of the transforms: 1e-11 of max|phi|. The mixed solve refines with FP64 residuals until the relative
+residual is <= 1e-12 and must then meet the field target of 1e-10 of max|phi|.
+"""
+import numpy as np
+import pytest
+
+from oracle import poisson as poisson_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def run(ftc, scope, hip, b, h, mixed, bc_xy=(0, 0), tol=1e-12, max_solves=8):
+    from __graft_entry__ import _module
+
+    pressure = _module("pressure")
+    nx, ny, nz = b.shape
+    phi = np.zeros((nx + 4, ny + 4, nz + 4), order="F")
+    phi[0, 0, 0] = 123.0  # a halo value the solve must preserve
+    rhs = np.zeros((nx + 2, ny + 2, nz + 2), order="F")
+    rhs[1:-1, 1:-1, 1:-1] = b
+    with scope.Domain(hip) as dom:
+        hp = dom.bind(phi, lower=(-1, -1, -1))
+        hr = dom.bind(rhs, lower=(0, 0, 0))
+        solver = pressure.PressureSolver(hip, dom, (nx, ny, nz), (nx * h[0], ny * h[1], nz * h[2]), mixed, bc_xy)
+        solver.refinement(tol, max_solves, True)
+        ext = solver.solve(hp, hr)
+        assert ext.started == 1 and ext.committed == 1
+        report = solver.report()
+        dom.observe(hp)
+        solver.destroy()
+    assert phi[0, 0, 0] == 123.0
+    return phi[2:-2, 2:-2, 2:-2].copy(), report
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("n", [(64, 48, 32), (37, 29, 17), (128, 128, 64)])
+def test_solve_matches_the_oracle(ftc, scope, hip, mixed, n):
+    h = (10.0, 10.0, 10.0)
+    b = poisson_oracle.divergence_rhs(*n, h, seed=sum(n))
+    got, report = run(ftc, scope, hip, b, h, mixed)
+    want = poisson_oracle.solve(b, h)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    assert err <= (1e-10 if mixed else 1e-11), (err, report.iterations, report.relative_residual)
+    assert report.relative_residual <= (1e-12 if mixed else 1e-12)
+    if mixed:
+        assert report.converged == 1 and report.iterations >= 2
+
+
+@pytest.mark.parametrize("bc_xy", [(4, 0), (0, 4), (4, 4)])
+def test_neumann_walls_in_x_or_y(ftc, scope, hip, bc_xy):
+    n, h = (40, 36, 12), (2.0, 3.0, 1.0)
+    kinds = tuple("periodic" if c == 0 else "neumann" for c in bc_xy)
+    rng = np.random.default_rng(5)
+    b = rng.uniform(-1, 1, n)
+    b -= b.mean()
+    got, _ = run(ftc, scope, hip, b, h, False, bc_xy)
+    want = poisson_oracle.solve(b, h, kinds)
+    assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
+
+
+def test_manufactured_eigenmode(ftc, scope, hip):
+    nx, ny, nz, h = 48, 32, 24, (1.0, 1.0, 1.0)
+    i, j, k = np.arange(nx), np.arange(ny), np.arange(nz)
+    phi = (np.cos(2 * np.pi * 3 * i / nx)[:, None, None] * np.sin(2 * np.pi * 2 * j / ny)[None, :, None]
+           * np.cos(np.pi * 2 * (k + 0.5) / nz)[None, None, :])
+    lam = ((2 * np.cos(2 * np.pi * 3 / nx) - 2) + (2 * np.cos(2 * np.pi * 2 / ny) - 2) + (2 * np.cos(np.pi * 2 / nz) - 2))
+    got, _ = run(ftc, scope, hip, lam * phi, h, False)
+    assert np.abs(got - phi).max() <= 1e-12
+
+
+def test_unsupported_configurations_are_rejected(ftc, scope, hip):
+    from __graft_entry__ import _module
+
+    pressure = _module("pressure")
+    with scope.Domain(hip) as dom:
+        with pytest.raises(scope.ScopeError) as e:
+            pressure.PressureSolver(hip, dom, (16, 16, 1), (1.0, 1.0, 1.0))
+        assert e.value.status == scope.ARGUMENT

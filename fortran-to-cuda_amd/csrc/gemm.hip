@@ -3,10 +3,11 @@
 //
 // Lane layouts used (wave64, l = lane):
 //   v_mfma_f64_16x16x4f64   A(16x4):  lane l holds A[l%16][l/16];  B(4x16): B[l/16][l%16];
-//                           D(16x16): lane l, element r holds D[4*(l/16)+r][l%16]
+//                           D(16x16): lane l, element r holds D[l/16 + 4*r][l%16]
 //   v_mfma_f32_16x16x32_bf16 A(16x32): lane l holds A[l%16][8*(l/16) .. 8*(l/16)+7];
-//                           B(32x16): B[8*(l/16) .. +7][l%16];  D as above (FP32)
-// tests/test_gemm_gpu.py checks both against the probe before trusting any product.
+//                           B(32x16): B[8*(l/16) .. +7][l%16];
+//                           D(16x16, FP32): lane l, element r holds D[4*(l/16) + r][l%16]
+// tests/test_dense_gpu.py checks both against the probe before trusting any product.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_f64_kernel(const GemmCall g
         for (int y = 0; y < TN; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * WTM + x * 16 + 4 * (lane / 16) + r;
+                const int row = m0 + wm * WTM + x * 16 + lane / 16 + 4 * r;
                 const int col = n0 + wn * WTN + y * 16 + lane % 16;
                 if (row < g.M && col < g.N) {
                     const long long ci = (long long)row + (long long)col * g.ldc;

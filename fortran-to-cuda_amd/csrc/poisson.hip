@@ -68,6 +68,23 @@ __global__ void poisson_residual_kernel(const double *__restrict__ phi, long lon
     r[out] = bv - lap;
 }
 
+// The mean of phi from the 256 fixed-order partial sums (folded in index order), on the device.
+__global__ void poisson_mean_kernel(const double *__restrict__ final256, double count, double *__restrict__ mean)
+{
+    double acc = 0.0;
+    for (int f = 0; f < kReduceFinal; ++f) acc = acc + final256[f];
+    *mean = acc / count;
+}
+
+__global__ void poisson_subtract_kernel(double *__restrict__ phi, long long p1, long long p2, int nx, int ny,
+                                        const double *__restrict__ mean)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    double &v = phi[i + j * p1 + (long long)k * p2];
+    v = v - *mean;
+}
+
 // One FP64 Thomas solve per (p, q) column of the transformed right-hand side (p fastest, so a wavefront
 // reads 64 consecutive values of each z plane). The Thomas coefficients of each column live in LDS.
 template <class T>
@@ -262,6 +279,23 @@ static int norm2(poisson_solver_handle_v1 &s, const ReduceBox &box, hipStream_t 
     return FORT_SCOPE_OK;
 }
 
+// phi -= mean(phi) over the interior, in FP64 with a fixed-order sum, without a host synchronisation.
+// The singular problem's solution is the zero-mean one; the approximate (split-bf16) transforms leak a
+// small constant into phi that no residual sees, so the mixed solve removes it explicitly.
+static int remove_mean(poisson_solver_handle_v1 &s, View3 phi, hipStream_t stream)
+{
+    ReduceBox box{phi.x, s.nx, s.ny, s.nz, phi.ld1, phi.ld2};
+    FTC_CHECK(reduce_launch(kReduceSum, box, s.slots, s.final256, stream));
+    double *mean = s.final256 + kReduceFinal;
+    hipLaunchKernelGGL(poisson_mean_kernel, dim3(1), dim3(1), 0, stream, s.final256,
+                       (double)s.nx * s.ny * s.nz, mean);
+    const dim3 grid((s.nx + 63) / 64, (s.ny + 3) / 4, s.nz);
+    hipLaunchKernelGGL(poisson_subtract_kernel, grid, dim3(64, 4), 0, stream, phi.x, phi.ld1, phi.ld2, s.nx, s.ny,
+                       mean);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "mean removal failed");
+    return FORT_SCOPE_OK;
+}
+
 static std::size_t align_up(std::size_t x) { return (x + 255) & ~std::size_t(255); }
 
 static std::size_t arena_size(const poisson_solver_handle_v1 &s)
@@ -272,7 +306,7 @@ static std::size_t arena_size(const poisson_solver_handle_v1 &s)
     b += 2 * align_up((std::size_t)s.nx * s.nx * 4) + 2 * align_up((std::size_t)s.ny * s.ny * 4);
     b += align_up((std::size_t)s.nx * 8) + align_up((std::size_t)s.ny * 8);
     b += 2 * align_up(N * wbytes) + align_up(N * 8);
-    b += align_up((kReduceSlots + kReduceFinal) * 8);
+    b += align_up((kReduceSlots + kReduceFinal + 1) * 8);  // slots, 256 partials, the mean
     return b;
 }
 
@@ -349,7 +383,7 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
     s->w0 = carve(N * wbytes);
     s->w1 = carve(N * wbytes);
     s->res = (double *)carve(N * 8);
-    s->slots = (double *)carve((kReduceSlots + kReduceFinal) * 8);
+    s->slots = (double *)carve((kReduceSlots + kReduceFinal + 1) * 8);
     s->final256 = s->slots + kReduceSlots;
     std::vector<double> qx, lx, qy, ly;
     basis(nx, s->hx, s->perx, qx, lx);
@@ -468,6 +502,7 @@ int poisson_solver_execute_device_v1(poisson_solver_handle_v1 *s, void *phi, siz
         st = apply_inverse(*s, s->res, s->nx, plane, pv, 1.0, stream);
         s->report.iterations++;
     }
+    if (st == FORT_SCOPE_OK && s->mixed) st = remove_mean(*s, pv, stream);
     if (st) return perr(map_status(st), fort_scope_error());
     return POISSON_SOLVER_OK;
 }

@@ -71,8 +71,13 @@ class PressureSolver:
     def __init__(self, lib, domain: "scope.Domain", n, lengths, mixed=False, bc_xy=(PERIODIC, PERIODIC)):
         self.lib = bind(lib)
         self.domain = domain
-        cfg = AdapterConfig(1, 0, solver_config(n, lengths, mixed, bc_xy), (c_int64 * 3)(-1, -1, -1),
-                            (c_int64 * 3)(0, 0, 0))
+        # The provider's workspace cap is what the domain has not claimed yet
+        # (benchmarks/elmm/sources/resident_pressure_state.f90:29-50).
+        budget, unclaimed = ctypes.c_size_t(), ctypes.c_size_t()
+        scope.check(lib, lib.fort_scope_domain_budget_get_v1(domain.domain, ctypes.byref(budget),
+                                                             ctypes.byref(unclaimed)))
+        cfg = AdapterConfig(1, 0, solver_config(n, lengths, mixed, bc_xy, unclaimed.value),
+                            (c_int64 * 3)(-1, -1, -1), (c_int64 * 3)(0, 0, 0))
         self.handle = c_void_p()
         status = lib.elmm_gpu_pressure_create_v1(domain.domain, ctypes.byref(cfg), ctypes.byref(self.handle))
         if status:

@@ -19,9 +19,10 @@ def probe(hip, kind, a_lane, a_elem, b_lane, b_elem):
 
 
 def test_f64_mfma_lane_layout(hip):
+    # The FP64 accumulator interleaves rows across its four registers: element r of lane l is row l/16 + 4r.
     lanes = np.arange(64)[:, None]
     regs = np.arange(4)[None, :]
-    out_row = 4 * (lanes // 16) + regs
+    out_row = lanes // 16 + 4 * regs
     out_col = np.broadcast_to(lanes % 16, (64, 4))
     for a in range(64):
         got = probe(hip, 0, a, 0, -1, 0)

@@ -46,6 +46,97 @@ def traffic_from_profiles(name: str):
     return None if entry is None else entry.get("bytes_per_launch")
 
 
+def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 10.0), reps=10, check=None):
+    """Pressure Poisson projection solve: FP64 direct solve vs the mixed solve (split-bf16 transforms,
+    FP64 residuals and refinement, three solves), both resident, timed with HIP events over `reps` solves.
+    Returns the section dict and the two solutions (interiors)."""
+    pressure = graft._module("pressure")
+    nx, ny, nz = n
+    b = fields.divergence_rhs(nx, ny, nz, h)
+    rhs = np.zeros((nx + 2, ny + 2, nz + 2), order="F")
+    rhs[1:-1, 1:-1, 1:-1] = b
+    out = {}
+    sols = {}
+    dom = scope.Domain(lib)
+    hr = dom.bind(rhs, lower=(0, 0, 0))
+    stream = ctypes.c_void_p(dom.stream())
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    lib.ftc_event_create(ctypes.byref(ev0))
+    lib.ftc_event_create(ctypes.byref(ev1))
+    for name, mixed, solves in (("fp64", False, 1), ("mixed", True, 3)):
+        phi = np.zeros((nx + 4, ny + 4, nz + 4), order="F")
+        hp = dom.bind(phi, lower=(-1, -1, -1))
+        solver = pressure.PressureSolver(lib, dom, n, (nx * h[0], ny * h[1], nz * h[2]), mixed)
+        solver.refinement(1e-12, 8, True)
+        solver.solve(hp, hr)  # warm-up, uploads, and the measured residual of a converged solve
+        rep = solver.report()
+        solver.refinement(1e-12, solves, False)  # fixed solve count, no host synchronisation
+        solver.solve(hp, hr)
+        dom.wait()
+        lib.ftc_event_record(ev0, stream)
+        for _ in range(reps):
+            solver.solve(hp, hr)
+        lib.ftc_event_record(ev1, stream)
+        ms = ctypes.c_float()
+        lib.ftc_event_elapsed_ms(ev0, ev1, ctypes.byref(ms))
+        dom.observe(hp)
+        sols[name] = phi[2:-2, 2:-2, 2:-2].copy()
+        solver.destroy()
+        out[name] = {"ms_per_solve": round(ms.value / reps, 4), "solves": solves if mixed else 1,
+                     "converged_iterations": rep.iterations, "relative_residual": rep.relative_residual}
+    dom.close()
+    N = nx * ny * nz
+    P = (nx + 2) * (ny + 2) * (nz + 2)
+    out["speedup_mixed_over_fp64"] = round(out["fp64"]["ms_per_solve"] / out["mixed"]["ms_per_solve"], 3)
+    scale = np.abs(sols["fp64"]).max()
+    out["max_rel_diff_mixed_vs_fp64"] = float(np.abs(sols["mixed"] - sols["fp64"]).max() / scale)
+    out["transform_flops_fp64"] = 4 * (nx + ny) * N * 2
+    out["fp64_mfma_tflops"] = round(out["transform_flops_fp64"] / (out["fp64"]["ms_per_solve"] * 1e-3) / 1e12, 2)
+    out["lower_bound_bytes"] = 8 * 2 * N
+    out["rhs_padded_cells"] = P
+    return out, sols, b
+
+
+def chain_section(ftc, scope, lib, n=4096, reps=3):
+    """D = A*B*C on n x n FP64 arrays, FP64 MFMA vs split-bf16 products, resident."""
+    rng = np.random.default_rng(20261123)
+    A, B, C = (np.asfortranarray(rng.uniform(-1.0, 1.0, (n, n))) for _ in range(3))
+    out = {}
+    results = {}
+    dom = scope.Domain(lib)
+    ha, hb, hc = (dom.bind(X) for X in (A, B, C))
+    stream = ctypes.c_void_p(dom.stream())
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    lib.ftc_event_create(ctypes.byref(ev0))
+    lib.ftc_event_create(ctypes.byref(ev1))
+    for name, prec, peak in (("fp64_mfma", 0, 78.6e12), ("bf16x3", 1, 2.5e15)):
+        D = np.zeros((n, n), order="F")
+        hd = dom.bind(D)
+
+        def run():
+            scope.check(lib, lib.cpp_fort_shared_chain_contraction(dom.ctx, 1, hd, ha, hb, hc, ftc.ip(n),
+                                                                   ftc.ip(prec)))
+            dom.flush()
+
+        run()
+        dom.wait()
+        lib.ftc_event_record(ev0, stream)
+        for _ in range(reps):
+            run()
+        lib.ftc_event_record(ev1, stream)
+        ms = ctypes.c_float()
+        lib.ftc_event_elapsed_ms(ev0, ev1, ctypes.byref(ms))
+        dom.observe(hd)
+        results[name] = D
+        t = ms.value / reps * 1e-3
+        flops = 4.0 * n ** 3
+        mfma_flops = flops * (3 if prec else 1)
+        out[name] = {"ms": round(t * 1e3, 3), "tflops_fp64_equivalent": round(flops / t / 1e12, 2),
+                     "mfma_utilisation": round(mfma_flops / t / peak, 4)}
+    dom.close()
+    return out, results, (A, B, C)
+
+
 def main() -> int:
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -57,6 +148,8 @@ def main() -> int:
     p.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps for the CPU baseline (0: skip)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only time a plain 2 GiB device copy (HBM ceiling and byte-counter calibration)")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the Poisson (FP64 vs mixed) and chain-contraction sections")
     args = p.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the LES domain would shard by "
@@ -133,10 +226,30 @@ def main() -> int:
     achieved = bytes_alg / (gpu_ms_per_step * 1e-3) / 1e9
     traffic = traffic_from_profiles("les_kernel<15,true,true>")
 
+    extras = {}
+    if not args.no_extras:
+        log("bench: Poisson 512x512x128, FP64 vs mixed")
+        extras["poisson"], poisson_sols, poisson_rhs = poisson_section(ftc, scope, lib, fields)
+        log("bench: chain contraction 4096")
+        extras["chain_contraction"], chain_results, chain_inputs = chain_section(ftc, scope, lib)
+
     cpu = None
     parity = None
     if args.cpu_steps:
         from oracle import binding as oracle
+        from oracle import poisson as poisson_oracle
+
+        if extras:
+            log("bench: oracle checks of the Poisson and chain sections")
+            want = poisson_oracle.solve(poisson_rhs, (10.0, 10.0, 10.0))
+            scale = np.abs(want).max()
+            for name, sol in poisson_sols.items():
+                extras["poisson"][name]["max_rel_error_vs_oracle"] = float(np.abs(sol - want).max() / scale)
+            A, B, C = chain_inputs
+            ref = (A @ B) @ C
+            for name, D in chain_results.items():
+                extras["chain_contraction"][name]["max_rel_error_vs_fp64_cpu"] = float(
+                    np.abs(D - ref).max() / np.abs(ref).max())
 
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
         log(f"bench: CPU baseline with {threads} threads")
@@ -175,6 +288,7 @@ def main() -> int:
         "graph": {"instantiations": gstats.instantiations, "replays": gstats.replays,
                   "parameter_updates": gstats.parameter_updates},
     }
+    line.update(extras)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -1,0 +1,881 @@
+// Batched ensembles of uneven LES subdomains (include/ftc_ensemble.h): every stage of the projected LES
+// step runs for all members in one persistent launch over a work list, the step is one captured hipGraph
+// per buffer parity, and a small autotuner picks the LES k chunk, the contraction tile and the
+// persistent grid. The per-member FP64 mode launches the very same kernels once per member per stage
+// (the baseline the batched modes are measured against), so batched FP64 equals it bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/ftc_ensemble.h"
+#include "../../include/ftc_les.h"
+#include "entry.hpp"
+#include "gemm.hpp"
+#include "gemm_kernels.hpp"
+#include "les_kernels.hpp"
+#include "poisson_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace ftc {
+
+static thread_local std::string g_ens_error;
+
+static int eerr(int status, const std::string &m)
+{
+    g_ens_error = m;
+    return status;
+}
+
+// Device-side description of one member.
+struct EnsMember {
+    int nx, ny, nz, reserved;
+    long long sj, sk;  // strides of the one-ghost field arrays
+    double *f[2][4];   // U, V, W, S of buffer set 0 and 1
+    double *phi, *rhs, *res;  // dense nx*ny*nz
+    void *w0, *w1;            // transform workspaces (double or float)
+    double *lx, *ly;
+    double *part;             // 256 partial sums, then the mean
+    double gx, gy, gz;        // dt/d
+    double rdx, rdy, rdz, rdt;
+    double rx, ry, rz;        // 1/h^2
+};
+
+struct TileItem {
+    int member, bx, by, bz;
+};
+struct GemmItem {
+    int call, mt, nt, b;
+};
+struct ColItem {
+    int member, block;
+};
+
+constexpr int kCols = 64;
+
+// ------------------------------------------------------------------------------------------------ kernels
+template <bool STEP>
+__global__ __launch_bounds__(NTH, 2) void ens_les_kernel(const LesArgs *args, const TileItem *items, int n)
+{
+    __shared__ SgsLds L;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const TileItem t = items[it];
+        les_sgs_body<STEP>(args[t.member], L, t.bx, t.by, t.bz);
+        __syncthreads();
+    }
+}
+
+// rhs = div(U*) / dt on every cell of the member (U* = buffer set `src`).
+__global__ __launch_bounds__(256) void ens_divergence_kernel(const EnsMember *ms, const TileItem *items, int n, int src)
+{
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const TileItem t = items[it];
+        const EnsMember &m = ms[t.member];
+        const int a = 1 + t.bx * 64 + threadIdx.x, b = 1 + t.by * 4 + threadIdx.y;
+        if (a > m.nx || b > m.ny) continue;
+        const double *U = m.f[src][0], *V = m.f[src][1], *W = m.f[src][2];
+        for (int c = 1; c <= m.nz; ++c) {
+            const long long p = a + b * m.sj + c * m.sk;
+            const long long q = (a - 1) + (long long)m.nx * ((b - 1) + (long long)m.ny * (c - 1));
+            m.rhs[q] = (((U[p] - U[p - 1]) * m.rdx + (V[p] - V[p - m.sj]) * m.rdy) + (W[p] - W[p - m.sk]) * m.rdz) * m.rdt;
+        }
+    }
+}
+
+// U -= dt grad(phi) on the interior faces of buffer set `dst`, in place, with the halo images.
+__global__ __launch_bounds__(256) void ens_projection_kernel(const EnsMember *ms, const LesArgs *args,
+                                                             const TileItem *items, int n, int dst)
+{
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const TileItem t = items[it];
+        const EnsMember &m = ms[t.member];
+        const LesArgs &g = args[t.member];
+        const int a = 1 + t.bx * 64 + threadIdx.x, b = 1 + t.by * 4 + threadIdx.y;
+        if (a > m.nx || b > m.ny) continue;
+        double *U = m.f[dst][0], *V = m.f[dst][1], *W = m.f[dst][2];
+        const long long plane = (long long)m.nx * m.ny;
+        for (int c = 1; c <= m.nz; ++c) {
+            const long long p = a + b * m.sj + c * m.sk;
+            const long long q = (a - 1) + (long long)m.nx * (b - 1) + plane * (c - 1);
+            const long long qx = (a % m.nx) + (long long)m.nx * (b - 1) + plane * (c - 1);
+            const long long qy = (a - 1) + (long long)m.nx * (b % m.ny) + plane * (c - 1);
+            put_with_images(g, U, p, U[p] - m.gx * (m.phi[qx] - m.phi[q]), a + 1, b + 1, c + 1, 0);
+            put_with_images(g, V, p, V[p] - m.gy * (m.phi[qy] - m.phi[q]), a + 1, b + 1, c + 1, 0);
+            if (c < m.nz) put_with_images(g, W, p, W[p] - m.gz * (m.phi[q + plane] - m.phi[q]), a + 1, b + 1, c + 1, 1);
+        }
+    }
+}
+
+// res = rhs - lap(phi): dense arrays, periodic x and y, Neumann z.
+__global__ __launch_bounds__(256) void ens_residual_kernel(const EnsMember *ms, const TileItem *items, int n)
+{
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const TileItem t = items[it];
+        const EnsMember &m = ms[t.member];
+        const int i = t.bx * 64 + threadIdx.x, j = t.by * 4 + threadIdx.y;
+        if (i >= m.nx || j >= m.ny) continue;
+        const int nx = m.nx, ny = m.ny, nz = m.nz;
+        const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+        const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+        const long long plane = (long long)nx * ny;
+        for (int k = 0; k < nz; ++k) {
+            const int km = k == 0 ? 0 : k - 1, kp = k == nz - 1 ? nz - 1 : k + 1;
+#define P(x, y, z) m.phi[(x) + (long long)nx * (y) + plane * (z)]
+            const double c0 = P(i, j, k);
+            const double lap = (((P(im, j, k) + P(ip, j, k)) - 2.0 * c0) * m.rx +
+                                ((P(i, jm, k) + P(i, jp, k)) - 2.0 * c0) * m.ry) +
+                               ((P(i, j, km) + P(i, j, kp)) - 2.0 * c0) * m.rz;
+#undef P
+            const long long q = i + (long long)nx * j + plane * k;
+            m.res[q] = m.rhs[q] - lap;
+        }
+    }
+}
+
+// Per member (one workgroup of 256): 256 strided partial sums of phi in order, folded in index order.
+__global__ __launch_bounds__(256) void ens_mean_kernel(const EnsMember *ms, int members)
+{
+    for (int mi = blockIdx.x; mi < members; mi += gridDim.x) {
+        const EnsMember &m = ms[mi];
+        const long long N = (long long)m.nx * m.ny * m.nz;
+        double acc = 0.0;
+        for (long long e = threadIdx.x; e < N; e += 256) acc = acc + m.phi[e];
+        m.part[threadIdx.x] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int f = 0; f < 256; ++f) s = s + m.part[f];
+            m.part[256] = s / (double)N;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void ens_subtract_kernel(const EnsMember *ms, const TileItem *items, int n)
+{
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const TileItem t = items[it];
+        const EnsMember &m = ms[t.member];
+        const int i = t.bx * 64 + threadIdx.x, j = t.by * 4 + threadIdx.y;
+        if (i >= m.nx || j >= m.ny) continue;
+        const double mean = m.part[256];
+        const long long plane = (long long)m.nx * m.ny;
+        for (int k = 0; k < m.nz; ++k) {
+            double &v = m.phi[i + (long long)m.nx * j + plane * k];
+            v = v - mean;
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kCols) void ens_tridiag_kernel(const EnsMember *ms, const ColItem *items, int n)
+{
+    extern __shared__ double lds[];
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const ColItem t = items[it];
+        const EnsMember &m = ms[t.member];
+        const long long plane = (long long)m.nx * m.ny;
+        const long long col = (long long)t.block * kCols + threadIdx.x;
+        if (col >= plane) continue;
+        tridiag_column<T>(static_cast<const T *>(m.w1), static_cast<T *>(m.w0), plane, m.nz,
+                          m.lx[col % m.nx] + m.ly[col / m.nx], m.rz, lds, lds + (long long)m.nz * kCols,
+                          threadIdx.x, kCols, col);
+    }
+}
+
+template <int BM, int BN>
+__global__ __launch_bounds__(kGemmThreads) void ens_gemm_f64_kernel(const GemmCall *calls, const GemmItem *items, int n)
+{
+    __shared__ F64Lds<BM, BN> L;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const GemmItem t = items[it];
+        gemm_f64_tile<BM, BN, 2, 2>(calls[t.call], L, t.mt, t.nt, t.b);
+        __syncthreads();
+    }
+}
+
+template <class TA, class TB, class TO, int BM, int BN>
+__global__ __launch_bounds__(kGemmThreads) void ens_gemm_bf16_kernel(const GemmCall *calls, const GemmItem *items, int n)
+{
+    __shared__ Bf16Lds<BM, BN> L;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const GemmItem t = items[it];
+        gemm_bf16x3_tile<TA, TB, TO, BM, BN, 2, 2>(calls[t.call], L, t.mt, t.nt, t.b);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ host
+static std::size_t up256(std::size_t x) { return (x + 255) & ~std::size_t(255); }
+
+// The basis of the periodic 1D second difference (same construction as the pressure provider).
+static void periodic_basis(int n, double h, std::vector<double> &Q, std::vector<double> &lam)
+{
+    Q.assign((std::size_t)n * n, 0.0);
+    lam.assign(n, 0.0);
+    const double pi = 3.14159265358979323846;
+    const double r2 = std::sqrt(2.0 / n), r1 = 1.0 / std::sqrt((double)n);
+    for (int i = 0; i < n; ++i) Q[i] = r1;
+    for (int f = 1; 2 * f < n; ++f) {
+        const double l = (2.0 * std::cos(2.0 * pi * f / n) - 2.0) / (h * h);
+        for (int i = 0; i < n; ++i) {
+            Q[(std::size_t)(2 * f - 1) * n + i] = r2 * std::cos(2.0 * pi * f * i / n);
+            Q[(std::size_t)(2 * f) * n + i] = r2 * std::sin(2.0 * pi * f * i / n);
+        }
+        lam[2 * f - 1] = l;
+        lam[2 * f] = l;
+    }
+    if (n % 2 == 0) {
+        for (int i = 0; i < n; ++i) Q[(std::size_t)(n - 1) * n + i] = (i % 2 ? -r1 : r1);
+        lam[n - 1] = -4.0 / (h * h);
+    }
+}
+
+enum Stage { kLes, kDiv, kG1, kG2, kTri, kG3, kG4, kRes, kG1c, kG4c, kMean, kSub, kProj, kStageCount };
+
+struct Range {
+    int start, count;
+};
+
+}  // namespace ftc
+
+using namespace ftc;
+
+struct ftc_ensemble {
+    int mode = 0, count = 0, solves = 3;
+    double dx = 0, dy = 0, dz = 0, dt = 0, cs = 0;
+    std::vector<int> nx, ny, nz;
+    // tuning
+    int les_kchunk = 16, gemm_tile = 1, blocks_per_cu = 4, cus = 256;
+    // device
+    void *arena = nullptr;
+    std::size_t arena_bytes = 0;
+    hipStream_t stream = nullptr;
+    std::vector<EnsMember> members;  // host copy
+    EnsMember *d_members = nullptr;
+    LesArgs *d_args[2] = {nullptr, nullptr};
+    std::vector<LesArgs> args[2];
+    GemmCall *d_calls = nullptr;  // kG1.. stage-major: calls[stage_slot * count + member]
+    TileItem *d_les = nullptr, *d_tiles = nullptr;
+    GemmItem *d_gemm[6] = {};
+    ColItem *d_cols = nullptr;
+    std::vector<Range> r_les, r_tiles, r_gemm[6], r_cols;
+    int n_les = 0, n_tiles = 0, n_gemm[6] = {}, n_cols = 0;
+    void *items_arena = nullptr;
+    int nzmax = 0;
+    int parity = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int64_t launches = 0;
+};
+
+namespace ftc {
+
+static bool mixed(const ftc_ensemble &e) { return e.mode == FTC_ENSEMBLE_MIXED_BATCHED; }
+
+// The contraction calls of member m for the six transform stages (G1, G2, G3, G4, G1c, G4c).
+static void member_calls(const ftc_ensemble &e, const EnsMember &m, const double *Qx, const double *QxT,
+                         const double *Qy, const double *QyT, const float *Qxf, const float *QxTf, const float *Qyf,
+                         const float *QyTf, GemmCall out[6])
+{
+    const int nx = m.nx, ny = m.ny, nz = m.nz;
+    const long long plane = (long long)nx * ny;
+    const bool mx = mixed(e);
+    const int wt = mx ? kFloat : kDouble;
+    const void *qxt = mx ? (const void *)QxTf : (const void *)QxT, *qy = mx ? (const void *)Qyf : (const void *)Qy;
+    const void *qyt = mx ? (const void *)QyTf : (const void *)QyT, *qx = mx ? (const void *)Qxf : (const void *)Qx;
+    out[0] = GemmCall{qxt, m.rhs, m.w0, wt, kDouble, wt, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0};
+    out[1] = GemmCall{m.w0, qy, m.w1, wt, wt, wt, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
+    out[2] = GemmCall{m.w0, qyt, m.w1, wt, wt, wt, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
+    out[3] = GemmCall{qx, m.w1, m.phi, wt, wt, kDouble, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0};
+    out[4] = GemmCall{qxt, m.res, m.w0, wt, kDouble, wt, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0};
+    out[5] = GemmCall{qx, m.w1, m.phi, wt, wt, kDouble, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 1.0};
+}
+
+static int tile_bm(int tile) { return tile == 0 ? 64 : 32; }
+
+// Builds (or rebuilds, after tuning) every work list and uploads it.
+static int build_items(ftc_ensemble &e)
+{
+    std::vector<TileItem> les, tiles;
+    std::vector<GemmItem> gemm[6];
+    std::vector<ColItem> cols;
+    e.r_les.clear();
+    e.r_tiles.clear();
+    e.r_cols.clear();
+    for (auto &r : e.r_gemm) r.clear();
+    const int bm = tile_bm(e.gemm_tile);
+    for (int m = 0; m < e.count; ++m) {
+        const int nx = e.nx[m], ny = e.ny[m], nz = e.nz[m];
+        const int kchunk = std::min(e.les_kchunk, nz);
+        Range r{(int)les.size(), 0};
+        for (int bz = 0; bz * kchunk < nz; ++bz)
+            for (int by = 0; by < (ny + TY - 1) / TY; ++by)
+                for (int bx = 0; bx < (nx + TX - 1) / TX; ++bx) les.push_back({m, bx, by, bz});
+        r.count = (int)les.size() - r.start;
+        e.r_les.push_back(r);
+        Range rt{(int)tiles.size(), 0};
+        for (int by = 0; by < (ny + 3) / 4; ++by)
+            for (int bx = 0; bx < (nx + 63) / 64; ++bx) tiles.push_back({m, bx, by, 0});
+        rt.count = (int)tiles.size() - rt.start;
+        e.r_tiles.push_back(rt);
+        for (int s = 0; s < 6; ++s) {
+            Range rg{(int)gemm[s].size(), 0};
+            const int mtiles = (nx + bm - 1) / bm, ntiles = (ny + bm - 1) / bm;
+            for (int b = 0; b < nz; ++b)
+                for (int nt = 0; nt < ntiles; ++nt)
+                    for (int mt = 0; mt < mtiles; ++mt) gemm[s].push_back({s * e.count + m, mt, nt, b});
+            rg.count = (int)gemm[s].size() - rg.start;
+            e.r_gemm[s].push_back(rg);
+        }
+        Range rc{(int)cols.size(), 0};
+        const long long plane = (long long)nx * ny;
+        for (int b = 0; b * (long long)kCols < plane; ++b) cols.push_back({m, b});
+        rc.count = (int)cols.size() - rc.start;
+        e.r_cols.push_back(rc);
+    }
+    e.n_les = (int)les.size();
+    e.n_tiles = (int)tiles.size();
+    e.n_cols = (int)cols.size();
+    std::size_t bytes = up256(les.size() * sizeof(TileItem)) + up256(tiles.size() * sizeof(TileItem)) +
+                        up256(cols.size() * sizeof(ColItem));
+    for (int s = 0; s < 6; ++s) {
+        e.n_gemm[s] = (int)gemm[s].size();
+        bytes += up256(gemm[s].size() * sizeof(GemmItem));
+    }
+    if (e.items_arena) (void)hipFree(e.items_arena);
+    e.items_arena = nullptr;
+    if (hipMalloc(&e.items_arena, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return eerr(FORT_SCOPE_RESOURCE, "work-list allocation failed");
+    }
+    char *p = static_cast<char *>(e.items_arena);
+    auto put = [&](const void *src, std::size_t n) {
+        char *dst = p;
+        if (n && hipMemcpy(dst, src, n, hipMemcpyHostToDevice) != hipSuccess) return (char *)nullptr;
+        p += up256(n);
+        return dst;
+    };
+    e.d_les = (TileItem *)put(les.data(), les.size() * sizeof(TileItem));
+    e.d_tiles = (TileItem *)put(tiles.data(), tiles.size() * sizeof(TileItem));
+    e.d_cols = (ColItem *)put(cols.data(), cols.size() * sizeof(ColItem));
+    for (int s = 0; s < 6; ++s) e.d_gemm[s] = (GemmItem *)put(gemm[s].data(), gemm[s].size() * sizeof(GemmItem));
+    // LES arguments carry the k chunk.
+    for (int par = 0; par < 2; ++par) {
+        for (int m = 0; m < e.count; ++m) e.args[par][m].kchunk = std::min(e.les_kchunk, e.nz[m]);
+        if (hipMemcpy(e.d_args[par], e.args[par].data(), e.count * sizeof(LesArgs), hipMemcpyHostToDevice) != hipSuccess)
+            return eerr(FORT_SCOPE_RESOURCE, "argument upload failed");
+    }
+    return FORT_SCOPE_OK;
+}
+
+static unsigned grid_for(const ftc_ensemble &e, int items)
+{
+    const long long cap = (long long)e.cus * e.blocks_per_cu;
+    return (unsigned)std::max<long long>(1, std::min<long long>(items, cap));
+}
+
+// Launches one stage, for all members (member < 0, persistent grid) or for one member (its own grid).
+static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
+{
+    const bool all = member < 0;
+    auto pick = [&](const std::vector<Range> &r, int total, int *start) {
+        if (all) {
+            *start = 0;
+            return total;
+        }
+        *start = r[member].start;
+        return r[member].count;
+    };
+    const bool mx = mixed(e);
+    const int src = parity, dst = 1 - parity;
+    int start = 0, n = 0;
+    hipStream_t s = e.stream;
+    switch (stage) {
+    case kLes:
+        n = pick(e.r_les, e.n_les, &start);
+        if (n) hipLaunchKernelGGL(ens_les_kernel<true>, dim3(all ? grid_for(e, n) : n), dim3(TX, TY), 0, s,
+                                  e.d_args[parity], e.d_les + start, n);
+        break;
+    case kDiv:
+        n = pick(e.r_tiles, e.n_tiles, &start);
+        if (n) hipLaunchKernelGGL(ens_divergence_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
+                                  e.d_members, e.d_tiles + start, n, dst);
+        break;
+    case kProj:
+        n = pick(e.r_tiles, e.n_tiles, &start);
+        if (n) hipLaunchKernelGGL(ens_projection_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
+                                  e.d_members, e.d_args[parity], e.d_tiles + start, n, dst);
+        break;
+    case kRes:
+        n = pick(e.r_tiles, e.n_tiles, &start);
+        if (n) hipLaunchKernelGGL(ens_residual_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
+                                  e.d_members, e.d_tiles + start, n);
+        break;
+    case kSub:
+        n = pick(e.r_tiles, e.n_tiles, &start);
+        if (n) hipLaunchKernelGGL(ens_subtract_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
+                                  e.d_members, e.d_tiles + start, n);
+        break;
+    case kMean:
+        if (all) hipLaunchKernelGGL(ens_mean_kernel, dim3(std::min(e.count, e.cus * e.blocks_per_cu)), dim3(256), 0,
+                                    s, e.d_members, e.count);
+        else hipLaunchKernelGGL(ens_mean_kernel, dim3(1), dim3(256), 0, s, e.d_members + member, 1);
+        break;
+    case kTri: {
+        n = pick(e.r_cols, e.n_cols, &start);
+        const std::size_t lds = 2 * (std::size_t)e.nzmax * kCols * sizeof(double);
+        if (n) {
+            if (mx) hipLaunchKernelGGL(ens_tridiag_kernel<float>, dim3(all ? grid_for(e, n) : n), dim3(kCols), lds, s,
+                                       e.d_members, e.d_cols + start, n);
+            else hipLaunchKernelGGL(ens_tridiag_kernel<double>, dim3(all ? grid_for(e, n) : n), dim3(kCols), lds, s,
+                                    e.d_members, e.d_cols + start, n);
+        }
+        break;
+    }
+    default: {
+        // Transform stages: slot 0..5 = G1, G2, G3, G4, G1c, G4c.
+        const int slot = stage == kG1 ? 0 : stage == kG2 ? 1 : stage == kG3 ? 2 : stage == kG4 ? 3 : stage == kG1c ? 4 : 5;
+        n = pick(e.r_gemm[slot], e.n_gemm[slot], &start);
+        if (!n) break;
+        const dim3 grid(all ? grid_for(e, n) : n), block(kGemmThreads);
+        const GemmItem *items = e.d_gemm[slot] + start;
+        const bool big = e.gemm_tile == 0;
+        if (!mx) {
+            if (big) hipLaunchKernelGGL((ens_gemm_f64_kernel<64, 64>), grid, block, 0, s, e.d_calls, items, n);
+            else hipLaunchKernelGGL((ens_gemm_f64_kernel<32, 32>), grid, block, 0, s, e.d_calls, items, n);
+        } else if (slot == 0 || slot == 4) {
+            if (big) hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, double, float, 64, 64>), grid, block, 0, s, e.d_calls, items, n);
+            else hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, double, float, 32, 32>), grid, block, 0, s, e.d_calls, items, n);
+        } else if (slot == 1 || slot == 2) {
+            if (big) hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, float, 64, 64>), grid, block, 0, s, e.d_calls, items, n);
+            else hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, float, 32, 32>), grid, block, 0, s, e.d_calls, items, n);
+        } else {
+            if (big) hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, double, 64, 64>), grid, block, 0, s, e.d_calls, items, n);
+            else hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, double, 32, 32>), grid, block, 0, s, e.d_calls, items, n);
+        }
+    }
+    }
+}
+
+// The stage sequence of one step.
+static std::vector<int> step_stages(const ftc_ensemble &e)
+{
+    std::vector<int> st{kLes, kDiv, kG1, kG2, kTri, kG3, kG4};
+    if (mixed(e)) {
+        for (int c = 1; c < e.solves; ++c)
+            for (int x : {kRes, kG1c, kG2, kTri, kG3, kG4c}) st.push_back(x);
+        st.push_back(kMean);
+        st.push_back(kSub);
+    }
+    st.push_back(kProj);
+    return st;
+}
+
+static int enqueue_step(ftc_ensemble &e, int parity)
+{
+    const std::vector<int> st = step_stages(e);
+    if (e.mode == FTC_ENSEMBLE_FP64_PER_MEMBER) {
+        for (int m = 0; m < e.count; ++m)
+            for (int stage : st) launch_stage(e, stage, m, parity);
+        e.launches = (int64_t)st.size() * e.count;
+    } else {
+        for (int stage : st) launch_stage(e, stage, -1, parity);
+        e.launches = (int64_t)st.size();
+    }
+    if (hipGetLastError() != hipSuccess) return eerr(FORT_SCOPE_EXECUTION, "stage launch failed");
+    return FORT_SCOPE_OK;
+}
+
+static void drop_graphs(ftc_ensemble &e)
+{
+    for (auto &g : e.graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// Captures one step per parity into an executable graph (batched modes).
+static int capture(ftc_ensemble &e)
+{
+    drop_graphs(e);
+    for (int par = 0; par < 2; ++par) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return eerr(FORT_SCOPE_RESOURCE, "graph capture failed to start");
+        int st = enqueue_step(e, par);
+        hipError_t end = hipStreamEndCapture(e.stream, &g);
+        if (st || end != hipSuccess) {
+            (void)hipGetLastError();
+            if (g) (void)hipGraphDestroy(g);
+            return st ? st : eerr(FORT_SCOPE_RESOURCE, "graph capture failed");
+        }
+        hipError_t inst = hipGraphInstantiate(&e.graph[par], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (inst != hipSuccess) {
+            (void)hipGetLastError();
+            return eerr(FORT_SCOPE_RESOURCE, "graph instantiation failed");
+        }
+    }
+    return FORT_SCOPE_OK;
+}
+
+static float time_ms(ftc_ensemble &e, const std::function<void()> &work, int reps)
+{
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    work();  // warm-up
+    std::vector<float> t;
+    for (int r = 0; r < reps; ++r) {
+        (void)hipEventRecord(a, e.stream);
+        work();
+        (void)hipEventRecord(b, e.stream);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        t.push_back(ms);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+}
+
+}  // namespace ftc
+
+extern "C" {
+
+const char *ftc_ensemble_error_v1(void) { return g_ens_error.c_str(); }
+
+int ftc_ensemble_create_v1(int count, const int32_t *sizes, double dx, double dy, double dz, double dt, double cs,
+                           int mode, ftc_ensemble **out)
+{
+    if (!out || !sizes || count <= 0) return eerr(FORT_SCOPE_ARGUMENT, "missing ensemble arguments");
+    *out = nullptr;
+    if (mode < 0 || mode > 2) return eerr(FORT_SCOPE_ARGUMENT, "unknown ensemble mode");
+    if (!(dx > 0 && dy > 0 && dz > 0 && dt > 0)) return eerr(FORT_SCOPE_ARGUMENT, "spacing and dt must be positive");
+    auto e = std::make_unique<ftc_ensemble>();
+    e->mode = mode;
+    e->count = count;
+    e->dx = dx, e->dy = dy, e->dz = dz, e->dt = dt, e->cs = cs;
+    for (int m = 0; m < count; ++m) {
+        const int a = sizes[3 * m], b = sizes[3 * m + 1], c = sizes[3 * m + 2];
+        if (a < 2 || b < 2 || c < 2 || a > 4096 || b > 4096 || c > 150)
+            return eerr(FORT_SCOPE_ARGUMENT, "member sizes must be 2..4096 (x, y) and 2..150 (z)");
+        e->nx.push_back(a);
+        e->ny.push_back(b);
+        e->nz.push_back(c);
+        e->nzmax = std::max(e->nzmax, c);
+    }
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return eerr(FORT_SCOPE_RESOURCE, "no device");
+    e->cus = prop.multiProcessorCount;
+    const bool mx = mode == FTC_ENSEMBLE_MIXED_BATCHED;
+    const std::size_t wbytes = mx ? 4 : 8;
+    // Arena: per member fields (2 sets x 4), phi, rhs, res, w0, w1, bases, eigenvalues, partials.
+    std::size_t bytes = 0;
+    for (int m = 0; m < count; ++m) {
+        const std::size_t P = (std::size_t)(e->nx[m] + 2) * (e->ny[m] + 2) * (e->nz[m] + 2);
+        const std::size_t N = (std::size_t)e->nx[m] * e->ny[m] * e->nz[m];
+        const std::size_t X = e->nx[m], Y = e->ny[m];
+        bytes += 8 * up256(P * 8) + 3 * up256(N * 8) + 2 * up256(N * wbytes);
+        bytes += 2 * up256(X * X * 8) + 2 * up256(Y * Y * 8) + 2 * up256(X * X * 4) + 2 * up256(Y * Y * 4);
+        bytes += up256(X * 8) + up256(Y * 8) + up256(257 * 8);
+    }
+    bytes += up256(count * sizeof(EnsMember)) + 2 * up256(count * sizeof(LesArgs)) + up256(6 * count * sizeof(GemmCall));
+    e->arena_bytes = bytes;
+    if (hipMalloc(&e->arena, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return eerr(FORT_SCOPE_RESOURCE, "ensemble allocation failed");
+    }
+    if (hipMemset(e->arena, 0, bytes) != hipSuccess) return eerr(FORT_SCOPE_RESOURCE, "ensemble clear failed");
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return eerr(FORT_SCOPE_RESOURCE, "stream creation failed");
+    char *p = static_cast<char *>(e->arena);
+    auto carve = [&](std::size_t n) {
+        char *r = p;
+        p += up256(n);
+        return r;
+    };
+    std::vector<GemmCall> calls(6 * count);
+    e->args[0].resize(count);
+    e->args[1].resize(count);
+    for (int m = 0; m < count; ++m) {
+        EnsMember mm{};
+        mm.nx = e->nx[m], mm.ny = e->ny[m], mm.nz = e->nz[m];
+        const std::size_t P = (std::size_t)(mm.nx + 2) * (mm.ny + 2) * (mm.nz + 2);
+        const std::size_t N = (std::size_t)mm.nx * mm.ny * mm.nz;
+        mm.sj = mm.nx + 2;
+        mm.sk = (long long)(mm.nx + 2) * (mm.ny + 2);
+        for (int s = 0; s < 2; ++s)
+            for (int f = 0; f < 4; ++f) mm.f[s][f] = (double *)carve(P * 8);
+        mm.phi = (double *)carve(N * 8);
+        mm.rhs = (double *)carve(N * 8);
+        mm.res = (double *)carve(N * 8);
+        mm.w0 = carve(N * wbytes);
+        mm.w1 = carve(N * wbytes);
+        const int X = mm.nx, Y = mm.ny;
+        double *Qx = (double *)carve((std::size_t)X * X * 8), *QxT = (double *)carve((std::size_t)X * X * 8);
+        double *Qy = (double *)carve((std::size_t)Y * Y * 8), *QyT = (double *)carve((std::size_t)Y * Y * 8);
+        float *Qxf = (float *)carve((std::size_t)X * X * 4), *QxTf = (float *)carve((std::size_t)X * X * 4);
+        float *Qyf = (float *)carve((std::size_t)Y * Y * 4), *QyTf = (float *)carve((std::size_t)Y * Y * 4);
+        mm.lx = (double *)carve((std::size_t)X * 8);
+        mm.ly = (double *)carve((std::size_t)Y * 8);
+        mm.part = (double *)carve(257 * 8);
+        mm.gx = dt / dx, mm.gy = dt / dy, mm.gz = dt / dz;
+        mm.rdx = 1.0 / dx, mm.rdy = 1.0 / dy, mm.rdz = 1.0 / dz, mm.rdt = 1.0 / dt;
+        mm.rx = 1.0 / (dx * dx), mm.ry = 1.0 / (dy * dy), mm.rz = 1.0 / (dz * dz);
+        std::vector<double> qx, lx, qy, ly;
+        periodic_basis(X, dx, qx, lx);
+        periodic_basis(Y, dy, qy, ly);
+        std::vector<double> qxt(qx.size()), qyt(qy.size());
+        for (int a = 0; a < X; ++a)
+            for (int b = 0; b < X; ++b) qxt[(std::size_t)b * X + a] = qx[(std::size_t)a * X + b];
+        for (int a = 0; a < Y; ++a)
+            for (int b = 0; b < Y; ++b) qyt[(std::size_t)b * Y + a] = qy[(std::size_t)a * Y + b];
+        std::vector<float> fqx(qx.begin(), qx.end()), fqxt(qxt.begin(), qxt.end()), fqy(qy.begin(), qy.end()),
+            fqyt(qyt.begin(), qyt.end());
+        bool ok = hipMemcpy(Qx, qx.data(), qx.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(QxT, qxt.data(), qxt.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(Qy, qy.data(), qy.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(QyT, qyt.data(), qyt.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(Qxf, fqx.data(), fqx.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(QxTf, fqxt.data(), fqxt.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(Qyf, fqy.data(), fqy.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(QyTf, fqyt.data(), fqyt.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(mm.lx, lx.data(), lx.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(mm.ly, ly.data(), ly.size() * 8, hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok) return eerr(FORT_SCOPE_RESOURCE, "basis upload failed");
+        GemmCall c6[6];
+        member_calls(*e, mm, Qx, QxT, Qy, QyT, Qxf, QxTf, Qyf, QyTf, c6);
+        for (int s = 0; s < 6; ++s) calls[s * count + m] = c6[s];
+        for (int par = 0; par < 2; ++par) {
+            LesArgs a{};
+            a.U = mm.f[par][0], a.V = mm.f[par][1], a.W = mm.f[par][2], a.S = mm.f[par][3];
+            a.OU = mm.f[1 - par][0], a.OV = mm.f[1 - par][1], a.OW = mm.f[1 - par][2], a.OS = mm.f[1 - par][3];
+            a.nx = mm.nx, a.ny = mm.ny, a.nz = mm.nz, a.kw = mm.nz;
+            a.sj = mm.sj, a.sk = mm.sk;
+            a.dt = dt;
+            double c19[19];
+            ftc_les_coefficients(dx, dy, dz, cs, c19);
+            for (int c = 0; c < 3; ++c) a.c25[c] = c19[c], a.c50[c] = c19[3 + c], a.c125[c] = c19[6 + c];
+            for (int c = 0; c < 10; ++c) a.sgs[c] = c19[9 + c];
+            e->args[par][m] = a;
+        }
+        e->members.push_back(mm);
+    }
+    e->d_members = (EnsMember *)carve(count * sizeof(EnsMember));
+    e->d_args[0] = (LesArgs *)carve(count * sizeof(LesArgs));
+    e->d_args[1] = (LesArgs *)carve(count * sizeof(LesArgs));
+    e->d_calls = (GemmCall *)carve(6 * count * sizeof(GemmCall));
+    if (hipMemcpy(e->d_members, e->members.data(), count * sizeof(EnsMember), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(e->d_calls, calls.data(), calls.size() * sizeof(GemmCall), hipMemcpyHostToDevice) != hipSuccess)
+        return eerr(FORT_SCOPE_RESOURCE, "description upload failed");
+    FTC_CHECK(build_items(*e));
+    if (mode != FTC_ENSEMBLE_FP64_PER_MEMBER) FTC_CHECK(capture(*e));
+    *out = e.release();
+    return FORT_SCOPE_OK;
+}
+
+int ftc_ensemble_upload_v1(ftc_ensemble *e, int member, const double *u, const double *v, const double *w,
+                           const double *s)
+{
+    if (!e || member < 0 || member >= e->count || !u || !v || !w || !s) return eerr(FORT_SCOPE_ARGUMENT, "bad upload");
+    const EnsMember &m = e->members[member];
+    const std::size_t bytes = (std::size_t)(m.nx + 2) * (m.ny + 2) * (m.nz + 2) * 8;
+    const double *src[4] = {u, v, w, s};
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return eerr(FORT_SCOPE_EXECUTION, "stream failed");
+    for (int set = 0; set < 2; ++set)
+        for (int f = 0; f < 4; ++f)
+            if (hipMemcpy(m.f[set][f], src[f], bytes, hipMemcpyHostToDevice) != hipSuccess)
+                return eerr(FORT_SCOPE_EXECUTION, "upload failed");
+    e->parity = 0;
+    return FORT_SCOPE_OK;
+}
+
+int ftc_ensemble_download_v1(ftc_ensemble *e, int member, double *u, double *v, double *w, double *s)
+{
+    if (!e || member < 0 || member >= e->count || !u || !v || !w || !s) return eerr(FORT_SCOPE_ARGUMENT, "bad download");
+    const EnsMember &m = e->members[member];
+    const std::size_t bytes = (std::size_t)(m.nx + 2) * (m.ny + 2) * (m.nz + 2) * 8;
+    double *dst[4] = {u, v, w, s};
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return eerr(FORT_SCOPE_EXECUTION, "stream failed");
+    for (int f = 0; f < 4; ++f)
+        if (hipMemcpy(dst[f], m.f[e->parity][f], bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            return eerr(FORT_SCOPE_EXECUTION, "download failed");
+    return FORT_SCOPE_OK;
+}
+
+int ftc_ensemble_download_pressure_v1(ftc_ensemble *e, int member, double *phi)
+{
+    if (!e || member < 0 || member >= e->count || !phi) return eerr(FORT_SCOPE_ARGUMENT, "bad download");
+    const EnsMember &m = e->members[member];
+    if (hipStreamSynchronize(e->stream) != hipSuccess ||
+        hipMemcpy(phi, m.phi, (std::size_t)m.nx * m.ny * m.nz * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return eerr(FORT_SCOPE_EXECUTION, "download failed");
+    return FORT_SCOPE_OK;
+}
+
+int ftc_ensemble_set_solves_v1(ftc_ensemble *e, int solves)
+{
+    if (!e || solves < 1 || solves > 16) return eerr(FORT_SCOPE_ARGUMENT, "solves must be 1..16");
+    e->solves = solves;
+    if (e->mode != FTC_ENSEMBLE_FP64_PER_MEMBER) return capture(*e);
+    return FORT_SCOPE_OK;
+}
+
+int ftc_ensemble_step_v1(ftc_ensemble *e, int steps)
+{
+    if (!e || steps < 0) return eerr(FORT_SCOPE_ARGUMENT, "bad step count");
+    for (int n = 0; n < steps; ++n) {
+        if (e->mode == FTC_ENSEMBLE_FP64_PER_MEMBER) {
+            FTC_CHECK(enqueue_step(*e, e->parity));
+        } else if (hipGraphLaunch(e->graph[e->parity], e->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            return eerr(FORT_SCOPE_EXECUTION, "graph launch failed");
+        }
+        e->parity = 1 - e->parity;
+    }
+    return FORT_SCOPE_OK;
+}
+
+int ftc_ensemble_synchronize_v1(ftc_ensemble *e)
+{
+    if (!e) return eerr(FORT_SCOPE_ARGUMENT, "missing ensemble");
+    return hipStreamSynchronize(e->stream) == hipSuccess ? FORT_SCOPE_OK : eerr(FORT_SCOPE_EXECUTION, "stream failed");
+}
+
+int ftc_ensemble_stream_v1(ftc_ensemble *e, void **stream)
+{
+    if (!e || !stream) return eerr(FORT_SCOPE_ARGUMENT, "missing ensemble");
+    *stream = e->stream;
+    return FORT_SCOPE_OK;
+}
+
+int ftc_ensemble_autotune_v1(ftc_ensemble *e, const char *cache_path)
+{
+    if (!e) return eerr(FORT_SCOPE_ARGUMENT, "missing ensemble");
+    std::ostringstream key;
+    key << "mode=" << e->mode << ";solves=" << e->solves << ";cus=" << e->cus << ";sizes=";
+    for (int m = 0; m < e->count; ++m) key << e->nx[m] << "x" << e->ny[m] << "x" << e->nz[m] << ",";
+    std::map<std::string, std::string> cache;
+    if (cache_path) {
+        std::ifstream in(cache_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            const auto tab = line.find('\t');
+            if (tab != std::string::npos) cache[line.substr(0, tab)] = line.substr(tab + 1);
+        }
+        auto hit = cache.find(key.str());
+        if (hit != cache.end() && std::sscanf(hit->second.c_str(), "%d %d %d", &e->les_kchunk, &e->gemm_tile,
+                                              &e->blocks_per_cu) == 3) {
+            FTC_CHECK(build_items(*e));
+            return e->mode != FTC_ENSEMBLE_FP64_PER_MEMBER ? capture(*e) : FORT_SCOPE_OK;
+        }
+    }
+    // Coordinate search in a fixed order: the LES k chunk on the LES stage, the contraction tile on the
+    // transform stages, then the persistent grid on the whole step. The fields are scratch for the
+    // search: the caller uploads (again) afterwards.
+    const bool per_member = e->mode == FTC_ENSEMBLE_FP64_PER_MEMBER;
+    auto run_stages = [&](std::initializer_list<int> stages) {
+        for (int st : stages) {
+            if (per_member)
+                for (int m = 0; m < e->count; ++m) launch_stage(*e, st, m, 0);
+            else launch_stage(*e, st, -1, 0);
+        }
+    };
+    float best = 1e30f;
+    int best_k = e->les_kchunk;
+    for (int k : {8, 16, 32, 64, 160}) {
+        e->les_kchunk = k;
+        FTC_CHECK(build_items(*e));
+        const float t = time_ms(*e, [&] { run_stages({kLes}); }, 3);
+        if (t < best) best = t, best_k = k;
+    }
+    e->les_kchunk = best_k;
+    best = 1e30f;
+    int best_t = e->gemm_tile;
+    for (int tile : {0, 1}) {
+        e->gemm_tile = tile;
+        FTC_CHECK(build_items(*e));
+        const float t = time_ms(*e, [&] { run_stages({kG1, kG2, kG3, kG4}); }, 3);
+        if (t < best) best = t, best_t = tile;
+    }
+    e->gemm_tile = best_t;
+    FTC_CHECK(build_items(*e));
+    if (!per_member) {
+        best = 1e30f;
+        int best_b = e->blocks_per_cu;
+        for (int b : {2, 4, 8, 16}) {
+            e->blocks_per_cu = b;
+            const float t = time_ms(*e, [&] { (void)enqueue_step(*e, 0); }, 3);
+            if (t < best) best = t, best_b = b;
+        }
+        e->blocks_per_cu = best_b;
+    }
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return eerr(FORT_SCOPE_EXECUTION, "tuning run failed");
+    if (cache_path) {
+        cache[key.str()] = std::to_string(e->les_kchunk) + " " + std::to_string(e->gemm_tile) + " " +
+                           std::to_string(e->blocks_per_cu);
+        std::ofstream out(cache_path, std::ios::trunc);
+        for (auto &kv : cache) out << kv.first << '\t' << kv.second << '\n';
+    }
+    return per_member ? FORT_SCOPE_OK : capture(*e);
+}
+
+int ftc_ensemble_get_info_v1(ftc_ensemble *e, ftc_ensemble_info_v1 *info)
+{
+    if (!e || !info) return eerr(FORT_SCOPE_ARGUMENT, "missing ensemble or output");
+    info->version = 1;
+    info->mode = e->mode;
+    info->members = e->count;
+    info->solves = mixed(*e) ? e->solves : 1;
+    info->les_kchunk = e->les_kchunk;
+    info->gemm_tile = e->gemm_tile;
+    info->blocks_per_cu = e->blocks_per_cu;
+    info->reserved = 0;
+    info->launches_per_step = (int64_t)step_stages(*e).size() * (e->mode == FTC_ENSEMBLE_FP64_PER_MEMBER ? e->count : 1);
+    double bytes = 0;
+    int64_t cells = 0;
+    for (int m = 0; m < e->count; ++m) {
+        const double P = (double)(e->nx[m] + 2) * (e->ny[m] + 2) * (e->nz[m] + 2);
+        const double N = (double)e->nx[m] * e->ny[m] * e->nz[m];
+        cells += (int64_t)N;
+        // LES step 8(4P+4I); divergence 8(3P+N); pressure floor 8(2N); projection 8(3P+N+3I).
+        bytes += 8 * (4 * P + 4 * N) + 8 * (3 * P + N) + 8 * 2 * N + 8 * (3 * P + N + 3 * N);
+    }
+    info->cells = cells;
+    info->algorithmic_bytes = bytes;
+    info->device_bytes = e->arena_bytes;
+    return FORT_SCOPE_OK;
+}
+
+int ftc_ensemble_destroy_v1(ftc_ensemble **pe)
+{
+    if (!pe) return eerr(FORT_SCOPE_ARGUMENT, "missing ensemble pointer");
+    ftc_ensemble *e = *pe;
+    if (!e) return FORT_SCOPE_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graphs(*e);
+    if (e->items_arena) (void)hipFree(e->items_arena);
+    if (e->arena) (void)hipFree(e->arena);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    *pe = nullptr;
+    return FORT_SCOPE_OK;
+}
+
+}  // extern "C"

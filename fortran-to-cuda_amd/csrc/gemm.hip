@@ -1,13 +1,6 @@
 // MFMA contractions for gfx950 (see gemm.hpp for the operation and precisions), the dense chain
-// contraction entry D = A*B*C, and an in-bounds probe of the MFMA lane layouts the kernels assume.
-//
-// Lane layouts used (wave64, l = lane):
-//   v_mfma_f64_16x16x4f64   A(16x4):  lane l holds A[l%16][l/16];  B(4x16): B[l/16][l%16];
-//                           D(16x16): lane l, element r holds D[l/16 + 4*r][l%16]
-//   v_mfma_f32_16x16x32_bf16 A(16x32): lane l holds A[l%16][8*(l/16) .. 8*(l/16)+7];
-//                           B(32x16): B[8*(l/16) .. +7][l%16];
-//                           D(16x16, FP32): lane l, element r holds D[4*(l/16) + r][l%16]
-// tests/test_dense_gpu.py checks both against the probe before trusting any product.
+// contraction entry D = A*B*C, and an in-bounds probe of the MFMA lane layouts the kernels assume
+// (the layouts are listed in gemm_kernels.hpp).
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -17,186 +10,9 @@
 #include "../../include/ftc_dense.h"
 #include "entry.hpp"
 #include "gemm.hpp"
+#include "gemm_kernels.hpp"
 
 namespace ftc {
-
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-constexpr int kGemmThreads = 256;
-
-// ------------------------------------------------------------------------------------------------ FP64
-template <int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(kGemmThreads) void gemm_f64_kernel(const GemmCall g)
-{
-    constexpr int BK = 16;
-    constexpr int WTM = BM / WM, WTN = BN / WN;
-    constexpr int TM = WTM / 16, TN = WTN / 16;
-    static_assert(WM * WN == kGemmThreads / 64, "four waves");
-    __shared__ double As[BK][BM + 1];
-    __shared__ double Bs[BK][BN + 1];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = wave % WM, wn = wave / WM;
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-    const long long b = blockIdx.z;
-    const double *A = static_cast<const double *>(g.A) + b * g.sa;
-    const double *B = static_cast<const double *>(g.B) + b * g.sb;
-    f64x4 acc[TM][TN];
-#pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TN; ++y) acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
-        double ra[BM * BK / kGemmThreads], rb[BN * BK / kGemmThreads];
-#pragma unroll
-        for (int e = 0; e < BM * BK / kGemmThreads; ++e) {
-            const int idx = t + e * kGemmThreads, m = idx % BM, k = idx / BM;
-            const bool ok = m0 + m < g.M && k0 + k < g.K;
-            ra[e] = ok ? A[(long long)(m0 + m) + (long long)(k0 + k) * g.lda] : 0.0;
-        }
-#pragma unroll
-        for (int e = 0; e < BN * BK / kGemmThreads; ++e) {
-            const int idx = t + e * kGemmThreads, k = idx % BK, n = idx / BK;
-            const bool ok = n0 + n < g.N && k0 + k < g.K;
-            rb[e] = ok ? B[(long long)(k0 + k) + (long long)(n0 + n) * g.ldb] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < BM * BK / kGemmThreads; ++e) {
-            const int idx = t + e * kGemmThreads;
-            As[idx / BM][idx % BM] = ra[e];
-        }
-#pragma unroll
-        for (int e = 0; e < BN * BK / kGemmThreads; ++e) {
-            const int idx = t + e * kGemmThreads;
-            Bs[idx % BK][idx / BK] = rb[e];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            double af[TM], bfr[TN];
-#pragma unroll
-            for (int x = 0; x < TM; ++x) af[x] = As[kk + lane / 16][wm * WTM + x * 16 + lane % 16];
-#pragma unroll
-            for (int y = 0; y < TN; ++y) bfr[y] = Bs[kk + lane / 16][wn * WTN + y * 16 + lane % 16];
-#pragma unroll
-            for (int x = 0; x < TM; ++x)
-#pragma unroll
-                for (int y = 0; y < TN; ++y)
-                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bfr[y], acc[x][y], 0, 0, 0);
-        }
-    }
-    double *C = static_cast<double *>(g.C) + b * g.sc;
-#pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TN; ++y)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * WTM + x * 16 + lane / 16 + 4 * r;
-                const int col = n0 + wn * WTN + y * 16 + lane % 16;
-                if (row < g.M && col < g.N) {
-                    const long long ci = (long long)row + (long long)col * g.ldc;
-                    double v = g.alpha * acc[x][y][r];
-                    if (g.beta != 0.0) v = v + g.beta * C[ci];
-                    C[ci] = v;
-                }
-            }
-}
-
-// ------------------------------------------------------------------------------------------------ bf16x3
-template <class TA, class TB, class TO, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(kGemmThreads) void gemm_bf16x3_kernel(const GemmCall g)
-{
-    constexpr int BK = 32, PAD = 8;
-    constexpr int WTM = BM / WM, WTN = BN / WN;
-    constexpr int TM = WTM / 16, TN = WTN / 16;
-    static_assert(WM * WN == kGemmThreads / 64, "four waves");
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[BM][BK + PAD];
-    __shared__ __attribute__((aligned(16))) __bf16 Al[BM][BK + PAD];
-    __shared__ __attribute__((aligned(16))) __bf16 Bh[BN][BK + PAD];
-    __shared__ __attribute__((aligned(16))) __bf16 Bl[BN][BK + PAD];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = wave % WM, wn = wave / WM;
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-    const long long b = blockIdx.z;
-    const TA *A = static_cast<const TA *>(g.A) + b * g.sa;
-    const TB *B = static_cast<const TB *>(g.B) + b * g.sb;
-    f32x4 acc[TM][TN];
-#pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TN; ++y) acc[x][y] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
-        float ra[BM * BK / kGemmThreads], rb[BN * BK / kGemmThreads];
-#pragma unroll
-        for (int e = 0; e < BM * BK / kGemmThreads; ++e) {
-            const int idx = t + e * kGemmThreads, m = idx % BM, k = idx / BM;
-            const bool ok = m0 + m < g.M && k0 + k < g.K;
-            ra[e] = ok ? (float)A[(long long)(m0 + m) + (long long)(k0 + k) * g.lda] : 0.f;
-        }
-#pragma unroll
-        for (int e = 0; e < BN * BK / kGemmThreads; ++e) {
-            const int idx = t + e * kGemmThreads, k = idx % BK, n = idx / BK;
-            const bool ok = n0 + n < g.N && k0 + k < g.K;
-            rb[e] = ok ? (float)B[(long long)(k0 + k) + (long long)(n0 + n) * g.ldb] : 0.f;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < BM * BK / kGemmThreads; ++e) {
-            const int idx = t + e * kGemmThreads, m = idx % BM, k = idx / BM;
-            const __bf16 hi = (__bf16)ra[e];
-            Ah[m][k] = hi;
-            Al[m][k] = (__bf16)(ra[e] - (float)hi);
-        }
-#pragma unroll
-        for (int e = 0; e < BN * BK / kGemmThreads; ++e) {
-            const int idx = t + e * kGemmThreads, k = idx % BK, n = idx / BK;
-            const __bf16 hi = (__bf16)rb[e];
-            Bh[n][k] = hi;
-            Bl[n][k] = (__bf16)(rb[e] - (float)hi);
-        }
-        __syncthreads();
-        bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
-#pragma unroll
-        for (int x = 0; x < TM; ++x) {
-            const int row = wm * WTM + x * 16 + lane % 16;
-            ah[x] = *reinterpret_cast<const bf16x8 *>(&Ah[row][8 * (lane / 16)]);
-            al[x] = *reinterpret_cast<const bf16x8 *>(&Al[row][8 * (lane / 16)]);
-        }
-#pragma unroll
-        for (int y = 0; y < TN; ++y) {
-            const int col = wn * WTN + y * 16 + lane % 16;
-            bh[y] = *reinterpret_cast<const bf16x8 *>(&Bh[col][8 * (lane / 16)]);
-            bl[y] = *reinterpret_cast<const bf16x8 *>(&Bl[col][8 * (lane / 16)]);
-        }
-#pragma unroll
-        for (int x = 0; x < TM; ++x)
-#pragma unroll
-            for (int y = 0; y < TN; ++y) {
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
-            }
-    }
-    TO *C = static_cast<TO *>(g.C) + b * g.sc;
-#pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TN; ++y)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * WTM + x * 16 + 4 * (lane / 16) + r;
-                const int col = n0 + wn * WTN + y * 16 + lane % 16;
-                if (row < g.M && col < g.N) {
-                    const long long ci = (long long)row + (long long)col * g.ldc;
-                    double v = g.alpha * (double)acc[x][y][r];
-                    if (g.beta != 0.0) v = v + g.beta * (double)C[ci];
-                    C[ci] = (TO)v;
-                }
-            }
-}
 
 // ------------------------------------------------------------------------------------------------ host
 struct Tile {

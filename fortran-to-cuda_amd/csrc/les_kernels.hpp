@@ -1,0 +1,568 @@
+// Device code of the staggered-grid LES stencils (see les.hip for the entries and the numerical
+// contract): the fused k-marching kernels and, for persistent grouped launches, the per-tile body of the
+// LDS-staged Smagorinsky kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#pragma clang fp contract(off)
+
+namespace ftc {
+
+constexpr int TX = 64;
+constexpr int TY = 4;
+
+struct LesArgs {
+    const double *U, *V, *W, *S;
+    double *OU, *OV, *OW, *OS;
+    int nx, ny, nz;  // i loop 2..nx+1, j loop 2..ny+1, U/V/S k loop 2..nz+1
+    int kw;          // W k loop 2..kw
+    int kchunk;
+    int reserved;
+    long long sj, sk;                   // strides of j and k in elements
+    double c25[3], c50[3], c125[3];     // 0.25/d, 0.5/d, 0.125/d for d = dx, dy, dz
+    double sgs[10];                     // 1/d, 0.25/d, 1/d^2 (x, y, z each), (Cs*Delta)^2
+    double dt;
+};
+
+enum : int { kU = 1, kV = 2, kW = 4, kS = 8 };
+
+__device__ __forceinline__ double cdu_point(const LesArgs &a, long long p)
+{
+    const double *U = a.U, *V = a.V, *W = a.W;
+    const long long sj = a.sj, sk = a.sk;
+    const double u0 = U[p], uxp = U[p + 1], uxm = U[p - 1], uyp = U[p + sj], uym = U[p - sj];
+    const double uzp = U[p + sk], uzm = U[p - sk];
+    const double v0 = V[p], vxp = V[p + 1], vym = V[p - sj], vxpym = V[p + 1 - sj];
+    const double w0 = W[p], wxp = W[p + 1], wzm = W[p - sk], wxpzm = W[p + 1 - sk];
+    const double Ax = a.c25[0], Ay = a.c25[1], Az = a.c25[2];
+    double x = Ax * (uxp + u0) * (uxp + u0) - Ax * (u0 + uxm) * (u0 + uxm);
+    double y = Ay * (uyp + u0) * (vxp + v0) - Ay * (u0 + uym) * (vxpym + vym);
+    double z = Az * (uzp + u0) * (wxp + w0) - Az * (u0 + uzm) * (wxpzm + wzm);
+    double t = -((x + y) + z);
+    double Vadv = ((v0 + vxp) + vym) + vxpym;
+    double Wadv = ((w0 + wxp) + wzm) + wxpzm;
+    t = t - ((a.c50[0] * (uxp - uxm) * u0 + a.c125[1] * (uyp - uym) * Vadv) + a.c125[2] * (uzp - uzm) * Wadv);
+    return t * 0.5;
+}
+
+__device__ __forceinline__ double cdv_point(const LesArgs &a, long long p)
+{
+    const double *U = a.U, *V = a.V, *W = a.W;
+    const long long sj = a.sj, sk = a.sk;
+    const double v0 = V[p], vxp = V[p + 1], vxm = V[p - 1], vyp = V[p + sj], vym = V[p - sj];
+    const double vzp = V[p + sk], vzm = V[p - sk];
+    const double u0 = U[p], uyp = U[p + sj], uxm = U[p - 1], uxmyp = U[p - 1 + sj];
+    const double w0 = W[p], wyp = W[p + sj], wzm = W[p - sk], wypzm = W[p + sj - sk];
+    const double Ax = a.c25[0], Ay = a.c25[1], Az = a.c25[2];
+    double y = Ay * (vyp + v0) * (vyp + v0) - Ay * (v0 + vym) * (v0 + vym);
+    double x = Ax * (vxp + v0) * (uyp + u0) - Ax * (v0 + vxm) * (uxmyp + uxm);
+    double z = Az * (vzp + v0) * (wyp + w0) - Az * (v0 + vzm) * (wypzm + wzm);
+    double t = -((y + x) + z);
+    double Uadv = ((u0 + uyp) + uxm) + uxmyp;
+    double Wadv = ((w0 + wyp) + wzm) + wypzm;
+    t = t - ((a.c125[0] * (vxp - vxm) * Uadv + a.c50[1] * (vyp - vym) * v0) + a.c125[2] * (vzp - vzm) * Wadv);
+    return t * 0.5;
+}
+
+__device__ __forceinline__ double cdw_point(const LesArgs &a, long long p)
+{
+    const double *U = a.U, *V = a.V, *W = a.W;
+    const long long sj = a.sj, sk = a.sk;
+    const double w0 = W[p], wxp = W[p + 1], wxm = W[p - 1], wyp = W[p + sj], wym = W[p - sj];
+    const double wzp = W[p + sk], wzm = W[p - sk];
+    const double u0 = U[p], uzp = U[p + sk], uxm = U[p - 1], uxmzp = U[p - 1 + sk];
+    const double v0 = V[p], vzp = V[p + sk], vym = V[p - sj], vymzp = V[p - sj + sk];
+    const double Ax = a.c25[0], Ay = a.c25[1], Az = a.c25[2];
+    double z = Az * (wzp + w0) * (wzp + w0) - Az * (w0 + wzm) * (w0 + wzm);
+    double y = Ay * (wyp + w0) * (vzp + v0) - Ay * (w0 + wym) * (vym + vymzp);
+    double x = Ax * (wxp + w0) * (uzp + u0) - Ax * (w0 + wxm) * (uxmzp + uxm);
+    double t = -((z + y) + x);
+    double Uadv = ((u0 + uzp) + uxm) + uxmzp;
+    double Vadv = ((v0 + vzp) + vym) + vymzp;
+    t = t - ((a.c125[0] * (wxp - wxm) * Uadv + a.c125[1] * (wyp - wym) * Vadv) + a.c50[2] * (wzp - wzm) * w0);
+    return t * 0.5;
+}
+
+__device__ __forceinline__ double sadv_point(const LesArgs &a, long long p)
+{
+    const double *U = a.U, *V = a.V, *W = a.W, *S = a.S;
+    const long long sj = a.sj, sk = a.sk;
+    const double s0 = S[p];
+    const double Cx = a.c50[0], Cy = a.c50[1], Cz = a.c50[2];
+    double x = Cx * U[p] * (S[p + 1] + s0) - Cx * U[p - 1] * (s0 + S[p - 1]);
+    double y = Cy * V[p] * (S[p + sj] + s0) - Cy * V[p - sj] * (s0 + S[p - sj]);
+    double z = Cz * W[p] * (S[p + sk] + s0) - Cz * W[p - sk] * (s0 + S[p - sk]);
+    return -((x + y) + z);
+}
+
+// Smagorinsky viscosity at an interior cell whose linear index is p.
+__device__ __forceinline__ double nu_point(const LesArgs &a, long long p)
+{
+    const double *U = a.U, *V = a.V, *W = a.W;
+    const long long sj = a.sj, sk = a.sk;
+    const double *q = a.sgs;
+    double s11 = (U[p] - U[p - 1]) * q[0];
+    double s22 = (V[p] - V[p - sj]) * q[1];
+    double s33 = (W[p] - W[p - sk]) * q[2];
+    double dudy = ((U[p + sj] + U[p - 1 + sj]) - (U[p - sj] + U[p - 1 - sj])) * q[4];
+    double dvdx = ((V[p + 1] + V[p + 1 - sj]) - (V[p - 1] + V[p - 1 - sj])) * q[3];
+    double dudz = ((U[p + sk] + U[p - 1 + sk]) - (U[p - sk] + U[p - 1 - sk])) * q[5];
+    double dwdx = ((W[p + 1] + W[p + 1 - sk]) - (W[p - 1] + W[p - 1 - sk])) * q[3];
+    double dvdz = ((V[p + sk] + V[p - sj + sk]) - (V[p - sk] + V[p - sj - sk])) * q[5];
+    double dwdy = ((W[p + sj] + W[p + sj - sk]) - (W[p - sj] + W[p - sj - sk])) * q[4];
+    double s12 = 0.5 * (dudy + dvdx);
+    double s13 = 0.5 * (dudz + dwdx);
+    double s23 = 0.5 * (dvdz + dwdy);
+    double ss = 2.0 * ((s11 * s11 + s22 * s22) + s33 * s33) + 4.0 * ((s12 * s12 + s13 * s13) + s23 * s23);
+    return q[9] * sqrt(ss);
+}
+
+template <int R>
+using NuRing = double[R][TY + 2][TX + 2];
+
+// Fills ring slot kp % 3 with nu at plane kp over the tile and its one-cell ring. A ghost cell takes the
+// value of its image: periodic in x and y, zero-gradient in z.
+__device__ __forceinline__ void nu_plane(const LesArgs &a, NuRing<3> &ring, int kp, int i0, int j0, int tid)
+{
+    const int kk = kp < 2 ? 2 : (kp > a.nz + 1 ? a.nz + 1 : kp);
+    double(*plane)[TX + 2] = ring[kp % 3];
+    for (int c = tid; c < (TX + 2) * (TY + 2); c += TX * TY) {
+        const int li = c % (TX + 2), lj = c / (TX + 2);
+        const int gi = i0 - 1 + li, gj = j0 - 1 + lj;
+        double value = 0.0;
+        if (gi <= a.nx + 2 && gj <= a.ny + 2) {
+            const int ii = gi == 1 ? a.nx + 1 : (gi == a.nx + 2 ? 2 : gi);
+            const int jj = gj == 1 ? a.ny + 1 : (gj == a.ny + 2 ? 2 : gj);
+            value = nu_point(a, (long long)(ii - 1) + (long long)(jj - 1) * a.sj + (long long)(kk - 1) * a.sk);
+        }
+        plane[lj][li] = value;
+    }
+}
+
+// Writes a new interior value and every ghost whose image it is (the result of the x, then y, then z
+// halo passes). zmode: 0 tangential velocity (sign change at the walls), 1 wall-normal velocity (no z
+// ghost), 2 scalar (zero gradient).
+__device__ __forceinline__ void put_with_images(const LesArgs &a, double *F, long long p, double val, int i, int j,
+                                                int k, int zmode)
+{
+    long long ox[3] = {0, 0, 0}, oy[3] = {0, 0, 0}, oz[2] = {0, 0};
+    int nxo = 1, nyo = 1, nzo = 0;
+    if (i == 2) ox[nxo++] = a.nx;
+    if (i == a.nx + 1) ox[nxo++] = -a.nx;
+    if (j == 2) oy[nyo++] = (long long)a.ny * a.sj;
+    if (j == a.ny + 1) oy[nyo++] = -(long long)a.ny * a.sj;
+    if (zmode != 1) {
+        if (k == 2) oz[nzo++] = -a.sk;
+        if (k == a.nz + 1) oz[nzo++] = a.sk;
+    }
+    const double zval = zmode == 0 ? -val : val;
+    for (int x = 0; x < nxo; ++x)
+        for (int y = 0; y < nyo; ++y) {
+            const long long q = p + ox[x] + oy[y];
+            F[q] = val;
+            for (int z = 0; z < nzo; ++z) F[q + oz[z]] = zval;
+        }
+}
+
+template <int MASK, bool SGS, bool STEP>
+__global__ __launch_bounds__(TX *TY) void les_kernel(const LesArgs a)
+{
+    __shared__ NuRing<SGS ? 3 : 1> ring;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const int i0 = 2 + blockIdx.x * TX, j0 = 2 + blockIdx.y * TY;
+    const int i = i0 + tx, j = j0 + ty;
+    int klast = (MASK & (kU | kV | kS)) ? a.nz + 1 : 0;
+    if ((MASK & kW) && a.kw > klast) klast = a.kw;
+    const int k0 = 2 + (int)blockIdx.z * a.kchunk;
+    if (k0 > klast) return;  // uniform over the workgroup
+    const int k1 = min(k0 + a.kchunk - 1, klast);
+    const bool inside = i <= a.nx + 1 && j <= a.ny + 1;
+    if constexpr (SGS) {
+        nu_plane(a, ring, k0 - 1, i0, j0, tid);
+        nu_plane(a, ring, k0, i0, j0, tid);
+    }
+    const double dt = a.dt;
+    const double *q = a.sgs;
+    for (int k = k0; k <= k1; ++k) {
+        if constexpr (SGS) {
+            nu_plane(a, ring, k + 1, i0, j0, tid);
+            __syncthreads();
+        }
+        if (inside) {
+            const long long p = (long long)(i - 1) + (long long)(j - 1) * a.sj + (long long)(k - 1) * a.sk;
+            const int lx = tx + 1, ly = ty + 1;
+            const int slm = (k - 1) % 3, slz = k % 3, slp = (k + 1) % 3;
+#define NU(di, dj, dk) ring[(dk) < 0 ? slm : ((dk) > 0 ? slp : slz)][ly + (dj)][lx + (di)]
+            const long long sj = a.sj, sk = a.sk;
+            if ((MASK & kU) && k <= a.nz + 1) {
+                double t = cdu_point(a, p);
+                if constexpr (SGS) {
+                    const double *U = a.U;
+                    const double u0 = U[p];
+                    double fxp = NU(1, 0, 0) * (U[p + 1] - u0);
+                    double fxm = NU(0, 0, 0) * (u0 - U[p - 1]);
+                    double exy0 = 0.25 * (((NU(0, 0, 0) + NU(1, 0, 0)) + NU(0, 1, 0)) + NU(1, 1, 0));
+                    double exym = 0.25 * (((NU(0, -1, 0) + NU(1, -1, 0)) + NU(0, 0, 0)) + NU(1, 0, 0));
+                    double fyp = exy0 * (U[p + sj] - u0);
+                    double fym = exym * (u0 - U[p - sj]);
+                    double exz0 = 0.25 * (((NU(0, 0, 0) + NU(1, 0, 0)) + NU(0, 0, 1)) + NU(1, 0, 1));
+                    double exzm = 0.25 * (((NU(0, 0, -1) + NU(1, 0, -1)) + NU(0, 0, 0)) + NU(1, 0, 0));
+                    double fzp = exz0 * (U[p + sk] - u0);
+                    double fzm = exzm * (u0 - U[p - sk]);
+                    t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                }
+                if constexpr (STEP) put_with_images(a, a.OU, p, a.U[p] + dt * t, i, j, k, 0);
+                else a.OU[p] = t;
+            }
+            if ((MASK & kV) && k <= a.nz + 1) {
+                double t = cdv_point(a, p);
+                if constexpr (SGS) {
+                    const double *V = a.V;
+                    const double v0 = V[p];
+                    double exy0 = 0.25 * (((NU(0, 0, 0) + NU(1, 0, 0)) + NU(0, 1, 0)) + NU(1, 1, 0));
+                    double exyxm = 0.25 * (((NU(-1, 0, 0) + NU(0, 0, 0)) + NU(-1, 1, 0)) + NU(0, 1, 0));
+                    double fxp = exy0 * (V[p + 1] - v0);
+                    double fxm = exyxm * (v0 - V[p - 1]);
+                    double fyp = NU(0, 1, 0) * (V[p + sj] - v0);
+                    double fym = NU(0, 0, 0) * (v0 - V[p - sj]);
+                    double eyz0 = 0.25 * (((NU(0, 0, 0) + NU(0, 1, 0)) + NU(0, 0, 1)) + NU(0, 1, 1));
+                    double eyzm = 0.25 * (((NU(0, 0, -1) + NU(0, 1, -1)) + NU(0, 0, 0)) + NU(0, 1, 0));
+                    double fzp = eyz0 * (V[p + sk] - v0);
+                    double fzm = eyzm * (v0 - V[p - sk]);
+                    t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                }
+                if constexpr (STEP) put_with_images(a, a.OV, p, a.V[p] + dt * t, i, j, k, 0);
+                else a.OV[p] = t;
+            }
+            if ((MASK & kW) && k <= a.kw) {
+                double t = cdw_point(a, p);
+                if constexpr (SGS) {
+                    const double *W = a.W;
+                    const double w0 = W[p];
+                    double exz0 = 0.25 * (((NU(0, 0, 0) + NU(1, 0, 0)) + NU(0, 0, 1)) + NU(1, 0, 1));
+                    double exzxm = 0.25 * (((NU(-1, 0, 0) + NU(0, 0, 0)) + NU(-1, 0, 1)) + NU(0, 0, 1));
+                    double fxp = exz0 * (W[p + 1] - w0);
+                    double fxm = exzxm * (w0 - W[p - 1]);
+                    double eyz0 = 0.25 * (((NU(0, 0, 0) + NU(0, 1, 0)) + NU(0, 0, 1)) + NU(0, 1, 1));
+                    double eyzym = 0.25 * (((NU(0, -1, 0) + NU(0, 0, 0)) + NU(0, -1, 1)) + NU(0, 0, 1));
+                    double fyp = eyz0 * (W[p + sj] - w0);
+                    double fym = eyzym * (w0 - W[p - sj]);
+                    double fzp = NU(0, 0, 1) * (W[p + sk] - w0);
+                    double fzm = NU(0, 0, 0) * (w0 - W[p - sk]);
+                    t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                }
+                if constexpr (STEP) put_with_images(a, a.OW, p, a.W[p] + dt * t, i, j, k, 1);
+                else a.OW[p] = t;
+            }
+            if ((MASK & kS) && k <= a.nz + 1) {
+                double t = sadv_point(a, p);
+                if constexpr (SGS) {
+                    const double *S = a.S;
+                    const double s0 = S[p];
+                    double fxp = (0.5 * (NU(0, 0, 0) + NU(1, 0, 0))) * (S[p + 1] - s0);
+                    double fxm = (0.5 * (NU(-1, 0, 0) + NU(0, 0, 0))) * (s0 - S[p - 1]);
+                    double fyp = (0.5 * (NU(0, 0, 0) + NU(0, 1, 0))) * (S[p + sj] - s0);
+                    double fym = (0.5 * (NU(0, -1, 0) + NU(0, 0, 0))) * (s0 - S[p - sj]);
+                    double fzp = (0.5 * (NU(0, 0, 0) + NU(0, 0, 1))) * (S[p + sk] - s0);
+                    double fzm = (0.5 * (NU(0, 0, -1) + NU(0, 0, 0))) * (s0 - S[p - sk]);
+                    t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                }
+                if constexpr (STEP) put_with_images(a, a.OS, p, a.S[p] + dt * t, i, j, k, 2);
+                else a.OS[p] = t;
+            }
+#undef NU
+        }
+        if constexpr (SGS) __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ v2
+// The Smagorinsky variants stage every input plane through LDS once: a workgroup keeps a ring of four
+// velocity planes (k-1 .. k+2) of its tile with a two-cell ring, three scalar planes and three viscosity
+// planes with a one-cell ring. The global loads of plane k+3 are issued into registers while plane k is
+// computed (software pipelining), so each input value is read from HBM about once per workgroup and every
+// stencil operand after that comes from LDS.
+//
+// Precondition (documented at the entries): the x and y ghost layers of U, V, W and S equal their
+// periodic images, as every field written by the step or the halo fill is. The staged tile is gathered
+// through periodic images, so a ghost cell's viscosity is the viscosity of its image.
+constexpr int VX = TX + 4, VY = TY + 4;  // velocity tile with a two-cell ring
+constexpr int CX = TX + 2, CY = TY + 2;  // scalar and viscosity tile with a one-cell ring
+constexpr int NTH = TX * TY;
+constexpr int VCELLS = VX * VY, CCELLS = CX * CY;
+constexpr int VR = (VCELLS + NTH - 1) / NTH, CR = (CCELLS + NTH - 1) / NTH;
+
+struct SgsLds {
+    double vel[4][3][VY][VX];
+    double sca[3][CY][CX];
+    double nu[3][CY][CX];
+};
+
+__device__ __forceinline__ int periodic_image(int c, int n)
+{
+    int r = (c - 2) % n;
+    if (r < 0) r += n;
+    return r + 2;
+}
+
+// One workgroup's share of the staged LES kernel: the (bx, by) tile of columns over k chunk bz.
+template <bool STEP>
+__device__ __forceinline__ void les_sgs_body(const LesArgs &a, SgsLds &L, int bx, int by, int bz)
+{
+    const double *__restrict__ gU = a.U;
+    const double *__restrict__ gV = a.V;
+    const double *__restrict__ gW = a.W;
+    const double *__restrict__ gS = a.S;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const int i0 = 2 + bx * TX, j0 = 2 + by * TY;
+    const int i = i0 + tx, j = j0 + ty;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int k0 = 2 + bz * a.kchunk;
+    if (k0 > nz + 1) return;  // uniform over the workgroup
+    const int k1 = min(k0 + a.kchunk - 1, nz + 1);
+    const long long sj = a.sj, sk = a.sk;
+
+    long long voff[VR], coff[CR];
+    int vcell[VR], ccell[CR];
+#pragma unroll
+    for (int r = 0; r < VR; ++r) {
+        const int c = tid + r * NTH;
+        vcell[r] = c < VCELLS ? c : -1;
+        const int cc = c < VCELLS ? c : 0;
+        const int ii = periodic_image(i0 - 2 + cc % VX, nx), jj = periodic_image(j0 - 2 + cc / VX, ny);
+        voff[r] = (long long)(ii - 1) + (long long)(jj - 1) * sj;
+    }
+#pragma unroll
+    for (int r = 0; r < CR; ++r) {
+        const int c = tid + r * NTH;
+        ccell[r] = c < CCELLS ? c : -1;
+        const int cc = c < CCELLS ? c : 0;
+        const int ii = periodic_image(i0 - 1 + cc % CX, nx), jj = periodic_image(j0 - 1 + cc / CX, ny);
+        coff[r] = (long long)(ii - 1) + (long long)(jj - 1) * sj;
+    }
+    double rv[VR][3], rs[CR];
+    auto fetch_vel = [&](int kp) {
+        if (kp < 1 || kp > nz + 2) return;
+        const long long base = (long long)(kp - 1) * sk;
+#pragma unroll
+        for (int r = 0; r < VR; ++r)
+            if (vcell[r] >= 0) {
+                rv[r][0] = gU[voff[r] + base];
+                rv[r][1] = gV[voff[r] + base];
+                rv[r][2] = gW[voff[r] + base];
+            }
+    };
+    auto put_vel = [&](int kp) {
+        if (kp < 1 || kp > nz + 2) return;
+        const int slot = kp & 3;
+#pragma unroll
+        for (int r = 0; r < VR; ++r)
+            if (vcell[r] >= 0) {
+                (&L.vel[slot][0][0][0])[vcell[r]] = rv[r][0];
+                (&L.vel[slot][1][0][0])[vcell[r]] = rv[r][1];
+                (&L.vel[slot][2][0][0])[vcell[r]] = rv[r][2];
+            }
+    };
+    auto fetch_sca = [&](int kp) {
+        if (kp < 1 || kp > nz + 2) return;
+        const long long base = (long long)(kp - 1) * sk;
+#pragma unroll
+        for (int r = 0; r < CR; ++r)
+            if (ccell[r] >= 0) rs[r] = gS[coff[r] + base];
+    };
+    auto put_sca = [&](int kp) {
+        if (kp < 1 || kp > nz + 2) return;
+        const int slot = kp % 3;
+#pragma unroll
+        for (int r = 0; r < CR; ++r)
+            if (ccell[r] >= 0) (&L.sca[slot][0][0])[ccell[r]] = rs[r];
+    };
+    const double *q = a.sgs;
+    auto nu_plane = [&](int qk) {
+        const int kk = qk < 2 ? 2 : (qk > nz + 1 ? nz + 1 : qk);
+        const int s0 = kk & 3, sm = (kk - 1) & 3, sp = (kk + 1) & 3;
+        double(*nuo)[CX] = L.nu[qk % 3];
+        for (int c = tid; c < CCELLS; c += NTH) {
+            const int li = c % CX, lj = c / CX;
+            const int vx = li + 1, vy = lj + 1;
+#define QU(s, di, dj) L.vel[s][0][vy + (dj)][vx + (di)]
+#define QV(s, di, dj) L.vel[s][1][vy + (dj)][vx + (di)]
+#define QW(s, di, dj) L.vel[s][2][vy + (dj)][vx + (di)]
+            double s11 = (QU(s0, 0, 0) - QU(s0, -1, 0)) * q[0];
+            double s22 = (QV(s0, 0, 0) - QV(s0, 0, -1)) * q[1];
+            double s33 = (QW(s0, 0, 0) - QW(sm, 0, 0)) * q[2];
+            double dudy = ((QU(s0, 0, 1) + QU(s0, -1, 1)) - (QU(s0, 0, -1) + QU(s0, -1, -1))) * q[4];
+            double dvdx = ((QV(s0, 1, 0) + QV(s0, 1, -1)) - (QV(s0, -1, 0) + QV(s0, -1, -1))) * q[3];
+            double dudz = ((QU(sp, 0, 0) + QU(sp, -1, 0)) - (QU(sm, 0, 0) + QU(sm, -1, 0))) * q[5];
+            double dwdx = ((QW(s0, 1, 0) + QW(sm, 1, 0)) - (QW(s0, -1, 0) + QW(sm, -1, 0))) * q[3];
+            double dvdz = ((QV(sp, 0, 0) + QV(sp, 0, -1)) - (QV(sm, 0, 0) + QV(sm, 0, -1))) * q[5];
+            double dwdy = ((QW(s0, 0, 1) + QW(sm, 0, 1)) - (QW(s0, 0, -1) + QW(sm, 0, -1))) * q[4];
+#undef QU
+#undef QV
+#undef QW
+            double s12 = 0.5 * (dudy + dvdx);
+            double s13 = 0.5 * (dudz + dwdx);
+            double s23 = 0.5 * (dvdz + dwdy);
+            double ss = 2.0 * ((s11 * s11 + s22 * s22) + s33 * s33) + 4.0 * ((s12 * s12 + s13 * s13) + s23 * s23);
+            nuo[lj][li] = q[9] * sqrt(ss);
+        }
+    };
+
+    for (int kp = k0 - 2; kp <= k0 + 1; ++kp) {
+        fetch_vel(kp);
+        put_vel(kp);
+    }
+    for (int kp = k0 - 1; kp <= k0; ++kp) {
+        fetch_sca(kp);
+        put_sca(kp);
+    }
+    fetch_vel(k0 + 2);
+    fetch_sca(k0 + 1);
+    __syncthreads();
+    nu_plane(k0 - 1);
+    nu_plane(k0);
+    __syncthreads();
+
+    const bool inside = i <= nx + 1 && j <= ny + 1;
+    const double dt = a.dt;
+    const double c25x = a.c25[0], c25y = a.c25[1], c25z = a.c25[2];
+    for (int k = k0; k <= k1; ++k) {
+        put_vel(k + 2);
+        put_sca(k + 1);
+        fetch_vel(k + 3);
+        fetch_sca(k + 2);
+        __syncthreads();
+        nu_plane(k + 1);
+        __syncthreads();
+        if (inside) {
+            const int sm = (k - 1) & 3, s0 = k & 3, sp = (k + 1) & 3;
+            const int cm = (k - 1) % 3, c0 = k % 3, cp = (k + 1) % 3;
+#define VU(s, di, dj) L.vel[s][0][ty + 2 + (dj)][tx + 2 + (di)]
+#define VV(s, di, dj) L.vel[s][1][ty + 2 + (dj)][tx + 2 + (di)]
+#define VW(s, di, dj) L.vel[s][2][ty + 2 + (dj)][tx + 2 + (di)]
+#define SS(s, di, dj) L.sca[s][ty + 1 + (dj)][tx + 1 + (di)]
+#define NU(di, dj, dk) L.nu[(dk) < 0 ? cm : ((dk) > 0 ? cp : c0)][ty + 1 + (dj)][tx + 1 + (di)]
+            const double u0 = VU(s0, 0, 0), uxp = VU(s0, 1, 0), uxm = VU(s0, -1, 0), uyp = VU(s0, 0, 1);
+            const double uym = VU(s0, 0, -1), uzp = VU(sp, 0, 0), uzm = VU(sm, 0, 0);
+            const double v0 = VV(s0, 0, 0), vxp = VV(s0, 1, 0), vxm = VV(s0, -1, 0), vyp = VV(s0, 0, 1);
+            const double vym = VV(s0, 0, -1), vzp = VV(sp, 0, 0), vzm = VV(sm, 0, 0);
+            const double w0 = VW(s0, 0, 0), wxp = VW(s0, 1, 0), wxm = VW(s0, -1, 0), wyp = VW(s0, 0, 1);
+            const double wym = VW(s0, 0, -1), wzp = VW(sp, 0, 0), wzm = VW(sm, 0, 0);
+            const double n000 = NU(0, 0, 0), n100 = NU(1, 0, 0), nm00 = NU(-1, 0, 0);
+            const double n010 = NU(0, 1, 0), n0m0 = NU(0, -1, 0), n001 = NU(0, 0, 1), n00m = NU(0, 0, -1);
+            const long long p = (long long)(i - 1) + (long long)(j - 1) * sj + (long long)(k - 1) * sk;
+            // U: CDU (cdu.f90:48-107) then the diffusion of u.
+            {
+                const double vxpym = VV(s0, 1, -1), wxpzm = VW(sm, 1, 0);
+                double x = c25x * (uxp + u0) * (uxp + u0) - c25x * (u0 + uxm) * (u0 + uxm);
+                double y = c25y * (uyp + u0) * (vxp + v0) - c25y * (u0 + uym) * (vxpym + vym);
+                double z = c25z * (uzp + u0) * (wxp + w0) - c25z * (u0 + uzm) * (wxpzm + wzm);
+                double t = -((x + y) + z);
+                double Vadv = ((v0 + vxp) + vym) + vxpym;
+                double Wadv = ((w0 + wxp) + wzm) + wxpzm;
+                t = t - ((a.c50[0] * (uxp - uxm) * u0 + a.c125[1] * (uyp - uym) * Vadv) +
+                         a.c125[2] * (uzp - uzm) * Wadv);
+                t = t * 0.5;
+                double fxp = n100 * (uxp - u0);
+                double fxm = n000 * (u0 - uxm);
+                double exy0 = 0.25 * (((n000 + n100) + n010) + NU(1, 1, 0));
+                double exym = 0.25 * (((n0m0 + NU(1, -1, 0)) + n000) + n100);
+                double fyp = exy0 * (uyp - u0);
+                double fym = exym * (u0 - uym);
+                double exz0 = 0.25 * (((n000 + n100) + n001) + NU(1, 0, 1));
+                double exzm = 0.25 * (((n00m + NU(1, 0, -1)) + n000) + n100);
+                double fzp = exz0 * (uzp - u0);
+                double fzm = exzm * (u0 - uzm);
+                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                if constexpr (STEP) put_with_images(a, a.OU, p, u0 + dt * t, i, j, k, 0);
+                else a.OU[p] = t;
+            }
+            // V: CDV (cvd.f90:44-98) then the diffusion of v.
+            {
+                const double uxmyp = VU(s0, -1, 1), wypzm = VW(sm, 0, 1);
+                double y = c25y * (vyp + v0) * (vyp + v0) - c25y * (v0 + vym) * (v0 + vym);
+                double x = c25x * (vxp + v0) * (uyp + u0) - c25x * (v0 + vxm) * (uxmyp + uxm);
+                double z = c25z * (vzp + v0) * (wyp + w0) - c25z * (v0 + vzm) * (wypzm + wzm);
+                double t = -((y + x) + z);
+                double Uadv = ((u0 + uyp) + uxm) + uxmyp;
+                double Wadv = ((w0 + wyp) + wzm) + wypzm;
+                t = t - ((a.c125[0] * (vxp - vxm) * Uadv + a.c50[1] * (vyp - vym) * v0) +
+                         a.c125[2] * (vzp - vzm) * Wadv);
+                t = t * 0.5;
+                double exy0 = 0.25 * (((n000 + n100) + n010) + NU(1, 1, 0));
+                double exyxm = 0.25 * (((nm00 + n000) + NU(-1, 1, 0)) + n010);
+                double fxp = exy0 * (vxp - v0);
+                double fxm = exyxm * (v0 - vxm);
+                double fyp = n010 * (vyp - v0);
+                double fym = n000 * (v0 - vym);
+                double eyz0 = 0.25 * (((n000 + n010) + n001) + NU(0, 1, 1));
+                double eyzm = 0.25 * (((n00m + NU(0, 1, -1)) + n000) + n010);
+                double fzp = eyz0 * (vzp - v0);
+                double fzm = eyzm * (v0 - vzm);
+                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                if constexpr (STEP) put_with_images(a, a.OV, p, v0 + dt * t, i, j, k, 0);
+                else a.OV[p] = t;
+            }
+            // W on the interior faces: CDW (cdw.f90:44-98) then the diffusion of w.
+            if (k <= a.kw) {
+                const double uxmzp = VU(sp, -1, 0), vymzp = VV(sp, 0, -1);
+                double z = c25z * (wzp + w0) * (wzp + w0) - c25z * (w0 + wzm) * (w0 + wzm);
+                double y = c25y * (wyp + w0) * (vzp + v0) - c25y * (w0 + wym) * (vym + vymzp);
+                double x = c25x * (wxp + w0) * (uzp + u0) - c25x * (w0 + wxm) * (uxmzp + uxm);
+                double t = -((z + y) + x);
+                double Uadv = ((u0 + uzp) + uxm) + uxmzp;
+                double Vadv = ((v0 + vzp) + vym) + vymzp;
+                t = t - ((a.c125[0] * (wxp - wxm) * Uadv + a.c125[1] * (wyp - wym) * Vadv) +
+                         a.c50[2] * (wzp - wzm) * w0);
+                t = t * 0.5;
+                double exz0 = 0.25 * (((n000 + n100) + n001) + NU(1, 0, 1));
+                double exzxm = 0.25 * (((nm00 + n000) + NU(-1, 0, 1)) + n001);
+                double fxp = exz0 * (wxp - w0);
+                double fxm = exzxm * (w0 - wxm);
+                double eyz0 = 0.25 * (((n000 + n010) + n001) + NU(0, 1, 1));
+                double eyzym = 0.25 * (((n0m0 + n000) + NU(0, -1, 1)) + n001);
+                double fyp = eyz0 * (wyp - w0);
+                double fym = eyzym * (w0 - wym);
+                double fzp = n001 * (wzp - w0);
+                double fzm = n000 * (w0 - wzm);
+                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                if constexpr (STEP) put_with_images(a, a.OW, p, w0 + dt * t, i, j, k, 1);
+                else a.OW[p] = t;
+            }
+            // S: flux-form advection then the diffusion of s.
+            {
+                const double s0v = SS(c0, 0, 0), sxp = SS(c0, 1, 0), sxm = SS(c0, -1, 0), syp = SS(c0, 0, 1);
+                const double sym = SS(c0, 0, -1), szp = SS(cp, 0, 0), szm = SS(cm, 0, 0);
+                const double Cx = a.c50[0], Cy = a.c50[1], Cz = a.c50[2];
+                double x = Cx * u0 * (sxp + s0v) - Cx * uxm * (s0v + sxm);
+                double y = Cy * v0 * (syp + s0v) - Cy * vym * (s0v + sym);
+                double z = Cz * w0 * (szp + s0v) - Cz * wzm * (s0v + szm);
+                double t = -((x + y) + z);
+                double fxp = (0.5 * (n000 + n100)) * (sxp - s0v);
+                double fxm = (0.5 * (nm00 + n000)) * (s0v - sxm);
+                double fyp = (0.5 * (n000 + n010)) * (syp - s0v);
+                double fym = (0.5 * (n0m0 + n000)) * (s0v - sym);
+                double fzp = (0.5 * (n000 + n001)) * (szp - s0v);
+                double fzm = (0.5 * (n00m + n000)) * (s0v - szm);
+                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                if constexpr (STEP) put_with_images(a, a.OS, p, s0v + dt * t, i, j, k, 2);
+                else a.OS[p] = t;
+            }
+#undef VU
+#undef VV
+#undef VW
+#undef SS
+#undef NU
+        }
+        __syncthreads();
+    }
+}
+
+template <bool STEP>
+__global__ __launch_bounds__(NTH, 2) void les_sgs_kernel(const LesArgs a)
+{
+    __shared__ SgsLds L;
+    les_sgs_body<STEP>(a, L, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+}  // namespace ftc

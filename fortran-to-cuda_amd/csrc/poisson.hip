@@ -24,6 +24,7 @@
 #include "entry.hpp"
 #include "gemm.hpp"
 #include "reduce.hpp"
+#include "poisson_kernels.hpp"
 
 #pragma clang fp contract(off)
 
@@ -41,107 +42,6 @@ struct View3 {
     double *x;  // interior origin
     long long ld1, ld2;
 };
-
-// r = b - A phi over the interior (phi == nullptr: r = b). phi's neighbours wrap periodically or mirror
-// (Neumann on the staggered face) instead of reading its ghost layers.
-__global__ void poisson_residual_kernel(const double *__restrict__ phi, long long p1, long long p2,
-                                        const double *__restrict__ b, long long b1, long long b2,
-                                        double *__restrict__ r, int nx, int ny, int nz, double rx, double ry,
-                                        double rz, int perx, int pery)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-    if (i >= nx || j >= ny) return;
-    const double bv = b[i + j * b1 + k * b2];
-    const long long out = (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
-    if (!phi) {
-        r[out] = bv;
-        return;
-    }
-    const int im = i == 0 ? (perx ? nx - 1 : 0) : i - 1, ip = i == nx - 1 ? (perx ? 0 : nx - 1) : i + 1;
-    const int jm = j == 0 ? (pery ? ny - 1 : 0) : j - 1, jp = j == ny - 1 ? (pery ? 0 : ny - 1) : j + 1;
-    const int km = k == 0 ? 0 : k - 1, kp = k == nz - 1 ? nz - 1 : k + 1;
-#define P(a, c, e) phi[(a) + (c) * p1 + (long long)(e) * p2]
-    const double c0 = P(i, j, k);
-    const double lap = (((P(im, j, k) + P(ip, j, k)) - 2.0 * c0) * rx + ((P(i, jm, k) + P(i, jp, k)) - 2.0 * c0) * ry) +
-                       ((P(i, j, km) + P(i, j, kp)) - 2.0 * c0) * rz;
-#undef P
-    r[out] = bv - lap;
-}
-
-// The mean of phi from the 256 fixed-order partial sums (folded in index order), on the device.
-__global__ void poisson_mean_kernel(const double *__restrict__ final256, double count, double *__restrict__ mean)
-{
-    double acc = 0.0;
-    for (int f = 0; f < kReduceFinal; ++f) acc = acc + final256[f];
-    *mean = acc / count;
-}
-
-__global__ void poisson_subtract_kernel(double *__restrict__ phi, long long p1, long long p2, int nx, int ny,
-                                        const double *__restrict__ mean)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-    if (i >= nx || j >= ny) return;
-    double &v = phi[i + j * p1 + (long long)k * p2];
-    v = v - *mean;
-}
-
-// One FP64 Thomas solve per (p, q) column of the transformed right-hand side (p fastest, so a wavefront
-// reads 64 consecutive values of each z plane). The Thomas coefficients of each column live in LDS.
-template <class T>
-__global__ void tridiag_z_kernel(const T *__restrict__ in, T *__restrict__ out, int nx, int ny, int nz,
-                                 const double *__restrict__ lx, const double *__restrict__ ly, double rz, int cols)
-{
-    extern __shared__ double lds[];
-    double *cp = lds, *dp = lds + (long long)nz * cols;
-    const int c = threadIdx.x;
-    const long long col = (long long)blockIdx.x * cols + c;
-    const long long plane = (long long)nx * ny;
-    if (col >= plane) return;
-    const double lam = lx[col % nx] + ly[col / nx];
-#define IN(k) ((double)in[col + (long long)(k) * plane])
-#define OUT(k, v) (out[col + (long long)(k) * plane] = (T)(v))
-    if (lam == 0.0) {
-        // The constant null mode: the solution of zero mean. psi_0 = 0 fixes the constant, the first
-        // nz-1 equations give the rest, then the mean is removed (the last equation holds when the
-        // right-hand side has zero mean).
-        double prev = 0.0, cur = 0.0, sum = 0.0;
-        dp[c] = 0.0;
-        if (nz > 1) {
-            cur = prev + IN(0) / rz;
-            dp[(long long)cols + c] = cur;
-            sum = cur;
-            for (int k = 1; k < nz - 1; ++k) {
-                const double next = (IN(k) / rz - prev) + 2.0 * cur;
-                prev = cur;
-                cur = next;
-                dp[(long long)(k + 1) * cols + c] = cur;
-                sum = sum + cur;
-            }
-        }
-        const double mean = sum / nz;
-        for (int k = 0; k < nz; ++k) OUT(k, dp[(long long)k * cols + c] - mean);
-        return;
-    }
-    double cprev = 0.0, dprev = 0.0;
-    for (int k = 0; k < nz; ++k) {
-        double diag = -2.0 * rz + lam;
-        if (k == 0) diag = diag + rz;
-        if (k == nz - 1) diag = diag + rz;
-        const double m = 1.0 / (diag - rz * cprev);
-        cprev = rz * m;
-        dprev = (IN(k) - rz * dprev) * m;
-        cp[(long long)k * cols + c] = cprev;
-        dp[(long long)k * cols + c] = dprev;
-    }
-    double psi = dprev;
-    OUT(nz - 1, psi);
-    for (int k = nz - 2; k >= 0; --k) {
-        psi = dp[(long long)k * cols + c] - cp[(long long)k * cols + c] * psi;
-        OUT(k, psi);
-    }
-#undef IN
-#undef OUT
-}
 
 // Orthonormal eigenvector basis of the 1D second difference on n cells of width h, column m = mode m,
 // and its eigenvalues. periodic: 1/sqrt(n); sqrt(2/n) cos / sin(2 pi f i / n); (-1)^i / sqrt(n) for even n.

@@ -110,6 +110,20 @@ def les_fields(nx: int, ny: int, nz: int, dx: float = 10.0, dy: float = 10.0, dz
     return U, V, W, S
 
 
+def divergence_rhs(nx: int, ny: int, nz: int, h, seed: int = SEED) -> np.ndarray:
+    """A pressure right-hand side: the divergence of a seeded uniform(-1, 1) staggered velocity field
+    (periodic in x and y, w = 0 on the z walls), minus its mean. Returns an (nx, ny, nz) Fortran array."""
+    d = uniform(seed ^ 0xD1F, 3 * nx * ny * nz + nx * ny)
+    u = (2.0 * d[:nx * ny * nz] - 1.0).reshape((nx, ny, nz), order="F")
+    v = (2.0 * d[nx * ny * nz:2 * nx * ny * nz] - 1.0).reshape((nx, ny, nz), order="F")
+    w = np.zeros((nx, ny, nz + 1), order="F")
+    w[:, :, 1:nz] = (2.0 * d[2 * nx * ny * nz:2 * nx * ny * nz + nx * ny * (nz - 1)] - 1.0).reshape(
+        (nx, ny, nz - 1), order="F")
+    div = ((u - np.roll(u, 1, axis=0)) / h[0] + (v - np.roll(v, 1, axis=1)) / h[1]
+           + (w[:, :, 1:] - w[:, :, :-1]) / h[2])
+    return np.asfortranarray(div - div.mean())
+
+
 def ensemble_sizes(count: int = 64, seed: int = SEED):
     """Subdomain sizes of the batched ensemble: nx, ny uniform in [32, 128], nz uniform in [32, 96]."""
     d = uniform(seed ^ 0xE45E, 3 * count)

@@ -52,6 +52,38 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
+def divergence(U, V, W, dx, dy, dz, dt) -> np.ndarray:
+    nx, ny, nz = (d - 2 for d in U.shape)
+    rhs = np.zeros((nx, ny, nz), order="F")
+    fn = lib().orc_divergence
+    fn.restype = None
+    fn.argtypes = [_V, _V, _V, _V, _D, _D, _D, _D, _I, _I, _I]
+    fn(ptr(rhs), ptr(U), ptr(V), ptr(W), dx, dy, dz, dt, nx, ny, nz)
+    return rhs
+
+
+def projection(U, V, W, phi, dx, dy, dz, dt) -> None:
+    """In place: U -= dt grad(phi) on the interior faces, then the halos (oracle/step.c)."""
+    nx, ny, nz = (d - 2 for d in U.shape)
+    phi = np.asfortranarray(phi)
+    fn = lib().orc_projection
+    fn.restype = None
+    fn.argtypes = [_V, _V, _V, _V, _D, _D, _D, _D, _I, _I, _I]
+    fn(ptr(U), ptr(V), ptr(W), ptr(phi), dx, dy, dz, dt, nx, ny, nz)
+
+
+def projected_step(U, V, W, S, dx, dy, dz, dt, cs):
+    """One step of the ensemble: explicit LES step, divergence, FP64 pressure (oracle/poisson.py),
+    projection. Returns (U, V, W, S, phi)."""
+    from oracle import poisson
+
+    Un, Vn, Wn, Sn = les_step(U, V, W, S, dx, dy, dz, dt, cs)
+    rhs = divergence(Un, Vn, Wn, dx, dy, dz, dt)
+    phi = np.asfortranarray(poisson.solve(rhs, (dx, dy, dz)))
+    projection(Un, Vn, Wn, phi, dx, dy, dz, dt)
+    return Un, Vn, Wn, Sn, phi
+
+
 def fixed_order_reduce(x: np.ndarray, lo, hi, op: int, initial: float) -> float:
     """The fixed-order reduction of the box [lo, hi) of a rank-3 Fortran array (oracle/reduce.c)."""
     l = lib()

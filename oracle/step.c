@@ -73,3 +73,50 @@ void orc_les_step(double *Un, double *Vn, double *Wn, double *Sn, const double *
     orc_halo_les(Sn, 2, nx, ny, nz, e1, e2, e3);
     free(T);
 }
+
+/* The projection half of the step (ELMM's prepoisson and postpoisson, benchmarks/elmm/elmm.toml:212-217,
+ * whose sources are not in the reference; written from the FD2 operators of the pressure provider,
+ * benchmarks/elmm/sources/gpu_pressure.cpp:51-69):
+ *   rhs(c) = (((U(i) - U(i-1))/dx + (V(j) - V(j-1))/dy) + (W(k) - W(k-1))/dz) / dt   at every cell,
+ *   U(i) -= (dt/dx) (phi(c+1) - phi(c)) on x faces (periodic), likewise V; W on the interior z faces,
+ * then the halo fill of U, V (tangential) and W (wall-normal). rhs and phi are dense nx*ny*nz arrays.
+ * Pinned by: the divergence of a projected field with an exact pressure is zero to rounding (div grad is
+ * the 7-point Laplacian of the pressure solve), tests/test_oracle.py. */
+void orc_divergence(double *rhs, const double *U, const double *V, const double *W, double dx, double dy,
+                    double dz, double dt, int nx, int ny, int nz)
+{
+    int e1 = nx + 2, e2 = ny + 2;
+    double rdx = 1.0 / dx, rdy = 1.0 / dy, rdz = 1.0 / dz, rdt = 1.0 / dt;
+#pragma omp parallel for
+    for (int c = 1; c <= nz; ++c)
+        for (int b = 1; b <= ny; ++b)
+            for (int a = 1; a <= nx; ++a) {
+                size_t p = ORC_IX(a + 1, b + 1, c + 1);
+                size_t q = (size_t)(a - 1) + (size_t)nx * ((size_t)(b - 1) + (size_t)ny * (size_t)(c - 1));
+                rhs[q] = (((U[p] - U[p - 1]) * rdx + (V[p] - V[p - e1]) * rdy) +
+                          (W[p] - W[p - (size_t)e1 * e2]) * rdz) * rdt;
+            }
+}
+
+void orc_projection(double *U, double *V, double *W, const double *phi, double dx, double dy, double dz, double dt,
+                    int nx, int ny, int nz)
+{
+    int e1 = nx + 2, e2 = ny + 2, e3 = nz + 2;
+    double gx = dt / dx, gy = dt / dy, gz = dt / dz;
+    size_t plane = (size_t)nx * ny;
+#pragma omp parallel for
+    for (int c = 1; c <= nz; ++c)
+        for (int b = 1; b <= ny; ++b)
+            for (int a = 1; a <= nx; ++a) {
+                size_t p = ORC_IX(a + 1, b + 1, c + 1);
+                size_t q = (size_t)(a - 1) + (size_t)nx * (size_t)(b - 1) + plane * (size_t)(c - 1);
+                size_t qx = (size_t)(a % nx) + (size_t)nx * (size_t)(b - 1) + plane * (size_t)(c - 1);
+                size_t qy = (size_t)(a - 1) + (size_t)nx * (size_t)(b % ny) + plane * (size_t)(c - 1);
+                U[p] = U[p] - gx * (phi[qx] - phi[q]);
+                V[p] = V[p] - gy * (phi[qy] - phi[q]);
+                if (c < nz) W[p] = W[p] - gz * (phi[q + plane] - phi[q]);
+            }
+    orc_halo_les(U, 0, nx, ny, nz, e1, e2, e3);
+    orc_halo_les(V, 0, nx, ny, nz, e1, e2, e3);
+    orc_halo_les(W, 1, nx, ny, nz, e1, e2, e3);
+}

@@ -1,0 +1,106 @@
+"""The projected LES step and the batched ensemble.
+
+CPU: the oracle's projection leaves a divergence-free field (div grad is exactly the pressure solve's
+7-point Laplacian), which pins the divergence and projection restatements.
+
+GPU: the per-member FP64 ensemble against the oracle (LES step bit-exact, pressure within the direct
+solve's rounding, so fields within 1e-12 of their scale); the batched FP64 ensemble bit for bit equal to
+the per-member one (same kernels, same order); the mixed batched ensemble within 1e-10 of the FP64 one
+(the north-star field error bound); the autotuner's cache round trip.
+"""
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+
+SIZES = [(40, 36, 12), (33, 47, 9), (64, 32, 16), (8, 6, 4)]
+D, DT, CS = (10.0, 10.0, 10.0), 0.2, 0.17
+
+
+def member_fields(fields, n, seed):
+    return fields.les_fields(*n, *D, seed=seed)
+
+
+def test_projected_oracle_step_is_divergence_free(oracle, fields):
+    U, V, W, S = member_fields(fields, (24, 20, 10), 11)
+    Un, Vn, Wn, Sn, phi = oracle.projected_step(U, V, W, S, *D, DT, CS)
+    before = oracle.divergence(*oracle.les_step(U, V, W, S, *D, DT, CS)[:3], *D, DT)
+    after = oracle.divergence(Un, Vn, Wn, *D, DT)
+    assert np.abs(after).max() <= 1e-10 * np.abs(before).max()
+    assert abs(phi.mean()) <= 1e-12 * np.abs(phi).max()
+
+
+def run(hip, ens_mod, fields, mode, steps, tune=None, solves=None):
+    e = ens_mod.Ensemble(hip, SIZES, D, DT, CS, mode)
+    try:
+        if solves:
+            e.solves(solves)
+        if tune is not None:
+            e.autotune(tune)
+        for m, n in enumerate(SIZES):
+            e.upload(m, *member_fields(fields, n, 100 + m))
+        e.step(steps)
+        e.synchronize()
+        out = [e.download(m) + [e.pressure(m)] for m in range(len(SIZES))]
+        info = e.info()
+    finally:
+        e.destroy()
+    return out, info
+
+
+@pytest.fixture(scope="module")
+def ens_mod(ftc):
+    import __graft_entry__ as graft
+
+    return graft._module("ensemble")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("steps", [1, 2])
+def test_fp64_members_match_the_oracle(hip, ens_mod, oracle, fields, steps):
+    got, info = run(hip, ens_mod, fields, ens_mod.FP64_PER_MEMBER, steps)
+    assert info.launches_per_step == 8 * len(SIZES)
+    for m, n in enumerate(SIZES):
+        state = list(member_fields(fields, n, 100 + m))
+        for _ in range(steps):
+            *state, phi = oracle.projected_step(*state, *D, DT, CS)
+        for name, g, w in zip("UVWS", got[m][:4], state):
+            scale = np.abs(w).max()
+            assert np.abs(g - w).max() <= 1e-12 * scale, (m, name, np.abs(g - w).max() / scale)
+        assert np.abs(got[m][4] - phi).max() <= 1e-11 * np.abs(phi).max()
+
+
+@pytest.mark.gpu
+def test_batched_fp64_equals_per_member_fp64_bit_for_bit(hip, ens_mod, fields):
+    ref, _ = run(hip, ens_mod, fields, ens_mod.FP64_PER_MEMBER, 3)
+    got, info = run(hip, ens_mod, fields, ens_mod.FP64_BATCHED, 3)
+    assert info.launches_per_step == 8
+    for m in range(len(SIZES)):
+        for g, w in zip(got[m], ref[m]):
+            assert bits_equal(g, w)
+
+
+@pytest.mark.gpu
+def test_mixed_batched_is_within_1e10_of_fp64(hip, ens_mod, fields):
+    ref, _ = run(hip, ens_mod, fields, ens_mod.FP64_PER_MEMBER, 2)
+    got, info = run(hip, ens_mod, fields, ens_mod.MIXED_BATCHED, 2)
+    assert info.solves == 3
+    for m in range(len(SIZES)):
+        for name, g, w in zip("UVWSP", got[m], ref[m]):
+            err = np.abs(g - w).max() / np.abs(w).max()
+            assert err <= 1e-10, (m, name, err)
+
+
+@pytest.mark.gpu
+def test_autotuned_ensemble_reuses_its_cache_and_keeps_fp64_results(hip, ens_mod, fields, tmp_path):
+    cache = str(tmp_path / "tune.tsv")
+    ref, _ = run(hip, ens_mod, fields, ens_mod.FP64_BATCHED, 2)
+    got, info = run(hip, ens_mod, fields, ens_mod.FP64_BATCHED, 2, tune=cache)
+    text = open(cache).read()
+    assert "mode=1" in text
+    again, info2 = run(hip, ens_mod, fields, ens_mod.FP64_BATCHED, 2, tune=cache)
+    assert (info2.les_kchunk, info2.gemm_tile, info2.blocks_per_cu) == (info.les_kchunk, info.gemm_tile,
+                                                                         info.blocks_per_cu)
+    for m in range(len(SIZES)):
+        for g, w, a in zip(got[m], ref[m], again[m]):
+            assert bits_equal(g, w) and bits_equal(a, w)

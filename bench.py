@@ -137,6 +137,54 @@ def chain_section(ftc, scope, lib, n=4096, reps=3):
     return out, results, (A, B, C)
 
 
+def ensemble_section(lib, fields, count=64, steps=10, tune_cache=None):
+    """64 uneven subdomains (nx, ny in [32, 128], nz in [32, 96] from the seed), one projected LES step each:
+    per-member plain FP64 launches vs batched FP64 vs batched mixed (autotuned), all graph-free or
+    one graph per step as the mode says; plus the mixed-vs-FP64 field error after two steps."""
+    ens = graft._module("ensemble")
+    sizes = fields.ensemble_sizes(count)
+    d, dt, cs = (10.0, 10.0, 10.0), 0.2, 0.17
+    init = [fields.les_fields(*n, *d, seed=1000 + m) for m, n in enumerate(sizes)]
+    out = {"members": count, "cells": int(sum(a * b * c for a, b, c in sizes))}
+    finals = {}
+    for name, mode in (("fp64_per_member", ens.FP64_PER_MEMBER), ("fp64_batched", ens.FP64_BATCHED),
+                       ("mixed_batched", ens.MIXED_BATCHED)):
+        e = ens.Ensemble(lib, sizes, d, dt, cs, mode)
+        if mode != ens.FP64_PER_MEMBER:
+            e.autotune(tune_cache)
+        for m in range(count):
+            e.upload(m, *init[m])
+        e.step(2)
+        e.synchronize()
+        finals[name] = [e.download(m)[:3] for m in range(count)]
+        stream = e.stream()
+        ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+        lib.ftc_event_create(ctypes.byref(ev0))
+        lib.ftc_event_create(ctypes.byref(ev1))
+        e.step(1)
+        e.synchronize()
+        lib.ftc_event_record(ev0, stream)
+        e.step(steps)
+        lib.ftc_event_record(ev1, stream)
+        ms = ctypes.c_float()
+        lib.ftc_event_elapsed_ms(ev0, ev1, ctypes.byref(ms))
+        info = e.info()
+        e.destroy()
+        out[name] = {"ms_per_step": round(ms.value / steps, 4), "launches_per_step": info.launches_per_step,
+                     "les_kchunk": info.les_kchunk, "gemm_tile": info.gemm_tile,
+                     "blocks_per_cu": info.blocks_per_cu, "solves": info.solves}
+        out["algorithmic_bytes_per_step"] = info.algorithmic_bytes
+    base = out["fp64_per_member"]["ms_per_step"]
+    out["speedup_mixed_batched_over_fp64_per_member"] = round(base / out["mixed_batched"]["ms_per_step"], 3)
+    out["speedup_fp64_batched_over_fp64_per_member"] = round(base / out["fp64_batched"]["ms_per_step"], 3)
+    err = 0.0
+    for m in range(count):
+        for g, w in zip(finals["mixed_batched"][m], finals["fp64_per_member"][m]):
+            err = max(err, float(np.abs(g - w).max() / np.abs(w).max()))
+    out["max_rel_field_error_mixed_vs_fp64_after_2_steps"] = err
+    return out
+
+
 def main() -> int:
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -232,6 +280,8 @@ def main() -> int:
         extras["poisson"], poisson_sols, poisson_rhs = poisson_section(ftc, scope, lib, fields)
         log("bench: chain contraction 4096")
         extras["chain_contraction"], chain_results, chain_inputs = chain_section(ftc, scope, lib)
+        log("bench: 64-member ensemble")
+        extras["ensemble"] = ensemble_section(lib, fields)
 
     cpu = None
     parity = None

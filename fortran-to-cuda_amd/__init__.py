@@ -41,7 +41,7 @@ class GemmDesc(ctypes.Structure):
                 ("type_c", c_int), ("m", c_int), ("n", c_int), ("k", c_int), ("batch", c_int),
                 ("lda", ctypes.c_longlong), ("ldb", ctypes.c_longlong), ("ldc", ctypes.c_longlong),
                 ("stride_a", ctypes.c_longlong), ("stride_b", ctypes.c_longlong), ("stride_c", ctypes.c_longlong),
-                ("alpha", c_double), ("beta", c_double)]
+                ("alpha", c_double), ("beta", c_double), ("flags", c_int)]
 HELPER_SIGNATURES = {
     "ftc_les_coefficients": (None, [c_double, c_double, c_double, c_double, _DP]),
     "ftc_event_create": (c_int, [POINTER(c_void_p)]),

@@ -177,7 +177,8 @@ __global__ __launch_bounds__(256) void ens_subtract_kernel(const EnsMember *ms, 
     }
 }
 
-template <class T>
+// QFAST: the mixed chain keeps mode (p, q) at column p*ny + q, the FP64 chain at p + q*nx.
+template <class T, bool QFAST>
 __global__ __launch_bounds__(kCols) void ens_tridiag_kernel(const EnsMember *ms, const ColItem *items, int n)
 {
     extern __shared__ double lds[];
@@ -187,9 +188,9 @@ __global__ __launch_bounds__(kCols) void ens_tridiag_kernel(const EnsMember *ms,
         const long long plane = (long long)m.nx * m.ny;
         const long long col = (long long)t.block * kCols + threadIdx.x;
         if (col >= plane) continue;
-        tridiag_column<T>(static_cast<const T *>(m.w1), static_cast<T *>(m.w0), plane, m.nz,
-                          m.lx[col % m.nx] + m.ly[col / m.nx], m.rz, lds, lds + (long long)m.nz * kCols,
-                          threadIdx.x, kCols, col);
+        const double lam = QFAST ? m.lx[col / m.ny] + m.ly[col % m.ny] : m.lx[col % m.nx] + m.ly[col / m.nx];
+        tridiag_column<T>(static_cast<const T *>(m.w1), static_cast<T *>(m.w0), plane, m.nz, lam, m.rz, lds,
+                          lds + (long long)m.nz * kCols, threadIdx.x, kCols, col);
     }
 }
 
@@ -204,13 +205,13 @@ __global__ __launch_bounds__(kGemmThreads) void ens_gemm_f64_kernel(const GemmCa
     }
 }
 
-template <class TA, class TB, class TO, int BM, int BN>
+template <class TA, class TB, class TO, int BM, int BN, bool TOUT>
 __global__ __launch_bounds__(kGemmThreads) void ens_gemm_bf16_kernel(const GemmCall *calls, const GemmItem *items, int n)
 {
     __shared__ Bf16Lds<BM, BN> L;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const GemmItem t = items[it];
-        gemm_bf16x3_tile<TA, TB, TO, BM, BN, 2, 2>(calls[t.call], L, t.mt, t.nt, t.b);
+        gemm_bf16x3_tn_tile<TA, TB, TO, BM, BN, TOUT>(calls[t.call], L, t.mt, t.nt, t.b);
         __syncthreads();
     }
 }
@@ -289,16 +290,26 @@ static void member_calls(const ftc_ensemble &e, const EnsMember &m, const double
 {
     const int nx = m.nx, ny = m.ny, nz = m.nz;
     const long long plane = (long long)nx * ny;
-    const bool mx = mixed(e);
-    const int wt = mx ? kFloat : kDouble;
-    const void *qxt = mx ? (const void *)QxTf : (const void *)QxT, *qy = mx ? (const void *)Qyf : (const void *)Qy;
-    const void *qyt = mx ? (const void *)QyTf : (const void *)QyT, *qx = mx ? (const void *)Qxf : (const void *)Qx;
-    out[0] = GemmCall{qxt, m.rhs, m.w0, wt, kDouble, wt, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0};
-    out[1] = GemmCall{m.w0, qy, m.w1, wt, wt, wt, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
-    out[2] = GemmCall{m.w0, qyt, m.w1, wt, wt, wt, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
-    out[3] = GemmCall{qx, m.w1, m.phi, wt, wt, kDouble, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0};
-    out[4] = GemmCall{qxt, m.res, m.w0, wt, kDouble, wt, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0};
-    out[5] = GemmCall{qx, m.w1, m.phi, wt, wt, kDouble, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 1.0};
+    if (mixed(e)) {
+        // Every operand K-contiguous (the pressure provider's split-bf16 chain): W0 and W1 keep the mode
+        // index q fastest between G1 and G3.
+        const int ak = kGemmAKContiguous, at = kGemmAKContiguous | kGemmTransposedC;
+        const int f = kFloat, d = kDouble;
+        out[0] = GemmCall{Qxf, m.rhs, m.w0, f, d, f, nx, ny, nx, nz, nx, nx, ny, 0, plane, plane, 1.0, 0.0, at};
+        out[1] = GemmCall{m.w0, Qyf, m.w1, f, f, f, nx, ny, ny, nz, ny, ny, ny, plane, 0, plane, 1.0, 0.0, at};
+        out[2] = GemmCall{m.w0, QyTf, m.w1, f, f, f, nx, ny, ny, nz, ny, ny, nx, plane, 0, plane, 1.0, 0.0, ak};
+        out[3] = GemmCall{QxTf, m.w1, m.phi, f, f, d, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0, ak};
+        out[4] = GemmCall{Qxf, m.res, m.w0, f, d, f, nx, ny, nx, nz, nx, nx, ny, 0, plane, plane, 1.0, 0.0, at};
+        out[5] = GemmCall{QxTf, m.w1, m.phi, f, f, d, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 1.0, ak};
+        return;
+    }
+    const int d = kDouble;
+    out[0] = GemmCall{QxT, m.rhs, m.w0, d, d, d, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0};
+    out[1] = GemmCall{m.w0, Qy, m.w1, d, d, d, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
+    out[2] = GemmCall{m.w0, QyT, m.w1, d, d, d, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
+    out[3] = GemmCall{Qx, m.w1, m.phi, d, d, d, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0};
+    out[4] = GemmCall{QxT, m.res, m.w0, d, d, d, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0};
+    out[5] = GemmCall{Qx, m.w1, m.phi, d, d, d, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 1.0};
 }
 
 static int tile_bm(int tile) { return tile == 0 ? 64 : 32; }
@@ -435,10 +446,10 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
         n = pick(e.r_cols, e.n_cols, &start);
         const std::size_t lds = 2 * (std::size_t)e.nzmax * kCols * sizeof(double);
         if (n) {
-            if (mx) hipLaunchKernelGGL(ens_tridiag_kernel<float>, dim3(all ? grid_for(e, n) : n), dim3(kCols), lds, s,
-                                       e.d_members, e.d_cols + start, n);
-            else hipLaunchKernelGGL(ens_tridiag_kernel<double>, dim3(all ? grid_for(e, n) : n), dim3(kCols), lds, s,
-                                    e.d_members, e.d_cols + start, n);
+            if (mx) hipLaunchKernelGGL((ens_tridiag_kernel<float, true>), dim3(all ? grid_for(e, n) : n), dim3(kCols),
+                                       lds, s, e.d_members, e.d_cols + start, n);
+            else hipLaunchKernelGGL((ens_tridiag_kernel<double, false>), dim3(all ? grid_for(e, n) : n), dim3(kCols),
+                                    lds, s, e.d_members, e.d_cols + start, n);
         }
         break;
     }
@@ -454,14 +465,17 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
             if (big) hipLaunchKernelGGL((ens_gemm_f64_kernel<64, 64>), grid, block, 0, s, e.d_calls, items, n);
             else hipLaunchKernelGGL((ens_gemm_f64_kernel<32, 32>), grid, block, 0, s, e.d_calls, items, n);
         } else if (slot == 0 || slot == 4) {
-            if (big) hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, double, float, 64, 64>), grid, block, 0, s, e.d_calls, items, n);
-            else hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, double, float, 32, 32>), grid, block, 0, s, e.d_calls, items, n);
-        } else if (slot == 1 || slot == 2) {
-            if (big) hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, float, 64, 64>), grid, block, 0, s, e.d_calls, items, n);
-            else hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, float, 32, 32>), grid, block, 0, s, e.d_calls, items, n);
+            if (big) hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, double, float, 64, 64, true>), grid, block, 0, s, e.d_calls, items, n);
+            else hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, double, float, 32, 32, true>), grid, block, 0, s, e.d_calls, items, n);
+        } else if (slot == 1) {
+            if (big) hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, float, 64, 64, true>), grid, block, 0, s, e.d_calls, items, n);
+            else hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, float, 32, 32, true>), grid, block, 0, s, e.d_calls, items, n);
+        } else if (slot == 2) {
+            if (big) hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, float, 64, 64, false>), grid, block, 0, s, e.d_calls, items, n);
+            else hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, float, 32, 32, false>), grid, block, 0, s, e.d_calls, items, n);
         } else {
-            if (big) hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, double, 64, 64>), grid, block, 0, s, e.d_calls, items, n);
-            else hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, double, 32, 32>), grid, block, 0, s, e.d_calls, items, n);
+            if (big) hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, double, 64, 64, false>), grid, block, 0, s, e.d_calls, items, n);
+            else hipLaunchKernelGGL((ens_gemm_bf16_kernel<float, float, double, 32, 32, false>), grid, block, 0, s, e.d_calls, items, n);
         }
     }
     }

@@ -46,6 +46,16 @@ static const void *bf16_by_tile(int tile)
     }
 }
 
+template <class TA, class TB, class TO, bool TOUT>
+static const void *tn_by_tile(int tile)
+{
+    switch (tile) {
+    case 0: return reinterpret_cast<const void *>(&gemm_bf16x3_tn_kernel<TA, TB, TO, 128, 128, TOUT>);
+    case 1: return reinterpret_cast<const void *>(&gemm_bf16x3_tn_kernel<TA, TB, TO, 64, 64, TOUT>);
+    default: return reinterpret_cast<const void *>(&gemm_bf16x3_tn_kernel<TA, TB, TO, 32, 32, TOUT>);
+    }
+}
+
 int gemm_default_tile(const GemmCall &c, int precision)
 {
     // The largest tile that still yields about two workgroups per CU of a 256-CU device; small
@@ -65,12 +75,26 @@ static int select_kernel(const GemmCall &c, int precision, int tile, const void 
     const Tile &T = kTiles[tile];
     *grid = dim3((c.M + T.bm - 1) / T.bm, (c.N + T.bn - 1) / T.bn, c.batch);
     if (precision == kF64) {
-        if (c.typeA != kDouble || c.typeB != kDouble || c.typeC != kDouble)
-            return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "FP64 contraction takes double operands");
+        if (c.typeA != kDouble || c.typeB != kDouble || c.typeC != kDouble || c.flags)
+            return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "FP64 contraction takes plain double operands");
         *fn = tile == 0 ? f64_kernel<128, 128>() : (tile == 1 ? f64_kernel<64, 64>() : f64_kernel<32, 32>());
         return FORT_SCOPE_OK;
     }
     if (precision != kBf16x3) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unknown contraction precision");
+    if (c.flags & kGemmAKContiguous) {
+        const bool tout = (c.flags & kGemmTransposedC) != 0;
+        const int combo = c.typeA * 8 + c.typeB * 4 + c.typeC * 2 + (tout ? 1 : 0);
+        switch (combo) {
+        case kFloat * 8 + kDouble * 4 + kFloat * 2 + 1: *fn = tn_by_tile<float, double, float, true>(tile); break;
+        case kFloat * 8 + kFloat * 4 + kFloat * 2 + 1: *fn = tn_by_tile<float, float, float, true>(tile); break;
+        case kFloat * 8 + kFloat * 4 + kFloat * 2 + 0: *fn = tn_by_tile<float, float, float, false>(tile); break;
+        case kFloat * 8 + kFloat * 4 + kDouble * 2 + 0: *fn = tn_by_tile<float, float, double, false>(tile); break;
+        case kDouble * 8 + kDouble * 4 + kDouble * 2 + 0: *fn = tn_by_tile<double, double, double, false>(tile); break;
+        default: return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unsupported K-contiguous operand types");
+        }
+        return FORT_SCOPE_OK;
+    }
+    if (c.flags) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "a transposed C needs a K-contiguous A");
     // The operand type combinations the chain contraction and the spectral transforms use.
     const int combo = c.typeA * 4 + c.typeB * 2 + c.typeC;
     switch (combo) {
@@ -89,7 +113,11 @@ static int validate(const GemmCall &c)
     if (c.M < 0 || c.N < 0 || c.K < 0 || c.batch < 0)
         return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "negative contraction extent");
     if (c.batch > 65535) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "batch above 65535");
-    if ((c.M > 0 && c.lda < c.M) || (c.K > 0 && c.ldb < c.K) || (c.M > 0 && c.ldc < c.M))
+    if (c.flags & ~(kGemmAKContiguous | kGemmTransposedC))
+        return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unknown contraction layout flags");
+    const long long amin = (c.flags & kGemmAKContiguous) ? c.K : c.M;
+    const long long cmin = (c.flags & kGemmTransposedC) ? c.N : c.M;
+    if ((amin > 0 && c.lda < amin) || (c.K > 0 && c.ldb < c.K) || (cmin > 0 && c.ldc < cmin))
         return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "leading dimension below the row count");
     if ((long long)(c.N + 31) / 32 > 65535)
         return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "too many column tiles");
@@ -160,7 +188,7 @@ int ftc_gemm_device(const ftc_gemm_desc *d, int precision, int tile, void *strea
 {
     if (!d) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "missing contraction descriptor");
     GemmCall c{d->a, d->b, d->c, d->type_a, d->type_b, d->type_c, d->m, d->n, d->k, d->batch, d->lda, d->ldb,
-               d->ldc, d->stride_a, d->stride_b, d->stride_c, d->alpha, d->beta};
+               d->ldc, d->stride_a, d->stride_b, d->stride_c, d->alpha, d->beta, d->flags};
     return gemm_launch(c, precision, static_cast<hipStream_t>(stream), tile);
 }
 

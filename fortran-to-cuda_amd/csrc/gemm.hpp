@@ -19,6 +19,9 @@ namespace ftc {
 
 enum GemmPrecision : int { kF64 = 0, kBf16x3 = 1 };
 enum GemmType : int { kDouble = 0, kFloat = 1 };
+// Layout flags (split-bf16 only): A stored K-contiguous, A(m, k) = A[m*lda + k]; C written transposed,
+// C(m, n) = C[m*ldc + n]. With both operands K-contiguous every tile row is one vector load.
+enum GemmFlags : int { kGemmAKContiguous = 1, kGemmTransposedC = 2 };
 
 struct GemmCall {
     const void *A;
@@ -29,6 +32,7 @@ struct GemmCall {
     long long lda, ldb, ldc;
     long long sa, sb, sc;
     double alpha, beta;
+    int flags;
 };
 
 // Launch geometry chosen for a call (block tile and grid); `tile` indexes the compiled tile shapes.

@@ -221,4 +221,147 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_bf16x3_kernel(const GemmCal
     gemm_bf16x3_tile<TA, TB, TO, BM, BN, WM, WN>(g, L, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// ------------------------------------------------------------------------------------------------ bf16x3, K-contiguous
+// Split-bf16 tile with both operands K-contiguous: A(m, k) = A[m*lda + k], B(k, n) = B[k + n*ldb]. Four
+// consecutive threads read one 32-element K run of a row (one or two cache lines), split it in registers
+// and store 16-byte hi and lo vectors straight into the MFMA fragment layout. TOUT writes C transposed.
+template <class T>
+__device__ __forceinline__ void split8(const T (&v)[8], bf16x8 &hi, bf16x8 &lo)
+{
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float f = (float)v[q];
+        const __bf16 h = (__bf16)f;
+        hi[q] = h;
+        lo[q] = (__bf16)(f - (float)h);
+    }
+}
+
+// Eight consecutive elements src[0..valid) (zeros past `valid` or when !ok); one 32- or 64-byte vector
+// read when the run is whole and 16-byte aligned.
+template <class T>
+__device__ __forceinline__ void load8(const T *src, bool ok, int valid, T (&v)[8])
+{
+    if (ok && valid >= 8 && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+        if constexpr (sizeof(T) == 4) {
+            const float4 a = reinterpret_cast<const float4 *>(src)[0], b = reinterpret_cast<const float4 *>(src)[1];
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double2 a = reinterpret_cast<const double2 *>(src)[q];
+                v[2 * q] = a.x;
+                v[2 * q + 1] = a.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = (ok && q < valid) ? src[q] : (T)0;
+    }
+}
+
+template <class TA, class TB, class TO, int BM, int BN, bool TOUT>
+__device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<BM, BN> &L, int mt, int nt, long long b)
+{
+    constexpr int BK = 32, WM = 2, WN = 2;
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int TM = WTM / 16, TN = WTN / 16;
+    constexpr int RA = (BM + 63) / 64, RB = (BN + 63) / 64;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int m0 = mt * BM, n0 = nt * BN;
+    const int row = t >> 2, grp = t & 3;
+    const TA *A = static_cast<const TA *>(g.A) + b * g.sa;
+    const TB *B = static_cast<const TB *>(g.B) + b * g.sb;
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y) acc[x][y] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        const int kb = k0 + 8 * grp;
+        TA ra[RA][8];
+        TB rb[RB][8];
+#pragma unroll
+        for (int e = 0; e < RA; ++e) {
+            const int m = row + 64 * e;
+            const bool mok = m < BM && m0 + m < g.M;
+            load8(A + (long long)(m0 + m) * g.lda + kb, mok, g.K - kb, ra[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < RB; ++e) {
+            const int n = row + 64 * e;
+            const bool nok = n < BN && n0 + n < g.N;
+            load8(B + kb + (long long)(n0 + n) * g.ldb, nok, g.K - kb, rb[e]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < RA; ++e) {
+            const int m = row + 64 * e;
+            if (m < BM) {
+                bf16x8 hi, lo;
+                split8(ra[e], hi, lo);
+                *reinterpret_cast<bf16x8 *>(&L.Ah[m][8 * grp]) = hi;
+                *reinterpret_cast<bf16x8 *>(&L.Al[m][8 * grp]) = lo;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < RB; ++e) {
+            const int n = row + 64 * e;
+            if (n < BN) {
+                bf16x8 hi, lo;
+                split8(rb[e], hi, lo);
+                *reinterpret_cast<bf16x8 *>(&L.Bh[n][8 * grp]) = hi;
+                *reinterpret_cast<bf16x8 *>(&L.Bl[n][8 * grp]) = lo;
+            }
+        }
+        __syncthreads();
+        bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+        for (int x = 0; x < TM; ++x) {
+            const int r = wm * WTM + x * 16 + lane % 16;
+            ah[x] = *reinterpret_cast<const bf16x8 *>(&L.Ah[r][8 * (lane / 16)]);
+            al[x] = *reinterpret_cast<const bf16x8 *>(&L.Al[r][8 * (lane / 16)]);
+        }
+#pragma unroll
+        for (int y = 0; y < TN; ++y) {
+            const int c = wn * WTN + y * 16 + lane % 16;
+            bh[y] = *reinterpret_cast<const bf16x8 *>(&L.Bh[c][8 * (lane / 16)]);
+            bl[y] = *reinterpret_cast<const bf16x8 *>(&L.Bl[c][8 * (lane / 16)]);
+        }
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+            for (int y = 0; y < TN; ++y) {
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+            }
+    }
+    TO *C = static_cast<TO *>(g.C) + b * g.sc;
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = m0 + wm * WTM + x * 16 + 4 * (lane / 16) + r;
+                const int cc = n0 + wn * WTN + y * 16 + lane % 16;
+                if (rr < g.M && cc < g.N) {
+                    const long long ci = TOUT ? (long long)rr * g.ldc + cc : (long long)rr + (long long)cc * g.ldc;
+                    double v = g.alpha * (double)acc[x][y][r];
+                    if (g.beta != 0.0) v = v + g.beta * (double)C[ci];
+                    C[ci] = (TO)v;
+                }
+            }
+}
+
+template <class TA, class TB, class TO, int BM, int BN, bool TOUT>
+__global__ __launch_bounds__(kGemmThreads) void gemm_bf16x3_tn_kernel(const GemmCall g)
+{
+    __shared__ Bf16Lds<BM, BN> L;
+    gemm_bf16x3_tn_tile<TA, TB, TO, BM, BN, TOUT>(g, L, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 }  // namespace ftc

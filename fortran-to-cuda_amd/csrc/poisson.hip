@@ -130,30 +130,40 @@ static int apply_inverse(poisson_solver_handle_v1 &s, const double *in, long lon
     const int nx = s.nx, ny = s.ny, nz = s.nz;
     const long long plane = (long long)nx * ny;
     const int prec = s.mixed ? kBf16x3 : kF64;
-    const int wt = s.mixed ? kFloat : kDouble;
-    const void *QxT = s.mixed ? (const void *)s.QxTf : (const void *)s.QxT;
-    const void *Qy = s.mixed ? (const void *)s.Qyf : (const void *)s.Qy;
-    const void *QyT = s.mixed ? (const void *)s.QyTf : (const void *)s.QyT;
-    const void *Qx = s.mixed ? (const void *)s.Qxf : (const void *)s.Qx;
-    GemmCall g1{QxT, in, s.w0, wt, kDouble, wt, nx, ny, nx, nz, nx, in1, nx, 0, in2, plane, 1.0, 0.0};
-    FTC_CHECK(gemm_launch(g1, prec, stream));
-    GemmCall g2{s.w0, Qy, s.w1, wt, wt, wt, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
-    FTC_CHECK(gemm_launch(g2, prec, stream));
     const unsigned blocks = (unsigned)((plane + s.cols - 1) / s.cols);
     const std::size_t lds = 2 * (std::size_t)nz * s.cols * sizeof(double);
     const double rz = 1.0 / (s.hz * s.hz);
-    if (s.mixed)
+    if (s.mixed) {
+        // Split-bf16 chain with every contraction operand K-contiguous: W0 = (Qx^T f)^T and W1 = W0 Qy keep
+        // the mode index q fastest, so G2 and G3 read rows of W0 as vectors; G3 turns W1 back to i fastest.
+        const int ak = kGemmAKContiguous, at = kGemmAKContiguous | kGemmTransposedC;
+        GemmCall g1{s.Qxf, in, s.w0, kFloat, kDouble, kFloat, nx, ny, nx, nz, nx, in1, ny, 0, in2, plane, 1.0, 0.0, at};
+        FTC_CHECK(gemm_launch(g1, prec, stream));
+        GemmCall g2{s.w0, s.Qyf, s.w1, kFloat, kFloat, kFloat, nx, ny, ny, nz, ny, ny, ny, plane, 0, plane, 1.0, 0.0, at};
+        FTC_CHECK(gemm_launch(g2, prec, stream));
         hipLaunchKernelGGL(tridiag_z_kernel<float>, dim3(blocks), dim3(s.cols), lds, stream,
                            static_cast<const float *>(s.w1), static_cast<float *>(s.w0), nx, ny, nz, s.lx, s.ly, rz,
-                           s.cols);
-    else
-        hipLaunchKernelGGL(tridiag_z_kernel<double>, dim3(blocks), dim3(s.cols), lds, stream,
-                           static_cast<const double *>(s.w1), static_cast<double *>(s.w0), nx, ny, nz, s.lx, s.ly,
-                           rz, s.cols);
+                           s.cols, 1);
+        if (hipGetLastError() != hipSuccess)
+            return fort_scope_report_error(FORT_SCOPE_EXECUTION, "tridiagonal launch failed");
+        GemmCall g3{s.w0, s.QyTf, s.w1, kFloat, kFloat, kFloat, nx, ny, ny, nz, ny, ny, nx, plane, 0, plane, 1.0, 0.0, ak};
+        FTC_CHECK(gemm_launch(g3, prec, stream));
+        GemmCall g4{s.QxTf, s.w1, out.x, kFloat, kFloat, kDouble, nx, ny, nx, nz, nx, nx, out.ld1, 0, plane, out.ld2,
+                    1.0, beta, ak};
+        return gemm_launch(g4, prec, stream);
+    }
+    GemmCall g1{s.QxT, in, s.w0, kDouble, kDouble, kDouble, nx, ny, nx, nz, nx, in1, nx, 0, in2, plane, 1.0, 0.0};
+    FTC_CHECK(gemm_launch(g1, prec, stream));
+    GemmCall g2{s.w0, s.Qy, s.w1, kDouble, kDouble, kDouble, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
+    FTC_CHECK(gemm_launch(g2, prec, stream));
+    hipLaunchKernelGGL(tridiag_z_kernel<double>, dim3(blocks), dim3(s.cols), lds, stream,
+                       static_cast<const double *>(s.w1), static_cast<double *>(s.w0), nx, ny, nz, s.lx, s.ly, rz,
+                       s.cols, 0);
     if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "tridiagonal launch failed");
-    GemmCall g3{s.w0, QyT, s.w1, wt, wt, wt, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
+    GemmCall g3{s.w0, s.QyT, s.w1, kDouble, kDouble, kDouble, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
     FTC_CHECK(gemm_launch(g3, prec, stream));
-    GemmCall g4{Qx, s.w1, out.x, wt, wt, kDouble, nx, ny, nx, nz, nx, nx, out.ld1, 0, plane, out.ld2, 1.0, beta};
+    GemmCall g4{s.Qx, s.w1, out.x, kDouble, kDouble, kDouble, nx, ny, nx, nz, nx, nx, out.ld1, 0, plane, out.ld2, 1.0,
+                beta};
     return gemm_launch(g4, prec, stream);
 }
 

@@ -112,15 +112,17 @@ __device__ __forceinline__ void tridiag_column(const T *__restrict__ in, T *__re
 
 template <class T>
 __global__ void tridiag_z_kernel(const T *__restrict__ in, T *__restrict__ out, int nx, int ny, int nz,
-                                 const double *__restrict__ lx, const double *__restrict__ ly, double rz, int cols)
+                                 const double *__restrict__ lx, const double *__restrict__ ly, double rz, int cols,
+                                 int qfast)
 {
     extern __shared__ double lds[];
     const int c = threadIdx.x;
     const long long col = (long long)blockIdx.x * cols + c;
     const long long plane = (long long)nx * ny;
     if (col >= plane) return;
-    tridiag_column<T>(in, out, plane, nz, lx[col % nx] + ly[col / nx], rz, lds, lds + (long long)nz * cols, c, cols,
-                      col);
+    // Mode (p, q) sits at column p + q*nx, or at p*ny + q when the transforms keep q fastest.
+    const double lam = qfast ? lx[col / ny] + ly[col % ny] : lx[col % nx] + ly[col / nx];
+    tridiag_column<T>(in, out, plane, nz, lam, rz, lds, lds + (long long)nz * cols, c, cols, col);
 }
 
 }  // namespace ftc

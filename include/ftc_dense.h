@@ -15,9 +15,12 @@ extern "C" {
 
 enum ftc_gemm_precision { FTC_GEMM_F64 = 0, FTC_GEMM_BF16X3 = 1 };
 enum ftc_gemm_type { FTC_GEMM_DOUBLE = 0, FTC_GEMM_FLOAT = 1 };
+enum ftc_gemm_flags { FTC_GEMM_A_K_CONTIGUOUS = 1, FTC_GEMM_TRANSPOSED_C = 2 };
 
 /* C_b = alpha * A_b * B_b + beta * C_b for b < batch, column-major, A_b = a + b*stride_a (m x k, lda),
- * B_b = b + b*stride_b (k x n, ldb), C_b = c + b*stride_c (m x n, ldc); beta == 0 does not read C. */
+ * B_b = b + b*stride_b (k x n, ldb), C_b = c + b*stride_c (m x n, ldc); beta == 0 does not read C.
+ * flags (split-bf16 only): FTC_GEMM_A_K_CONTIGUOUS stores A(m, k) at a[m*lda + k];
+ * FTC_GEMM_TRANSPOSED_C writes C(m, n) at c[m*ldc + n] (requires a K-contiguous A). */
 typedef struct ftc_gemm_desc {
     const void *a;
     const void *b;
@@ -27,6 +30,7 @@ typedef struct ftc_gemm_desc {
     long long lda, ldb, ldc;
     long long stride_a, stride_b, stride_c;
     double alpha, beta;
+    int flags;
 } ftc_gemm_desc;
 
 /* Device-pointer contraction on `stream` (a hipStream_t). precision FTC_GEMM_F64: FP64 MFMA, double

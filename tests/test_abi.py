@@ -52,3 +52,24 @@ def test_headers_compile_as_c_and_cxx(tmp_path):
                         str(tmp_path / "probe.o")], check=True)
         subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-x", "c++", "-I", str(INCLUDE), "-c",
                         str(src), "-o", str(tmp_path / "probe2.o")], check=True)
+
+
+def test_contraction_descriptor_is_checked_on_the_host(ftc):
+    # Rejections happen before any device call, so they run without a GPU; the descriptor's layout (with
+    # its trailing flags word) matches ftc_gemm_desc in ftc_dense.h.
+    import ctypes
+
+    lib = ftc.load()
+    assert ctypes.sizeof(ftc.GemmDesc) == 128
+    cases = [(0, (1, 1, 1), 0, b"FP64 contraction takes plain double operands"),
+             (0, (0, 0, 0), 1, b"FP64 contraction takes plain double operands"),
+             (1, (1, 1, 1), 2, b"a transposed C needs a K-contiguous A"),
+             (1, (1, 1, 1), 4, b"unknown contraction layout flags"),
+             (1, (0, 1, 0), 1, b"unsupported K-contiguous operand types")]
+    for precision, (ta, tb, tc), flags, message in cases:
+        d = ftc.GemmDesc(4096, 8192, 12288, ta, tb, tc, 8, 8, 8, 1, 8, 8, 8, 0, 0, 0, 1.0, 0.0, flags)
+        assert lib.ftc_gemm_device(ctypes.byref(d), precision, -1, None) == 1
+        assert lib.fort_scope_error() == message
+    short = ftc.GemmDesc(4096, 8192, 12288, 1, 1, 1, 8, 8, 16, 1, 8, 16, 8, 0, 0, 0, 1.0, 0.0, 1)
+    assert lib.ftc_gemm_device(ctypes.byref(short), 1, -1, None) == 1  # K-contiguous lda below K
+    assert lib.fort_scope_error() == b"leading dimension below the row count"

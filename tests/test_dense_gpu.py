@@ -52,7 +52,8 @@ def test_bf16_mfma_lane_layout(hip):
         assert probe(hip, 1, a, ea, b, eb).sum() == (1.0 if k(a, ea) == k(b, eb) else 0.0)
 
 
-def device_gemm(ftc, scope, hip, A, B, C, precision, batch, tile=-1, alpha=1.0, beta=0.0, m=None, n=None, k=None):
+def device_gemm(ftc, scope, hip, A, B, C, precision, batch, tile=-1, alpha=1.0, beta=0.0, m=None, n=None, k=None,
+                flags=0):
     """Runs ftc_gemm_device on registered arrays (A: lda x K x batch, etc.) and returns C."""
     lda, ldb, ldc = A.shape[0], B.shape[0], C.shape[0]
     m = m or A.shape[0]
@@ -63,8 +64,8 @@ def device_gemm(ftc, scope, hip, A, B, C, precision, batch, tile=-1, alpha=1.0, 
         hs = [dom.bind(X) for X in (A, B, C)]
         binds = []
         keep = []
-        for h, flags in zip(hs, (scope.READ_ALL, scope.READ_ALL, scope.READ_ALL | scope.OVERWRITE_ALL)):
-            acc, kp = scope.make_access(flags)
+        for h, access in zip(hs, (scope.READ_ALL, scope.READ_ALL, scope.READ_ALL | scope.OVERWRITE_ALL)):
+            acc, kp = scope.make_access(access)
             keep.append(kp)
             binds.append(scope.PlanBinding(h, acc))
         arr = (scope.PlanBinding * 3)(*binds)
@@ -72,7 +73,8 @@ def device_gemm(ftc, scope, hip, A, B, C, precision, batch, tile=-1, alpha=1.0, 
         scope.check(hip, hip.fort_scope_access_begin_v1(dom.ctx, arr, 3, 1, devs))
         desc = ftc.GemmDesc(devs[0], devs[1], devs[2], types[A.dtype], types[B.dtype], types[C.dtype], m, n, k,
                             batch, lda, ldb, ldc, A.shape[0] * A.shape[1] if A.ndim == 3 else 0,
-                            B.shape[0] * B.shape[1] if B.ndim == 3 else 0, C.shape[0] * C.shape[1], alpha, beta)
+                            B.shape[0] * B.shape[1] if B.ndim == 3 else 0, C.shape[0] * C.shape[1], alpha, beta,
+                            flags)
         stream = ctypes.c_void_p(dom.stream())
         scope.check(hip, hip.ftc_gemm_device(ctypes.byref(desc), precision, tile, stream))
         for h in hs:
@@ -112,6 +114,43 @@ def test_float_operands_and_outputs(ftc, scope, hip):
     C = device_gemm(ftc, scope, hip, A, B, np.zeros((70, 40), order="F"), 1, 1)
     want = A.astype(np.float64) @ B.astype(np.float64)
     assert np.abs(C - want).max() <= 1e-4 * np.abs(want).max()
+
+
+F, D = np.float32, np.float64
+
+
+@pytest.mark.parametrize("types,tout", [((F, D, F), True), ((F, F, F), True), ((F, F, F), False),
+                                        ((F, F, D), False), ((D, D, D), False)])
+@pytest.mark.parametrize("m,n,k,batch,pad", [(128, 128, 128, 1, 0), (100, 37, 53, 3, 1), (64, 256, 96, 2, 8),
+                                             (33, 70, 200, 1, 3)])
+@pytest.mark.parametrize("tile", [-1, 0, 1, 2])
+def test_k_contiguous_split_bf16(ftc, scope, hip, types, tout, m, n, k, batch, pad, tile):
+    # A stored K-contiguous (A(m, k) at m*lda + k, i.e. the Fortran array At(k, m)); C optionally written
+    # transposed (Ct(n, m)). pad > 0 gives leading dimensions that break the 16-byte vector path.
+    rng = np.random.default_rng(m + 7 * n + 13 * k + pad)
+    At = np.asfortranarray(rng.uniform(-1, 1, (k + pad, m, batch)).astype(types[0]))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k + pad, n, batch)).astype(types[1]))
+    shape = (n + pad, m, batch) if tout else (m + pad, n, batch)
+    C0 = np.asfortranarray(rng.uniform(-1, 1, shape).astype(types[2]))
+    flags = 1 | (2 if tout else 0)
+    C = device_gemm(ftc, scope, hip, At, B, C0.copy(order="F"), 1, batch, tile=tile, alpha=1.25, beta=0.5, m=m, n=n,
+                    k=k, flags=flags)
+    for b in range(batch):
+        want = 1.25 * (At[:k, :, b].astype(D).T @ B[:k, :, b].astype(D))
+        got = C[:n, :, b].T if tout else C[:m, :, b]
+        want = want + 0.5 * (C0[:n, :, b].T if tout else C0[:m, :, b]).astype(D)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        assert err <= 1e-4, (b, err)
+        rest = (C[n:, :, b], C0[n:, :, b]) if tout else (C[m:, :, b], C0[m:, :, b])
+        assert np.array_equal(*rest)  # padding past the logical extent untouched
+
+
+def test_layout_flags_are_checked_before_any_launch(ftc, scope, hip):
+    # Rejected on the host: layout flags with FP64, a transposed C without a K-contiguous A, unknown bits.
+    A = np.zeros((8, 8), order="F")
+    for precision, flags in ((0, 1), (1, 2), (1, 4), (1, 1 | 8)):
+        with pytest.raises(RuntimeError):
+            device_gemm(ftc, scope, hip, A, A.copy(order="F"), A.copy(order="F"), precision, 1, flags=flags)
 
 
 @pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 2e-4)])

@@ -90,7 +90,8 @@ def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 1
     out["speedup_mixed_over_fp64"] = round(out["fp64"]["ms_per_solve"] / out["mixed"]["ms_per_solve"], 3)
     scale = np.abs(sols["fp64"]).max()
     out["max_rel_diff_mixed_vs_fp64"] = float(np.abs(sols["mixed"] - sols["fp64"]).max() / scale)
-    out["transform_flops_fp64"] = 4 * (nx + ny) * N * 2
+    # Four transforms per solve, each 2*N*K flops with K = nx (x transforms) or ny (y transforms).
+    out["transform_flops_fp64"] = 2 * N * (2 * nx + 2 * ny)
     out["fp64_mfma_tflops"] = round(out["transform_flops_fp64"] / (out["fp64"]["ms_per_solve"] * 1e-3) / 1e12, 2)
     out["lower_bound_bytes"] = 8 * 2 * N
     out["rhs_padded_cells"] = P

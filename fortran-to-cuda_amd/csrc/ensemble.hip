@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kCols) void ens_tridiag_kernel(const EnsMember *ms,
         if (col >= plane) continue;
         const double lam = QFAST ? m.lx[col / m.ny] + m.ly[col % m.ny] : m.lx[col % m.nx] + m.ly[col / m.nx];
         tridiag_column<T>(static_cast<const T *>(m.w1), static_cast<T *>(m.w0), plane, m.nz, lam, m.rz, lds,
-                          lds + (long long)m.nz * kCols, threadIdx.x, kCols, col);
+                          threadIdx.x, kCols, col);
     }
 }
 
@@ -444,7 +444,7 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
         break;
     case kTri: {
         n = pick(e.r_cols, e.n_cols, &start);
-        const std::size_t lds = 2 * (std::size_t)e.nzmax * kCols * sizeof(double);
+        const std::size_t lds = 2 * (std::size_t)tridiag_chunks(e.nzmax) * kCols * sizeof(double);
         if (n) {
             if (mx) hipLaunchKernelGGL((ens_tridiag_kernel<float, true>), dim3(all ? grid_for(e, n) : n), dim3(kCols),
                                        lds, s, e.d_members, e.d_cols + start, n);

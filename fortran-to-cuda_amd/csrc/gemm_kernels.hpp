@@ -21,6 +21,20 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kGemmThreads = 256;
 
+// The tile of this workgroup in a one-call launch over grid (mtiles, ntiles, batch). Workgroups are handed
+// to the eight XCDs round-robin in launch order, so consecutive tiles would land in eight different L2s;
+// the remap gives each XCD one contiguous run of the tile order (mt fastest), so the tiles that re-read one
+// operand panel (same nt and b, or same mt and b) run under one L2.
+__device__ __forceinline__ void gemm_block_tile(int &mt, int &nt, long long &b)
+{
+    const unsigned mtiles = gridDim.x, ntiles = gridDim.y, total = mtiles * ntiles * gridDim.z;
+    unsigned L = blockIdx.x + mtiles * (blockIdx.y + ntiles * blockIdx.z);
+    if (total % 8 == 0) L = (L % 8) * (total / 8) + L / 8;
+    mt = (int)(L % mtiles);
+    nt = (int)((L / mtiles) % ntiles);
+    b = (long long)(L / (mtiles * ntiles));
+}
+
 // ------------------------------------------------------------------------------------------------ FP64
 template <int BM, int BN>
 struct F64Lds {
@@ -28,13 +42,17 @@ struct F64Lds {
     double Bs[16][BN + 1];
 };
 
-// One BM x BN tile (mt, nt) of batch entry b of an FP64 contraction.
+// One BM x BN tile (mt, nt) of batch entry b of an FP64 contraction. The global loads of K-slice s+1 are
+// issued before the MFMAs of slice s, so their latency hides behind the products. The MFMA operands are
+// swapped (the accumulators hold C^T tiles: lane l, element r is C(l%16, l/16 + 4r)), so sixteen
+// consecutive lanes store sixteen consecutive rows of the column-major C.
 template <int BM, int BN, int WM, int WN>
 __device__ __forceinline__ void gemm_f64_tile(const GemmCall &g, F64Lds<BM, BN> &L, int mt, int nt, long long b)
 {
     constexpr int BK = 16;
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int TM = WTM / 16, TN = WTN / 16;
+    constexpr int NA = BM * BK / kGemmThreads, NB = BN * BK / kGemmThreads;
     static_assert(WM * WN == kGemmThreads / 64, "four waves");
     auto &As = L.As;
     auto &Bs = L.Bs;
@@ -48,32 +66,36 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmCall &g, F64Lds<BM, BN> 
     for (int x = 0; x < TM; ++x)
 #pragma unroll
         for (int y = 0; y < TN; ++y) acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
-        double ra[BM * BK / kGemmThreads], rb[BN * BK / kGemmThreads];
+    double ra[NA], rb[NB];
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int e = 0; e < BM * BK / kGemmThreads; ++e) {
+        for (int e = 0; e < NA; ++e) {
             const int idx = t + e * kGemmThreads, m = idx % BM, k = idx / BM;
             const bool ok = m0 + m < g.M && k0 + k < g.K;
             ra[e] = ok ? A[(long long)(m0 + m) + (long long)(k0 + k) * g.lda] : 0.0;
         }
 #pragma unroll
-        for (int e = 0; e < BN * BK / kGemmThreads; ++e) {
+        for (int e = 0; e < NB; ++e) {
             const int idx = t + e * kGemmThreads, k = idx % BK, n = idx / BK;
             const bool ok = n0 + n < g.N && k0 + k < g.K;
             rb[e] = ok ? B[(long long)(k0 + k) + (long long)(n0 + n) * g.ldb] : 0.0;
         }
+    };
+    load(0);
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < BM * BK / kGemmThreads; ++e) {
+        for (int e = 0; e < NA; ++e) {
             const int idx = t + e * kGemmThreads;
             As[idx / BM][idx % BM] = ra[e];
         }
 #pragma unroll
-        for (int e = 0; e < BN * BK / kGemmThreads; ++e) {
+        for (int e = 0; e < NB; ++e) {
             const int idx = t + e * kGemmThreads;
             Bs[idx % BK][idx / BK] = rb[e];
         }
         __syncthreads();
+        if (k0 + BK < g.K) load(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             double af[TM], bfr[TN];
@@ -85,7 +107,7 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmCall &g, F64Lds<BM, BN> 
             for (int x = 0; x < TM; ++x)
 #pragma unroll
                 for (int y = 0; y < TN; ++y)
-                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bfr[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bfr[y], af[x], acc[x][y], 0, 0, 0);
         }
     }
     double *C = static_cast<double *>(g.C) + b * g.sc;
@@ -95,8 +117,8 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmCall &g, F64Lds<BM, BN> 
         for (int y = 0; y < TN; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * WTM + x * 16 + lane / 16 + 4 * r;
-                const int col = n0 + wn * WTN + y * 16 + lane % 16;
+                const int row = m0 + wm * WTM + x * 16 + lane % 16;
+                const int col = n0 + wn * WTN + y * 16 + lane / 16 + 4 * r;
                 if (row < g.M && col < g.N) {
                     const long long ci = (long long)row + (long long)col * g.ldc;
                     double v = g.alpha * acc[x][y][r];
@@ -110,7 +132,10 @@ template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__(kGemmThreads) void gemm_f64_kernel(const GemmCall g)
 {
     __shared__ F64Lds<BM, BN> L;
-    gemm_f64_tile<BM, BN, WM, WN>(g, L, blockIdx.x, blockIdx.y, blockIdx.z);
+    int mt, nt;
+    long long b;
+    gemm_block_tile(mt, nt, b);
+    gemm_f64_tile<BM, BN, WM, WN>(g, L, mt, nt, b);
 }
 
 // ------------------------------------------------------------------------------------------------ bf16x3
@@ -218,7 +243,10 @@ template <class TA, class TB, class TO, int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__(kGemmThreads) void gemm_bf16x3_kernel(const GemmCall g)
 {
     __shared__ Bf16Lds<BM, BN> L;
-    gemm_bf16x3_tile<TA, TB, TO, BM, BN, WM, WN>(g, L, blockIdx.x, blockIdx.y, blockIdx.z);
+    int mt, nt;
+    long long b;
+    gemm_block_tile(mt, nt, b);
+    gemm_bf16x3_tile<TA, TB, TO, BM, BN, WM, WN>(g, L, mt, nt, b);
 }
 
 // ------------------------------------------------------------------------------------------------ bf16x3, K-contiguous
@@ -279,10 +307,11 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
     for (int x = 0; x < TM; ++x)
 #pragma unroll
         for (int y = 0; y < TN; ++y) acc[x][y] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
+    TA ra[RA][8];
+    TB rb[RB][8];
+    // Global loads of K-slice s+1 are issued before the MFMAs of slice s.
+    auto load = [&](int k0) {
         const int kb = k0 + 8 * grp;
-        TA ra[RA][8];
-        TB rb[RB][8];
 #pragma unroll
         for (int e = 0; e < RA; ++e) {
             const int m = row + 64 * e;
@@ -295,6 +324,9 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
             const bool nok = n < BN && n0 + n < g.N;
             load8(B + kb + (long long)(n0 + n) * g.ldb, nok, g.K - kb, rb[e]);
         }
+    };
+    load(0);
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < RA; ++e) {
@@ -317,6 +349,7 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
             }
         }
         __syncthreads();
+        if (k0 + BK < g.K) load(k0 + BK);
         bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
         for (int x = 0; x < TM; ++x) {
@@ -330,13 +363,22 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
             bh[y] = *reinterpret_cast<const bf16x8 *>(&L.Bh[c][8 * (lane / 16)]);
             bl[y] = *reinterpret_cast<const bf16x8 *>(&L.Bl[c][8 * (lane / 16)]);
         }
+        // TOUT: the accumulator tile is C (lane l holds column l%16), so a row of the transposed output is
+        // contiguous across lanes. Otherwise the operands are swapped and the tile is C^T (lane l holds row
+        // l%16), so a column of the column-major output is.
 #pragma unroll
         for (int x = 0; x < TM; ++x)
 #pragma unroll
             for (int y = 0; y < TN; ++y) {
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+                if constexpr (TOUT) {
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+                } else {
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], al[x], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[y], ah[x], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], ah[x], acc[x][y], 0, 0, 0);
+                }
             }
     }
     TO *C = static_cast<TO *>(g.C) + b * g.sc;
@@ -346,8 +388,8 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
         for (int y = 0; y < TN; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int rr = m0 + wm * WTM + x * 16 + 4 * (lane / 16) + r;
-                const int cc = n0 + wn * WTN + y * 16 + lane % 16;
+                const int rr = m0 + wm * WTM + x * 16 + (TOUT ? 4 * (lane / 16) + r : lane % 16);
+                const int cc = n0 + wn * WTN + y * 16 + (TOUT ? lane % 16 : 4 * (lane / 16) + r);
                 if (rr < g.M && cc < g.N) {
                     const long long ci = TOUT ? (long long)rr * g.ldc + cc : (long long)rr + (long long)cc * g.ldc;
                     double v = g.alpha * (double)acc[x][y][r];
@@ -361,7 +403,10 @@ template <class TA, class TB, class TO, int BM, int BN, bool TOUT>
 __global__ __launch_bounds__(kGemmThreads) void gemm_bf16x3_tn_kernel(const GemmCall g)
 {
     __shared__ Bf16Lds<BM, BN> L;
-    gemm_bf16x3_tn_tile<TA, TB, TO, BM, BN, TOUT>(g, L, blockIdx.x, blockIdx.y, blockIdx.z);
+    int mt, nt;
+    long long b;
+    gemm_block_tile(mt, nt, b);
+    gemm_bf16x3_tn_tile<TA, TB, TO, BM, BN, TOUT>(g, L, mt, nt, b);
 }
 
 }  // namespace ftc

@@ -131,7 +131,7 @@ static int apply_inverse(poisson_solver_handle_v1 &s, const double *in, long lon
     const long long plane = (long long)nx * ny;
     const int prec = s.mixed ? kBf16x3 : kF64;
     const unsigned blocks = (unsigned)((plane + s.cols - 1) / s.cols);
-    const std::size_t lds = 2 * (std::size_t)nz * s.cols * sizeof(double);
+    const std::size_t lds = 2 * (std::size_t)tridiag_chunks(nz) * s.cols * sizeof(double);
     const double rz = 1.0 / (s.hz * s.hz);
     if (s.mixed) {
         // Split-bf16 chain with every contraction operand K-contiguous: W0 = (Qx^T f)^T and W1 = W0 Qy keep
@@ -256,9 +256,8 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
     s->pery = c->boundary_conditions[2] == 0;
     s->mixed = (c->flags & POISSON_SOLVER_MIXED) != 0;
     s->max_solves = s->mixed ? 6 : 1;
-    s->cols = 2 * (std::size_t)s->nz * 64 * 8 <= 160 * 1024 ? 64 : 32;
-    if (2 * (std::size_t)s->nz * s->cols * 8 > 160 * 1024)
-        return perr(POISSON_SOLVER_UNSUPPORTED, "nz above 320 exceeds the LDS of the z solve");
+    s->cols = 64;
+    if (s->nz > kPoissonMaxNz) return perr(POISSON_SOLVER_UNSUPPORTED, "nz above 1024 exceeds the LDS of the z solve");
     s->arena_bytes = arena_size(*s);
     if (c->device_budget_bytes && s->arena_bytes > c->device_budget_bytes)
         return perr(POISSON_SOLVER_RESOURCE, "workspace exceeds device_budget_bytes");

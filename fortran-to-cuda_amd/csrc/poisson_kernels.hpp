@@ -52,62 +52,116 @@ static __global__ void poisson_subtract_kernel(double *__restrict__ phi, long lo
     v = v - *mean;
 }
 
-// One FP64 Thomas solve per (p, q) column of the transformed right-hand side (p fastest, so a wavefront
-// reads 64 consecutive values of each z plane). The whole column is first staged into LDS with
-// independent loads (the sweep itself is a dependent chain), then solved and written back from LDS.
-// Solves column `col` (p + nx*q) of a plane-major (nx*ny per z plane) array; cp and dp are this block's
-// LDS arrays of nz*cols doubles and c the thread's column within the block.
+// One FP64 Thomas solve per (p, q) column of the transformed right-hand side; a wavefront holds 64
+// consecutive columns, so every z plane is read and written as one contiguous run.
+//
+// The elimination factors depend on lambda and k only, and the sweep is a dependent chain, so the solve
+// keeps no column in LDS: the forward sweep stores just the state (c_{k-1}, d_{k-1}) entering each chunk
+// of kTriChunk points (ck: 2 * chunks * cols doubles of LDS), and back substitution re-runs each chunk
+// from its entry state into registers, re-reading that chunk of the input, before substituting through
+// it. The re-run repeats the forward arithmetic operation for operation, so the result is the plain
+// Thomas solve's, while LDS per column drops from 16*nz bytes to 16*chunks and many more waves fit a CU.
+constexpr int kTriChunk = 16;
+// Largest nz of the z solve: its checkpoints then take 64 KB of LDS per 64 columns.
+constexpr int kPoissonMaxNz = 1024;
+
+__host__ __device__ constexpr int tridiag_chunks(int nz) { return (nz + kTriChunk - 1) / kTriChunk; }
+
 template <class T>
-__device__ __forceinline__ void tridiag_column(const T *__restrict__ in, T *__restrict__ out, long long plane,
-                                               int nz, double lam, double rz, double *cp, double *dp, int c,
-                                               int cols, long long col)
+__device__ __forceinline__ void tridiag_load_chunk(const T *__restrict__ src, long long plane, int k0, int nz,
+                                                   double (&v)[kTriChunk])
 {
-    for (int k = 0; k < nz; ++k) dp[(long long)k * cols + c] = (double)in[col + (long long)k * plane];
-#define IN(k) (dp[(long long)(k) * cols + c])
-#define OUT(k, v) (dp[(long long)(k) * cols + c] = (v))
-    double *result = dp;
-    if (lam == 0.0) {
-        // The constant null mode: the solution of zero mean. psi_0 = 0 fixes the constant, the first
-        // nz-1 equations give the rest (kept in cp), then the mean is removed (the last equation holds
-        // when the right-hand side has zero mean).
-        double prev = 0.0, cur = 0.0, sum = 0.0;
-        cp[c] = 0.0;
-        if (nz > 1) {
-            cur = prev + IN(0) / rz;
-            cp[(long long)cols + c] = cur;
-            sum = cur;
-            for (int k = 1; k < nz - 1; ++k) {
-                const double next = (IN(k) / rz - prev) + 2.0 * cur;
-                prev = cur;
-                cur = next;
-                cp[(long long)(k + 1) * cols + c] = cur;
-                sum = sum + cur;
-            }
-        }
-        const double mean = sum / nz;
-        for (int k = 0; k < nz; ++k) cp[(long long)k * cols + c] = cp[(long long)k * cols + c] - mean;
-        result = cp;
-    } else {
-        double cprev = 0.0, dprev = 0.0;
-        for (int k = 0; k < nz; ++k) {
+#pragma unroll
+    for (int i = 0; i < kTriChunk; ++i) v[i] = k0 + i < nz ? (double)src[(long long)(k0 + i) * plane] : 0.0;
+}
+
+// Forward elimination over one chunk: c = rz*m, d = (f - rz*d)*m with m = 1/(diag_k - rz*c).
+template <bool KEEP>
+__device__ __forceinline__ void tridiag_forward_chunk(const double (&v)[kTriChunk], int k0, int nz, double lam,
+                                                      double rz, double &c, double &d, double (&cl)[kTriChunk],
+                                                      double (&dl)[kTriChunk])
+{
+#pragma unroll
+    for (int i = 0; i < kTriChunk; ++i) {
+        const int k = k0 + i;
+        if (k < nz) {
             double diag = -2.0 * rz + lam;
             if (k == 0) diag = diag + rz;
             if (k == nz - 1) diag = diag + rz;
-            const double m = 1.0 / (diag - rz * cprev);
-            cprev = rz * m;
-            dprev = (IN(k) - rz * dprev) * m;
-            cp[(long long)k * cols + c] = cprev;
-            OUT(k, dprev);
-        }
-        double psi = dprev;
-        for (int k = nz - 2; k >= 0; --k) {
-            psi = dp[(long long)k * cols + c] - cp[(long long)k * cols + c] * psi;
-            OUT(k, psi);
+            const double m = 1.0 / (diag - rz * c);
+            c = rz * m;
+            d = (v[i] - rz * d) * m;
+            if (KEEP) {
+                cl[i] = c;
+                dl[i] = d;
+            }
         }
     }
-    for (int k = 0; k < nz; ++k) out[col + (long long)k * plane] = (T)result[(long long)k * cols + c];
-#undef IN
-#undef OUT
+}
+
+// The constant null mode (lambda == 0): the solution of zero mean. psi_0 = 0 fixes the constant, the first
+// nz-1 equations give the rest, and the mean is removed (the last equation holds when the right-hand side
+// has zero mean). Two passes over the column: the sum, then the values again minus the mean.
+template <class T>
+__device__ __forceinline__ void tridiag_null_mode(const T *__restrict__ src, T *__restrict__ dst, long long plane,
+                                                  int nz, double rz)
+{
+    double mean = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+        double prev = 0.0, cur = 0.0, sum = 0.0;
+        if (pass == 1) dst[0] = (T)(0.0 - mean);
+        if (nz > 1) {
+            cur = prev + (double)src[0] / rz;
+            if (pass == 1) dst[plane] = (T)(cur - mean);
+            sum = cur;
+            for (int k = 1; k < nz - 1; ++k) {
+                const double next = ((double)src[(long long)k * plane] / rz - prev) + 2.0 * cur;
+                prev = cur;
+                cur = next;
+                if (pass == 1) dst[(long long)(k + 1) * plane] = (T)(cur - mean);
+                sum = sum + cur;
+            }
+        }
+        mean = sum / nz;
+    }
+}
+
+// Solves column `col` of a plane-major (plane = nx*ny values per z level) array; c is the thread's column
+// within its block of `cols` and ck the block's checkpoint LDS.
+template <class T>
+__device__ __forceinline__ void tridiag_column(const T *__restrict__ in, T *__restrict__ out, long long plane,
+                                               int nz, double lam, double rz, double *ck, int c, int cols,
+                                               long long col)
+{
+    const T *src = in + col;
+    T *dst = out + col;
+    if (lam == 0.0) {
+        tridiag_null_mode<T>(src, dst, plane, nz, rz);
+        return;
+    }
+    const int chunks = tridiag_chunks(nz);
+    double cs = 0.0, ds = 0.0, v[kTriChunk], cl[kTriChunk], dl[kTriChunk];
+    for (int j = 0; j < chunks; ++j) {
+        ck[(2 * j) * cols + c] = cs;
+        ck[(2 * j + 1) * cols + c] = ds;
+        tridiag_load_chunk<T>(src, plane, j * kTriChunk, nz, v);
+        tridiag_forward_chunk<false>(v, j * kTriChunk, nz, lam, rz, cs, ds, cl, dl);
+    }
+    double psi = 0.0;
+    for (int j = chunks - 1; j >= 0; --j) {
+        const int k0 = j * kTriChunk;
+        double cc = ck[(2 * j) * cols + c], dd = ck[(2 * j + 1) * cols + c];
+        tridiag_load_chunk<T>(src, plane, k0, nz, v);
+        tridiag_forward_chunk<true>(v, k0, nz, lam, rz, cc, dd, cl, dl);
+#pragma unroll
+        for (int i = kTriChunk - 1; i >= 0; --i) {
+            const int k = k0 + i;
+            if (k < nz) {
+                psi = k == nz - 1 ? dl[i] : dl[i] - cl[i] * psi;
+                dst[(long long)k * plane] = (T)psi;
+            }
+        }
+    }
 }
 
 template <class T>
@@ -122,7 +176,7 @@ __global__ void tridiag_z_kernel(const T *__restrict__ in, T *__restrict__ out, 
     if (col >= plane) return;
     // Mode (p, q) sits at column p + q*nx, or at p*ny + q when the transforms keep q fastest.
     const double lam = qfast ? lx[col / ny] + ly[col % ny] : lx[col % nx] + ly[col / nx];
-    tridiag_column<T>(in, out, plane, nz, lam, rz, lds, lds + (long long)nz * cols, c, cols, col);
+    tridiag_column<T>(in, out, plane, nz, lam, rz, lds, c, cols, col);
 }
 
 }  // namespace ftc

@@ -36,21 +36,57 @@ __device__ __forceinline__ double identity()
     else return HUGE_VAL;
 }
 
-// Element of iteration t of the loop box (column-major over the box).
-__device__ __forceinline__ double at(const ReduceBox &b, long long t)
+// The box position (i, j, k) of an iteration, and the position step of `slots` iterations: walking a
+// slot's iterations by adding the step (with carries) avoids 64-bit divisions in the loop.
+struct BoxStep {
+    long long di, dj, dk;
+};
+
+__device__ __forceinline__ void advance(const ReduceBox &b, const BoxStep &st, long long &i, long long &j, long long &k)
 {
-    const long long i = t % b.n1, r = t / b.n1;
-    const long long j = r % b.n2, k = r / b.n2;
-    return b.x[i + j * b.ld1 + k * b.ld2];
+    i += st.di;
+    if (i >= b.n1) {
+        i -= b.n1;
+        ++j;
+    }
+    j += st.dj;
+    if (j >= b.n2) {
+        j -= b.n2;
+        ++k;
+    }
+    k += st.dk;
 }
 
+// Slot s folds iterations s, s + slots, ... in order. The loads of kUnroll consecutive iterations are
+// issued together; the combines stay in iteration order.
 template <int OP>
-__global__ void reduce_slots(const ReduceBox b, long long n, int slots, double *out)
+__global__ void reduce_slots(const ReduceBox b, long long n, int slots, BoxStep st, double *out)
 {
+    constexpr int kUnroll = 8;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= slots) return;
     double acc = identity<OP>();
-    for (long long t = s; t < n; t += slots) acc = combine<OP>(acc, at(b, t));
+    long long i = 0, j = 0, k = 0, t = s;
+    if (t < n) {  // n > 0, so the extents are positive
+        const long long r = t / b.n1;
+        i = t % b.n1;
+        j = r % b.n2;
+        k = r / b.n2;
+    }
+    for (; t + (long long)(kUnroll - 1) * slots < n; t += (long long)kUnroll * slots) {
+        double v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            v[u] = b.x[i + j * b.ld1 + k * b.ld2];
+            advance(b, st, i, j, k);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) acc = combine<OP>(acc, v[u]);
+    }
+    for (; t < n; t += slots) {
+        acc = combine<OP>(acc, b.x[i + j * b.ld1 + k * b.ld2]);
+        advance(b, st, i, j, k);
+    }
     out[s] = acc;
 }
 
@@ -66,15 +102,23 @@ __global__ void reduce_fold(const double *partial, int count, double *out)
     out[f] = acc;
 }
 
+static BoxStep box_step(const ReduceBox &b, long long n, long long slots)
+{
+    if (n == 0) return BoxStep{0, 0, 0};
+    const long long q = slots / b.n1;
+    return BoxStep{slots % b.n1, q % b.n2, q / b.n2};
+}
+
 template <int OP>
 static int launch(const ReduceBox &b, double *slots, double *final256, hipStream_t stream)
 {
     const long long n = b.n1 * b.n2 * b.n3;
     if (n <= kReduceDirect) {
-        hipLaunchKernelGGL(reduce_slots<OP>, dim3(1), dim3(kReduceFinal), 0, stream, b, n, kReduceFinal, final256);
+        hipLaunchKernelGGL(reduce_slots<OP>, dim3(1), dim3(kReduceFinal), 0, stream, b, n, kReduceFinal,
+                           box_step(b, n, kReduceFinal), final256);
     } else {
         hipLaunchKernelGGL(reduce_slots<OP>, dim3(kReduceSlots / 256), dim3(256), 0, stream, b, n, kReduceSlots,
-                           slots);
+                           box_step(b, n, kReduceSlots), slots);
         hipLaunchKernelGGL(reduce_fold<OP>, dim3(1), dim3(kReduceFinal), 0, stream, slots, kReduceSlots, final256);
     }
     if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "reduction launch failed");

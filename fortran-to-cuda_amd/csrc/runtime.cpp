@@ -1401,12 +1401,12 @@ int fort_scope_note_launch(fort_scope_t h)
     return with(h, [&](Context &c) { c.domain->stats.launches++; });
 }
 
-int fort_scope_graph_enqueue_v1(fort_scope_t h, uint64_t code_key, const fort_scope_graph_task_v1 *task,
-                                const std::size_t *argument_sizes)
+// Accepts one launch with its bindings into the context's queue (the effects are committed now; the
+// launch runs in acceptance order on the context stream).
+static void accept_task(Context &c, uint64_t code_key, const fort_scope_kernel_v1 &k, const std::size_t *argument_sizes,
+                        const fort_scope_plan_binding *bindings, std::size_t binding_count)
 {
-    return with(h, [&](Context &c) {
-        require(task != nullptr, FORT_SCOPE_ARGUMENT, "missing task");
-        const fort_scope_kernel_v1 &k = task->kernel;
+    {
         require(k.function != nullptr, FORT_SCOPE_ARGUMENT, "missing kernel function");
         require(!k.argument_count || (k.arguments && argument_sizes), FORT_SCOPE_ARGUMENT, "missing arguments");
         uint64_t threads = uint64_t(k.grid[0]) * k.grid[1] * k.grid[2] * k.block[0] * k.block[1] * k.block[2];
@@ -1416,8 +1416,8 @@ int fort_scope_graph_enqueue_v1(fort_scope_t h, uint64_t code_key, const fort_sc
 #endif
         // No drain: tasks on one context run in acceptance order on its stream, and uploads only touch
         // cells no accepted task holds on the device.
-        std::vector<void *> devices(task->binding_count);
-        prepare_device(c, task->bindings, task->binding_count, devices.data(), true);
+        std::vector<void *> devices(binding_count);
+        prepare_device(c, bindings, binding_count, devices.data(), true);
         Task t;
         t.code_key = code_key;
         t.function = k.function;
@@ -1439,8 +1439,8 @@ int fort_scope_graph_enqueue_v1(fort_scope_t h, uint64_t code_key, const fort_sc
         t.args.resize(off);
         for (std::size_t i = 0; i < k.argument_count; ++i)
             std::memcpy(t.args.data() + t.offsets[i], k.arguments[i], t.sizes[i]);
-        for (std::size_t i = 0; i < task->binding_count; ++i) {
-            Buffer &b = buffer(c, task->bindings[i].buffer);
+        for (std::size_t i = 0; i < binding_count; ++i) {
+            Buffer &b = buffer(c, bindings[i].buffer);
             b.queued++;
             t.buffers.push_back(&b);
         }
@@ -1451,6 +1451,99 @@ int fort_scope_graph_enqueue_v1(fort_scope_t h, uint64_t code_key, const fort_sc
         c.qstats.pending_tasks = c.queue.size();
         if (c.queue.size() > c.qstats.peak_pending_tasks) c.qstats.peak_pending_tasks = c.queue.size();
         if (!c.queue_enabled || (c.queue_work && c.queued_threads >= c.queue_work)) flush(c);
+    }
+}
+
+int fort_scope_graph_enqueue_v1(fort_scope_t h, uint64_t code_key, const fort_scope_graph_task_v1 *task,
+                                const std::size_t *argument_sizes)
+{
+    return with(h, [&](Context &c) {
+        require(task != nullptr, FORT_SCOPE_ARGUMENT, "missing task");
+        accept_task(c, code_key, task->kernel, argument_sizes, task->bindings, task->binding_count);
+    });
+}
+
+// scoped_runtime.h:401-413: members are checked pairwise on their actual sections, then merged into one
+// binding per buffer whose sections are the union of the members' sections.
+int fort_scope_graph_enqueue_members_v1(fort_scope_t h, uint64_t code_key, const fort_scope_kernel_v1 *kernel,
+                                        const std::size_t *argument_sizes, const fort_scope_graph_member_v1 *members,
+                                        std::size_t member_count)
+{
+    return with(h, [&](Context &c) {
+        require(kernel != nullptr, FORT_SCOPE_ARGUMENT, "missing kernel");
+        require(!member_count || members, FORT_SCOPE_ARGUMENT, "missing members");
+        struct Use {
+            Buffer *buf;
+            fort_buffer_t handle;
+            Effects e;
+            std::size_t member;
+        };
+        std::vector<Use> uses;
+        for (std::size_t m = 0; m < member_count; ++m) {
+            require(!members[m].binding_count || members[m].bindings, FORT_SCOPE_ARGUMENT, "missing member bindings");
+            for (std::size_t i = 0; i < members[m].binding_count; ++i) {
+                const fort_scope_plan_binding &pb = members[m].bindings[i];
+                Buffer &b = buffer(c, pb.buffer);
+                for (const Use &u : uses)
+                    require(u.member != m || u.buf != &b, FORT_SCOPE_ALIAS, "a member's bindings repeat a buffer");
+                uses.push_back({&b, pb.buffer, effects(b, &pb.access), m});
+            }
+        }
+        auto touches = [](const std::vector<Box> &writes, const Effects &o) {
+            for (const Box &w : writes)
+                for (const std::vector<Box> *list : {&o.reads, &o.writes, &o.overwrites})
+                    for (const Box &x : *list)
+                        if (w.intersects(x)) return true;
+            return false;
+        };
+        for (const Use &a : uses)
+            for (const Use &b : uses)
+                if (a.member != b.member && a.buf == b.buf && (touches(a.e.writes, b.e) || touches(a.e.overwrites, b.e)))
+                    fail(FORT_SCOPE_BOUNDARY, "launch members overlap: one writes a section another member uses");
+        // One binding per distinct buffer, listing every member's boxes as explicit sections.
+        std::vector<fort_buffer_t> order;
+        for (const Use &u : uses)
+            if (std::find(order.begin(), order.end(), u.handle) == order.end()) order.push_back(u.handle);
+        std::size_t nbox = 0;
+        for (const Use &u : uses) nbox += u.e.reads.size() + u.e.writes.size() + u.e.overwrites.size();
+        std::vector<std::size_t> coords(2 * kMaxRank * (nbox ? nbox : 1));
+        std::vector<fort_scope_section> sections(nbox ? nbox : 1);
+        std::size_t used = 0;
+        auto section_of = [&](const Box &x) {
+            std::size_t *lo = coords.data() + 2 * kMaxRank * used, *hi = lo + kMaxRank;
+            for (int d = 0; d < x.rank; ++d) {
+                lo[d] = x.lo[d];
+                hi[d] = x.hi[d];
+            }
+            sections[used] = fort_scope_section{lo, hi};
+            return &sections[used++];
+        };
+        std::vector<fort_scope_plan_binding> merged;
+        for (fort_buffer_t handle : order) {
+            fort_scope_plan_binding pb{};
+            pb.buffer = handle;
+            const fort_scope_section *firsts[3] = {nullptr, nullptr, nullptr};
+            std::size_t counts[3] = {0, 0, 0};
+            for (int kind = 0; kind < 3; ++kind)
+                for (const Use &u : uses) {
+                    if (u.handle != handle) continue;
+                    const std::vector<Box> &list = kind == 0 ? u.e.reads : (kind == 1 ? u.e.writes : u.e.overwrites);
+                    for (const Box &x : list) {
+                        const fort_scope_section *sp = section_of(x);
+                        if (!firsts[kind]) firsts[kind] = sp;
+                        counts[kind]++;
+                    }
+                }
+            pb.access.flags = 0;
+            pb.access.read_count = counts[0];
+            pb.access.reads = firsts[0];
+            pb.access.write_count = counts[1];
+            pb.access.writes = firsts[1];
+            pb.access.overwrite_count = counts[2];
+            pb.access.overwrites = firsts[2];
+            merged.push_back(pb);
+        }
+        accept_task(c, code_key, *kernel, argument_sizes, merged.data(), merged.size());
     });
 }
 

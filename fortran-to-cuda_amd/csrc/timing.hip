@@ -10,11 +10,19 @@ __global__ void ftc_copy_kernel(const double *__restrict__ src, double *__restri
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) dst[e] = src[e];
 }
 
+__global__ void ftc_iota(double *__restrict__ x, long long n, double scale)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = scale * (double)i;
+}
+
 extern "C" {
 
 // Measures a plain device-to-device copy of n doubles (reps launches of a grid-stride copy kernel on a
 // private stream): the practical HBM ceiling, and the calibration kernel for the FETCH_SIZE/WRITE_SIZE
 // counters (each launch reads and writes exactly 8*n bytes). Writes the average milliseconds per launch.
+const void *ftc_iota_kernel(void) { return reinterpret_cast<const void *>(&ftc_iota); }
+
 int ftc_copy_probe(size_t n, int reps, float *ms_per_launch)
 {
     if (!ms_per_launch || n == 0 || reps <= 0) return FORT_SCOPE_ARGUMENT;

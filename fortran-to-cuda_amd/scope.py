@@ -138,6 +138,8 @@ RUNTIME_SIGNATURES = {
     "fort_scope_gpu_leave": (c_int, [c_uint64, c_int]),
     "fort_scope_note_launch": (c_int, [c_uint64]),
     "fort_scope_graph_enqueue_v1": (c_int, [c_uint64, c_uint64, c_void_p, POINTER(c_size_t)]),
+    "fort_scope_graph_enqueue_members_v1": (c_int, [c_uint64, c_uint64, c_void_p, POINTER(c_size_t), c_void_p,
+                                                    c_size_t]),
     "fort_scope_graph_flush_v1": (c_int, [c_uint64]),
     "fort_scope_graph_queue_cancel_v1": (c_int, [c_uint64]),
     "fort_scope_graph_queue_stats_get_v1": (c_int, [c_uint64, POINTER(QueueStats)]),

@@ -183,6 +183,18 @@ int fort_scope_gpu_leave(fort_scope_t context, int previous_device);            
 int fort_scope_note_launch(fort_scope_t context);                                   /* :348 */
 int fort_scope_graph_enqueue_v1(fort_scope_t context, uint64_t code_key, const fort_scope_graph_task_v1 *task,
                                 const size_t *argument_sizes);                      /* :399-400 */
+/* One launch whose kernel runs several independent members, each with its own effects. Accepted only when
+ * no member reads or writes a section that another member writes (judged on the actual sections);
+ * otherwise returns BOUNDARY, queues nothing, and the caller runs the members one after another. The
+ * accepted launch carries the union of the members' effects. The CPU backend returns RESOURCE for an
+ * accepted nonempty launch, after the overlap check. */
+typedef struct fort_scope_graph_member_v1 {    /* scoped_runtime.h:407-410 */
+    const fort_scope_plan_binding *bindings;
+    size_t binding_count;
+} fort_scope_graph_member_v1;
+int fort_scope_graph_enqueue_members_v1(fort_scope_t context, uint64_t code_key, const fort_scope_kernel_v1 *kernel,
+                                        const size_t *argument_sizes, const fort_scope_graph_member_v1 *members,
+                                        size_t member_count);                       /* :411-413 */
 int fort_scope_graph_flush_v1(fort_scope_t context);                                /* :414 */
 int fort_scope_graph_queue_cancel_v1(fort_scope_t context);                         /* :415 */
 int fort_scope_graph_queue_stats_get_v1(fort_scope_t context, fort_scope_graph_queue_stats_v1 *stats); /* :416 */

@@ -73,3 +73,38 @@ def test_contraction_descriptor_is_checked_on_the_host(ftc):
     short = ftc.GemmDesc(4096, 8192, 12288, 1, 1, 1, 8, 8, 16, 1, 8, 16, 8, 0, 0, 0, 1.0, 0.0, 1)
     assert lib.ftc_gemm_device(ctypes.byref(short), 1, -1, None) == 1  # K-contiguous lda below K
     assert lib.fort_scope_error() == b"leading dimension below the row count"
+
+
+def test_fortran_interfaces_call_the_library(ftc, tmp_path):
+    # The bind(C) interface modules of fortran-to-cuda_amd/fortran/ compile, link against libftc_hip.so and
+    # reach the entries from Fortran: native mode declines (9) before any device work, and an unknown mode
+    # is an argument error (1). No GPU is needed for either.
+    import shutil
+
+    fc = shutil.which("gfortran") or shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not Path(fc).exists() and not shutil.which(fc):
+        pytest.skip("no Fortran compiler")
+    srcdir = ftc.PACKAGE / "fortran"
+    prog = tmp_path / "probe.f90"
+    prog.write_text("""program probe
+  use iso_c_binding
+  use fort_shared_momentumadvection_cdu, only: cdu => run
+  use fort_shared_fixed_order_reduction, only: reduce => run
+  implicit none
+  real(c_double) :: d = 10.0d0, v = 0.0d0
+  integer(c_int) :: n = 16, op = 0
+  integer(c_long_long) :: lo(3) = 0, hi(3) = 1
+  print '(i0)', cdu(0_c_int64_t, 0_c_int, 1_c_int64_t, 2_c_int64_t, 3_c_int64_t, 4_c_int64_t, d, d, d, n, n, n)
+  print '(i0)', reduce(0_c_int64_t, 7_c_int, 1_c_int64_t, op, lo, hi, v)
+end program probe
+""")
+    flag = "-J" if "gfortran" in fc else "-module-dir"
+    objs = []
+    for src in sorted(srcdir.glob("*.f90")) + [prog]:
+        obj = tmp_path / (src.stem + ".o")
+        subprocess.run([fc, "-c", flag, str(tmp_path), str(src), "-o", str(obj)], check=True, cwd=tmp_path)
+        objs.append(str(obj))
+    exe = tmp_path / "probe"
+    subprocess.run([fc, *objs, "-o", str(exe), str(ftc.HIP_LIBRARY), f"-Wl,-rpath,{ftc.PACKAGE}"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert out == ["9", "1"]

@@ -181,6 +181,58 @@ def test_device_kernels_are_refused_by_the_cpu_backend(scope, cpu_runtime):
         assert dom.stats().allocations == 0
 
 
+def test_member_launches_are_checked_on_their_actual_sections(scope, cpu_runtime):
+    # scoped_runtime.h:401-413: members that write a section another member uses are refused with BOUNDARY
+    # (nothing queued, context still usable); disjoint members are accepted, which the CPU backend then
+    # refuses with RESOURCE because it runs no device kernels.
+    lib = cpu_runtime
+    a = np.zeros(16, order="F")
+    b = np.zeros(16, order="F")
+
+    class Kernel(ctypes.Structure):
+        _fields_ = [("version", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("function", c_void_p),
+                    ("grid", ctypes.c_uint32 * 3), ("block", ctypes.c_uint32 * 3), ("shared_bytes", c_size_t),
+                    ("arguments", c_void_p), ("argument_count", c_size_t)]
+
+    class Member(ctypes.Structure):
+        _fields_ = [("bindings", c_void_p), ("binding_count", c_size_t)]
+
+    arg = ctypes.c_double(1.0)
+    argv = (c_void_p * 1)(ctypes.addressof(arg))
+    sizes = (c_size_t * 1)(8)
+    k = Kernel(1, 0, c_void_p(1), (ctypes.c_uint32 * 3)(1, 1, 1), (ctypes.c_uint32 * 3)(64, 1, 1), 0,
+               ctypes.cast(argv, c_void_p), 1)
+    with new_domain(scope, lib) as dom:
+        ha, hb = dom.bind(a), dom.bind(b)
+        keep = []
+
+        def member(*uses):
+            binds = []
+            for h, kw in uses:
+                acc, kp = scope.make_access(**kw)
+                keep.append(kp)
+                binds.append(scope.PlanBinding(h, acc))
+            arr = (scope.PlanBinding * len(binds))(*binds)
+            keep.append(arr)
+            return Member(ctypes.cast(arr, c_void_p), len(binds))
+
+        def launch(*members):
+            ms = (Member * len(members))(*members)
+            return lib.fort_scope_graph_enqueue_members_v1(dom.ctx, 7, ctypes.byref(k), sizes, ms, len(members))
+
+        w = lambda lo, hi: {"writes": [((lo,), (hi,))]}
+        r = lambda lo, hi: {"reads": [((lo,), (hi,))]}
+        assert launch(member((ha, w(0, 8))), member((ha, w(8, 16)))) == scope.RESOURCE
+        assert launch(member((ha, w(0, 9))), member((ha, r(8, 16)))) == scope.BOUNDARY
+        assert launch(member((ha, r(0, 16))), member((ha, r(0, 16)), (hb, w(0, 16)))) == scope.RESOURCE
+        assert launch(member((ha, r(0, 4))), member((hb, {"overwrites": [((0,), (16,))]}), (ha, w(4, 16)))) \
+            == scope.RESOURCE
+        assert launch(member((ha, {"flags": scope.WRITE_ALL})), member((ha, r(15, 16)))) == scope.BOUNDARY
+        assert launch(member((ha, r(0, 4)), (ha, r(4, 8)))) == scope.ALIAS
+        assert dom.stats().allocations == 0
+        dom.observe(ha)  # a refused launch does not poison the context
+
+
 def test_scratch_lease(scope, cpu_runtime):
     lib = cpu_runtime
     with new_domain(scope, lib) as dom:

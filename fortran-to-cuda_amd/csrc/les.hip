@@ -53,7 +53,8 @@ static void fill_coefficients(LesArgs &a, double dx, double dy, double dz, doubl
 static int choose_kchunk(int nx, int ny, int kplanes)
 {
     const long long tiles = (long long)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY);
-    const long long target = 2048;
+    // About four rounds of resident workgroups (three per CU on 256 CUs for the staged kernel).
+    const long long target = std::getenv("FTC_LES_V2") ? 2048 : 3072;
     int chunks = (int)std::max<long long>(1, (target + tiles - 1) / tiles);
     int kchunk = (kplanes + chunks - 1) / chunks;
     if (const char *v = std::getenv("FTC_LES_KCHUNK")) kchunk = std::atoi(v);
@@ -205,9 +206,13 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     b[6] = {w, {box(0, X + 2, 0, Y + 2, 0, Z + 1)}, {}, {}};
     b[7] = {s, {full}, {}, {}};
     const void *fn;
+    // FTC_LES_UNSTAGED selects the unstaged kernel and FTC_LES_V2 the four-plane staging, for comparisons.
     const bool staged = sgs && !std::getenv("FTC_LES_UNSTAGED");
-    if (staged) fn = step ? reinterpret_cast<const void *>(&les_sgs_kernel<true>)
-                          : reinterpret_cast<const void *>(&les_sgs_kernel<false>);
+    const bool v2 = std::getenv("FTC_LES_V2") != nullptr;
+    if (staged && v2) fn = step ? reinterpret_cast<const void *>(&les_sgs_kernel<true>)
+                                : reinterpret_cast<const void *>(&les_sgs_kernel<false>);
+    else if (staged) fn = step ? reinterpret_cast<const void *>(&les_sgs3_kernel<true>)
+                               : reinterpret_cast<const void *>(&les_sgs3_kernel<false>);
     else if (step) fn = sgs ? kernel_of<15, true, true>() : kernel_of<15, false, true>();
     else fn = sgs ? kernel_of<15, true, false>() : kernel_of<15, false, false>();
     const uint64_t key = 0x1e5000000000000full ^ ((uint64_t)step << 8) ^ ((uint64_t)sgs << 9);

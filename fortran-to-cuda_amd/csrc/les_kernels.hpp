@@ -565,4 +565,334 @@ __global__ __launch_bounds__(NTH, 2) void les_sgs_kernel(const LesArgs a)
     les_sgs_body<STEP>(a, L, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// ------------------------------------------------------------------------------------------------ v3
+// The v2 staging with the z-direction fluxes carried in registers. In flux form the lower z-face term of
+// cell k is the upper z-face term of cell k-1, computed from the same operands in the same order, so a
+// thread marching k keeps the upper terms of step k-1 (and its own column's k-1 velocities) and reads
+// nothing of plane k-1 from LDS. The rings shrink to three velocity planes (k .. k+2), two scalar planes
+// and two viscosity planes: 52 KB, three workgroups per CU. Every value is the one v2 and the oracle
+// compute; only where it is kept changes.
+struct Sgs3Lds {
+    double vel[3][3][VY][VX];  // slot = plane % 3
+    double sca[2][CY][CX];     // slot = plane & 1
+    double nu[2][CY][CX];      // slot = plane & 1
+};
+
+// The upper z-face terms of a cell (advection and diffusion of u, v, w and s), which are the lower terms
+// of the cell above.
+struct ZCarry {
+    double cu, du, cv, dv, cw, dw, cs, ds;  // flux-form terms
+    double u, v, w, wxp, wyp;               // U, V, W at (i, j), W at (i+1, j) and (i, j+1)
+};
+
+template <bool STEP>
+__device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int bx, int by, int bz)
+{
+    const double *__restrict__ gU = a.U;
+    const double *__restrict__ gV = a.V;
+    const double *__restrict__ gW = a.W;
+    const double *__restrict__ gS = a.S;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const int i0 = 2 + bx * TX, j0 = 2 + by * TY;
+    const int i = i0 + tx, j = j0 + ty;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int k0 = 2 + bz * a.kchunk;
+    if (k0 > nz + 1) return;  // uniform over the workgroup
+    const int k1 = min(k0 + a.kchunk - 1, nz + 1);
+    const long long sj = a.sj, sk = a.sk;
+
+    long long voff[VR], coff[CR];
+    int vcell[VR], ccell[CR];
+#pragma unroll
+    for (int r = 0; r < VR; ++r) {
+        const int c = tid + r * NTH;
+        vcell[r] = c < VCELLS ? c : -1;
+        const int cc = c < VCELLS ? c : 0;
+        const int ii = periodic_image(i0 - 2 + cc % VX, nx), jj = periodic_image(j0 - 2 + cc / VX, ny);
+        voff[r] = (long long)(ii - 1) + (long long)(jj - 1) * sj;
+    }
+#pragma unroll
+    for (int r = 0; r < CR; ++r) {
+        const int c = tid + r * NTH;
+        ccell[r] = c < CCELLS ? c : -1;
+        const int cc = c < CCELLS ? c : 0;
+        const int ii = periodic_image(i0 - 1 + cc % CX, nx), jj = periodic_image(j0 - 1 + cc / CX, ny);
+        coff[r] = (long long)(ii - 1) + (long long)(jj - 1) * sj;
+    }
+    double rv[VR][3], rs[CR];
+    auto fetch_vel = [&](int kp) {
+        if (kp < 1 || kp > nz + 2) return;
+        const long long base = (long long)(kp - 1) * sk;
+#pragma unroll
+        for (int r = 0; r < VR; ++r)
+            if (vcell[r] >= 0) {
+                rv[r][0] = gU[voff[r] + base];
+                rv[r][1] = gV[voff[r] + base];
+                rv[r][2] = gW[voff[r] + base];
+            }
+    };
+    auto put_vel = [&](int kp) {
+        if (kp < 1 || kp > nz + 2) return;
+        const int slot = kp % 3;
+#pragma unroll
+        for (int r = 0; r < VR; ++r)
+            if (vcell[r] >= 0) {
+                (&L.vel[slot][0][0][0])[vcell[r]] = rv[r][0];
+                (&L.vel[slot][1][0][0])[vcell[r]] = rv[r][1];
+                (&L.vel[slot][2][0][0])[vcell[r]] = rv[r][2];
+            }
+    };
+    auto fetch_sca = [&](int kp) {
+        if (kp < 1 || kp > nz + 2) return;
+        const long long base = (long long)(kp - 1) * sk;
+#pragma unroll
+        for (int r = 0; r < CR; ++r)
+            if (ccell[r] >= 0) rs[r] = gS[coff[r] + base];
+    };
+    auto put_sca = [&](int kp) {
+        if (kp < 1 || kp > nz + 2) return;
+        const int slot = kp & 1;
+#pragma unroll
+        for (int r = 0; r < CR; ++r)
+            if (ccell[r] >= 0) (&L.sca[slot][0][0])[ccell[r]] = rs[r];
+    };
+    const double *q = a.sgs;
+    // nu at plane qk (clamped to the interior planes: zero gradient in z) over the tile and its ring.
+    auto nu_plane = [&](int qk) {
+        const int kk = qk < 2 ? 2 : (qk > nz + 1 ? nz + 1 : qk);
+        const int s0 = kk % 3, sm = (kk + 2) % 3, sp = (kk + 1) % 3;
+        double(*nuo)[CX] = L.nu[qk & 1];
+        for (int c = tid; c < CCELLS; c += NTH) {
+            const int li = c % CX, lj = c / CX;
+            const int vx = li + 1, vy = lj + 1;
+#define QU(s, di, dj) L.vel[s][0][vy + (dj)][vx + (di)]
+#define QV(s, di, dj) L.vel[s][1][vy + (dj)][vx + (di)]
+#define QW(s, di, dj) L.vel[s][2][vy + (dj)][vx + (di)]
+            double s11 = (QU(s0, 0, 0) - QU(s0, -1, 0)) * q[0];
+            double s22 = (QV(s0, 0, 0) - QV(s0, 0, -1)) * q[1];
+            double s33 = (QW(s0, 0, 0) - QW(sm, 0, 0)) * q[2];
+            double dudy = ((QU(s0, 0, 1) + QU(s0, -1, 1)) - (QU(s0, 0, -1) + QU(s0, -1, -1))) * q[4];
+            double dvdx = ((QV(s0, 1, 0) + QV(s0, 1, -1)) - (QV(s0, -1, 0) + QV(s0, -1, -1))) * q[3];
+            double dudz = ((QU(sp, 0, 0) + QU(sp, -1, 0)) - (QU(sm, 0, 0) + QU(sm, -1, 0))) * q[5];
+            double dwdx = ((QW(s0, 1, 0) + QW(sm, 1, 0)) - (QW(s0, -1, 0) + QW(sm, -1, 0))) * q[3];
+            double dvdz = ((QV(sp, 0, 0) + QV(sp, 0, -1)) - (QV(sm, 0, 0) + QV(sm, 0, -1))) * q[5];
+            double dwdy = ((QW(s0, 0, 1) + QW(sm, 0, 1)) - (QW(s0, 0, -1) + QW(sm, 0, -1))) * q[4];
+#undef QU
+#undef QV
+#undef QW
+            double s12 = 0.5 * (dudy + dvdx);
+            double s13 = 0.5 * (dudz + dwdx);
+            double s23 = 0.5 * (dvdz + dwdy);
+            double ss = 2.0 * ((s11 * s11 + s22 * s22) + s33 * s33) + 4.0 * ((s12 * s12 + s13 * s13) + s23 * s23);
+            nuo[lj][li] = q[9] * sqrt(ss);
+        }
+    };
+
+    const bool inside = i <= nx + 1 && j <= ny + 1;
+    const double dt = a.dt;
+    const double c25x = a.c25[0], c25y = a.c25[1], c25z = a.c25[2];
+    const double Cz = a.c50[2];
+#define VU(s, di, dj) L.vel[s][0][ty + 2 + (dj)][tx + 2 + (di)]
+#define VV(s, di, dj) L.vel[s][1][ty + 2 + (dj)][tx + 2 + (di)]
+#define VW(s, di, dj) L.vel[s][2][ty + 2 + (dj)][tx + 2 + (di)]
+#define SS(s, di, dj) L.sca[s][ty + 1 + (dj)][tx + 1 + (di)]
+#define NV(s, di, dj) L.nu[s][ty + 1 + (dj)][tx + 1 + (di)]
+
+    // Prologue: planes k0-2 .. k0 give nu(k0-1); plane k0+1 replaces k0-2 and gives nu(k0); then the upper
+    // terms of plane k0-1 (the lower terms of k0) are formed as a step forms them.
+    for (int kp = k0 - 2; kp <= k0; ++kp) {
+        fetch_vel(kp);
+        put_vel(kp);
+    }
+    for (int kp = k0 - 1; kp <= k0; ++kp) {
+        fetch_sca(kp);
+        put_sca(kp);
+    }
+    fetch_vel(k0 + 1);
+    // nu(k0-1) is clamped to plane 2 when k0 = 2 and then needs plane k0+1 (plane k0-2 does not exist).
+    const bool clamped = k0 - 1 < 2;
+    __syncthreads();
+    if (!clamped) nu_plane(k0 - 1);
+    __syncthreads();
+    put_vel(k0 + 1);
+    __syncthreads();
+    if (clamped) nu_plane(k0 - 1);
+    nu_plane(k0);
+    __syncthreads();
+    ZCarry zc{};
+    if (inside) {
+        const int kv = k0 - 1;
+        const int s0 = kv % 3, sp = (kv + 1) % 3, c0 = kv & 1, cp = (kv + 1) & 1;
+        const double u0 = VU(s0, 0, 0), uzp = VU(sp, 0, 0);
+        const double v0 = VV(s0, 0, 0), vzp = VV(sp, 0, 0);
+        const double w0 = VW(s0, 0, 0), wxp = VW(s0, 1, 0), wyp = VW(s0, 0, 1), wzp = VW(sp, 0, 0);
+        const double n000 = NV(c0, 0, 0), n100 = NV(c0, 1, 0), n010 = NV(c0, 0, 1), n001 = NV(cp, 0, 0);
+        zc.cu = c25z * (uzp + u0) * (wxp + w0);
+        zc.du = (0.25 * (((n000 + n100) + n001) + NV(cp, 1, 0))) * (uzp - u0);
+        zc.cv = c25z * (vzp + v0) * (wyp + w0);
+        zc.dv = (0.25 * (((n000 + n010) + n001) + NV(cp, 0, 1))) * (vzp - v0);
+        zc.cw = c25z * (wzp + w0) * (wzp + w0);
+        zc.dw = n001 * (wzp - w0);
+        const double s0v = SS(c0, 0, 0), szp = SS(cp, 0, 0);
+        zc.cs = Cz * w0 * (szp + s0v);
+        zc.ds = (0.5 * (n000 + n001)) * (szp - s0v);
+        zc.u = u0;
+        zc.v = v0;
+        zc.w = w0;
+        zc.wxp = wxp;
+        zc.wyp = wyp;
+    }
+    fetch_vel(k0 + 2);
+    fetch_sca(k0 + 1);
+    __syncthreads();
+
+    for (int k = k0; k <= k1; ++k) {
+        put_vel(k + 2);
+        put_sca(k + 1);
+        fetch_vel(k + 3);
+        fetch_sca(k + 2);
+        __syncthreads();
+        nu_plane(k + 1);
+        __syncthreads();
+        if (inside) {
+            const int s0 = k % 3, sp = (k + 1) % 3;
+            const int c0 = k & 1, cp = (k + 1) & 1;
+            const double u0 = VU(s0, 0, 0), uxp = VU(s0, 1, 0), uxm = VU(s0, -1, 0), uyp = VU(s0, 0, 1);
+            const double uym = VU(s0, 0, -1), uzp = VU(sp, 0, 0), uzm = zc.u;
+            const double v0 = VV(s0, 0, 0), vxp = VV(s0, 1, 0), vxm = VV(s0, -1, 0), vyp = VV(s0, 0, 1);
+            const double vym = VV(s0, 0, -1), vzp = VV(sp, 0, 0), vzm = zc.v;
+            const double w0 = VW(s0, 0, 0), wxp = VW(s0, 1, 0), wxm = VW(s0, -1, 0), wyp = VW(s0, 0, 1);
+            const double wym = VW(s0, 0, -1), wzp = VW(sp, 0, 0), wzm = zc.w;
+            const double n000 = NV(c0, 0, 0), n100 = NV(c0, 1, 0), nm00 = NV(c0, -1, 0);
+            const double n010 = NV(c0, 0, 1), n0m0 = NV(c0, 0, -1), n001 = NV(cp, 0, 0);
+            const long long p = (long long)(i - 1) + (long long)(j - 1) * sj + (long long)(k - 1) * sk;
+            ZCarry up;
+            // U: CDU (cdu.f90:48-107) then the diffusion of u.
+            {
+                const double vxpym = VV(s0, 1, -1), wxpzm = zc.wxp;
+                double x = c25x * (uxp + u0) * (uxp + u0) - c25x * (u0 + uxm) * (u0 + uxm);
+                double y = c25y * (uyp + u0) * (vxp + v0) - c25y * (u0 + uym) * (vxpym + vym);
+                up.cu = c25z * (uzp + u0) * (wxp + w0);
+                double z = up.cu - zc.cu;
+                double t = -((x + y) + z);
+                double Vadv = ((v0 + vxp) + vym) + vxpym;
+                double Wadv = ((w0 + wxp) + wzm) + wxpzm;
+                t = t - ((a.c50[0] * (uxp - uxm) * u0 + a.c125[1] * (uyp - uym) * Vadv) +
+                         a.c125[2] * (uzp - uzm) * Wadv);
+                t = t * 0.5;
+                double fxp = n100 * (uxp - u0);
+                double fxm = n000 * (u0 - uxm);
+                double exy0 = 0.25 * (((n000 + n100) + n010) + NV(c0, 1, 1));
+                double exym = 0.25 * (((n0m0 + NV(c0, 1, -1)) + n000) + n100);
+                double fyp = exy0 * (uyp - u0);
+                double fym = exym * (u0 - uym);
+                double exz0 = 0.25 * (((n000 + n100) + n001) + NV(cp, 1, 0));
+                up.du = exz0 * (uzp - u0);
+                double fzp = up.du;
+                double fzm = zc.du;
+                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                if constexpr (STEP) put_with_images(a, a.OU, p, u0 + dt * t, i, j, k, 0);
+                else a.OU[p] = t;
+            }
+            // V: CDV (cvd.f90:44-98) then the diffusion of v.
+            {
+                const double uxmyp = VU(s0, -1, 1), wypzm = zc.wyp;
+                double y = c25y * (vyp + v0) * (vyp + v0) - c25y * (v0 + vym) * (v0 + vym);
+                double x = c25x * (vxp + v0) * (uyp + u0) - c25x * (v0 + vxm) * (uxmyp + uxm);
+                up.cv = c25z * (vzp + v0) * (wyp + w0);
+                double z = up.cv - zc.cv;
+                double t = -((y + x) + z);
+                double Uadv = ((u0 + uyp) + uxm) + uxmyp;
+                double Wadv = ((w0 + wyp) + wzm) + wypzm;
+                t = t - ((a.c125[0] * (vxp - vxm) * Uadv + a.c50[1] * (vyp - vym) * v0) +
+                         a.c125[2] * (vzp - vzm) * Wadv);
+                t = t * 0.5;
+                double exy0 = 0.25 * (((n000 + n100) + n010) + NV(c0, 1, 1));
+                double exyxm = 0.25 * (((nm00 + n000) + NV(c0, -1, 1)) + n010);
+                double fxp = exy0 * (vxp - v0);
+                double fxm = exyxm * (v0 - vxm);
+                double fyp = n010 * (vyp - v0);
+                double fym = n000 * (v0 - vym);
+                double eyz0 = 0.25 * (((n000 + n010) + n001) + NV(cp, 0, 1));
+                up.dv = eyz0 * (vzp - v0);
+                double fzp = up.dv;
+                double fzm = zc.dv;
+                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                if constexpr (STEP) put_with_images(a, a.OV, p, v0 + dt * t, i, j, k, 0);
+                else a.OV[p] = t;
+            }
+            // W on the interior faces: CDW (cdw.f90:44-98) then the diffusion of w.
+            up.cw = zc.cw;
+            up.dw = zc.dw;
+            if (k <= a.kw) {
+                const double uxmzp = VU(sp, -1, 0), vymzp = VV(sp, 0, -1);
+                up.cw = c25z * (wzp + w0) * (wzp + w0);
+                double z = up.cw - zc.cw;
+                double y = c25y * (wyp + w0) * (vzp + v0) - c25y * (w0 + wym) * (vym + vymzp);
+                double x = c25x * (wxp + w0) * (uzp + u0) - c25x * (w0 + wxm) * (uxmzp + uxm);
+                double t = -((z + y) + x);
+                double Uadv = ((u0 + uzp) + uxm) + uxmzp;
+                double Vadv = ((v0 + vzp) + vym) + vymzp;
+                t = t - ((a.c125[0] * (wxp - wxm) * Uadv + a.c125[1] * (wyp - wym) * Vadv) +
+                         a.c50[2] * (wzp - wzm) * w0);
+                t = t * 0.5;
+                double exz0 = 0.25 * (((n000 + n100) + n001) + NV(cp, 1, 0));
+                double exzxm = 0.25 * (((nm00 + n000) + NV(cp, -1, 0)) + n001);
+                double fxp = exz0 * (wxp - w0);
+                double fxm = exzxm * (w0 - wxm);
+                double eyz0 = 0.25 * (((n000 + n010) + n001) + NV(cp, 0, 1));
+                double eyzym = 0.25 * (((n0m0 + n000) + NV(cp, 0, -1)) + n001);
+                double fyp = eyz0 * (wyp - w0);
+                double fym = eyzym * (w0 - wym);
+                up.dw = n001 * (wzp - w0);
+                double fzp = up.dw;
+                double fzm = zc.dw;
+                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                if constexpr (STEP) put_with_images(a, a.OW, p, w0 + dt * t, i, j, k, 1);
+                else a.OW[p] = t;
+            }
+            // S: flux-form advection then the diffusion of s.
+            {
+                const double s0v = SS(c0, 0, 0), sxp = SS(c0, 1, 0), sxm = SS(c0, -1, 0), syp = SS(c0, 0, 1);
+                const double sym = SS(c0, 0, -1), szp = SS(cp, 0, 0);
+                const double Cx = a.c50[0], Cy = a.c50[1];
+                double x = Cx * u0 * (sxp + s0v) - Cx * uxm * (s0v + sxm);
+                double y = Cy * v0 * (syp + s0v) - Cy * vym * (s0v + sym);
+                up.cs = Cz * w0 * (szp + s0v);
+                double z = up.cs - zc.cs;
+                double t = -((x + y) + z);
+                double fxp = (0.5 * (n000 + n100)) * (sxp - s0v);
+                double fxm = (0.5 * (nm00 + n000)) * (s0v - sxm);
+                double fyp = (0.5 * (n000 + n010)) * (syp - s0v);
+                double fym = (0.5 * (n0m0 + n000)) * (s0v - sym);
+                up.ds = (0.5 * (n000 + n001)) * (szp - s0v);
+                double fzp = up.ds;
+                double fzm = zc.ds;
+                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                if constexpr (STEP) put_with_images(a, a.OS, p, s0v + dt * t, i, j, k, 2);
+                else a.OS[p] = t;
+            }
+            up.u = u0;
+            up.v = v0;
+            up.w = w0;
+            up.wxp = wxp;
+            up.wyp = wyp;
+            zc = up;
+        }
+        __syncthreads();
+    }
+#undef VU
+#undef VV
+#undef VW
+#undef SS
+#undef NV
+}
+
+template <bool STEP>
+__global__ __launch_bounds__(NTH, 3) void les_sgs3_kernel(const LesArgs a)
+{
+    __shared__ Sgs3Lds L;
+    les_sgs3_body<STEP>(a, L, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 }  // namespace ftc

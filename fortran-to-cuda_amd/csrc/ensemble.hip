@@ -67,7 +67,7 @@ constexpr int kCols = 64;
 template <bool STEP>
 __global__ __launch_bounds__(NTH, 3) void ens_les_kernel(const LesArgs *args, const TileItem *items, int n)
 {
-    __shared__ Sgs3Lds L;
+    __shared__ Sgs3Lds<1> L;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const TileItem t = items[it];
         les_sgs3_body<STEP>(args[t.member], L, t.bx, t.by, t.bz);

@@ -50,11 +50,12 @@ static void fill_coefficients(LesArgs &a, double dx, double dy, double dz, doubl
 
 // Launch geometry: enough k chunks that the grid holds at least ~8 workgroups per CU of a 256-CU
 // device, without chunks shorter than 4 planes (each chunk recomputes two viscosity planes).
-static int choose_kchunk(int nx, int ny, int kplanes)
+// The k chunk of a launch over tiles of TX x (TY*rows) columns: about four rounds of the resident
+// workgroups (`resident` per CU on 256 CUs).
+static int choose_kchunk(int nx, int ny, int kplanes, int rows = 1, int resident = 2)
 {
-    const long long tiles = (long long)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY);
-    // About four rounds of resident workgroups (three per CU on 256 CUs for the staged kernel).
-    const long long target = std::getenv("FTC_LES_V2") ? 2048 : 3072;
+    const long long tiles = (long long)((nx + TX - 1) / TX) * ((ny + TY * rows - 1) / (TY * rows));
+    const long long target = 4LL * 256 * resident;
     int chunks = (int)std::max<long long>(1, (target + tiles - 1) / tiles);
     int kchunk = (kplanes + chunks - 1) / chunks;
     if (const char *v = std::getenv("FTC_LES_KCHUNK")) kchunk = std::atoi(v);
@@ -67,9 +68,9 @@ static const void *kernel_of()
     return reinterpret_cast<const void *>(&les_kernel<MASK, SGS, STEP>);
 }
 
-static dim3 grid_of(const LesArgs &a, int kplanes)
+static dim3 grid_of(const LesArgs &a, int kplanes, int rows = 1)
 {
-    return dim3((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY, (kplanes + a.kchunk - 1) / a.kchunk);
+    return dim3((a.nx + TX - 1) / TX, (a.ny + TY * rows - 1) / (TY * rows), (kplanes + a.kchunk - 1) / a.kchunk);
 }
 
 static Sec3 box(std::size_t i0, std::size_t i1, std::size_t j0, std::size_t j1, std::size_t k0, std::size_t k1)
@@ -185,7 +186,14 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     a.sk = (long long)(nx + 2) * (ny + 2);
     a.dt = step ? *dt : 0.0;
     fill_coefficients(a, *dx, *dy, *dz, *cs);
-    a.kchunk = choose_kchunk(nx, ny, nz);
+    // The staged kernel: three-plane ring with one row per thread (three workgroups per CU) or two rows per
+    // thread (FTC_LES_ROWS=2, two per CU); FTC_LES_V2 selects the four-plane ring, FTC_LES_UNSTAGED the
+    // unstaged kernel, both for comparisons.
+    const bool staged = sgs && !std::getenv("FTC_LES_UNSTAGED");
+    const bool v2 = std::getenv("FTC_LES_V2") != nullptr;
+    const char *rows_env = std::getenv("FTC_LES_ROWS");
+    const int rows = staged && !v2 && rows_env && std::atoi(rows_env) == 2 ? 2 : 1;
+    a.kchunk = choose_kchunk(nx, ny, nz, rows, !staged || v2 ? 2 : (rows == 2 ? 2 : 3));
     const std::size_t X = nx, Y = ny, Z = nz;
     const Sec3 full = box(0, X + 2, 0, Y + 2, 0, Z + 2);
     std::vector<Binding3> b(8);
@@ -206,17 +214,16 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     b[6] = {w, {box(0, X + 2, 0, Y + 2, 0, Z + 1)}, {}, {}};
     b[7] = {s, {full}, {}, {}};
     const void *fn;
-    // FTC_LES_UNSTAGED selects the unstaged kernel and FTC_LES_V2 the four-plane staging, for comparisons.
-    const bool staged = sgs && !std::getenv("FTC_LES_UNSTAGED");
-    const bool v2 = std::getenv("FTC_LES_V2") != nullptr;
     if (staged && v2) fn = step ? reinterpret_cast<const void *>(&les_sgs_kernel<true>)
                                 : reinterpret_cast<const void *>(&les_sgs_kernel<false>);
-    else if (staged) fn = step ? reinterpret_cast<const void *>(&les_sgs3_kernel<true>)
-                               : reinterpret_cast<const void *>(&les_sgs3_kernel<false>);
+    else if (staged && rows == 2) fn = step ? reinterpret_cast<const void *>(&les_sgs3_kernel<true, 2>)
+                                            : reinterpret_cast<const void *>(&les_sgs3_kernel<false, 2>);
+    else if (staged) fn = step ? reinterpret_cast<const void *>(&les_sgs3_kernel<true, 1>)
+                               : reinterpret_cast<const void *>(&les_sgs3_kernel<false, 1>);
     else if (step) fn = sgs ? kernel_of<15, true, true>() : kernel_of<15, false, true>();
     else fn = sgs ? kernel_of<15, true, false>() : kernel_of<15, false, false>();
     const uint64_t key = 0x1e5000000000000full ^ ((uint64_t)step << 8) ^ ((uint64_t)sgs << 9);
-    return enqueue_kernel(ctx, key, fn, grid_of(a, nz), dim3(TX, TY, 1), 0, &a, sizeof a, b);
+    return enqueue_kernel(ctx, key, fn, grid_of(a, nz, rows), dim3(TX, TY, 1), 0, &a, sizeof a, b);
 }
 
 }  // namespace ftc

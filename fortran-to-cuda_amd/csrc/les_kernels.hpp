@@ -566,16 +566,100 @@ __global__ __launch_bounds__(NTH, 2) void les_sgs_kernel(const LesArgs a)
 }
 
 // ------------------------------------------------------------------------------------------------ v3
+// put_with_images with the column's x/y image offsets computed once per thread (they do not depend on k)
+// and the z images decided per plane (uniform over the workgroup): a cell off every boundary costs one
+// store. The outputs are not read again by the launch, so they are written nontemporally.
+struct ColumnImages {
+    int ox0, ox1;            // x offsets of the ghost images of the column (nox of them)
+    long long oy0, oy1;      // y offsets (noy of them)
+    int nox, noy;
+};
+
+__device__ __forceinline__ ColumnImages column_images(const LesArgs &a, int i, int j)
+{
+    ColumnImages c{0, 0, 0, 0, 0, 0};
+    if (i == 2) c.ox0 = a.nx, c.nox = 1;
+    if (i == a.nx + 1) {
+        if (c.nox) c.ox1 = -a.nx;
+        else c.ox0 = -a.nx;
+        c.nox++;
+    }
+    if (j == 2) c.oy0 = (long long)a.ny * a.sj, c.noy = 1;
+    if (j == a.ny + 1) {
+        if (c.noy) c.oy1 = -(long long)a.ny * a.sj;
+        else c.oy0 = -(long long)a.ny * a.sj;
+        c.noy++;
+    }
+    return c;
+}
+
+// The z ghost offsets of plane k for fields other than W (W has no z ghost written by the step).
+struct PlaneImages {
+    long long oz0, oz1;
+    int noz;
+};
+
+__device__ __forceinline__ PlaneImages plane_images(const LesArgs &a, int k)
+{
+    PlaneImages z{0, 0, 0};
+    if (k == 2) z.oz0 = -a.sk, z.noz = 1;
+    if (k == a.nz + 1) {
+        if (z.noz) z.oz1 = a.sk;
+        else z.oz0 = a.sk;
+        z.noz++;
+    }
+    return z;
+}
+
+__device__ __forceinline__ void put_value(double *F, long long q, double v) { __builtin_nontemporal_store(v, &F[q]); }
+
+// Writes val at p and at every ghost image put_with_images writes, in the same cases: the x and y images
+// (and their combinations), and for zmode != 1 the z images, negated for zmode 0.
+__device__ __forceinline__ void put_column(double *F, long long p, double val, const ColumnImages &c,
+                                           const PlaneImages &z, int zmode)
+{
+    const double zval = zmode == 0 ? -val : val;
+    const int noz = zmode == 1 ? 0 : z.noz;
+    auto with_z = [&](long long q) {
+        put_value(F, q, val);
+        if (noz > 0) put_value(F, q + z.oz0, zval);
+        if (noz > 1) put_value(F, q + z.oz1, zval);
+    };
+    with_z(p);
+    if (c.nox | c.noy) {
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int y = 0; y < 3; ++y) {
+                if ((x == 0 && y == 0) || x > c.nox || y > c.noy) continue;
+                const long long q = p + (x == 1 ? c.ox0 : (x == 2 ? c.ox1 : 0)) + (y == 1 ? c.oy0 : (y == 2 ? c.oy1 : 0));
+                with_z(q);
+            }
+    }
+}
+
 // The v2 staging with the z-direction fluxes carried in registers. In flux form the lower z-face term of
 // cell k is the upper z-face term of cell k-1, computed from the same operands in the same order, so a
 // thread marching k keeps the upper terms of step k-1 (and its own column's k-1 velocities) and reads
 // nothing of plane k-1 from LDS. The rings shrink to three velocity planes (k .. k+2), two scalar planes
 // and two viscosity planes: 52 KB, three workgroups per CU. Every value is the one v2 and the oracle
 // compute; only where it is kept changes.
+// ROWS = 2 gives each thread two adjacent rows of a 64 x 8 tile (80 KB, two workgroups per CU): less halo
+// per cell (viscosity ring 1.29x instead of 1.55x, velocity ring 1.59x instead of 2.13x) and the per-step
+// bookkeeping shared by two cells.
+template <int ROWS>
+struct Sgs3Tile {
+    static constexpr int TYT = TY * ROWS;            // rows of the tile
+    static constexpr int VYT = TYT + 4, CYT = TYT + 2;
+    static constexpr int VC = VX * VYT, CC = CX * CYT;
+    static constexpr int VRT = (VC + NTH - 1) / NTH, CRT = (CC + NTH - 1) / NTH;
+};
+
+template <int ROWS>
 struct Sgs3Lds {
-    double vel[3][3][VY][VX];  // slot = plane % 3
-    double sca[2][CY][CX];     // slot = plane & 1
-    double nu[2][CY][CX];      // slot = plane & 1
+    double vel[3][3][Sgs3Tile<ROWS>::VYT][VX];  // slot = plane % 3
+    double sca[2][Sgs3Tile<ROWS>::CYT][CX];     // slot = plane & 1
+    double nu[2][Sgs3Tile<ROWS>::CYT][CX];      // slot = plane & 1
 };
 
 // The upper z-face terms of a cell (advection and diffusion of u, v, w and s), which are the lower terms
@@ -585,16 +669,18 @@ struct ZCarry {
     double u, v, w, wxp, wyp;               // U, V, W at (i, j), W at (i+1, j) and (i, j+1)
 };
 
-template <bool STEP>
-__device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int bx, int by, int bz)
+template <bool STEP, int ROWS = 1>
+__device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS> &L, int bx, int by, int bz)
 {
+    using T = Sgs3Tile<ROWS>;
+    constexpr int VR = T::VRT, CR = T::CRT, VCELLS = T::VC, CCELLS = T::CC;
     const double *__restrict__ gU = a.U;
     const double *__restrict__ gV = a.V;
     const double *__restrict__ gW = a.W;
     const double *__restrict__ gS = a.S;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int i0 = 2 + bx * TX, j0 = 2 + by * TY;
-    const int i = i0 + tx, j = j0 + ty;
+    const int i0 = 2 + bx * TX, j0 = 2 + by * T::TYT;
+    const int i = i0 + tx;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int k0 = 2 + bz * a.kchunk;
     if (k0 > nz + 1) return;  // uniform over the workgroup
@@ -688,15 +774,14 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int 
         }
     };
 
-    const bool inside = i <= nx + 1 && j <= ny + 1;
     const double dt = a.dt;
     const double c25x = a.c25[0], c25y = a.c25[1], c25z = a.c25[2];
     const double Cz = a.c50[2];
-#define VU(s, di, dj) L.vel[s][0][ty + 2 + (dj)][tx + 2 + (di)]
-#define VV(s, di, dj) L.vel[s][1][ty + 2 + (dj)][tx + 2 + (di)]
-#define VW(s, di, dj) L.vel[s][2][ty + 2 + (dj)][tx + 2 + (di)]
-#define SS(s, di, dj) L.sca[s][ty + 1 + (dj)][tx + 1 + (di)]
-#define NV(s, di, dj) L.nu[s][ty + 1 + (dj)][tx + 1 + (di)]
+#define VU(s, di, dj) L.vel[s][0][ly + 2 + (dj)][tx + 2 + (di)]
+#define VV(s, di, dj) L.vel[s][1][ly + 2 + (dj)][tx + 2 + (di)]
+#define VW(s, di, dj) L.vel[s][2][ly + 2 + (dj)][tx + 2 + (di)]
+#define SS(s, di, dj) L.sca[s][ly + 1 + (dj)][tx + 1 + (di)]
+#define NV(s, di, dj) L.nu[s][ly + 1 + (dj)][tx + 1 + (di)]
 
     // Prologue: planes k0-2 .. k0 give nu(k0-1); plane k0+1 replaces k0-2 and gives nu(k0); then the upper
     // terms of plane k0-1 (the lower terms of k0) are formed as a step forms them.
@@ -719,8 +804,12 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int 
     if (clamped) nu_plane(k0 - 1);
     nu_plane(k0);
     __syncthreads();
-    ZCarry zc{};
-    if (inside) {
+    ZCarry zcr[ROWS] = {};
+#pragma unroll
+    for (int rr = 0; rr < ROWS; ++rr) {
+        const int ly = ty * ROWS + rr, j = j0 + ly;
+        if (i <= nx + 1 && j <= ny + 1) {
+        ZCarry &zc = zcr[rr];
         const int kv = k0 - 1;
         const int s0 = kv % 3, sp = (kv + 1) % 3, c0 = kv & 1, cp = (kv + 1) & 1;
         const double u0 = VU(s0, 0, 0), uzp = VU(sp, 0, 0);
@@ -741,6 +830,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int 
         zc.w = w0;
         zc.wxp = wxp;
         zc.wyp = wyp;
+        }
     }
     fetch_vel(k0 + 2);
     fetch_sca(k0 + 1);
@@ -754,7 +844,13 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int 
         __syncthreads();
         nu_plane(k + 1);
         __syncthreads();
-        if (inside) {
+#pragma unroll
+        for (int rr = 0; rr < ROWS; ++rr) {
+            const int ly = ty * ROWS + rr, j = j0 + ly;
+            if (i <= nx + 1 && j <= ny + 1) {
+            ZCarry &zc = zcr[rr];
+            const ColumnImages ci = column_images(a, i, j);
+            const PlaneImages zi = plane_images(a, k);
             const int s0 = k % 3, sp = (k + 1) % 3;
             const int c0 = k & 1, cp = (k + 1) & 1;
             const double u0 = VU(s0, 0, 0), uxp = VU(s0, 1, 0), uxm = VU(s0, -1, 0), uyp = VU(s0, 0, 1);
@@ -791,7 +887,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int 
                 double fzp = up.du;
                 double fzm = zc.du;
                 t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
-                if constexpr (STEP) put_with_images(a, a.OU, p, u0 + dt * t, i, j, k, 0);
+                if constexpr (STEP) put_column(a.OU, p, u0 + dt * t, ci, zi, 0);
                 else a.OU[p] = t;
             }
             // V: CDV (cvd.f90:44-98) then the diffusion of v.
@@ -818,7 +914,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int 
                 double fzp = up.dv;
                 double fzm = zc.dv;
                 t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
-                if constexpr (STEP) put_with_images(a, a.OV, p, v0 + dt * t, i, j, k, 0);
+                if constexpr (STEP) put_column(a.OV, p, v0 + dt * t, ci, zi, 0);
                 else a.OV[p] = t;
             }
             // W on the interior faces: CDW (cdw.f90:44-98) then the diffusion of w.
@@ -848,7 +944,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int 
                 double fzp = up.dw;
                 double fzm = zc.dw;
                 t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
-                if constexpr (STEP) put_with_images(a, a.OW, p, w0 + dt * t, i, j, k, 1);
+                if constexpr (STEP) put_column(a.OW, p, w0 + dt * t, ci, zi, 1);
                 else a.OW[p] = t;
             }
             // S: flux-form advection then the diffusion of s.
@@ -869,7 +965,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int 
                 double fzp = up.ds;
                 double fzm = zc.ds;
                 t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
-                if constexpr (STEP) put_with_images(a, a.OS, p, s0v + dt * t, i, j, k, 2);
+                if constexpr (STEP) put_column(a.OS, p, s0v + dt * t, ci, zi, 2);
                 else a.OS[p] = t;
             }
             up.u = u0;
@@ -878,6 +974,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int 
             up.wxp = wxp;
             up.wyp = wyp;
             zc = up;
+            }
         }
         __syncthreads();
     }
@@ -888,11 +985,11 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds &L, int 
 #undef NV
 }
 
-template <bool STEP>
-__global__ __launch_bounds__(NTH, 3) void les_sgs3_kernel(const LesArgs a)
+template <bool STEP, int ROWS>
+__global__ __launch_bounds__(NTH, ROWS == 1 ? 3 : 2) void les_sgs3_kernel(const LesArgs a)
 {
-    __shared__ Sgs3Lds L;
-    les_sgs3_body<STEP>(a, L, blockIdx.x, blockIdx.y, blockIdx.z);
+    __shared__ Sgs3Lds<ROWS> L;
+    les_sgs3_body<STEP, ROWS>(a, L, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 }  // namespace ftc

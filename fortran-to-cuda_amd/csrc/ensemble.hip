@@ -65,7 +65,10 @@ constexpr int kCols = 64;
 
 // ------------------------------------------------------------------------------------------------ kernels
 template <bool STEP>
-__global__ __launch_bounds__(NTH, 3) void ens_les_kernel(const LesArgs *args, const TileItem *items, int n)
+// args is __restrict__: no store of the step writes a member's argument block, so its fields can stay in
+// registers across the body's stores.
+__global__ __launch_bounds__(NTH, 3) void ens_les_kernel(const LesArgs *__restrict__ args,
+                                                         const TileItem *__restrict__ items, int n)
 {
     __shared__ Sgs3Lds<1> L;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
@@ -80,10 +83,11 @@ __global__ __launch_bounds__(256) void ens_divergence_kernel(const EnsMember *ms
 {
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const TileItem t = items[it];
-        const EnsMember &m = ms[t.member];
+        const EnsMember m = ms[t.member];
         const int a = 1 + t.bx * 64 + threadIdx.x, b = 1 + t.by * 4 + threadIdx.y;
         if (a > m.nx || b > m.ny) continue;
-        const double *U = m.f[src][0], *V = m.f[src][1], *W = m.f[src][2];
+        const double *U = src ? m.f[1][0] : m.f[0][0], *V = src ? m.f[1][1] : m.f[0][1];
+        const double *W = src ? m.f[1][2] : m.f[0][2];
         for (int c = 1; c <= m.nz; ++c) {
             const long long p = a + b * m.sj + c * m.sk;
             const long long q = (a - 1) + (long long)m.nx * ((b - 1) + (long long)m.ny * (c - 1));
@@ -92,26 +96,33 @@ __global__ __launch_bounds__(256) void ens_divergence_kernel(const EnsMember *ms
     }
 }
 
-// U -= dt grad(phi) on the interior faces of buffer set `dst`, in place, with the halo images.
+// U -= dt grad(phi) on the interior faces of buffer set `dst`, in place, with the halo images (the same
+// stores and images as put_with_images; the column's image offsets and phi's periodic neighbours are
+// formed once per column).
 __global__ __launch_bounds__(256) void ens_projection_kernel(const EnsMember *ms, const LesArgs *args,
                                                              const TileItem *items, int n, int dst)
 {
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const TileItem t = items[it];
-        const EnsMember &m = ms[t.member];
-        const LesArgs &g = args[t.member];
+        const EnsMember m = ms[t.member];
+        const LesArgs g = args[t.member];
         const int a = 1 + t.bx * 64 + threadIdx.x, b = 1 + t.by * 4 + threadIdx.y;
         if (a > m.nx || b > m.ny) continue;
-        double *U = m.f[dst][0], *V = m.f[dst][1], *W = m.f[dst][2];
+        double *U = dst ? m.f[1][0] : m.f[0][0], *V = dst ? m.f[1][1] : m.f[0][1];
+        double *W = dst ? m.f[1][2] : m.f[0][2];
         const long long plane = (long long)m.nx * m.ny;
+        const ColumnImages ci = column_images(g, a + 1, b + 1);
+        const long long q0 = (a - 1) + (long long)m.nx * (b - 1);
+        const long long qx0 = (a == m.nx ? 0 : a) + (long long)m.nx * (b - 1);  // (a % nx), row b-1
+        const long long qy0 = (a - 1) + (long long)m.nx * (b == m.ny ? 0 : b);  // column a-1, row (b % ny)
         for (int c = 1; c <= m.nz; ++c) {
             const long long p = a + b * m.sj + c * m.sk;
-            const long long q = (a - 1) + (long long)m.nx * (b - 1) + plane * (c - 1);
-            const long long qx = (a % m.nx) + (long long)m.nx * (b - 1) + plane * (c - 1);
-            const long long qy = (a - 1) + (long long)m.nx * (b % m.ny) + plane * (c - 1);
-            put_with_images(g, U, p, U[p] - m.gx * (m.phi[qx] - m.phi[q]), a + 1, b + 1, c + 1, 0);
-            put_with_images(g, V, p, V[p] - m.gy * (m.phi[qy] - m.phi[q]), a + 1, b + 1, c + 1, 0);
-            if (c < m.nz) put_with_images(g, W, p, W[p] - m.gz * (m.phi[q + plane] - m.phi[q]), a + 1, b + 1, c + 1, 1);
+            const long long o = plane * (c - 1);
+            const double f = m.phi[q0 + o];
+            const PlaneImages zi = plane_images(g, c + 1);
+            put_column(U, p, U[p] - m.gx * (m.phi[qx0 + o] - f), ci, zi, 0);
+            put_column(V, p, V[p] - m.gy * (m.phi[qy0 + o] - f), ci, zi, 0);
+            if (c < m.nz) put_column(W, p, W[p] - m.gz * (m.phi[q0 + o + plane] - f), ci, zi, 1);
         }
     }
 }
@@ -121,7 +132,7 @@ __global__ __launch_bounds__(256) void ens_residual_kernel(const EnsMember *ms, 
 {
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const TileItem t = items[it];
-        const EnsMember &m = ms[t.member];
+        const EnsMember m = ms[t.member];
         const int i = t.bx * 64 + threadIdx.x, j = t.by * 4 + threadIdx.y;
         if (i >= m.nx || j >= m.ny) continue;
         const int nx = m.nx, ny = m.ny, nz = m.nz;
@@ -142,30 +153,53 @@ __global__ __launch_bounds__(256) void ens_residual_kernel(const EnsMember *ms, 
     }
 }
 
-// Per member (one workgroup of 256): 256 strided partial sums of phi in order, folded in index order.
-__global__ __launch_bounds__(256) void ens_mean_kernel(const EnsMember *ms, int members)
+// Partial sums of phi for the mean removal, one per tile item (64 x 4 columns over all planes): each column
+// summed over k in order, then the 256 column sums in thread order. part[base + it] receives item it.
+__global__ __launch_bounds__(256) void ens_mean_partial_kernel(const EnsMember *ms, const TileItem *items, int n,
+                                                               double *part, int base)
 {
-    for (int mi = blockIdx.x; mi < members; mi += gridDim.x) {
-        const EnsMember &m = ms[mi];
-        const long long N = (long long)m.nx * m.ny * m.nz;
+    __shared__ double col[256];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const TileItem t = items[it];
+        const EnsMember m = ms[t.member];
+        const int i = t.bx * 64 + threadIdx.x, j = t.by * 4 + threadIdx.y;
         double acc = 0.0;
-        for (long long e = threadIdx.x; e < N; e += 256) acc = acc + m.phi[e];
-        m.part[threadIdx.x] = acc;
+        if (i < m.nx && j < m.ny) {
+            const long long plane = (long long)m.nx * m.ny;
+            const double *ph = m.phi + i + (long long)m.nx * j;
+            for (int k = 0; k < m.nz; ++k) acc = acc + ph[plane * k];
+        }
+        col[tid] = acc;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int f = 0; f < 256; ++f) s = s + m.part[f];
-            m.part[256] = s / (double)N;
+        if (tid == 0) {
+            double sum = 0.0;
+            for (int c = 0; c < 256; ++c) sum = sum + col[c];
+            part[base + it] = sum;
         }
         __syncthreads();
     }
+}
+
+// The mean of members m0 .. m0+count-1: each member's tile partials folded in tile order, stored in
+// part[256] of the member.
+__global__ __launch_bounds__(64) void ens_mean_fold_kernel(const EnsMember *ms, const int2 *tiles, const double *part,
+                                                           int m0, int count)
+{
+    const int m = m0 + blockIdx.x * 64 + threadIdx.x;
+    if (m >= m0 + count) return;
+    const EnsMember mm = ms[m];
+    const int2 r = tiles[m];
+    double sum = 0.0;
+    for (int q = 0; q < r.y; ++q) sum = sum + part[r.x + q];
+    mm.part[256] = sum / ((double)mm.nx * mm.ny * mm.nz);
 }
 
 __global__ __launch_bounds__(256) void ens_subtract_kernel(const EnsMember *ms, const TileItem *items, int n)
 {
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const TileItem t = items[it];
-        const EnsMember &m = ms[t.member];
+        const EnsMember m = ms[t.member];
         const int i = t.bx * 64 + threadIdx.x, j = t.by * 4 + threadIdx.y;
         if (i >= m.nx || j >= m.ny) continue;
         const double mean = m.part[256];
@@ -184,7 +218,7 @@ __global__ __launch_bounds__(kCols) void ens_tridiag_kernel(const EnsMember *ms,
     extern __shared__ double lds[];
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const ColItem t = items[it];
-        const EnsMember &m = ms[t.member];
+        const EnsMember m = ms[t.member];
         const long long plane = (long long)m.nx * m.ny;
         const long long col = (long long)t.block * kCols + threadIdx.x;
         if (col >= plane) continue;
@@ -270,6 +304,8 @@ struct ftc_ensemble {
     TileItem *d_les = nullptr, *d_tiles = nullptr;
     GemmItem *d_gemm[6] = {};
     ColItem *d_cols = nullptr;
+    double *d_tile_part = nullptr;  // one mean partial per tile item
+    int2 *d_tile_ranges = nullptr;  // (first tile item, count) per member
     std::vector<Range> r_les, r_tiles, r_gemm[6], r_cols;
     int n_les = 0, n_tiles = 0, n_gemm[6] = {}, n_cols = 0;
     void *items_arena = nullptr;
@@ -358,7 +394,8 @@ static int build_items(ftc_ensemble &e)
     e.n_tiles = (int)tiles.size();
     e.n_cols = (int)cols.size();
     std::size_t bytes = up256(les.size() * sizeof(TileItem)) + up256(tiles.size() * sizeof(TileItem)) +
-                        up256(cols.size() * sizeof(ColItem));
+                        up256(cols.size() * sizeof(ColItem)) + up256(tiles.size() * sizeof(double)) +
+                        up256(e.count * sizeof(int2));
     for (int s = 0; s < 6; ++s) {
         e.n_gemm[s] = (int)gemm[s].size();
         bytes += up256(gemm[s].size() * sizeof(GemmItem));
@@ -379,6 +416,11 @@ static int build_items(ftc_ensemble &e)
     e.d_les = (TileItem *)put(les.data(), les.size() * sizeof(TileItem));
     e.d_tiles = (TileItem *)put(tiles.data(), tiles.size() * sizeof(TileItem));
     e.d_cols = (ColItem *)put(cols.data(), cols.size() * sizeof(ColItem));
+    std::vector<int2> ranges(e.count);
+    for (int m = 0; m < e.count; ++m) ranges[m] = make_int2(e.r_tiles[m].start, e.r_tiles[m].count);
+    e.d_tile_ranges = (int2 *)put(ranges.data(), ranges.size() * sizeof(int2));
+    e.d_tile_part = (double *)p;
+    p += up256(tiles.size() * sizeof(double));
     for (int s = 0; s < 6; ++s) e.d_gemm[s] = (GemmItem *)put(gemm[s].data(), gemm[s].size() * sizeof(GemmItem));
     // LES arguments carry the k chunk.
     for (int par = 0; par < 2; ++par) {
@@ -438,9 +480,11 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
                                   e.d_members, e.d_tiles + start, n);
         break;
     case kMean:
-        if (all) hipLaunchKernelGGL(ens_mean_kernel, dim3(std::min(e.count, e.cus * e.blocks_per_cu)), dim3(256), 0,
-                                    s, e.d_members, e.count);
-        else hipLaunchKernelGGL(ens_mean_kernel, dim3(1), dim3(256), 0, s, e.d_members + member, 1);
+        n = pick(e.r_tiles, e.n_tiles, &start);
+        if (n) hipLaunchKernelGGL(ens_mean_partial_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
+                                  e.d_members, e.d_tiles + start, n, e.d_tile_part, start);
+        hipLaunchKernelGGL(ens_mean_fold_kernel, dim3(all ? (e.count + 63) / 64 : 1), dim3(64), 0, s, e.d_members,
+                           e.d_tile_ranges, e.d_tile_part, all ? 0 : member, all ? e.count : 1);
         break;
     case kTri: {
         n = pick(e.r_cols, e.n_cols, &start);

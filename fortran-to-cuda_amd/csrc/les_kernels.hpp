@@ -742,7 +742,6 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS> &L
         for (int r = 0; r < CR; ++r)
             if (ccell[r] >= 0) (&L.sca[slot][0][0])[ccell[r]] = rs[r];
     };
-    const double *q = a.sgs;
     // nu at plane qk (clamped to the interior planes: zero gradient in z) over the tile and its ring.
     auto nu_plane = [&](int qk) {
         const int kk = qk < 2 ? 2 : (qk > nz + 1 ? nz + 1 : qk);
@@ -754,15 +753,15 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS> &L
 #define QU(s, di, dj) L.vel[s][0][vy + (dj)][vx + (di)]
 #define QV(s, di, dj) L.vel[s][1][vy + (dj)][vx + (di)]
 #define QW(s, di, dj) L.vel[s][2][vy + (dj)][vx + (di)]
-            double s11 = (QU(s0, 0, 0) - QU(s0, -1, 0)) * q[0];
-            double s22 = (QV(s0, 0, 0) - QV(s0, 0, -1)) * q[1];
-            double s33 = (QW(s0, 0, 0) - QW(sm, 0, 0)) * q[2];
-            double dudy = ((QU(s0, 0, 1) + QU(s0, -1, 1)) - (QU(s0, 0, -1) + QU(s0, -1, -1))) * q[4];
-            double dvdx = ((QV(s0, 1, 0) + QV(s0, 1, -1)) - (QV(s0, -1, 0) + QV(s0, -1, -1))) * q[3];
-            double dudz = ((QU(sp, 0, 0) + QU(sp, -1, 0)) - (QU(sm, 0, 0) + QU(sm, -1, 0))) * q[5];
-            double dwdx = ((QW(s0, 1, 0) + QW(sm, 1, 0)) - (QW(s0, -1, 0) + QW(sm, -1, 0))) * q[3];
-            double dvdz = ((QV(sp, 0, 0) + QV(sp, 0, -1)) - (QV(sm, 0, 0) + QV(sm, 0, -1))) * q[5];
-            double dwdy = ((QW(s0, 0, 1) + QW(sm, 0, 1)) - (QW(s0, 0, -1) + QW(sm, 0, -1))) * q[4];
+            double s11 = (QU(s0, 0, 0) - QU(s0, -1, 0)) * a.sgs[0];
+            double s22 = (QV(s0, 0, 0) - QV(s0, 0, -1)) * a.sgs[1];
+            double s33 = (QW(s0, 0, 0) - QW(sm, 0, 0)) * a.sgs[2];
+            double dudy = ((QU(s0, 0, 1) + QU(s0, -1, 1)) - (QU(s0, 0, -1) + QU(s0, -1, -1))) * a.sgs[4];
+            double dvdx = ((QV(s0, 1, 0) + QV(s0, 1, -1)) - (QV(s0, -1, 0) + QV(s0, -1, -1))) * a.sgs[3];
+            double dudz = ((QU(sp, 0, 0) + QU(sp, -1, 0)) - (QU(sm, 0, 0) + QU(sm, -1, 0))) * a.sgs[5];
+            double dwdx = ((QW(s0, 1, 0) + QW(sm, 1, 0)) - (QW(s0, -1, 0) + QW(sm, -1, 0))) * a.sgs[3];
+            double dvdz = ((QV(sp, 0, 0) + QV(sp, 0, -1)) - (QV(sm, 0, 0) + QV(sm, 0, -1))) * a.sgs[5];
+            double dwdy = ((QW(s0, 0, 1) + QW(sm, 0, 1)) - (QW(s0, 0, -1) + QW(sm, 0, -1))) * a.sgs[4];
 #undef QU
 #undef QV
 #undef QW
@@ -770,7 +769,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS> &L
             double s13 = 0.5 * (dudz + dwdx);
             double s23 = 0.5 * (dvdz + dwdy);
             double ss = 2.0 * ((s11 * s11 + s22 * s22) + s33 * s33) + 4.0 * ((s12 * s12 + s13 * s13) + s23 * s23);
-            nuo[lj][li] = q[9] * sqrt(ss);
+            nuo[lj][li] = a.sgs[9] * sqrt(ss);
         }
     };
 
@@ -886,7 +885,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS> &L
                 up.du = exz0 * (uzp - u0);
                 double fzp = up.du;
                 double fzm = zc.du;
-                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                t = t + (((fxp - fxm) * a.sgs[6] + (fyp - fym) * a.sgs[7]) + (fzp - fzm) * a.sgs[8]);
                 if constexpr (STEP) put_column(a.OU, p, u0 + dt * t, ci, zi, 0);
                 else a.OU[p] = t;
             }
@@ -913,7 +912,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS> &L
                 up.dv = eyz0 * (vzp - v0);
                 double fzp = up.dv;
                 double fzm = zc.dv;
-                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                t = t + (((fxp - fxm) * a.sgs[6] + (fyp - fym) * a.sgs[7]) + (fzp - fzm) * a.sgs[8]);
                 if constexpr (STEP) put_column(a.OV, p, v0 + dt * t, ci, zi, 0);
                 else a.OV[p] = t;
             }
@@ -943,7 +942,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS> &L
                 up.dw = n001 * (wzp - w0);
                 double fzp = up.dw;
                 double fzm = zc.dw;
-                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                t = t + (((fxp - fxm) * a.sgs[6] + (fyp - fym) * a.sgs[7]) + (fzp - fzm) * a.sgs[8]);
                 if constexpr (STEP) put_column(a.OW, p, w0 + dt * t, ci, zi, 1);
                 else a.OW[p] = t;
             }
@@ -964,7 +963,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS> &L
                 up.ds = (0.5 * (n000 + n001)) * (szp - s0v);
                 double fzp = up.ds;
                 double fzm = zc.ds;
-                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
+                t = t + (((fxp - fxm) * a.sgs[6] + (fyp - fym) * a.sgs[7]) + (fzp - fzm) * a.sgs[8]);
                 if constexpr (STEP) put_column(a.OS, p, s0v + dt * t, ci, zi, 2);
                 else a.OS[p] = t;
             }

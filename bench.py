@@ -217,8 +217,13 @@ def main() -> int:
         n = 1 << 28
         ms = ctypes.c_float()
         scope.check(lib, lib.ftc_copy_probe(n, 20, ctypes.byref(ms)))
-        print(json.dumps({"copy_probe": {"bytes_per_launch": 16 * n, "ms_per_launch": round(ms.value, 4),
-                                         "GB_per_s": round(16 * n / (ms.value * 1e-3) / 1e9, 1)}}), flush=True)
+        out = {"copy_probe": {"bytes_per_launch": 16 * n, "ms_per_launch": round(ms.value, 4),
+                              "GB_per_s": round(16 * n / (ms.value * 1e-3) / 1e9, 1)}}
+        for kind, name, factor in ((1, "copy_nontemporal", 16), (2, "read", 8), (3, "write", 8)):
+            scope.check(lib, lib.ftc_bandwidth_probe(kind, n, 10, ctypes.byref(ms)))
+            out[name] = {"bytes_per_launch": factor * n, "ms_per_launch": round(ms.value, 4),
+                         "GB_per_s": round(factor * n / (ms.value * 1e-3) / 1e9, 1)}
+        print(json.dumps(out), flush=True)
         return 0
 
     nx, ny, nz = args.nx, args.ny, args.nz
@@ -273,7 +278,7 @@ def main() -> int:
     I = nx * ny * nz
     bytes_alg = 8 * (4 * P + 4 * I)
     achieved = bytes_alg / (gpu_ms_per_step * 1e-3) / 1e9
-    traffic = traffic_from_profiles("les_kernel<15,true,true>")
+    traffic = traffic_from_profiles("ftc::les_sgs3_kernel<true, 1>")
 
     extras = {}
     if not args.no_extras:

@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256) void ens_divergence_kernel(const EnsMember *ms
         if (a > m.nx || b > m.ny) continue;
         const double *U = src ? m.f[1][0] : m.f[0][0], *V = src ? m.f[1][1] : m.f[0][1];
         const double *W = src ? m.f[1][2] : m.f[0][2];
+#pragma unroll 4
         for (int c = 1; c <= m.nz; ++c) {
             const long long p = a + b * m.sj + c * m.sk;
             const long long q = (a - 1) + (long long)m.nx * ((b - 1) + (long long)m.ny * (c - 1));
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(256) void ens_projection_kernel(const EnsMember *ms
         const long long q0 = (a - 1) + (long long)m.nx * (b - 1);
         const long long qx0 = (a == m.nx ? 0 : a) + (long long)m.nx * (b - 1);  // (a % nx), row b-1
         const long long qy0 = (a - 1) + (long long)m.nx * (b == m.ny ? 0 : b);  // column a-1, row (b % ny)
+#pragma unroll 4
         for (int c = 1; c <= m.nz; ++c) {
             const long long p = a + b * m.sj + c * m.sk;
             const long long o = plane * (c - 1);
@@ -139,6 +141,7 @@ __global__ __launch_bounds__(256) void ens_residual_kernel(const EnsMember *ms, 
         const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
         const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
         const long long plane = (long long)nx * ny;
+#pragma unroll 4
         for (int k = 0; k < nz; ++k) {
             const int km = k == 0 ? 0 : k - 1, kp = k == nz - 1 ? nz - 1 : k + 1;
 #define P(x, y, z) m.phi[(x) + (long long)nx * (y) + plane * (z)]
@@ -168,6 +171,7 @@ __global__ __launch_bounds__(256) void ens_mean_partial_kernel(const EnsMember *
         if (i < m.nx && j < m.ny) {
             const long long plane = (long long)m.nx * m.ny;
             const double *ph = m.phi + i + (long long)m.nx * j;
+#pragma unroll 8
             for (int k = 0; k < m.nz; ++k) acc = acc + ph[plane * k];
         }
         col[tid] = acc;
@@ -204,6 +208,7 @@ __global__ __launch_bounds__(256) void ens_subtract_kernel(const EnsMember *ms, 
         if (i >= m.nx || j >= m.ny) continue;
         const double mean = m.part[256];
         const long long plane = (long long)m.nx * m.ny;
+#pragma unroll 4
         for (int k = 0; k < m.nz; ++k) {
             double &v = m.phi[i + (long long)m.nx * j + plane * k];
             v = v - mean;

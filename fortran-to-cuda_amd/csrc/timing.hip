@@ -24,6 +24,54 @@ __global__ __launch_bounds__(256) void ftc_copy_kernel(const double *__restrict_
     if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) dst[n - 1] = src[n - 1];
 }
 
+typedef double ftc_d2v __attribute__((ext_vector_type(2)));
+
+// The copy above with nontemporal loads and stores.
+__global__ __launch_bounds__(256) void ftc_copy_nt_kernel(const double *__restrict__ src, double *__restrict__ dst,
+                                                          size_t n)
+{
+    const ftc_d2v *__restrict__ s2 = reinterpret_cast<const ftc_d2v *>(src);
+    ftc_d2v *__restrict__ d2 = reinterpret_cast<ftc_d2v *>(dst);
+    const size_t n2 = n / 2, stride = (size_t)gridDim.x * blockDim.x;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; e + 3 * stride < n2; e += 4 * stride) {
+        const ftc_d2v a0 = __builtin_nontemporal_load(&s2[e]), a1 = __builtin_nontemporal_load(&s2[e + stride]);
+        const ftc_d2v a2 = __builtin_nontemporal_load(&s2[e + 2 * stride]);
+        const ftc_d2v a3 = __builtin_nontemporal_load(&s2[e + 3 * stride]);
+        __builtin_nontemporal_store(a0, &d2[e]);
+        __builtin_nontemporal_store(a1, &d2[e + stride]);
+        __builtin_nontemporal_store(a2, &d2[e + 2 * stride]);
+        __builtin_nontemporal_store(a3, &d2[e + 3 * stride]);
+    }
+    for (; e < n2; e += stride) d2[e] = s2[e];
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) dst[n - 1] = src[n - 1];
+}
+
+// Reads n doubles (16 bytes per lane, four in flight); the sum is stored only if it equals `never`, which
+// the zero-filled input never gives, so the loads cannot be dropped.
+__global__ __launch_bounds__(256) void ftc_read_kernel(const double *__restrict__ src, double *__restrict__ out,
+                                                       size_t n, double never)
+{
+    const double2 *__restrict__ s2 = reinterpret_cast<const double2 *>(src);
+    const size_t n2 = n / 2, stride = (size_t)gridDim.x * blockDim.x;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    for (; e + 3 * stride < n2; e += 4 * stride) {
+        const double2 a0 = s2[e], a1 = s2[e + stride], a2 = s2[e + 2 * stride], a3 = s2[e + 3 * stride];
+        acc += ((a0.x + a0.y) + (a1.x + a1.y)) + ((a2.x + a2.y) + (a3.x + a3.y));
+    }
+    for (; e < n2; e += stride) acc += s2[e].x + s2[e].y;
+    if (acc == never) out[0] = acc;
+}
+
+// Writes n doubles (16 bytes per lane).
+__global__ __launch_bounds__(256) void ftc_write_kernel(double *__restrict__ dst, size_t n)
+{
+    double2 *__restrict__ d2 = reinterpret_cast<double2 *>(dst);
+    const size_t n2 = n / 2, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += stride) d2[e] = double2{1.0, 2.0};
+}
+
 __global__ void ftc_iota(double *__restrict__ x, long long n, double scale)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -53,6 +101,49 @@ int ftc_copy_probe(size_t n, int reps, float *ms_per_launch)
         hipLaunchKernelGGL(ftc_copy_kernel, dim3(blocks), dim3(256), 0, s, a, b, n);
         (void)hipEventRecord(e0, s);
         for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(ftc_copy_kernel, dim3(blocks), dim3(256), 0, s, a, b, n);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) == hipSuccess) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            *ms_per_launch = ms / reps;
+            status = FORT_SCOPE_OK;
+        } else {
+            status = FORT_SCOPE_EXECUTION;
+        }
+    }
+    (void)hipGetLastError();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return status;
+}
+
+// The HBM rates a kernel can reach on this part: kind 0 copy (the kernel above), 1 copy with nontemporal
+// loads and stores, 2 read only, 3 write only, over n doubles (2n for the copies), reps timed launches after
+// one warm-up launch on a private stream. n must be even.
+int ftc_bandwidth_probe(int kind, size_t n, int reps, float *ms_per_launch)
+{
+    if (!ms_per_launch || n == 0 || (n & 1) || reps <= 0 || kind < 0 || kind > 3) return FORT_SCOPE_ARGUMENT;
+    double *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int status = FORT_SCOPE_RESOURCE;
+    if (hipMalloc(&a, n * sizeof(double)) == hipSuccess && hipMalloc(&b, n * sizeof(double)) == hipSuccess &&
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+        hipEventCreate(&e1) == hipSuccess) {
+        (void)hipMemsetAsync(a, 0, n * sizeof(double), s);
+        const unsigned blocks = 256 * 32;
+        auto launch = [&]() {
+            if (kind == 0) hipLaunchKernelGGL(ftc_copy_kernel, dim3(blocks), dim3(256), 0, s, a, b, n);
+            else if (kind == 1) hipLaunchKernelGGL(ftc_copy_nt_kernel, dim3(blocks), dim3(256), 0, s, a, b, n);
+            else if (kind == 2) hipLaunchKernelGGL(ftc_read_kernel, dim3(blocks), dim3(256), 0, s, a, b, n, 12345.678);
+            else hipLaunchKernelGGL(ftc_write_kernel, dim3(blocks), dim3(256), 0, s, b, n);
+        };
+        launch();
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r) launch();
         (void)hipEventRecord(e1, s);
         if (hipEventSynchronize(e1) == hipSuccess) {
             float ms = 0.0f;

@@ -68,6 +68,9 @@ int ftc_device_count(void);
 /* A plain device-to-device copy of n doubles, launched reps times: the measured HBM ceiling and the
  * calibration kernel of the HBM byte counters. No reference counterpart. */
 int ftc_copy_probe(size_t n, int reps, float *ms_per_launch);
+/* The HBM ceilings of this part: kind 0 copy (the probe above), 1 copy with nontemporal loads and stores,
+ * 2 read only, 3 write only, over n doubles (n even). No reference counterpart. */
+int ftc_bandwidth_probe(int kind, size_t n, int reps, float *ms_per_launch);
 /* The device function of a kernel (double *x, long long n, double scale) that writes x[i] = scale * i for
  * i < n with one thread per element (256-thread blocks): a task body for exercising the graph queue and
  * merged member launches from a caller that has no kernels of its own. No reference counterpart. */

@@ -228,8 +228,8 @@ __global__ __launch_bounds__(kCols) void ens_tridiag_kernel(const EnsMember *ms,
         const long long col = (long long)t.block * kCols + threadIdx.x;
         if (col >= plane) continue;
         const double lam = QFAST ? m.lx[col / m.ny] + m.ly[col % m.ny] : m.lx[col % m.nx] + m.ly[col / m.nx];
-        tridiag_column<T>(static_cast<const T *>(m.w1), static_cast<T *>(m.w0), plane, m.nz, lam, m.rz, lds,
-                          threadIdx.x, kCols, col);
+        tridiag_column<T, T>(static_cast<const T *>(m.w1), TriOut<T>{static_cast<T *>(m.w0), 0}, plane, m.nz, lam,
+                             m.rz, lds, threadIdx.x, kCols, col);
     }
 }
 

@@ -83,18 +83,31 @@ static int select_kernel(const GemmCall &c, int precision, int tile, const void 
     if (precision != kBf16x3) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unknown contraction precision");
     if (c.flags & kGemmAKContiguous) {
         const bool tout = (c.flags & kGemmTransposedC) != 0;
-        const int combo = c.typeA * 8 + c.typeB * 4 + c.typeC * 2 + (tout ? 1 : 0);
-        switch (combo) {
-        case kFloat * 8 + kDouble * 4 + kFloat * 2 + 1: *fn = tn_by_tile<float, double, float, true>(tile); break;
-        case kFloat * 8 + kFloat * 4 + kFloat * 2 + 1: *fn = tn_by_tile<float, float, float, true>(tile); break;
-        case kFloat * 8 + kFloat * 4 + kFloat * 2 + 0: *fn = tn_by_tile<float, float, float, false>(tile); break;
-        case kFloat * 8 + kFloat * 4 + kDouble * 2 + 0: *fn = tn_by_tile<float, float, double, false>(tile); break;
-        case kDouble * 8 + kDouble * 4 + kDouble * 2 + 0: *fn = tn_by_tile<double, double, double, false>(tile); break;
-        default: return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unsupported K-contiguous operand types");
+        const int combo = c.typeA * 27 + c.typeB * 9 + c.typeC * 3;
+        constexpr int D = kDouble, F = kFloat, S = kSplit;
+        if (tout) {
+            switch (combo) {
+            case F * 27 + D * 9 + F * 3: *fn = tn_by_tile<float, double, float, true>(tile); break;
+            case F * 27 + F * 9 + F * 3: *fn = tn_by_tile<float, float, float, true>(tile); break;
+            case S * 27 + D * 9 + S * 3: *fn = tn_by_tile<Split16, double, Split16, true>(tile); break;
+            case S * 27 + S * 9 + F * 3: *fn = tn_by_tile<Split16, Split16, float, true>(tile); break;
+            default: return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unsupported K-contiguous operand types");
+            }
+        } else {
+            switch (combo) {
+            case F * 27 + F * 9 + F * 3: *fn = tn_by_tile<float, float, float, false>(tile); break;
+            case F * 27 + F * 9 + D * 3: *fn = tn_by_tile<float, float, double, false>(tile); break;
+            case D * 27 + D * 9 + D * 3: *fn = tn_by_tile<double, double, double, false>(tile); break;
+            case S * 27 + S * 9 + S * 3: *fn = tn_by_tile<Split16, Split16, Split16, false>(tile); break;
+            case S * 27 + S * 9 + D * 3: *fn = tn_by_tile<Split16, Split16, double, false>(tile); break;
+            default: return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unsupported K-contiguous operand types");
+            }
         }
         return FORT_SCOPE_OK;
     }
     if (c.flags) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "a transposed C needs a K-contiguous A");
+    if (c.typeA == kSplit || c.typeB == kSplit || c.typeC == kSplit)
+        return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "split operands need a K-contiguous A");
     // The operand type combinations the chain contraction and the spectral transforms use.
     const int combo = c.typeA * 4 + c.typeB * 2 + c.typeC;
     switch (combo) {
@@ -187,6 +200,8 @@ extern "C" {
 int ftc_gemm_device(const ftc_gemm_desc *d, int precision, int tile, void *stream)
 {
     if (!d) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "missing contraction descriptor");
+    for (int t : {d->type_a, d->type_b, d->type_c})
+        if (t != kDouble && t != kFloat) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unknown operand type");
     GemmCall c{d->a, d->b, d->c, d->type_a, d->type_b, d->type_c, d->m, d->n, d->k, d->batch, d->lda, d->ldb,
                d->ldc, d->stride_a, d->stride_b, d->stride_c, d->alpha, d->beta, d->flags};
     return gemm_launch(c, precision, static_cast<hipStream_t>(stream), tile);

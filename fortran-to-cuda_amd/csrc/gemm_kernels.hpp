@@ -11,6 +11,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "gemm.hpp"
 
 namespace ftc {
@@ -289,6 +291,97 @@ __device__ __forceinline__ void load8(const T *src, bool ok, int valid, T (&v)[8
     }
 }
 
+// A kSplit operand: bf16 hi and lo planes (lo `lo` elements after hi).
+struct Split16 {};
+
+// Eight consecutive K elements of one operand row as staged in registers between the global load and the
+// LDS store: raw values (split on the way into LDS) or, for Split16, the hi and lo vectors themselves.
+template <class T>
+struct Raw8 {
+    T v[8];
+};
+template <>
+struct Raw8<Split16> {
+    bf16x8 h, l;
+};
+
+template <class T>
+__device__ __forceinline__ void load_raw8(const void *base, long long idx, long long lo, bool ok, int valid, Raw8<T> &r)
+{
+    (void)lo;
+    load8(static_cast<const T *>(base) + idx, ok, valid, r.v);
+}
+
+template <>
+__device__ __forceinline__ void load_raw8<Split16>(const void *base, long long idx, long long lo, bool ok, int valid,
+                                                   Raw8<Split16> &r)
+{
+    const __bf16 *h = static_cast<const __bf16 *>(base) + idx, *l = h + lo;
+    if (ok && valid >= 8 && ((reinterpret_cast<uintptr_t>(h) | reinterpret_cast<uintptr_t>(l)) & 15) == 0) {
+        r.h = *reinterpret_cast<const bf16x8 *>(h);
+        r.l = *reinterpret_cast<const bf16x8 *>(l);
+    } else {
+        const __bf16 z = (__bf16)0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            r.h[q] = (ok && q < valid) ? h[q] : z;
+            r.l[q] = (ok && q < valid) ? l[q] : z;
+        }
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void store_raw8(const Raw8<T> &r, __bf16 *hi, __bf16 *lo)
+{
+    bf16x8 h, l;
+    split8(r.v, h, l);
+    *reinterpret_cast<bf16x8 *>(hi) = h;
+    *reinterpret_cast<bf16x8 *>(lo) = l;
+}
+
+template <>
+__device__ __forceinline__ void store_raw8<Split16>(const Raw8<Split16> &r, __bf16 *hi, __bf16 *lo)
+{
+    *reinterpret_cast<bf16x8 *>(hi) = r.h;
+    *reinterpret_cast<bf16x8 *>(lo) = r.l;
+}
+
+// Output element access: plain types, or Split16 (the value rounded to float, then split as split8 does).
+template <class TO>
+__device__ __forceinline__ double load_out(const void *C, long long ci, long long lo)
+{
+    (void)lo;
+    return (double)static_cast<const TO *>(C)[ci];
+}
+template <>
+__device__ __forceinline__ double load_out<Split16>(const void *C, long long ci, long long lo)
+{
+    const __bf16 *c = static_cast<const __bf16 *>(C);
+    return (double)((float)c[ci] + (float)c[ci + lo]);
+}
+template <class TO>
+__device__ __forceinline__ void store_out(void *C, long long ci, long long lo, double v)
+{
+    (void)lo;
+    static_cast<TO *>(C)[ci] = (TO)v;
+}
+template <>
+__device__ __forceinline__ void store_out<Split16>(void *C, long long ci, long long lo, double v)
+{
+    __bf16 *c = static_cast<__bf16 *>(C);
+    const float f = (float)v;
+    const __bf16 h = (__bf16)f;
+    c[ci] = h;
+    c[ci + lo] = (__bf16)(f - (float)h);
+}
+
+template <class T>
+__device__ __forceinline__ const void *offset_elems(const void *p, long long e)
+{
+    using E = typename std::conditional<std::is_same<T, Split16>::value, __bf16, T>::type;
+    return static_cast<const E *>(p) + e;
+}
+
 template <class TA, class TB, class TO, int BM, int BN, bool TOUT>
 __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<BM, BN> &L, int mt, int nt, long long b)
 {
@@ -300,15 +393,15 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
     const int wm = wave % WM, wn = wave / WM;
     const int m0 = mt * BM, n0 = nt * BN;
     const int row = t >> 2, grp = t & 3;
-    const TA *A = static_cast<const TA *>(g.A) + b * g.sa;
-    const TB *B = static_cast<const TB *>(g.B) + b * g.sb;
+    const void *A = offset_elems<TA>(g.A, b * g.sa);
+    const void *B = offset_elems<TB>(g.B, b * g.sb);
     f32x4 acc[TM][TN];
 #pragma unroll
     for (int x = 0; x < TM; ++x)
 #pragma unroll
         for (int y = 0; y < TN; ++y) acc[x][y] = f32x4{0.f, 0.f, 0.f, 0.f};
-    TA ra[RA][8];
-    TB rb[RB][8];
+    Raw8<TA> ra[RA];
+    Raw8<TB> rb[RB];
     // Global loads of K-slice s+1 are issued before the MFMAs of slice s.
     auto load = [&](int k0) {
         const int kb = k0 + 8 * grp;
@@ -316,13 +409,13 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
         for (int e = 0; e < RA; ++e) {
             const int m = row + 64 * e;
             const bool mok = m < BM && m0 + m < g.M;
-            load8(A + (long long)(m0 + m) * g.lda + kb, mok, g.K - kb, ra[e]);
+            load_raw8<TA>(A, (long long)(m0 + m) * g.lda + kb, g.lo_a, mok, g.K - kb, ra[e]);
         }
 #pragma unroll
         for (int e = 0; e < RB; ++e) {
             const int n = row + 64 * e;
             const bool nok = n < BN && n0 + n < g.N;
-            load8(B + kb + (long long)(n0 + n) * g.ldb, nok, g.K - kb, rb[e]);
+            load_raw8<TB>(B, kb + (long long)(n0 + n) * g.ldb, g.lo_b, nok, g.K - kb, rb[e]);
         }
     };
     load(0);
@@ -331,22 +424,12 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
 #pragma unroll
         for (int e = 0; e < RA; ++e) {
             const int m = row + 64 * e;
-            if (m < BM) {
-                bf16x8 hi, lo;
-                split8(ra[e], hi, lo);
-                *reinterpret_cast<bf16x8 *>(&L.Ah[m][8 * grp]) = hi;
-                *reinterpret_cast<bf16x8 *>(&L.Al[m][8 * grp]) = lo;
-            }
+            if (m < BM) store_raw8<TA>(ra[e], &L.Ah[m][8 * grp], &L.Al[m][8 * grp]);
         }
 #pragma unroll
         for (int e = 0; e < RB; ++e) {
             const int n = row + 64 * e;
-            if (n < BN) {
-                bf16x8 hi, lo;
-                split8(rb[e], hi, lo);
-                *reinterpret_cast<bf16x8 *>(&L.Bh[n][8 * grp]) = hi;
-                *reinterpret_cast<bf16x8 *>(&L.Bl[n][8 * grp]) = lo;
-            }
+            if (n < BN) store_raw8<TB>(rb[e], &L.Bh[n][8 * grp], &L.Bl[n][8 * grp]);
         }
         __syncthreads();
         if (k0 + BK < g.K) load(k0 + BK);
@@ -381,7 +464,7 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
                 }
             }
     }
-    TO *C = static_cast<TO *>(g.C) + b * g.sc;
+    void *C = const_cast<void *>(offset_elems<TO>(g.C, b * g.sc));
 #pragma unroll
     for (int x = 0; x < TM; ++x)
 #pragma unroll
@@ -393,8 +476,8 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
                 if (rr < g.M && cc < g.N) {
                     const long long ci = TOUT ? (long long)rr * g.ldc + cc : (long long)rr + (long long)cc * g.ldc;
                     double v = g.alpha * (double)acc[x][y][r];
-                    if (g.beta != 0.0) v = v + g.beta * (double)C[ci];
-                    C[ci] = (TO)v;
+                    if (g.beta != 0.0) v = v + g.beta * load_out<TO>(C, ci, g.lo_c);
+                    store_out<TO>(C, ci, g.lo_c, v);
                 }
             }
 }

@@ -102,7 +102,7 @@ struct poisson_solver_handle_v1 {
     void *arena = nullptr;
     std::size_t arena_bytes = 0;
     double *Qx = nullptr, *QxT = nullptr, *Qy = nullptr, *QyT = nullptr, *lx = nullptr, *ly = nullptr;
-    float *Qxf = nullptr, *QxTf = nullptr, *Qyf = nullptr, *QyTf = nullptr;
+    void *Qxs = nullptr, *QxTs = nullptr, *Qys = nullptr, *QyTs = nullptr;  // split bases (bf16 hi, lo planes)
     void *w0 = nullptr, *w1 = nullptr;
     double *res = nullptr, *slots = nullptr, *final256 = nullptr;
     double *host_final = nullptr;  // pinned, 256 doubles
@@ -136,29 +136,35 @@ static int apply_inverse(poisson_solver_handle_v1 &s, const double *in, long lon
     if (s.mixed) {
         // Split-bf16 chain with every contraction operand K-contiguous: W0 = (Qx^T f)^T and W1 = W0 Qy keep
         // the mode index q fastest, so G2 and G3 read rows of W0 as vectors; G3 turns W1 back to i fastest.
+        // The bases, W0 (written by G1 and by the z solve) and W1 between G3 and G4 are kept split (bf16 hi
+        // and lo planes); W1 between G2 and the z solve is FP32.
         const int ak = kGemmAKContiguous, at = kGemmAKContiguous | kGemmTransposedC;
-        GemmCall g1{s.Qxf, in, s.w0, kFloat, kDouble, kFloat, nx, ny, nx, nz, nx, in1, ny, 0, in2, plane, 1.0, 0.0, at};
+        const long long N = plane * nz, Lx = (long long)nx * nx, Ly = (long long)ny * ny;
+        GemmCall g1{s.Qxs, in, s.w0, kSplit, kDouble, kSplit, nx, ny, nx, nz, nx, in1, ny, 0, in2, plane, 1.0, 0.0,
+                    at, Lx, 0, N};
         FTC_CHECK(gemm_launch(g1, prec, stream));
-        GemmCall g2{s.w0, s.Qyf, s.w1, kFloat, kFloat, kFloat, nx, ny, ny, nz, ny, ny, ny, plane, 0, plane, 1.0, 0.0, at};
+        GemmCall g2{s.w0, s.Qys, s.w1, kSplit, kSplit, kFloat, nx, ny, ny, nz, ny, ny, ny, plane, 0, plane, 1.0, 0.0,
+                    at, N, Ly, 0};
         FTC_CHECK(gemm_launch(g2, prec, stream));
-        hipLaunchKernelGGL(tridiag_z_kernel<float>, dim3(blocks), dim3(s.cols), lds, stream,
-                           static_cast<const float *>(s.w1), static_cast<float *>(s.w0), nx, ny, nz, s.lx, s.ly, rz,
-                           s.cols, 1);
+        hipLaunchKernelGGL((tridiag_z_kernel<float, TriSplit>), dim3(blocks), dim3(s.cols), lds, stream,
+                           static_cast<const float *>(s.w1), TriOut<TriSplit>{static_cast<__bf16 *>(s.w0), N}, nx,
+                           ny, nz, s.lx, s.ly, rz, s.cols, 1);
         if (hipGetLastError() != hipSuccess)
             return fort_scope_report_error(FORT_SCOPE_EXECUTION, "tridiagonal launch failed");
-        GemmCall g3{s.w0, s.QyTf, s.w1, kFloat, kFloat, kFloat, nx, ny, ny, nz, ny, ny, nx, plane, 0, plane, 1.0, 0.0, ak};
+        GemmCall g3{s.w0, s.QyTs, s.w1, kSplit, kSplit, kSplit, nx, ny, ny, nz, ny, ny, nx, plane, 0, plane, 1.0, 0.0,
+                    ak, N, Ly, N};
         FTC_CHECK(gemm_launch(g3, prec, stream));
-        GemmCall g4{s.QxTf, s.w1, out.x, kFloat, kFloat, kDouble, nx, ny, nx, nz, nx, nx, out.ld1, 0, plane, out.ld2,
-                    1.0, beta, ak};
+        GemmCall g4{s.QxTs, s.w1, out.x, kSplit, kSplit, kDouble, nx, ny, nx, nz, nx, nx, out.ld1, 0, plane, out.ld2,
+                    1.0, beta, ak, Lx, N, 0};
         return gemm_launch(g4, prec, stream);
     }
     GemmCall g1{s.QxT, in, s.w0, kDouble, kDouble, kDouble, nx, ny, nx, nz, nx, in1, nx, 0, in2, plane, 1.0, 0.0};
     FTC_CHECK(gemm_launch(g1, prec, stream));
     GemmCall g2{s.w0, s.Qy, s.w1, kDouble, kDouble, kDouble, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
     FTC_CHECK(gemm_launch(g2, prec, stream));
-    hipLaunchKernelGGL(tridiag_z_kernel<double>, dim3(blocks), dim3(s.cols), lds, stream,
-                       static_cast<const double *>(s.w1), static_cast<double *>(s.w0), nx, ny, nz, s.lx, s.ly, rz,
-                       s.cols, 0);
+    hipLaunchKernelGGL((tridiag_z_kernel<double, double>), dim3(blocks), dim3(s.cols), lds, stream,
+                       static_cast<const double *>(s.w1), TriOut<double>{static_cast<double *>(s.w0), 0}, nx, ny, nz,
+                       s.lx, s.ly, rz, s.cols, 0);
     if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "tridiagonal launch failed");
     GemmCall g3{s.w0, s.QyT, s.w1, kDouble, kDouble, kDouble, nx, ny, ny, nz, nx, ny, nx, plane, 0, plane, 1.0, 0.0};
     FTC_CHECK(gemm_launch(g3, prec, stream));
@@ -283,10 +289,10 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
     s->QxT = (double *)carve((std::size_t)nx * nx * 8);
     s->Qy = (double *)carve((std::size_t)ny * ny * 8);
     s->QyT = (double *)carve((std::size_t)ny * ny * 8);
-    s->Qxf = (float *)carve((std::size_t)nx * nx * 4);
-    s->QxTf = (float *)carve((std::size_t)nx * nx * 4);
-    s->Qyf = (float *)carve((std::size_t)ny * ny * 4);
-    s->QyTf = (float *)carve((std::size_t)ny * ny * 4);
+    s->Qxs = carve((std::size_t)nx * nx * 4);
+    s->QxTs = carve((std::size_t)nx * nx * 4);
+    s->Qys = carve((std::size_t)ny * ny * 4);
+    s->QyTs = carve((std::size_t)ny * ny * 4);
     s->lx = (double *)carve((std::size_t)nx * 8);
     s->ly = (double *)carve((std::size_t)ny * 8);
     s->w0 = carve(N * wbytes);
@@ -298,16 +304,17 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
     basis(nx, s->hx, s->perx, qx, lx);
     basis(ny, s->hy, s->pery, qy, ly);
     std::vector<double> qxt = transpose(qx, nx), qyt = transpose(qy, ny);
-    auto tofloat = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
-    std::vector<float> qxf = tofloat(qx), qxtf = tofloat(qxt), qyf = tofloat(qy), qytf = tofloat(qyt);
+    // The split-bf16 solve takes the bases pre-split (bf16 hi plane, then lo plane: the FP32 footprint).
+    std::vector<uint16_t> qxf = split_planes(qx), qxtf = split_planes(qxt), qyf = split_planes(qy),
+                          qytf = split_planes(qyt);
     bool ok = hipMemcpy(s->Qx, qx.data(), qx.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(s->QxT, qxt.data(), qxt.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(s->Qy, qy.data(), qy.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(s->QyT, qyt.data(), qyt.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(s->Qxf, qxf.data(), qxf.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(s->QxTf, qxtf.data(), qxtf.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(s->Qyf, qyf.data(), qyf.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(s->QyTf, qytf.data(), qytf.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->Qxs, qxf.data(), qxf.size() * 2, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->QxTs, qxtf.data(), qxtf.size() * 2, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->Qys, qyf.data(), qyf.size() * 2, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s->QyTs, qytf.data(), qytf.size() * 2, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(s->lx, lx.data(), lx.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(s->ly, ly.data(), ly.size() * 8, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {

@@ -102,23 +102,46 @@ __device__ __forceinline__ void tridiag_forward_chunk(const double (&v)[kTriChun
 // The constant null mode (lambda == 0): the solution of zero mean. psi_0 = 0 fixes the constant, the first
 // nz-1 equations give the rest, and the mean is removed (the last equation holds when the right-hand side
 // has zero mean). Two passes over the column: the sum, then the values again minus the mean.
-template <class T>
-__device__ __forceinline__ void tridiag_null_mode(const T *__restrict__ src, T *__restrict__ dst, long long plane,
+// Output of the z solve: a plain array of TOUT, or (TOUT = TriSplit) bf16 hi and lo planes, lo `lo`
+// elements after hi, holding the value rounded to float and split as the split-bf16 contractions split it.
+struct TriSplit {};
+
+template <class TOUT>
+struct TriOut {
+    TOUT *p;
+    long long lo;
+    __device__ __forceinline__ void put(long long e, double v) const { p[e] = (TOUT)v; }
+};
+template <>
+struct TriOut<TriSplit> {
+    __bf16 *p;
+    long long lo;
+    __device__ __forceinline__ void put(long long e, double v) const
+    {
+        const float f = (float)v;
+        const __bf16 h = (__bf16)f;
+        p[e] = h;
+        p[e + lo] = (__bf16)(f - (float)h);
+    }
+};
+
+template <class T, class TOUT>
+__device__ __forceinline__ void tridiag_null_mode(const T *__restrict__ src, const TriOut<TOUT> &dst, long long plane,
                                                   int nz, double rz)
 {
     double mean = 0.0;
     for (int pass = 0; pass < 2; ++pass) {
         double prev = 0.0, cur = 0.0, sum = 0.0;
-        if (pass == 1) dst[0] = (T)(0.0 - mean);
+        if (pass == 1) dst.put(0, 0.0 - mean);
         if (nz > 1) {
             cur = prev + (double)src[0] / rz;
-            if (pass == 1) dst[plane] = (T)(cur - mean);
+            if (pass == 1) dst.put(plane, cur - mean);
             sum = cur;
             for (int k = 1; k < nz - 1; ++k) {
                 const double next = ((double)src[(long long)k * plane] / rz - prev) + 2.0 * cur;
                 prev = cur;
                 cur = next;
-                if (pass == 1) dst[(long long)(k + 1) * plane] = (T)(cur - mean);
+                if (pass == 1) dst.put((long long)(k + 1) * plane, cur - mean);
                 sum = sum + cur;
             }
         }
@@ -128,15 +151,15 @@ __device__ __forceinline__ void tridiag_null_mode(const T *__restrict__ src, T *
 
 // Solves column `col` of a plane-major (plane = nx*ny values per z level) array; c is the thread's column
 // within its block of `cols` and ck the block's checkpoint LDS.
-template <class T>
-__device__ __forceinline__ void tridiag_column(const T *__restrict__ in, T *__restrict__ out, long long plane,
-                                               int nz, double lam, double rz, double *ck, int c, int cols,
-                                               long long col)
+template <class T, class TOUT = T>
+__device__ __forceinline__ void tridiag_column(const T *__restrict__ in, TriOut<TOUT> out, long long plane, int nz,
+                                               double lam, double rz, double *ck, int c, int cols, long long col)
 {
     const T *src = in + col;
-    T *dst = out + col;
+    TriOut<TOUT> dst = out;
+    dst.p += col;
     if (lam == 0.0) {
-        tridiag_null_mode<T>(src, dst, plane, nz, rz);
+        tridiag_null_mode<T, TOUT>(src, dst, plane, nz, rz);
         return;
     }
     const int chunks = tridiag_chunks(nz);
@@ -158,14 +181,14 @@ __device__ __forceinline__ void tridiag_column(const T *__restrict__ in, T *__re
             const int k = k0 + i;
             if (k < nz) {
                 psi = k == nz - 1 ? dl[i] : dl[i] - cl[i] * psi;
-                dst[(long long)k * plane] = (T)psi;
+                dst.put((long long)k * plane, psi);
             }
         }
     }
 }
 
-template <class T>
-__global__ void tridiag_z_kernel(const T *__restrict__ in, T *__restrict__ out, int nx, int ny, int nz,
+template <class T, class TOUT = T>
+__global__ void tridiag_z_kernel(const T *__restrict__ in, TriOut<TOUT> out, int nx, int ny, int nz,
                                  const double *__restrict__ lx, const double *__restrict__ ly, double rz, int cols,
                                  int qfast)
 {
@@ -176,7 +199,7 @@ __global__ void tridiag_z_kernel(const T *__restrict__ in, T *__restrict__ out, 
     if (col >= plane) return;
     // Mode (p, q) sits at column p + q*nx, or at p*ny + q when the transforms keep q fastest.
     const double lam = qfast ? lx[col / ny] + ly[col % ny] : lx[col % nx] + ly[col / nx];
-    tridiag_column<T>(in, out, plane, nz, lam, rz, lds, c, cols, col);
+    tridiag_column<T, TOUT>(in, out, plane, nz, lam, rz, lds, c, cols, col);
 }
 
 }  // namespace ftc

@@ -54,6 +54,8 @@ HELPER_SIGNATURES = {
     "ftc_gemm_device": (c_int, [POINTER(GemmDesc), c_int, c_int, c_void_p]),
     "ftc_gemm_tile_count": (c_int, []),
     "ftc_mfma_layout_probe": (c_int, [c_int, c_int, c_int, c_int, c_int, _DP]),
+    "ftc_divide_probe": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "ftc_intrinsic_probe": (c_int, [_IP]),
 }
 
 _loaded: dict[str, ctypes.CDLL] = {}

@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kGemmThreads) void ens_gemm_f64_kernel(const GemmCa
 template <class TA, class TB, class TO, int BM, int BN, bool TOUT>
 __global__ __launch_bounds__(kGemmThreads) void ens_gemm_bf16_kernel(const GemmCall *calls, const GemmItem *items, int n)
 {
-    __shared__ Bf16Lds<BM, BN> L;
+    __shared__ Bf16TnLds<BM, BN, 32> L;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const GemmItem t = items[it];
         gemm_bf16x3_tn_tile<TA, TB, TO, BM, BN, TOUT>(calls[t.call], L, t.mt, t.nt, t.b);

@@ -14,6 +14,7 @@
 #include <type_traits>
 
 #include "gemm.hpp"
+#include "xcd.hpp"
 
 namespace ftc {
 
@@ -31,7 +32,7 @@ __device__ __forceinline__ void gemm_block_tile(int &mt, int &nt, long long &b)
 {
     const unsigned mtiles = gridDim.x, ntiles = gridDim.y, total = mtiles * ntiles * gridDim.z;
     unsigned L = blockIdx.x + mtiles * (blockIdx.y + ntiles * blockIdx.z);
-    if (total % 8 == 0) L = (L % 8) * (total / 8) + L / 8;
+    L = xcd_contiguous(L, total);
     mt = (int)(L % mtiles);
     nt = (int)((L / mtiles) % ntiles);
     b = (long long)(L / (mtiles * ntiles));
@@ -128,6 +129,115 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmCall &g, F64Lds<BM, BN> 
                     C[ci] = v;
                 }
             }
+}
+
+// The same contraction with two LDS stages of BK elements of K: slice s+1 is stored into the other stage
+// while slice s is multiplied, so each slice costs one barrier, and the global loads of slice s+2 are in
+// flight meanwhile. Summation order over K is that of gemm_f64_tile (k ascending, four per MFMA).
+template <int BM, int BN, int BK>
+struct F64DbLds {
+    double As[2][BK][BM + 1];
+    double Bs[2][BK][BN + 1];
+};
+
+template <int BM, int BN, int WM, int WN, int BK>
+__device__ __forceinline__ void gemm_f64_db_tile(const GemmCall &g, F64DbLds<BM, BN, BK> &L, int mt, int nt,
+                                                 long long b)
+{
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int TM = WTM / 16, TN = WTN / 16;
+    constexpr int NA = BM * BK / kGemmThreads, NB = BN * BK / kGemmThreads;
+    static_assert(WM * WN == kGemmThreads / 64, "four waves");
+    static_assert(BK % 4 == 0 && NA * kGemmThreads == BM * BK && NB * kGemmThreads == BN * BK, "whole stages");
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int m0 = mt * BM, n0 = nt * BN;
+    const double *A = static_cast<const double *>(g.A) + b * g.sa;
+    const double *B = static_cast<const double *>(g.B) + b * g.sb;
+    f64x4 acc[TM][TN];
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y) acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
+    double ra[NA], rb[NB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < NA; ++e) {
+            const int idx = t + e * kGemmThreads, m = idx % BM, k = idx / BM;
+            const bool ok = m0 + m < g.M && k0 + k < g.K;
+            ra[e] = ok ? A[(long long)(m0 + m) + (long long)(k0 + k) * g.lda] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            const int idx = t + e * kGemmThreads, k = idx % BK, n = idx / BK;
+            const bool ok = n0 + n < g.N && k0 + k < g.K;
+            rb[e] = ok ? B[(long long)(k0 + k) + (long long)(n0 + n) * g.ldb] : 0.0;
+        }
+    };
+    auto put = [&](int stage) {
+#pragma unroll
+        for (int e = 0; e < NA; ++e) {
+            const int idx = t + e * kGemmThreads;
+            L.As[stage][idx / BM][idx % BM] = ra[e];
+        }
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            const int idx = t + e * kGemmThreads;
+            L.Bs[stage][idx % BK][idx / BK] = rb[e];
+        }
+    };
+    load(0);
+    put(0);
+    if (BK < g.K) load(BK);
+    __syncthreads();
+    int stage = 0;
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            double af[TM], bfr[TN];
+#pragma unroll
+            for (int x = 0; x < TM; ++x) af[x] = L.As[stage][kk + lane / 16][wm * WTM + x * 16 + lane % 16];
+#pragma unroll
+            for (int y = 0; y < TN; ++y) bfr[y] = L.Bs[stage][kk + lane / 16][wn * WTN + y * 16 + lane % 16];
+#pragma unroll
+            for (int x = 0; x < TM; ++x)
+#pragma unroll
+                for (int y = 0; y < TN; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bfr[y], af[x], acc[x][y], 0, 0, 0);
+        }
+        if (k0 + BK < g.K) {
+            put(stage ^ 1);  // slice k0 + BK; the other stage's readers passed the previous barrier
+            if (k0 + 2 * BK < g.K) load(k0 + 2 * BK);
+        }
+        __syncthreads();
+        stage ^= 1;
+    }
+    double *C = static_cast<double *>(g.C) + b * g.sc;
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm * WTM + x * 16 + lane % 16;
+                const int col = n0 + wn * WTN + y * 16 + lane / 16 + 4 * r;
+                if (row < g.M && col < g.N) {
+                    const long long ci = (long long)row + (long long)col * g.ldc;
+                    double v = g.alpha * acc[x][y][r];
+                    if (g.beta != 0.0) v = v + g.beta * C[ci];
+                    C[ci] = v;
+                }
+            }
+}
+
+template <int BM, int BN, int WM, int WN, int BK>
+__global__ __launch_bounds__(kGemmThreads) void gemm_f64_db_kernel(const GemmCall g)
+{
+    __shared__ F64DbLds<BM, BN, BK> L;
+    int mt, nt;
+    long long b;
+    gemm_block_tile(mt, nt, b);
+    gemm_f64_db_tile<BM, BN, WM, WN, BK>(g, L, mt, nt, b);
 }
 
 template <int BM, int BN, int WM, int WN>
@@ -382,17 +492,30 @@ __device__ __forceinline__ const void *offset_elems(const void *p, long long e)
     return static_cast<const E *>(p) + e;
 }
 
-template <class TA, class TB, class TO, int BM, int BN, bool TOUT>
-__device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<BM, BN> &L, int mt, int nt, long long b)
+// LDS of the K-contiguous tile: BK elements of K per row, padded by 8 so that the sixteen rows one
+// ds_read_b128 touches fall on disjoint banks.
+template <int BM, int BN, int BK>
+struct Bf16TnLds {
+    __attribute__((aligned(16))) __bf16 Ah[BM][BK + 8];
+    __attribute__((aligned(16))) __bf16 Al[BM][BK + 8];
+    __attribute__((aligned(16))) __bf16 Bh[BN][BK + 8];
+    __attribute__((aligned(16))) __bf16 Bl[BN][BK + 8];
+};
+
+template <class TA, class TB, class TO, int BM, int BN, bool TOUT, int BK = 32>
+__device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16TnLds<BM, BN, BK> &L, int mt, int nt,
+                                                    long long b)
 {
-    constexpr int BK = 32, WM = 2, WN = 2;
+    constexpr int WM = 2, WN = 2;
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int TM = WTM / 16, TN = WTN / 16;
-    constexpr int RA = (BM + 63) / 64, RB = (BN + 63) / 64;
+    constexpr int GR = BK / 8, ROWS = kGemmThreads / GR;  // threads per staged row, rows per pass
+    constexpr int RA = (BM + ROWS - 1) / ROWS, RB = (BN + ROWS - 1) / ROWS;
+    static_assert(BK == 32 || BK == 64, "one or two MFMA K-slices per stage");
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave % WM, wn = wave / WM;
     const int m0 = mt * BM, n0 = nt * BN;
-    const int row = t >> 2, grp = t & 3;
+    const int row = t / GR, grp = t % GR;
     const void *A = offset_elems<TA>(g.A, b * g.sa);
     const void *B = offset_elems<TB>(g.B, b * g.sb);
     f32x4 acc[TM][TN];
@@ -407,13 +530,13 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
         const int kb = k0 + 8 * grp;
 #pragma unroll
         for (int e = 0; e < RA; ++e) {
-            const int m = row + 64 * e;
+            const int m = row + ROWS * e;
             const bool mok = m < BM && m0 + m < g.M;
             load_raw8<TA>(A, (long long)(m0 + m) * g.lda + kb, g.lo_a, mok, g.K - kb, ra[e]);
         }
 #pragma unroll
         for (int e = 0; e < RB; ++e) {
-            const int n = row + 64 * e;
+            const int n = row + ROWS * e;
             const bool nok = n < BN && n0 + n < g.N;
             load_raw8<TB>(B, kb + (long long)(n0 + n) * g.ldb, g.lo_b, nok, g.K - kb, rb[e]);
         }
@@ -423,28 +546,30 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < RA; ++e) {
-            const int m = row + 64 * e;
+            const int m = row + ROWS * e;
             if (m < BM) store_raw8<TA>(ra[e], &L.Ah[m][8 * grp], &L.Al[m][8 * grp]);
         }
 #pragma unroll
         for (int e = 0; e < RB; ++e) {
-            const int n = row + 64 * e;
+            const int n = row + ROWS * e;
             if (n < BN) store_raw8<TB>(rb[e], &L.Bh[n][8 * grp], &L.Bl[n][8 * grp]);
         }
         __syncthreads();
         if (k0 + BK < g.K) load(k0 + BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 32) {
         bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
         for (int x = 0; x < TM; ++x) {
             const int r = wm * WTM + x * 16 + lane % 16;
-            ah[x] = *reinterpret_cast<const bf16x8 *>(&L.Ah[r][8 * (lane / 16)]);
-            al[x] = *reinterpret_cast<const bf16x8 *>(&L.Al[r][8 * (lane / 16)]);
+            ah[x] = *reinterpret_cast<const bf16x8 *>(&L.Ah[r][ks + 8 * (lane / 16)]);
+            al[x] = *reinterpret_cast<const bf16x8 *>(&L.Al[r][ks + 8 * (lane / 16)]);
         }
 #pragma unroll
         for (int y = 0; y < TN; ++y) {
             const int c = wn * WTN + y * 16 + lane % 16;
-            bh[y] = *reinterpret_cast<const bf16x8 *>(&L.Bh[c][8 * (lane / 16)]);
-            bl[y] = *reinterpret_cast<const bf16x8 *>(&L.Bl[c][8 * (lane / 16)]);
+            bh[y] = *reinterpret_cast<const bf16x8 *>(&L.Bh[c][ks + 8 * (lane / 16)]);
+            bl[y] = *reinterpret_cast<const bf16x8 *>(&L.Bl[c][ks + 8 * (lane / 16)]);
         }
         // TOUT: the accumulator tile is C (lane l holds column l%16), so a row of the transposed output is
         // contiguous across lanes. Otherwise the operands are swapped and the tile is C^T (lane l holds row
@@ -463,6 +588,7 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
                     acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], ah[x], acc[x][y], 0, 0, 0);
                 }
             }
+        }
     }
     void *C = const_cast<void *>(offset_elems<TO>(g.C, b * g.sc));
 #pragma unroll
@@ -482,14 +608,14 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16Lds<B
             }
 }
 
-template <class TA, class TB, class TO, int BM, int BN, bool TOUT>
+template <class TA, class TB, class TO, int BM, int BN, bool TOUT, int BK = 32>
 __global__ __launch_bounds__(kGemmThreads) void gemm_bf16x3_tn_kernel(const GemmCall g)
 {
-    __shared__ Bf16Lds<BM, BN> L;
+    __shared__ Bf16TnLds<BM, BN, BK> L;
     int mt, nt;
     long long b;
     gemm_block_tile(mt, nt, b);
-    gemm_bf16x3_tn_tile<TA, TB, TO, BM, BN, TOUT>(g, L, mt, nt, b);
+    gemm_bf16x3_tn_tile<TA, TB, TO, BM, BN, TOUT, BK>(g, L, mt, nt, b);
 }
 
 }  // namespace ftc

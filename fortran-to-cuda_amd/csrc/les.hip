@@ -194,6 +194,10 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     const char *rows_env = std::getenv("FTC_LES_ROWS");
     const int rows = staged && !v2 && rows_env && std::atoi(rows_env) == 2 ? 2 : 1;
     a.kchunk = choose_kchunk(nx, ny, nz, rows, !staged || v2 ? 2 : (rows == 2 ? 2 : 3));
+    // Tiles in launch order put each 64-column strip on one XCD, so the y neighbours that re-read a tile's
+    // ghost rows share its L2. The XCD-contiguous order (FTC_LES_XCD_ORDER) also shares the x neighbours'
+    // lines, but only between tiles resident at the same time; over four rounds of workgroups it read more.
+    a.tile_order = std::getenv("FTC_LES_XCD_ORDER") ? 1 : 0;
     const std::size_t X = nx, Y = ny, Z = nz;
     const Sec3 full = box(0, X + 2, 0, Y + 2, 0, Z + 2);
     std::vector<Binding3> b(8);

@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "xcd.hpp"
+
 #pragma clang fp contract(off)
 
 namespace ftc {
@@ -17,7 +19,7 @@ struct LesArgs {
     int nx, ny, nz;  // i loop 2..nx+1, j loop 2..ny+1, U/V/S k loop 2..nz+1
     int kw;          // W k loop 2..kw
     int kchunk;
-    int reserved;
+    int tile_order;  // 0: tiles in launch order; 1: each XCD takes a contiguous run of tiles (xcd.hpp)
     long long sj, sk;                   // strides of j and k in elements
     double c25[3], c50[3], c125[3];     // 0.25/d, 0.5/d, 0.125/d for d = dx, dy, dz
     double sgs[10];                     // 1/d, 0.25/d, 1/d^2 (x, y, z each), (Cs*Delta)^2
@@ -988,7 +990,12 @@ template <bool STEP, int ROWS>
 __global__ __launch_bounds__(NTH, ROWS == 1 ? 3 : 2) void les_sgs3_kernel(const LesArgs a)
 {
     __shared__ Sgs3Lds<ROWS> L;
-    les_sgs3_body<STEP, ROWS>(a, L, blockIdx.x, blockIdx.y, blockIdx.z);
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    unsigned t = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    // Tiles x fastest, then y, then the k chunk: with the XCD-contiguous order the x and y neighbours
+    // whose ghost rows and columns a tile re-reads run under the same L2 at about the same time.
+    if (a.tile_order) t = xcd_contiguous(t, gx * gy * gridDim.z);
+    les_sgs3_body<STEP, ROWS>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
 }
 
 }  // namespace ftc

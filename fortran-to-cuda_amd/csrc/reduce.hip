@@ -91,14 +91,26 @@ __global__ void reduce_slots(const ReduceBox b, long long n, int slots, BoxStep 
 }
 
 // Folds `count` partial results into 256 (thread f takes f, f + 256, ... in order). For sums of squares
-// the partial results are already squared, so the fold is a plain sum.
+// the partial results are already squared, so the fold is a plain sum. The loads of a batch of 32 are all
+// issued before the batch is combined, in order, so one memory latency covers 32 partials.
 template <int OP>
 __global__ void reduce_fold(const double *partial, int count, double *out)
 {
+    constexpr int kBatch = 32;
     const int f = threadIdx.x;
     constexpr int FOP = OP == kReduceSumSquares ? kReduceSum : (OP == kReduceMaxAbs ? kReduceMax : OP);
     double acc = identity<FOP>();
-    for (int t = f; t < count; t += kReduceFinal) acc = combine<FOP>(acc, partial[t]);
+    for (int t0 = f; t0 < count; t0 += kBatch * kReduceFinal) {
+        double v[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const int t = t0 + q * kReduceFinal;
+            v[q] = t < count ? partial[t] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q)
+            if (t0 + q * kReduceFinal < count) acc = combine<FOP>(acc, v[q]);
+    }
     out[f] = acc;
 }
 

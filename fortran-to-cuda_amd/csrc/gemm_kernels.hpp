@@ -377,20 +377,27 @@ __device__ __forceinline__ void split8(const T (&v)[8], bf16x8 &hi, bf16x8 &lo)
     }
 }
 
-// Eight consecutive elements src[0..valid) (zeros past `valid` or when !ok); one 32- or 64-byte vector
-// read when the run is whole and 16-byte aligned.
+// 16-byte vectors that only assume the alignment of their elements: gfx950 serves a global dwordx4 load
+// at any dword-aligned address, so a run that starts at an odd element (an interior that follows one ghost
+// cell) still moves in 16-byte pieces.
+typedef float f32x4_elem_aligned __attribute__((ext_vector_type(4), aligned(4)));
+typedef double f64x2_elem_aligned __attribute__((ext_vector_type(2), aligned(8)));
+
+// Eight consecutive elements src[0..valid) (zeros past `valid` or when !ok); 16-byte vector reads when the
+// run is whole.
 template <class T>
 __device__ __forceinline__ void load8(const T *src, bool ok, int valid, T (&v)[8])
 {
-    if (ok && valid >= 8 && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+    if (ok && valid >= 8) {
         if constexpr (sizeof(T) == 4) {
-            const float4 a = reinterpret_cast<const float4 *>(src)[0], b = reinterpret_cast<const float4 *>(src)[1];
+            const f32x4_elem_aligned a = reinterpret_cast<const f32x4_elem_aligned *>(src)[0];
+            const f32x4_elem_aligned b = reinterpret_cast<const f32x4_elem_aligned *>(src)[1];
             v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
             v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
         } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const double2 a = reinterpret_cast<const double2 *>(src)[q];
+                const f64x2_elem_aligned a = reinterpret_cast<const f64x2_elem_aligned *>(src)[q];
                 v[2 * q] = a.x;
                 v[2 * q + 1] = a.y;
             }
@@ -485,6 +492,58 @@ __device__ __forceinline__ void store_out<Split16>(void *C, long long ci, long l
     c[ci + lo] = (__bf16)(f - (float)h);
 }
 
+// Four adjacent output elements C[ci .. ci+3]: 16-byte vector accesses for FP32 and FP64 outputs (which
+// need only element alignment), two 8-byte plane accesses for Split16 when ci is even.
+template <class TO>
+__device__ __forceinline__ void load_out4(const void *C, long long ci, long long lo, double (&v)[4])
+{
+    if constexpr (std::is_same<TO, double>::value) {
+        const f64x2_elem_aligned *p = reinterpret_cast<const f64x2_elem_aligned *>(static_cast<const double *>(C) + ci);
+        const f64x2_elem_aligned a = p[0], b = p[1];
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    } else if constexpr (std::is_same<TO, float>::value) {
+        const f32x4_elem_aligned a = *reinterpret_cast<const f32x4_elem_aligned *>(static_cast<const float *>(C) + ci);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = load_out<TO>(C, ci + r, lo);
+    }
+}
+
+typedef __bf16 bf16x4_dword_aligned __attribute__((ext_vector_type(4), aligned(4)));
+
+template <class TO>
+__device__ __forceinline__ void store_out4(void *C, long long ci, long long lo, const double (&v)[4])
+{
+    if constexpr (std::is_same<TO, double>::value) {
+        f64x2_elem_aligned *p = reinterpret_cast<f64x2_elem_aligned *>(static_cast<double *>(C) + ci);
+        p[0] = f64x2_elem_aligned{v[0], v[1]};
+        p[1] = f64x2_elem_aligned{v[2], v[3]};
+    } else if constexpr (std::is_same<TO, float>::value) {
+        *reinterpret_cast<f32x4_elem_aligned *>(static_cast<float *>(C) + ci) =
+            f32x4_elem_aligned{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+    } else if constexpr (std::is_same<TO, Split16>::value) {
+        if (((ci | lo) & 1) == 0) {
+            bf16x4_dword_aligned h, l;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float f = (float)v[r];
+                h[r] = (__bf16)f;
+                l[r] = (__bf16)(f - (float)h[r]);
+            }
+            __bf16 *c = static_cast<__bf16 *>(C);
+            *reinterpret_cast<bf16x4_dword_aligned *>(c + ci) = h;
+            *reinterpret_cast<bf16x4_dword_aligned *>(c + ci + lo) = l;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) store_out<TO>(C, ci + r, lo, v[r]);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) store_out<TO>(C, ci + r, lo, v[r]);
+    }
+}
+
 template <class T>
 __device__ __forceinline__ const void *offset_elems(const void *p, long long e)
 {
@@ -492,14 +551,21 @@ __device__ __forceinline__ const void *offset_elems(const void *p, long long e)
     return static_cast<const E *>(p) + e;
 }
 
-// LDS of the K-contiguous tile: BK elements of K per row, padded by 8 so that the sixteen rows one
-// ds_read_b128 touches fall on disjoint banks.
+// LDS of the K-contiguous tile: BK elements of K per row in 16-byte groups of 8, rows padded by 8 elements
+// so that the sixteen rows one ds_read_b128 touches fall on disjoint banks. (An unpadded XOR-swizzled row
+// of 64 left the bank-conflict counter unchanged and made the FP64-input transform 40 % slower.)
+template <int BK>
+struct TnRow {
+    static constexpr int kWidth = BK + 8;
+    __device__ static __forceinline__ int at(int, int group) { return 8 * group; }
+};
+
 template <int BM, int BN, int BK>
 struct Bf16TnLds {
-    __attribute__((aligned(16))) __bf16 Ah[BM][BK + 8];
-    __attribute__((aligned(16))) __bf16 Al[BM][BK + 8];
-    __attribute__((aligned(16))) __bf16 Bh[BN][BK + 8];
-    __attribute__((aligned(16))) __bf16 Bl[BN][BK + 8];
+    __attribute__((aligned(16))) __bf16 Ah[BM][TnRow<BK>::kWidth];
+    __attribute__((aligned(16))) __bf16 Al[BM][TnRow<BK>::kWidth];
+    __attribute__((aligned(16))) __bf16 Bh[BN][TnRow<BK>::kWidth];
+    __attribute__((aligned(16))) __bf16 Bl[BN][TnRow<BK>::kWidth];
 };
 
 template <class TA, class TB, class TO, int BM, int BN, bool TOUT, int BK = 32>
@@ -547,12 +613,12 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16TnLds
 #pragma unroll
         for (int e = 0; e < RA; ++e) {
             const int m = row + ROWS * e;
-            if (m < BM) store_raw8<TA>(ra[e], &L.Ah[m][8 * grp], &L.Al[m][8 * grp]);
+            if (m < BM) store_raw8<TA>(ra[e], &L.Ah[m][TnRow<BK>::at(m, grp)], &L.Al[m][TnRow<BK>::at(m, grp)]);
         }
 #pragma unroll
         for (int e = 0; e < RB; ++e) {
             const int n = row + ROWS * e;
-            if (n < BN) store_raw8<TB>(rb[e], &L.Bh[n][8 * grp], &L.Bl[n][8 * grp]);
+            if (n < BN) store_raw8<TB>(rb[e], &L.Bh[n][TnRow<BK>::at(n, grp)], &L.Bl[n][TnRow<BK>::at(n, grp)]);
         }
         __syncthreads();
         if (k0 + BK < g.K) load(k0 + BK);
@@ -561,24 +627,25 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16TnLds
         bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
         for (int x = 0; x < TM; ++x) {
-            const int r = wm * WTM + x * 16 + lane % 16;
-            ah[x] = *reinterpret_cast<const bf16x8 *>(&L.Ah[r][ks + 8 * (lane / 16)]);
-            al[x] = *reinterpret_cast<const bf16x8 *>(&L.Al[r][ks + 8 * (lane / 16)]);
+            const int r = wm * WTM + x * 16 + lane % 16, at = TnRow<BK>::at(r, ks / 8 + lane / 16);
+            ah[x] = *reinterpret_cast<const bf16x8 *>(&L.Ah[r][at]);
+            al[x] = *reinterpret_cast<const bf16x8 *>(&L.Al[r][at]);
         }
 #pragma unroll
         for (int y = 0; y < TN; ++y) {
-            const int c = wn * WTN + y * 16 + lane % 16;
-            bh[y] = *reinterpret_cast<const bf16x8 *>(&L.Bh[c][ks + 8 * (lane / 16)]);
-            bl[y] = *reinterpret_cast<const bf16x8 *>(&L.Bl[c][ks + 8 * (lane / 16)]);
+            const int c = wn * WTN + y * 16 + lane % 16, at = TnRow<BK>::at(c, ks / 8 + lane / 16);
+            bh[y] = *reinterpret_cast<const bf16x8 *>(&L.Bh[c][at]);
+            bl[y] = *reinterpret_cast<const bf16x8 *>(&L.Bl[c][at]);
         }
-        // TOUT: the accumulator tile is C (lane l holds column l%16), so a row of the transposed output is
-        // contiguous across lanes. Otherwise the operands are swapped and the tile is C^T (lane l holds row
-        // l%16), so a column of the column-major output is.
+        // Each lane ends up with four output elements that are adjacent in memory. Column-major C: the
+        // accumulator tile is C (lane l holds rows 4(l/16) .. +3 of column l%16). TOUT (C stored
+        // transposed): the operands are swapped, the tile is C^T and lane l holds columns 4(l/16) .. +3 of
+        // row l%16.
 #pragma unroll
         for (int x = 0; x < TM; ++x)
 #pragma unroll
             for (int y = 0; y < TN; ++y) {
-                if constexpr (TOUT) {
+                if constexpr (!TOUT) {
                     acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
                     acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
                     acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
@@ -594,18 +661,31 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16TnLds
 #pragma unroll
     for (int x = 0; x < TM; ++x)
 #pragma unroll
-        for (int y = 0; y < TN; ++y)
+        for (int y = 0; y < TN; ++y) {
+            // (rr, cc) of element r = 0; element r is r rows (column-major) or r columns (TOUT) further on.
+            const int rr = m0 + wm * WTM + x * 16 + (TOUT ? lane % 16 : 4 * (lane / 16));
+            const int cc = n0 + wn * WTN + y * 16 + (TOUT ? 4 * (lane / 16) : lane % 16);
+            const long long ci = TOUT ? (long long)rr * g.ldc + cc : (long long)rr + (long long)cc * g.ldc;
+            const bool whole = TOUT ? rr < g.M && cc + 3 < g.N : rr + 3 < g.M && cc < g.N;
+            if (whole) {
+                double v[4];
+                if (g.beta != 0.0) load_out4<TO>(C, ci, g.lo_c, v);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int rr = m0 + wm * WTM + x * 16 + (TOUT ? 4 * (lane / 16) + r : lane % 16);
-                const int cc = n0 + wn * WTN + y * 16 + (TOUT ? lane % 16 : 4 * (lane / 16) + r);
-                if (rr < g.M && cc < g.N) {
-                    const long long ci = TOUT ? (long long)rr * g.ldc + cc : (long long)rr + (long long)cc * g.ldc;
-                    double v = g.alpha * (double)acc[x][y][r];
-                    if (g.beta != 0.0) v = v + g.beta * load_out<TO>(C, ci, g.lo_c);
-                    store_out<TO>(C, ci, g.lo_c, v);
+                for (int r = 0; r < 4; ++r)
+                    v[r] = g.beta != 0.0 ? g.alpha * (double)acc[x][y][r] + g.beta * v[r] : g.alpha * (double)acc[x][y][r];
+                store_out4<TO>(C, ci, g.lo_c, v);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int er = TOUT ? rr : rr + r, ec = TOUT ? cc + r : cc;
+                    if (er < g.M && ec < g.N) {
+                        double v = g.alpha * (double)acc[x][y][r];
+                        if (g.beta != 0.0) v = v + g.beta * load_out<TO>(C, ci + r, g.lo_c);
+                        store_out<TO>(C, ci + r, g.lo_c, v);
+                    }
                 }
             }
+        }
 }
 
 template <class TA, class TB, class TO, int BM, int BN, bool TOUT, int BK = 32>

@@ -98,6 +98,7 @@ static int select_kernel(const GemmCall &c, int precision, int tile, const void 
             case F * 27 + F * 9 + F * 3: *fn = tn_by_tile<float, float, float, true>(tile); break;
             case S * 27 + D * 9 + S * 3: *fn = tn_by_tile<Split16, double, Split16, true>(tile); break;
             case S * 27 + S * 9 + F * 3: *fn = tn_by_tile<Split16, Split16, float, true>(tile); break;
+            case S * 27 + S * 9 + S * 3: *fn = tn_by_tile<Split16, Split16, Split16, true>(tile); break;
             default: return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unsupported K-contiguous operand types");
             }
         } else {

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -105,6 +106,7 @@ struct poisson_solver_handle_v1 {
     void *Qxs = nullptr, *QxTs = nullptr, *Qys = nullptr, *QyTs = nullptr;  // split bases (bf16 hi, lo planes)
     void *w0 = nullptr, *w1 = nullptr;
     double *res = nullptr, *slots = nullptr, *final256 = nullptr;
+    double *partial = nullptr;     // mixed: per-block sums of squares of the residual
     double *host_final = nullptr;  // pinned, 256 doubles
     poisson_solver_report_v1 report{};
     std::mutex mutex;
@@ -124,8 +126,8 @@ static int map_status(int scope_status)
 
 // The contraction calls of one application of the approximate (mixed) or exact (FP64) inverse:
 // phi_out = beta * phi_out + Qx [T^-1 ((Qx^T in) Qy)] Qy^T.
-static int apply_inverse(poisson_solver_handle_v1 &s, const double *in, long long in1, long long in2, View3 out,
-                         double beta, hipStream_t stream)
+static int apply_inverse(poisson_solver_handle_v1 &s, const void *in, long long in1, long long in2, View3 out,
+                         double beta, hipStream_t stream, bool in_split = false)
 {
     const int nx = s.nx, ny = s.ny, nz = s.nz;
     const long long plane = (long long)nx * ny;
@@ -140,8 +142,9 @@ static int apply_inverse(poisson_solver_handle_v1 &s, const double *in, long lon
         // and lo planes); W1 between G2 and the z solve is FP32.
         const int ak = kGemmAKContiguous, at = kGemmAKContiguous | kGemmTransposedC;
         const long long N = plane * nz, Lx = (long long)nx * nx, Ly = (long long)ny * ny;
-        GemmCall g1{s.Qxs, in, s.w0, kSplit, kDouble, kSplit, nx, ny, nx, nz, nx, in1, ny, 0, in2, plane, 1.0, 0.0,
-                    at, Lx, 0, N};
+        // The first application reads the FP64 right-hand side; the corrections read the split residual.
+        GemmCall g1{s.Qxs, in, s.w0, kSplit, in_split ? kSplit : kDouble, kSplit, nx, ny, nx, nz, nx, in1, ny, 0, in2,
+                    plane, 1.0, 0.0, at, Lx, in_split ? N : 0, N};
         FTC_CHECK(gemm_launch(g1, prec, stream));
         GemmCall g2{s.w0, s.Qys, s.w1, kSplit, kSplit, kFloat, nx, ny, ny, nz, ny, ny, ny, plane, 0, plane, 1.0, 0.0,
                     at, N, Ly, 0};
@@ -173,6 +176,24 @@ static int apply_inverse(poisson_solver_handle_v1 &s, const double *in, long lon
     return gemm_launch(g4, prec, stream);
 }
 
+// The mixed solve's residual, split for the next application; with `partial`, also the per-block sums of
+// squares for its norm.
+static int residual_split(poisson_solver_handle_v1 &s, View3 phiv, View3 b, double *partial, hipStream_t stream)
+{
+    const dim3 grid((s.nx + 63) / 64, (s.ny + 3) / 4, s.nz);
+    const long long N = (long long)s.nx * s.ny * s.nz;
+    hipLaunchKernelGGL(poisson_residual_split_kernel, grid, dim3(64, 4), 0, stream, phiv.x, phiv.ld1, phiv.ld2, b.x,
+                       b.ld1, b.ld2, reinterpret_cast<__bf16 *>(s.res), N, partial, s.nx, s.ny, s.nz,
+                       1.0 / (s.hx * s.hx), 1.0 / (s.hy * s.hy), 1.0 / (s.hz * s.hz), s.perx ? 1 : 0, s.pery ? 1 : 0);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "residual launch failed");
+    return FORT_SCOPE_OK;
+}
+
+static long long residual_blocks(const poisson_solver_handle_v1 &s)
+{
+    return (long long)((s.nx + 63) / 64) * ((s.ny + 3) / 4) * s.nz;
+}
+
 static int residual(poisson_solver_handle_v1 &s, const double *phi, View3 phiv, View3 b, hipStream_t stream)
 {
     const dim3 grid((s.nx + 63) / 64, (s.ny + 3) / 4, s.nz);
@@ -183,16 +204,29 @@ static int residual(poisson_solver_handle_v1 &s, const double *phi, View3 phiv, 
     return FORT_SCOPE_OK;
 }
 
-// Fixed-order 2-norm of a box; synchronises the stream.
-static int norm2(poisson_solver_handle_v1 &s, const ReduceBox &box, hipStream_t stream, double *out)
+// Fixed-order reduction `op` of a box, folded on the host; synchronises the stream.
+static int reduce_to_host(poisson_solver_handle_v1 &s, int op, const ReduceBox &box, hipStream_t stream, double *out)
 {
-    FTC_CHECK(reduce_launch(kReduceSumSquares, box, s.slots, s.final256, stream));
+    FTC_CHECK(reduce_launch(op, box, s.slots, s.final256, stream));
     if (hipMemcpyAsync(s.host_final, s.final256, kReduceFinal * sizeof(double), hipMemcpyDeviceToHost, stream) !=
             hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
         return fort_scope_report_error(FORT_SCOPE_EXECUTION, "norm readback failed");
-    *out = std::sqrt(reduce_host_fold(kReduceSumSquares, 0.0, s.host_final));
+    *out = reduce_host_fold(op, 0.0, s.host_final);
     return FORT_SCOPE_OK;
+}
+
+// Fixed-order 2-norm of a box; synchronises the stream.
+static int norm2(poisson_solver_handle_v1 &s, const ReduceBox &box, hipStream_t stream, double *out)
+{
+    FTC_CHECK(reduce_to_host(s, kReduceSumSquares, box, stream, out));
+    *out = std::sqrt(*out);
+    return FORT_SCOPE_OK;
+}
+
+static int sum_of(poisson_solver_handle_v1 &s, const ReduceBox &box, hipStream_t stream, double *out)
+{
+    return reduce_to_host(s, kReduceSum, box, stream, out);
 }
 
 // phi -= mean(phi) over the interior, in FP64 with a fixed-order sum, without a host synchronisation.
@@ -223,6 +257,7 @@ static std::size_t arena_size(const poisson_solver_handle_v1 &s)
     b += align_up((std::size_t)s.nx * 8) + align_up((std::size_t)s.ny * 8);
     b += 2 * align_up(N * wbytes) + align_up(N * 8);
     b += align_up((kReduceSlots + kReduceFinal + 1) * 8);  // slots, 256 partials, the mean
+    b += align_up((std::size_t)residual_blocks(s) * 8);    // per-block residual sums of squares
     return b;
 }
 
@@ -298,6 +333,7 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
     s->w0 = carve(N * wbytes);
     s->w1 = carve(N * wbytes);
     s->res = (double *)carve(N * 8);
+    s->partial = (double *)carve((std::size_t)residual_blocks(*s) * 8);
     s->slots = (double *)carve((kReduceSlots + kReduceFinal + 1) * 8);
     s->final256 = s->slots + kReduceSlots;
     std::vector<double> qx, lx, qy, ly;
@@ -401,12 +437,21 @@ int poisson_solver_execute_device_v1(poisson_solver_handle_v1 *s, void *phi, siz
     while (st == FORT_SCOPE_OK) {
         const bool last = s->report.iterations >= s->max_solves;
         if (!s->measure && last) break;
-        st = residual(*s, pv.x, pv, bv, stream);
+        // FP64 solve: the FP64 residual and its fixed-order norm. Mixed solve: the split residual, whose
+        // norm is the fixed-order sum of the residual kernel's per-block sums of squares.
+        st = s->mixed ? residual_split(*s, pv, bv, s->measure ? s->partial : nullptr, stream)
+                      : residual(*s, pv.x, pv, bv, stream);
         if (st) break;
         if (s->measure) {
             double rn = 0.0;
-            ReduceBox box{s->res, s->nx, s->ny, s->nz, s->nx, plane};
-            st = norm2(*s, box, stream, &rn);
+            if (s->mixed) {
+                ReduceBox box{s->partial, residual_blocks(*s), 1, 1, residual_blocks(*s), residual_blocks(*s)};
+                st = sum_of(*s, box, stream, &rn);
+                rn = std::sqrt(rn);
+            } else {
+                ReduceBox box{s->res, s->nx, s->ny, s->nz, s->nx, plane};
+                st = norm2(*s, box, stream, &rn);
+            }
             if (st) break;
             s->report.relative_residual = bnorm > 0 ? rn / bnorm : rn;
             if (s->report.relative_residual <= s->tol) {
@@ -415,7 +460,7 @@ int poisson_solver_execute_device_v1(poisson_solver_handle_v1 *s, void *phi, siz
             }
             if (last) break;
         }
-        st = apply_inverse(*s, s->res, s->nx, plane, pv, 1.0, stream);
+        st = apply_inverse(*s, s->res, s->nx, plane, pv, 1.0, stream, s->mixed);
         s->report.iterations++;
     }
     if (st == FORT_SCOPE_OK && s->mixed) st = remove_mean(*s, pv, stream);

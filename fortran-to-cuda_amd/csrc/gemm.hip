@@ -79,6 +79,8 @@ int gemm_default_tile(const GemmCall &c, int precision)
 static int select_kernel(const GemmCall &c, int precision, int tile, const void **fn, dim3 *grid)
 {
     if (tile < 0 || tile >= kTileCount) tile = gemm_default_tile(c, precision);
+    if (c.shift && !(precision == kBf16x3 && (c.flags & kGemmAKContiguous)))
+        return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "an output shift needs the K-contiguous split-bf16 path");
     const Tile &T = kTiles[tile];
     *grid = dim3((c.M + T.bm - 1) / T.bm, (c.N + T.bn - 1) / T.bn, c.batch);
     if (precision == kF64) {

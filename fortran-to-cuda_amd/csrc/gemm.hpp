@@ -39,6 +39,8 @@ struct GemmCall {
     double alpha, beta;
     int flags;
     long long lo_a, lo_b, lo_c;  // lo-plane offsets (elements) of kSplit operands
+    // Split-bf16 K-contiguous calls only: when set, every stored C value is v - *shift (a device scalar).
+    const double *shift = nullptr;
 };
 
 // Host-side split of values into the kSplit layout: the value rounded to float, hi = its bf16 rounding

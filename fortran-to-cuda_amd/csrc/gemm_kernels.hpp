@@ -658,6 +658,8 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16TnLds
         }
     }
     void *C = const_cast<void *>(offset_elems<TO>(g.C, b * g.sc));
+    const bool shifted = g.shift != nullptr;
+    const double shift = shifted ? *g.shift : 0.0;
 #pragma unroll
     for (int x = 0; x < TM; ++x)
 #pragma unroll
@@ -671,8 +673,10 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16TnLds
                 double v[4];
                 if (g.beta != 0.0) load_out4<TO>(C, ci, g.lo_c, v);
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
+                for (int r = 0; r < 4; ++r) {
                     v[r] = g.beta != 0.0 ? g.alpha * (double)acc[x][y][r] + g.beta * v[r] : g.alpha * (double)acc[x][y][r];
+                    if (shifted) v[r] = v[r] - shift;
+                }
                 store_out4<TO>(C, ci, g.lo_c, v);
             } else {
 #pragma unroll
@@ -681,6 +685,7 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16TnLds
                     if (er < g.M && ec < g.N) {
                         double v = g.alpha * (double)acc[x][y][r];
                         if (g.beta != 0.0) v = v + g.beta * load_out<TO>(C, ci + r, g.lo_c);
+                        if (shifted) v = v - shift;
                         store_out<TO>(C, ci + r, g.lo_c, v);
                     }
                 }

@@ -107,6 +107,7 @@ struct poisson_solver_handle_v1 {
     void *w0 = nullptr, *w1 = nullptr;
     double *res = nullptr, *slots = nullptr, *final256 = nullptr;
     double *partial = nullptr;     // mixed: per-block sums of squares of the residual
+    double *phisum = nullptr;      // mixed: per-block sums of phi
     double *host_final = nullptr;  // pinned, 256 doubles
     poisson_solver_report_v1 report{};
     std::mutex mutex;
@@ -127,7 +128,7 @@ static int map_status(int scope_status)
 // The contraction calls of one application of the approximate (mixed) or exact (FP64) inverse:
 // phi_out = beta * phi_out + Qx [T^-1 ((Qx^T in) Qy)] Qy^T.
 static int apply_inverse(poisson_solver_handle_v1 &s, const void *in, long long in1, long long in2, View3 out,
-                         double beta, hipStream_t stream, bool in_split = false)
+                         double beta, hipStream_t stream, bool in_split = false, const double *shift = nullptr)
 {
     const int nx = s.nx, ny = s.ny, nz = s.nz;
     const long long plane = (long long)nx * ny;
@@ -158,7 +159,7 @@ static int apply_inverse(poisson_solver_handle_v1 &s, const void *in, long long 
                     ak, N, Ly, N};
         FTC_CHECK(gemm_launch(g3, prec, stream));
         GemmCall g4{s.QxTs, s.w1, out.x, kSplit, kSplit, kDouble, nx, ny, nx, nz, nx, nx, out.ld1, 0, plane, out.ld2,
-                    1.0, beta, ak, Lx, N, 0};
+                    1.0, beta, ak, Lx, N, 0, shift};
         return gemm_launch(g4, prec, stream);
     }
     GemmCall g1{s.QxT, in, s.w0, kDouble, kDouble, kDouble, nx, ny, nx, nz, nx, in1, nx, 0, in2, plane, 1.0, 0.0};
@@ -178,20 +179,37 @@ static int apply_inverse(poisson_solver_handle_v1 &s, const void *in, long long 
 
 // The mixed solve's residual, split for the next application; with `partial`, also the per-block sums of
 // squares for its norm.
-static int residual_split(poisson_solver_handle_v1 &s, View3 phiv, View3 b, double *partial, hipStream_t stream)
-{
-    const dim3 grid((s.nx + 63) / 64, (s.ny + 3) / 4, s.nz);
-    const long long N = (long long)s.nx * s.ny * s.nz;
-    hipLaunchKernelGGL(poisson_residual_split_kernel, grid, dim3(64, 4), 0, stream, phiv.x, phiv.ld1, phiv.ld2, b.x,
-                       b.ld1, b.ld2, reinterpret_cast<__bf16 *>(s.res), N, partial, s.nx, s.ny, s.nz,
-                       1.0 / (s.hx * s.hx), 1.0 / (s.hy * s.hy), 1.0 / (s.hz * s.hz), s.perx ? 1 : 0, s.pery ? 1 : 0);
-    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "residual launch failed");
-    return FORT_SCOPE_OK;
-}
-
 static long long residual_blocks(const poisson_solver_handle_v1 &s)
 {
     return (long long)((s.nx + 63) / 64) * ((s.ny + 3) / 4) * s.nz;
+}
+
+// The mixed solve's residual, split for the next application, and the mean of phi (from the residual
+// kernel's per-block sums, folded in fixed order) into mean_dev; with `partial`, also the per-block sums
+// of squares of the residual for its norm.
+static int residual_split(poisson_solver_handle_v1 &s, View3 phiv, View3 b, double *partial, double *mean_dev,
+                          hipStream_t stream)
+{
+    const dim3 grid((s.nx + 63) / 64, (s.ny + 3) / 4, s.nz);
+    const long long N = (long long)s.nx * s.ny * s.nz, nb = residual_blocks(s);
+    hipLaunchKernelGGL(poisson_residual_split_kernel, grid, dim3(64, 4), 0, stream, phiv.x, phiv.ld1, phiv.ld2, b.x,
+                       b.ld1, b.ld2, reinterpret_cast<__bf16 *>(s.res), N, partial, s.phisum, s.nx, s.ny, s.nz,
+                       1.0 / (s.hx * s.hx), 1.0 / (s.hy * s.hy), 1.0 / (s.hz * s.hz), s.perx ? 1 : 0, s.pery ? 1 : 0);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "residual launch failed");
+    ReduceBox box{s.phisum, nb, 1, 1, nb, nb};
+    FTC_CHECK(reduce_launch(kReduceSum, box, s.slots, s.final256, stream));
+    hipLaunchKernelGGL(poisson_mean_kernel, dim3(1), dim3(1), 0, stream, s.final256, (double)N, mean_dev);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "mean launch failed");
+    return FORT_SCOPE_OK;
+}
+
+static int subtract_mean(poisson_solver_handle_v1 &s, View3 phi, const double *mean_dev, hipStream_t stream)
+{
+    const dim3 grid((s.nx + 63) / 64, (s.ny + 3) / 4, s.nz);
+    hipLaunchKernelGGL(poisson_subtract_kernel, grid, dim3(64, 4), 0, stream, phi.x, phi.ld1, phi.ld2, s.nx, s.ny,
+                       mean_dev);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "mean removal failed");
+    return FORT_SCOPE_OK;
 }
 
 static int residual(poisson_solver_handle_v1 &s, const double *phi, View3 phiv, View3 b, hipStream_t stream)
@@ -257,7 +275,7 @@ static std::size_t arena_size(const poisson_solver_handle_v1 &s)
     b += align_up((std::size_t)s.nx * 8) + align_up((std::size_t)s.ny * 8);
     b += 2 * align_up(N * wbytes) + align_up(N * 8);
     b += align_up((kReduceSlots + kReduceFinal + 1) * 8);  // slots, 256 partials, the mean
-    b += align_up((std::size_t)residual_blocks(s) * 8);    // per-block residual sums of squares
+    b += 2 * align_up((std::size_t)residual_blocks(s) * 8);  // per-block residual sums of squares, of phi
     return b;
 }
 
@@ -334,6 +352,7 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
     s->w1 = carve(N * wbytes);
     s->res = (double *)carve(N * 8);
     s->partial = (double *)carve((std::size_t)residual_blocks(*s) * 8);
+    s->phisum = (double *)carve((std::size_t)residual_blocks(*s) * 8);
     s->slots = (double *)carve((kReduceSlots + kReduceFinal + 1) * 8);
     s->final256 = s->slots + kReduceSlots;
     std::vector<double> qx, lx, qy, ly;
@@ -427,6 +446,8 @@ int poisson_solver_execute_device_v1(poisson_solver_handle_v1 *s, void *phi, siz
     s->report.rhs_norm = -1.0;
     int st = FORT_SCOPE_OK;
     double bnorm = 0.0;
+    double *mean_dev = s->final256 + kReduceFinal;  // device scalar: phi's mean from the last residual
+    bool have_mean = false;
     if (s->measure) {
         ReduceBox box{bv.x, s->nx, s->ny, s->nz, bv.ld1, bv.ld2};
         st = norm2(*s, box, stream, &bnorm);
@@ -439,9 +460,10 @@ int poisson_solver_execute_device_v1(poisson_solver_handle_v1 *s, void *phi, siz
         if (!s->measure && last) break;
         // FP64 solve: the FP64 residual and its fixed-order norm. Mixed solve: the split residual, whose
         // norm is the fixed-order sum of the residual kernel's per-block sums of squares.
-        st = s->mixed ? residual_split(*s, pv, bv, s->measure ? s->partial : nullptr, stream)
+        st = s->mixed ? residual_split(*s, pv, bv, s->measure ? s->partial : nullptr, mean_dev, stream)
                       : residual(*s, pv.x, pv, bv, stream);
         if (st) break;
+        have_mean = s->mixed;
         if (s->measure) {
             double rn = 0.0;
             if (s->mixed) {
@@ -460,10 +482,19 @@ int poisson_solver_execute_device_v1(poisson_solver_handle_v1 *s, void *phi, siz
             }
             if (last) break;
         }
-        st = apply_inverse(*s, s->res, s->nx, plane, pv, 1.0, stream, s->mixed);
+        // The mixed correction also subtracts phi's current mean: the Laplacian does not see a constant,
+        // and the approximate transforms leak one (about the transform error times the correction).
+        st = apply_inverse(*s, s->res, s->nx, plane, pv, 1.0, stream, s->mixed, s->mixed ? mean_dev : nullptr);
+        have_mean = false;
         s->report.iterations++;
     }
-    if (st == FORT_SCOPE_OK && s->mixed) st = remove_mean(*s, pv, stream);
+    // Zero mean for the mixed solve: after a residual, subtract the mean it measured; after a single
+    // application, remove the mean explicitly. After a correction, what is left is the last correction's
+    // leak, as small as the field error that correction leaves.
+    if (st == FORT_SCOPE_OK && s->mixed) {
+        if (have_mean) st = subtract_mean(*s, pv, mean_dev, stream);
+        else if (s->report.iterations == 1) st = remove_mean(*s, pv, stream);
+    }
     if (st) return perr(map_status(st), fort_scope_error());
     return POISSON_SOLVER_OK;
 }

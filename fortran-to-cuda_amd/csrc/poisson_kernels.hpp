@@ -36,24 +36,24 @@ static __global__ void poisson_residual_kernel(const double *__restrict__ phi, l
 }
 
 // The mixed solve's residual: r = b - lap(phi) in FP64, stored as the split-bf16 contractions read it
-// (the value rounded to float, then bf16 hi and lo planes, lo `lo` elements after hi), and, when
-// `partial` is set, the sum of r*r over the block's 256 cells in a fixed tree order into
-// partial[block] (blocks in x, then y, then z order) for the residual norm.
+// (the value rounded to float, then bf16 hi and lo planes, lo `lo` elements after hi). Per block of 256
+// cells (blocks in x, then y, then z order), in a fixed tree order: the sum of phi into phisum[block]
+// and, when `partial` is set, the sum of r*r into partial[block] for the residual norm.
 static __global__ __launch_bounds__(256) void poisson_residual_split_kernel(
     const double *__restrict__ phi, long long p1, long long p2, const double *__restrict__ b, long long b1,
-    long long b2, __bf16 *__restrict__ r, long long lo, double *__restrict__ partial, int nx, int ny, int nz,
-    double rx, double ry, double rz, int perx, int pery)
+    long long b2, __bf16 *__restrict__ r, long long lo, double *__restrict__ partial, double *__restrict__ phisum,
+    int nx, int ny, int nz, double rx, double ry, double rz, int perx, int pery)
 {
-    __shared__ double sq[256];
+    __shared__ double sq[256], sp[256];
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     const int t = threadIdx.y * 64 + threadIdx.x;
-    double v = 0.0;
+    double v = 0.0, c0 = 0.0;
     if (i < nx && j < ny) {
         const int im = i == 0 ? (perx ? nx - 1 : 0) : i - 1, ip = i == nx - 1 ? (perx ? 0 : nx - 1) : i + 1;
         const int jm = j == 0 ? (pery ? ny - 1 : 0) : j - 1, jp = j == ny - 1 ? (pery ? 0 : ny - 1) : j + 1;
         const int km = k == 0 ? 0 : k - 1, kp = k == nz - 1 ? nz - 1 : k + 1;
 #define P(a, c, e) phi[(a) + (c) * p1 + (long long)(e) * p2]
-        const double c0 = P(i, j, k);
+        c0 = P(i, j, k);
         const double lap = (((P(im, j, k) + P(ip, j, k)) - 2.0 * c0) * rx +
                             ((P(i, jm, k) + P(i, jp, k)) - 2.0 * c0) * ry) +
                            ((P(i, j, km) + P(i, j, kp)) - 2.0 * c0) * rz;
@@ -65,15 +65,22 @@ static __global__ __launch_bounds__(256) void poisson_residual_split_kernel(
         r[out] = h;
         r[out + lo] = (__bf16)(f - (float)h);
     }
-    if (!partial) return;  // uniform over the launch
     sq[t] = v * v;
+    sp[t] = c0;
     __syncthreads();
 #pragma unroll
     for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) sq[t] = sq[t] + sq[t + w];
+        if (t < w) {
+            sq[t] = sq[t] + sq[t + w];
+            sp[t] = sp[t] + sp[t + w];
+        }
         __syncthreads();
     }
-    if (t == 0) partial[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = sq[0];
+    if (t == 0) {
+        const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+        phisum[blk] = sp[0];
+        if (partial) partial[blk] = sq[0];
+    }
 }
 
 // The mean of phi from the 256 fixed-order partial sums (folded in index order), on the device.

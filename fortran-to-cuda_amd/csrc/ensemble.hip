@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) void ens_subtract_kernel(const EnsMember *ms, 
 
 // QFAST: the mixed chain keeps mode (p, q) at column p*ny + q, the FP64 chain at p + q*nx.
 template <class T, bool QFAST>
-__global__ __launch_bounds__(kCols) void ens_tridiag_kernel(const EnsMember *ms, const ColItem *items, int n)
+__global__ __launch_bounds__(kCols, 3) void ens_tridiag_kernel(const EnsMember *ms, const ColItem *items, int n)
 {
     extern __shared__ double lds[];
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
@@ -228,8 +228,8 @@ __global__ __launch_bounds__(kCols) void ens_tridiag_kernel(const EnsMember *ms,
         const long long col = (long long)t.block * kCols + threadIdx.x;
         if (col >= plane) continue;
         const double lam = QFAST ? m.lx[col / m.ny] + m.ly[col % m.ny] : m.lx[col % m.nx] + m.ly[col / m.nx];
-        tridiag_column<T, T>(static_cast<const T *>(m.w1), TriOut<T>{static_cast<T *>(m.w0), 0}, plane, m.nz, lam,
-                             m.rz, lds, threadIdx.x, kCols, col);
+        tridiag_column_prefetch<T, T>(static_cast<const T *>(m.w1), TriOut<T>{static_cast<T *>(m.w0), 0}, plane,
+                                      m.nz, lam, m.rz, lds, threadIdx.x, kCols, col);
     }
 }
 

@@ -141,18 +141,22 @@ __global__ __launch_bounds__(256) void ens_residual_kernel(const EnsMember *ms, 
         const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
         const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
         const long long plane = (long long)nx * ny;
+        // phi at (i, j) over k is carried in registers (the k-1, k, k+1 window; Neumann ends repeat the
+        // end plane), so only the x and y neighbours are read per plane.
+#define P(x, y, z) m.phi[(x) + (long long)nx * (y) + plane * (z)]
+        double c0 = P(i, j, 0), zm = c0;
 #pragma unroll 4
         for (int k = 0; k < nz; ++k) {
-            const int km = k == 0 ? 0 : k - 1, kp = k == nz - 1 ? nz - 1 : k + 1;
-#define P(x, y, z) m.phi[(x) + (long long)nx * (y) + plane * (z)]
-            const double c0 = P(i, j, k);
+            const double zp = k == nz - 1 ? c0 : P(i, j, k + 1);
             const double lap = (((P(im, j, k) + P(ip, j, k)) - 2.0 * c0) * m.rx +
                                 ((P(i, jm, k) + P(i, jp, k)) - 2.0 * c0) * m.ry) +
-                               ((P(i, j, km) + P(i, j, kp)) - 2.0 * c0) * m.rz;
-#undef P
+                               ((zm + zp) - 2.0 * c0) * m.rz;
             const long long q = i + (long long)nx * j + plane * k;
             m.res[q] = m.rhs[q] - lap;
+            zm = c0;
+            c0 = zp;
         }
+#undef P
     }
 }
 

@@ -179,9 +179,15 @@ static int apply_inverse(poisson_solver_handle_v1 &s, const void *in, long long 
 
 // The mixed solve's residual, split for the next application; with `partial`, also the per-block sums of
 // squares for its norm.
+static dim3 residual_split_grid(const poisson_solver_handle_v1 &s)
+{
+    return dim3((s.nx + 63) / 64, (s.ny + 3) / 4, (s.nz + kResidualChunk - 1) / kResidualChunk);
+}
+
 static long long residual_blocks(const poisson_solver_handle_v1 &s)
 {
-    return (long long)((s.nx + 63) / 64) * ((s.ny + 3) / 4) * s.nz;
+    const dim3 g = residual_split_grid(s);
+    return (long long)g.x * g.y * g.z;
 }
 
 // The mixed solve's residual, split for the next application, and the mean of phi (from the residual
@@ -190,7 +196,7 @@ static long long residual_blocks(const poisson_solver_handle_v1 &s)
 static int residual_split(poisson_solver_handle_v1 &s, View3 phiv, View3 b, double *partial, double *mean_dev,
                           hipStream_t stream)
 {
-    const dim3 grid((s.nx + 63) / 64, (s.ny + 3) / 4, s.nz);
+    const dim3 grid = residual_split_grid(s);
     const long long N = (long long)s.nx * s.ny * s.nz, nb = residual_blocks(s);
     hipLaunchKernelGGL(poisson_residual_split_kernel, grid, dim3(64, 4), 0, stream, phiv.x, phiv.ld1, phiv.ld2, b.x,
                        b.ld1, b.ld2, reinterpret_cast<__bf16 *>(s.res), N, partial, s.phisum, s.nx, s.ny, s.nz,

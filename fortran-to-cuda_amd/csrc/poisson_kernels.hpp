@@ -36,37 +36,48 @@ static __global__ void poisson_residual_kernel(const double *__restrict__ phi, l
 }
 
 // The mixed solve's residual: r = b - lap(phi) in FP64, stored as the split-bf16 contractions read it
-// (the value rounded to float, then bf16 hi and lo planes, lo `lo` elements after hi). Per block of 256
-// cells (blocks in x, then y, then z order), in a fixed tree order: the sum of phi into phisum[block]
-// and, when `partial` is set, the sum of r*r into partial[block] for the residual norm.
+// (the value rounded to float, then bf16 hi and lo planes, lo `lo` elements after hi). A block covers
+// 64 x 4 columns over kResidualChunk planes; phi at (i, j) is carried in registers over k (the k-1, k, k+1
+// window, Neumann ends repeating the end plane). Per block (blocks in x, then y, then chunk order), in a
+// fixed order (each thread over its k in order, then a tree over the 256 threads): the sum of phi into
+// phisum[block] and, when `partial` is set, the sum of r*r into partial[block] for the residual norm.
+constexpr int kResidualChunk = 16;
+
 static __global__ __launch_bounds__(256) void poisson_residual_split_kernel(
     const double *__restrict__ phi, long long p1, long long p2, const double *__restrict__ b, long long b1,
     long long b2, __bf16 *__restrict__ r, long long lo, double *__restrict__ partial, double *__restrict__ phisum,
     int nx, int ny, int nz, double rx, double ry, double rz, int perx, int pery)
 {
     __shared__ double sq[256], sp[256];
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * kResidualChunk, k1 = min(k0 + kResidualChunk, nz);
     const int t = threadIdx.y * 64 + threadIdx.x;
-    double v = 0.0, c0 = 0.0;
+    double acc2 = 0.0, acc = 0.0;
     if (i < nx && j < ny) {
         const int im = i == 0 ? (perx ? nx - 1 : 0) : i - 1, ip = i == nx - 1 ? (perx ? 0 : nx - 1) : i + 1;
         const int jm = j == 0 ? (pery ? ny - 1 : 0) : j - 1, jp = j == ny - 1 ? (pery ? 0 : ny - 1) : j + 1;
-        const int km = k == 0 ? 0 : k - 1, kp = k == nz - 1 ? nz - 1 : k + 1;
 #define P(a, c, e) phi[(a) + (c) * p1 + (long long)(e) * p2]
-        c0 = P(i, j, k);
-        const double lap = (((P(im, j, k) + P(ip, j, k)) - 2.0 * c0) * rx +
-                            ((P(i, jm, k) + P(i, jp, k)) - 2.0 * c0) * ry) +
-                           ((P(i, j, km) + P(i, j, kp)) - 2.0 * c0) * rz;
+        double c0 = P(i, j, k0), zm = k0 == 0 ? c0 : P(i, j, k0 - 1);
+        for (int k = k0; k < k1; ++k) {
+            const double zp = k == nz - 1 ? c0 : P(i, j, k + 1);
+            const double lap = (((P(im, j, k) + P(ip, j, k)) - 2.0 * c0) * rx +
+                                ((P(i, jm, k) + P(i, jp, k)) - 2.0 * c0) * ry) +
+                               ((zm + zp) - 2.0 * c0) * rz;
+            const double v = b[i + j * b1 + k * b2] - lap;
+            const long long out = (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
+            const float f = (float)v;
+            const __bf16 h = (__bf16)f;
+            r[out] = h;
+            r[out + lo] = (__bf16)(f - (float)h);
+            acc2 = acc2 + v * v;
+            acc = acc + c0;
+            zm = c0;
+            c0 = zp;
+        }
 #undef P
-        v = b[i + j * b1 + k * b2] - lap;
-        const long long out = (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
-        const float f = (float)v;
-        const __bf16 h = (__bf16)f;
-        r[out] = h;
-        r[out + lo] = (__bf16)(f - (float)h);
     }
-    sq[t] = v * v;
-    sp[t] = c0;
+    sq[t] = acc2;
+    sp[t] = acc;
     __syncthreads();
 #pragma unroll
     for (int w = 128; w > 0; w >>= 1) {

@@ -101,6 +101,7 @@ static int select_kernel(const GemmCall &c, int precision, int tile, const void 
             case S * 27 + D * 9 + S * 3: *fn = tn_by_tile<Split16, double, Split16, true>(tile); break;
             case S * 27 + S * 9 + F * 3: *fn = tn_by_tile<Split16, Split16, float, true>(tile); break;
             case S * 27 + S * 9 + S * 3: *fn = tn_by_tile<Split16, Split16, Split16, true>(tile); break;
+            case S * 27 + D * 9 + F * 3: *fn = tn_by_tile<Split16, double, float, true>(tile); break;
             default: return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unsupported K-contiguous operand types");
             }
         } else {
@@ -110,6 +111,7 @@ static int select_kernel(const GemmCall &c, int precision, int tile, const void 
             case D * 27 + D * 9 + D * 3: *fn = tn_by_tile<double, double, double, false>(tile); break;
             case S * 27 + S * 9 + S * 3: *fn = tn_by_tile<Split16, Split16, Split16, false>(tile); break;
             case S * 27 + S * 9 + D * 3: *fn = tn_by_tile<Split16, Split16, double, false>(tile); break;
+            case F * 27 + D * 9 + D * 3: *fn = tn_by_tile<float, double, double, false>(tile); break;
             default: return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unsupported K-contiguous operand types");
             }
         }
@@ -174,6 +176,41 @@ static int gemm_enqueue(fort_scope_t ctx, const GemmCall &c, int precision, cons
     FTC_CHECK(select_kernel(c, precision, -1, &fn, &grid));
     return enqueue_kernel(ctx, 0x6e33000000000000ull | (uint64_t)precision, fn, grid, dim3(kGemmThreads), 0, &c,
                           sizeof c, b);
+}
+
+// ------------------------------------------------------------------------------------------------ chain
+// out = A^T split: out[m*ldo + k] (bf16 hi) and out[lo + m*ldo + k] (bf16 lo) hold A(m, k) = a[m + k*lda]
+// rounded to float and split as the contractions split it, so A becomes a K-contiguous split operand.
+// 64 x 64 tiles through LDS: columns of A are read and rows of the output written as contiguous runs.
+struct SplitTransposeArgs {
+    const double *a;
+    __bf16 *out;
+    int m, k;
+    long long lda, ldo, lo;
+};
+
+__global__ __launch_bounds__(256) void split_transpose_kernel(const SplitTransposeArgs p)
+{
+    __shared__ double tile[64][65];
+    const int m0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int kk = ty + 4 * r, m = m0 + tx, k = k0 + kk;
+        tile[kk][tx] = (m < p.m && k < p.k) ? p.a[(long long)m + (long long)k * p.lda] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int mm = ty + 4 * r, m = m0 + mm, k = k0 + tx;
+        if (m < p.m && k < p.k) {
+            const float f = (float)tile[tx][mm];
+            const __bf16 h = (__bf16)f;
+            const long long o = (long long)m * p.ldo + k;
+            p.out[o] = h;
+            p.out[p.lo + o] = (__bf16)(f - (float)h);
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------------ probe
@@ -266,18 +303,38 @@ int cpp_fort_shared_chain_contraction(fort_scope_t ctx, int mode, fort_buffer_t 
     FTC_CHECK(fort_scope_forget_definition(ctx, d));
     std::vector<void *> dev;
     FTC_CHECK(reserve(ctx, {d, a, b, c}, dev));
-    // T = A*B stays FP64 in both precisions: its traffic is negligible next to the 4n^3 flops.
+    // FP64: T = A*B into an FP64 scratch array, then D = T*C.
+    // Split-bf16: every operand K-contiguous. A is transposed and split once (At), T = A*B is written
+    // transposed in FP32 (row-major, so K-contiguous as the A operand of the second product), and B and C
+    // (column-major, K-contiguous) are split as they are loaded. At and T share the n*n*8-byte scratch.
     const std::size_t tbytes = (std::size_t)n * n * 8;
     fort_scope_scratch_lease_v1 lease{};
     FTC_CHECK(fort_scope_scratch_acquire_v1(ctx, tbytes, &lease));
     const std::size_t N = n;
     const Sec3 all2{{0, 0, 0}, {N, N, 1}};
-    GemmCall first{dev[1], dev[2], lease.device, kDouble, kDouble, kDouble, n, n, n, 1, n, n, n, 0, 0, 0, 1.0, 0.0};
-    GemmCall second{lease.device, dev[3], dev[0], kDouble, kDouble, kDouble, n, n, n, 1, n, n, n, 0, 0, 0, 1.0, 0.0};
+    const long long nn = (long long)n * n;
+    int status = FORT_SCOPE_OK;
     // Binding sections are rank 3 in the helper; a rank-2 array uses the first two bounds.
-    int status = gemm_enqueue(ctx, first, precision, {{a, {all2}, {}, {}}, {b, {all2}, {}, {}}});
-    if (status == FORT_SCOPE_OK)
-        status = gemm_enqueue(ctx, second, precision, {{c, {all2}, {}, {}}, {d, {}, {}, {all2}}});
+    if (precision == kF64) {
+        GemmCall first{dev[1], dev[2], lease.device, kDouble, kDouble, kDouble, n, n, n, 1, n, n, n, 0, 0, 0, 1.0, 0.0};
+        GemmCall second{lease.device, dev[3], dev[0], kDouble, kDouble, kDouble, n, n, n, 1, n, n, n, 0, 0, 0, 1.0, 0.0};
+        status = gemm_enqueue(ctx, first, precision, {{a, {all2}, {}, {}}, {b, {all2}, {}, {}}});
+        if (status == FORT_SCOPE_OK)
+            status = gemm_enqueue(ctx, second, precision, {{c, {all2}, {}, {}}, {d, {}, {}, {all2}}});
+    } else {
+        __bf16 *at = static_cast<__bf16 *>(lease.device);
+        float *t = reinterpret_cast<float *>(at + 2 * nn);
+        const SplitTransposeArgs st{static_cast<const double *>(dev[1]), at, n, n, n, n, nn};
+        status = enqueue_kernel(ctx, 0x5e71000000000000ull, reinterpret_cast<const void *>(&split_transpose_kernel),
+                                dim3((n + 63) / 64, (n + 63) / 64), dim3(256), 0, &st, sizeof st,
+                                {{a, {all2}, {}, {}}});
+        const int ak = kGemmAKContiguous, at_flags = kGemmAKContiguous | kGemmTransposedC;
+        GemmCall first{at, dev[2], t, kSplit, kDouble, kFloat, n, n, n, 1, n, n, n, 0, 0, 0, 1.0, 0.0, at_flags, nn, 0, 0};
+        GemmCall second{t, dev[3], dev[0], kFloat, kDouble, kDouble, n, n, n, 1, n, n, n, 0, 0, 0, 1.0, 0.0, ak, 0, 0, 0};
+        if (status == FORT_SCOPE_OK) status = gemm_enqueue(ctx, first, precision, {{b, {all2}, {}, {}}});
+        if (status == FORT_SCOPE_OK)
+            status = gemm_enqueue(ctx, second, precision, {{c, {all2}, {}, {}}, {d, {}, {}, {all2}}});
+    }
     int released = fort_scope_scratch_release_v1(ctx, lease.token);
     return status ? status : released;
 }

@@ -112,6 +112,7 @@ static int select_kernel(const GemmCall &c, int precision, int tile, const void 
             case S * 27 + S * 9 + S * 3: *fn = tn_by_tile<Split16, Split16, Split16, false>(tile); break;
             case S * 27 + S * 9 + D * 3: *fn = tn_by_tile<Split16, Split16, double, false>(tile); break;
             case F * 27 + D * 9 + D * 3: *fn = tn_by_tile<float, double, double, false>(tile); break;
+            case F * 27 + S * 9 + D * 3: *fn = tn_by_tile<float, Split16, double, false>(tile); break;
             default: return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unsupported K-contiguous operand types");
             }
         }
@@ -213,6 +214,24 @@ __global__ __launch_bounds__(256) void split_transpose_kernel(const SplitTranspo
     }
 }
 
+// out[i] (bf16 hi) and out[lo + i] (bf16 lo) hold a[i] rounded to float and split, for i < n: a
+// column-major operand that is already K-contiguous, split once instead of in every tile that loads it.
+struct SplitArgs {
+    const double *a;
+    __bf16 *out;
+    long long n, lo;
+};
+
+__global__ __launch_bounds__(256) void split_planes_kernel(const SplitArgs p)
+{
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < p.n; i += (long long)gridDim.x * 256) {
+        const float f = (float)p.a[i];
+        const __bf16 h = (__bf16)f;
+        p.out[i] = h;
+        p.out[p.lo + i] = (__bf16)(f - (float)h);
+    }
+}
+
 // ------------------------------------------------------------------------------------------------ probe
 __global__ void mfma_probe_f64(double *out, int a_lane, int b_lane)
 {
@@ -304,10 +323,11 @@ int cpp_fort_shared_chain_contraction(fort_scope_t ctx, int mode, fort_buffer_t 
     std::vector<void *> dev;
     FTC_CHECK(reserve(ctx, {d, a, b, c}, dev));
     // FP64: T = A*B into an FP64 scratch array, then D = T*C.
-    // Split-bf16: every operand K-contiguous. A is transposed and split once (At), T = A*B is written
-    // transposed in FP32 (row-major, so K-contiguous as the A operand of the second product), and B and C
-    // (column-major, K-contiguous) are split as they are loaded. At and T share the n*n*8-byte scratch.
-    const std::size_t tbytes = (std::size_t)n * n * 8;
+    // Split-bf16: every operand K-contiguous and split once. A is transposed and split (At), B and C
+    // (column-major, so already K-contiguous) are split in place order (Bs, Cs), and T = A*B is written
+    // transposed in FP32 (row-major: K-contiguous as the A operand of the second product). Scratch: At, Bs,
+    // Cs and T, n*n*4 bytes each.
+    const std::size_t tbytes = (std::size_t)n * n * (precision == kF64 ? 8 : 16);
     fort_scope_scratch_lease_v1 lease{};
     FTC_CHECK(fort_scope_scratch_acquire_v1(ctx, tbytes, &lease));
     const std::size_t N = n;
@@ -322,18 +342,25 @@ int cpp_fort_shared_chain_contraction(fort_scope_t ctx, int mode, fort_buffer_t 
         if (status == FORT_SCOPE_OK)
             status = gemm_enqueue(ctx, second, precision, {{c, {all2}, {}, {}}, {d, {}, {}, {all2}}});
     } else {
-        __bf16 *at = static_cast<__bf16 *>(lease.device);
-        float *t = reinterpret_cast<float *>(at + 2 * nn);
+        __bf16 *at = static_cast<__bf16 *>(lease.device), *bs = at + 2 * nn, *cs = bs + 2 * nn;
+        float *t = reinterpret_cast<float *>(cs + 2 * nn);
         const SplitTransposeArgs st{static_cast<const double *>(dev[1]), at, n, n, n, n, nn};
         status = enqueue_kernel(ctx, 0x5e71000000000000ull, reinterpret_cast<const void *>(&split_transpose_kernel),
                                 dim3((n + 63) / 64, (n + 63) / 64), dim3(256), 0, &st, sizeof st,
                                 {{a, {all2}, {}, {}}});
-        const int ak = kGemmAKContiguous, at_flags = kGemmAKContiguous | kGemmTransposedC;
-        GemmCall first{at, dev[2], t, kSplit, kDouble, kFloat, n, n, n, 1, n, n, n, 0, 0, 0, 1.0, 0.0, at_flags, nn, 0, 0};
-        GemmCall second{t, dev[3], dev[0], kFloat, kDouble, kDouble, n, n, n, 1, n, n, n, 0, 0, 0, 1.0, 0.0, ak, 0, 0, 0};
-        if (status == FORT_SCOPE_OK) status = gemm_enqueue(ctx, first, precision, {{b, {all2}, {}, {}}});
+        const unsigned sgrid = (unsigned)std::min<long long>((nn + 255) / 256, 4096);
+        const SplitArgs sb{static_cast<const double *>(dev[2]), bs, nn, nn}, sc{static_cast<const double *>(dev[3]), cs, nn, nn};
         if (status == FORT_SCOPE_OK)
-            status = gemm_enqueue(ctx, second, precision, {{c, {all2}, {}, {}}, {d, {}, {}, {all2}}});
+            status = enqueue_kernel(ctx, 0x5e72000000000000ull, reinterpret_cast<const void *>(&split_planes_kernel),
+                                    dim3(sgrid), dim3(256), 0, &sb, sizeof sb, {{b, {all2}, {}, {}}});
+        if (status == FORT_SCOPE_OK)
+            status = enqueue_kernel(ctx, 0x5e72000000000000ull, reinterpret_cast<const void *>(&split_planes_kernel),
+                                    dim3(sgrid), dim3(256), 0, &sc, sizeof sc, {{c, {all2}, {}, {}}});
+        const int ak = kGemmAKContiguous, at_flags = kGemmAKContiguous | kGemmTransposedC;
+        GemmCall first{at, bs, t, kSplit, kSplit, kFloat, n, n, n, 1, n, n, n, 0, 0, 0, 1.0, 0.0, at_flags, nn, nn, 0};
+        GemmCall second{t, cs, dev[0], kFloat, kSplit, kDouble, n, n, n, 1, n, n, n, 0, 0, 0, 1.0, 0.0, ak, 0, nn, 0};
+        if (status == FORT_SCOPE_OK) status = gemm_enqueue(ctx, first, precision, {});
+        if (status == FORT_SCOPE_OK) status = gemm_enqueue(ctx, second, precision, {{d, {}, {}, {all2}}});
     }
     int released = fort_scope_scratch_release_v1(ctx, lease.token);
     return status ? status : released;

@@ -187,13 +187,12 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     a.dt = step ? *dt : 0.0;
     fill_coefficients(a, *dx, *dy, *dz, *cs);
     // The staged kernel: three-plane ring with one row per thread (three workgroups per CU) or two rows per
-    // thread (FTC_LES_ROWS=2, two per CU); FTC_LES_V2 selects the four-plane ring, FTC_LES_UNSTAGED the
-    // unstaged kernel, both for comparisons.
+    // thread (FTC_LES_ROWS=2, two per CU); FTC_LES_UNSTAGED selects the unstaged kernel. Both variants are
+    // bit-identical to the default and kept for comparisons.
     const bool staged = sgs && !std::getenv("FTC_LES_UNSTAGED");
-    const bool v2 = std::getenv("FTC_LES_V2") != nullptr;
     const char *rows_env = std::getenv("FTC_LES_ROWS");
-    const int rows = staged && !v2 && rows_env && std::atoi(rows_env) == 2 ? 2 : 1;
-    a.kchunk = choose_kchunk(nx, ny, nz, rows, !staged || v2 ? 2 : (rows == 2 ? 2 : 3));
+    const int rows = staged && rows_env && std::atoi(rows_env) == 2 ? 2 : 1;
+    a.kchunk = choose_kchunk(nx, ny, nz, rows, !staged ? 2 : (rows == 2 ? 2 : 3));
     // Tiles in launch order put each 64-column strip on one XCD, so the y neighbours that re-read a tile's
     // ghost rows share its L2. The XCD-contiguous order (FTC_LES_XCD_ORDER) also shares the x neighbours'
     // lines, but only between tiles resident at the same time; over four rounds of workgroups it read more.
@@ -218,9 +217,7 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     b[6] = {w, {box(0, X + 2, 0, Y + 2, 0, Z + 1)}, {}, {}};
     b[7] = {s, {full}, {}, {}};
     const void *fn;
-    if (staged && v2) fn = step ? reinterpret_cast<const void *>(&les_sgs_kernel<true>)
-                                : reinterpret_cast<const void *>(&les_sgs_kernel<false>);
-    else if (staged && rows == 2) fn = step ? reinterpret_cast<const void *>(&les_sgs3_kernel<true, 2>)
+    if (staged && rows == 2) fn = step ? reinterpret_cast<const void *>(&les_sgs3_kernel<true, 2>)
                                             : reinterpret_cast<const void *>(&les_sgs3_kernel<false, 2>);
     else if (staged) fn = step ? reinterpret_cast<const void *>(&les_sgs3_kernel<true, 1>)
                                : reinterpret_cast<const void *>(&les_sgs3_kernel<false, 1>);

@@ -278,12 +278,11 @@ __global__ __launch_bounds__(TX *TY) void les_kernel(const LesArgs a)
     }
 }
 
-// ------------------------------------------------------------------------------------------------ v2
-// The Smagorinsky variants stage every input plane through LDS once: a workgroup keeps a ring of four
-// velocity planes (k-1 .. k+2) of its tile with a two-cell ring, three scalar planes and three viscosity
-// planes with a one-cell ring. The global loads of plane k+3 are issued into registers while plane k is
-// computed (software pipelining), so each input value is read from HBM about once per workgroup and every
-// stencil operand after that comes from LDS.
+// ------------------------------------------------------------------------------------------------ staging
+// The Smagorinsky kernel stages every input plane through LDS once: a workgroup keeps its tile's velocity
+// planes with a two-cell ring and its scalar and viscosity planes with a one-cell ring. The global loads
+// of the next plane are issued into registers while the current one is computed, so each input value is
+// read from HBM about once per workgroup and every stencil operand after that comes from LDS.
 //
 // Precondition (documented at the entries): the x and y ghost layers of U, V, W and S equal their
 // periodic images, as every field written by the step or the halo fill is. The staged tile is gathered
@@ -294,12 +293,6 @@ constexpr int NTH = TX * TY;
 constexpr int VCELLS = VX * VY, CCELLS = CX * CY;
 constexpr int VR = (VCELLS + NTH - 1) / NTH, CR = (CCELLS + NTH - 1) / NTH;
 
-struct SgsLds {
-    double vel[4][3][VY][VX];
-    double sca[3][CY][CX];
-    double nu[3][CY][CX];
-};
-
 __device__ __forceinline__ int periodic_image(int c, int n)
 {
     int r = (c - 2) % n;
@@ -307,267 +300,7 @@ __device__ __forceinline__ int periodic_image(int c, int n)
     return r + 2;
 }
 
-// One workgroup's share of the staged LES kernel: the (bx, by) tile of columns over k chunk bz.
-template <bool STEP>
-__device__ __forceinline__ void les_sgs_body(const LesArgs &a, SgsLds &L, int bx, int by, int bz)
-{
-    const double *__restrict__ gU = a.U;
-    const double *__restrict__ gV = a.V;
-    const double *__restrict__ gW = a.W;
-    const double *__restrict__ gS = a.S;
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int i0 = 2 + bx * TX, j0 = 2 + by * TY;
-    const int i = i0 + tx, j = j0 + ty;
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int k0 = 2 + bz * a.kchunk;
-    if (k0 > nz + 1) return;  // uniform over the workgroup
-    const int k1 = min(k0 + a.kchunk - 1, nz + 1);
-    const long long sj = a.sj, sk = a.sk;
-
-    long long voff[VR], coff[CR];
-    int vcell[VR], ccell[CR];
-#pragma unroll
-    for (int r = 0; r < VR; ++r) {
-        const int c = tid + r * NTH;
-        vcell[r] = c < VCELLS ? c : -1;
-        const int cc = c < VCELLS ? c : 0;
-        const int ii = periodic_image(i0 - 2 + cc % VX, nx), jj = periodic_image(j0 - 2 + cc / VX, ny);
-        voff[r] = (long long)(ii - 1) + (long long)(jj - 1) * sj;
-    }
-#pragma unroll
-    for (int r = 0; r < CR; ++r) {
-        const int c = tid + r * NTH;
-        ccell[r] = c < CCELLS ? c : -1;
-        const int cc = c < CCELLS ? c : 0;
-        const int ii = periodic_image(i0 - 1 + cc % CX, nx), jj = periodic_image(j0 - 1 + cc / CX, ny);
-        coff[r] = (long long)(ii - 1) + (long long)(jj - 1) * sj;
-    }
-    double rv[VR][3], rs[CR];
-    auto fetch_vel = [&](int kp) {
-        if (kp < 1 || kp > nz + 2) return;
-        const long long base = (long long)(kp - 1) * sk;
-#pragma unroll
-        for (int r = 0; r < VR; ++r)
-            if (vcell[r] >= 0) {
-                rv[r][0] = gU[voff[r] + base];
-                rv[r][1] = gV[voff[r] + base];
-                rv[r][2] = gW[voff[r] + base];
-            }
-    };
-    auto put_vel = [&](int kp) {
-        if (kp < 1 || kp > nz + 2) return;
-        const int slot = kp & 3;
-#pragma unroll
-        for (int r = 0; r < VR; ++r)
-            if (vcell[r] >= 0) {
-                (&L.vel[slot][0][0][0])[vcell[r]] = rv[r][0];
-                (&L.vel[slot][1][0][0])[vcell[r]] = rv[r][1];
-                (&L.vel[slot][2][0][0])[vcell[r]] = rv[r][2];
-            }
-    };
-    auto fetch_sca = [&](int kp) {
-        if (kp < 1 || kp > nz + 2) return;
-        const long long base = (long long)(kp - 1) * sk;
-#pragma unroll
-        for (int r = 0; r < CR; ++r)
-            if (ccell[r] >= 0) rs[r] = gS[coff[r] + base];
-    };
-    auto put_sca = [&](int kp) {
-        if (kp < 1 || kp > nz + 2) return;
-        const int slot = kp % 3;
-#pragma unroll
-        for (int r = 0; r < CR; ++r)
-            if (ccell[r] >= 0) (&L.sca[slot][0][0])[ccell[r]] = rs[r];
-    };
-    const double *q = a.sgs;
-    auto nu_plane = [&](int qk) {
-        const int kk = qk < 2 ? 2 : (qk > nz + 1 ? nz + 1 : qk);
-        const int s0 = kk & 3, sm = (kk - 1) & 3, sp = (kk + 1) & 3;
-        double(*nuo)[CX] = L.nu[qk % 3];
-        for (int c = tid; c < CCELLS; c += NTH) {
-            const int li = c % CX, lj = c / CX;
-            const int vx = li + 1, vy = lj + 1;
-#define QU(s, di, dj) L.vel[s][0][vy + (dj)][vx + (di)]
-#define QV(s, di, dj) L.vel[s][1][vy + (dj)][vx + (di)]
-#define QW(s, di, dj) L.vel[s][2][vy + (dj)][vx + (di)]
-            double s11 = (QU(s0, 0, 0) - QU(s0, -1, 0)) * q[0];
-            double s22 = (QV(s0, 0, 0) - QV(s0, 0, -1)) * q[1];
-            double s33 = (QW(s0, 0, 0) - QW(sm, 0, 0)) * q[2];
-            double dudy = ((QU(s0, 0, 1) + QU(s0, -1, 1)) - (QU(s0, 0, -1) + QU(s0, -1, -1))) * q[4];
-            double dvdx = ((QV(s0, 1, 0) + QV(s0, 1, -1)) - (QV(s0, -1, 0) + QV(s0, -1, -1))) * q[3];
-            double dudz = ((QU(sp, 0, 0) + QU(sp, -1, 0)) - (QU(sm, 0, 0) + QU(sm, -1, 0))) * q[5];
-            double dwdx = ((QW(s0, 1, 0) + QW(sm, 1, 0)) - (QW(s0, -1, 0) + QW(sm, -1, 0))) * q[3];
-            double dvdz = ((QV(sp, 0, 0) + QV(sp, 0, -1)) - (QV(sm, 0, 0) + QV(sm, 0, -1))) * q[5];
-            double dwdy = ((QW(s0, 0, 1) + QW(sm, 0, 1)) - (QW(s0, 0, -1) + QW(sm, 0, -1))) * q[4];
-#undef QU
-#undef QV
-#undef QW
-            double s12 = 0.5 * (dudy + dvdx);
-            double s13 = 0.5 * (dudz + dwdx);
-            double s23 = 0.5 * (dvdz + dwdy);
-            double ss = 2.0 * ((s11 * s11 + s22 * s22) + s33 * s33) + 4.0 * ((s12 * s12 + s13 * s13) + s23 * s23);
-            nuo[lj][li] = q[9] * sqrt(ss);
-        }
-    };
-
-    for (int kp = k0 - 2; kp <= k0 + 1; ++kp) {
-        fetch_vel(kp);
-        put_vel(kp);
-    }
-    for (int kp = k0 - 1; kp <= k0; ++kp) {
-        fetch_sca(kp);
-        put_sca(kp);
-    }
-    fetch_vel(k0 + 2);
-    fetch_sca(k0 + 1);
-    __syncthreads();
-    nu_plane(k0 - 1);
-    nu_plane(k0);
-    __syncthreads();
-
-    const bool inside = i <= nx + 1 && j <= ny + 1;
-    const double dt = a.dt;
-    const double c25x = a.c25[0], c25y = a.c25[1], c25z = a.c25[2];
-    for (int k = k0; k <= k1; ++k) {
-        put_vel(k + 2);
-        put_sca(k + 1);
-        fetch_vel(k + 3);
-        fetch_sca(k + 2);
-        __syncthreads();
-        nu_plane(k + 1);
-        __syncthreads();
-        if (inside) {
-            const int sm = (k - 1) & 3, s0 = k & 3, sp = (k + 1) & 3;
-            const int cm = (k - 1) % 3, c0 = k % 3, cp = (k + 1) % 3;
-#define VU(s, di, dj) L.vel[s][0][ty + 2 + (dj)][tx + 2 + (di)]
-#define VV(s, di, dj) L.vel[s][1][ty + 2 + (dj)][tx + 2 + (di)]
-#define VW(s, di, dj) L.vel[s][2][ty + 2 + (dj)][tx + 2 + (di)]
-#define SS(s, di, dj) L.sca[s][ty + 1 + (dj)][tx + 1 + (di)]
-#define NU(di, dj, dk) L.nu[(dk) < 0 ? cm : ((dk) > 0 ? cp : c0)][ty + 1 + (dj)][tx + 1 + (di)]
-            const double u0 = VU(s0, 0, 0), uxp = VU(s0, 1, 0), uxm = VU(s0, -1, 0), uyp = VU(s0, 0, 1);
-            const double uym = VU(s0, 0, -1), uzp = VU(sp, 0, 0), uzm = VU(sm, 0, 0);
-            const double v0 = VV(s0, 0, 0), vxp = VV(s0, 1, 0), vxm = VV(s0, -1, 0), vyp = VV(s0, 0, 1);
-            const double vym = VV(s0, 0, -1), vzp = VV(sp, 0, 0), vzm = VV(sm, 0, 0);
-            const double w0 = VW(s0, 0, 0), wxp = VW(s0, 1, 0), wxm = VW(s0, -1, 0), wyp = VW(s0, 0, 1);
-            const double wym = VW(s0, 0, -1), wzp = VW(sp, 0, 0), wzm = VW(sm, 0, 0);
-            const double n000 = NU(0, 0, 0), n100 = NU(1, 0, 0), nm00 = NU(-1, 0, 0);
-            const double n010 = NU(0, 1, 0), n0m0 = NU(0, -1, 0), n001 = NU(0, 0, 1), n00m = NU(0, 0, -1);
-            const long long p = (long long)(i - 1) + (long long)(j - 1) * sj + (long long)(k - 1) * sk;
-            // U: CDU (cdu.f90:48-107) then the diffusion of u.
-            {
-                const double vxpym = VV(s0, 1, -1), wxpzm = VW(sm, 1, 0);
-                double x = c25x * (uxp + u0) * (uxp + u0) - c25x * (u0 + uxm) * (u0 + uxm);
-                double y = c25y * (uyp + u0) * (vxp + v0) - c25y * (u0 + uym) * (vxpym + vym);
-                double z = c25z * (uzp + u0) * (wxp + w0) - c25z * (u0 + uzm) * (wxpzm + wzm);
-                double t = -((x + y) + z);
-                double Vadv = ((v0 + vxp) + vym) + vxpym;
-                double Wadv = ((w0 + wxp) + wzm) + wxpzm;
-                t = t - ((a.c50[0] * (uxp - uxm) * u0 + a.c125[1] * (uyp - uym) * Vadv) +
-                         a.c125[2] * (uzp - uzm) * Wadv);
-                t = t * 0.5;
-                double fxp = n100 * (uxp - u0);
-                double fxm = n000 * (u0 - uxm);
-                double exy0 = 0.25 * (((n000 + n100) + n010) + NU(1, 1, 0));
-                double exym = 0.25 * (((n0m0 + NU(1, -1, 0)) + n000) + n100);
-                double fyp = exy0 * (uyp - u0);
-                double fym = exym * (u0 - uym);
-                double exz0 = 0.25 * (((n000 + n100) + n001) + NU(1, 0, 1));
-                double exzm = 0.25 * (((n00m + NU(1, 0, -1)) + n000) + n100);
-                double fzp = exz0 * (uzp - u0);
-                double fzm = exzm * (u0 - uzm);
-                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
-                if constexpr (STEP) put_with_images(a, a.OU, p, u0 + dt * t, i, j, k, 0);
-                else a.OU[p] = t;
-            }
-            // V: CDV (cvd.f90:44-98) then the diffusion of v.
-            {
-                const double uxmyp = VU(s0, -1, 1), wypzm = VW(sm, 0, 1);
-                double y = c25y * (vyp + v0) * (vyp + v0) - c25y * (v0 + vym) * (v0 + vym);
-                double x = c25x * (vxp + v0) * (uyp + u0) - c25x * (v0 + vxm) * (uxmyp + uxm);
-                double z = c25z * (vzp + v0) * (wyp + w0) - c25z * (v0 + vzm) * (wypzm + wzm);
-                double t = -((y + x) + z);
-                double Uadv = ((u0 + uyp) + uxm) + uxmyp;
-                double Wadv = ((w0 + wyp) + wzm) + wypzm;
-                t = t - ((a.c125[0] * (vxp - vxm) * Uadv + a.c50[1] * (vyp - vym) * v0) +
-                         a.c125[2] * (vzp - vzm) * Wadv);
-                t = t * 0.5;
-                double exy0 = 0.25 * (((n000 + n100) + n010) + NU(1, 1, 0));
-                double exyxm = 0.25 * (((nm00 + n000) + NU(-1, 1, 0)) + n010);
-                double fxp = exy0 * (vxp - v0);
-                double fxm = exyxm * (v0 - vxm);
-                double fyp = n010 * (vyp - v0);
-                double fym = n000 * (v0 - vym);
-                double eyz0 = 0.25 * (((n000 + n010) + n001) + NU(0, 1, 1));
-                double eyzm = 0.25 * (((n00m + NU(0, 1, -1)) + n000) + n010);
-                double fzp = eyz0 * (vzp - v0);
-                double fzm = eyzm * (v0 - vzm);
-                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
-                if constexpr (STEP) put_with_images(a, a.OV, p, v0 + dt * t, i, j, k, 0);
-                else a.OV[p] = t;
-            }
-            // W on the interior faces: CDW (cdw.f90:44-98) then the diffusion of w.
-            if (k <= a.kw) {
-                const double uxmzp = VU(sp, -1, 0), vymzp = VV(sp, 0, -1);
-                double z = c25z * (wzp + w0) * (wzp + w0) - c25z * (w0 + wzm) * (w0 + wzm);
-                double y = c25y * (wyp + w0) * (vzp + v0) - c25y * (w0 + wym) * (vym + vymzp);
-                double x = c25x * (wxp + w0) * (uzp + u0) - c25x * (w0 + wxm) * (uxmzp + uxm);
-                double t = -((z + y) + x);
-                double Uadv = ((u0 + uzp) + uxm) + uxmzp;
-                double Vadv = ((v0 + vzp) + vym) + vymzp;
-                t = t - ((a.c125[0] * (wxp - wxm) * Uadv + a.c125[1] * (wyp - wym) * Vadv) +
-                         a.c50[2] * (wzp - wzm) * w0);
-                t = t * 0.5;
-                double exz0 = 0.25 * (((n000 + n100) + n001) + NU(1, 0, 1));
-                double exzxm = 0.25 * (((nm00 + n000) + NU(-1, 0, 1)) + n001);
-                double fxp = exz0 * (wxp - w0);
-                double fxm = exzxm * (w0 - wxm);
-                double eyz0 = 0.25 * (((n000 + n010) + n001) + NU(0, 1, 1));
-                double eyzym = 0.25 * (((n0m0 + n000) + NU(0, -1, 1)) + n001);
-                double fyp = eyz0 * (wyp - w0);
-                double fym = eyzym * (w0 - wym);
-                double fzp = n001 * (wzp - w0);
-                double fzm = n000 * (w0 - wzm);
-                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
-                if constexpr (STEP) put_with_images(a, a.OW, p, w0 + dt * t, i, j, k, 1);
-                else a.OW[p] = t;
-            }
-            // S: flux-form advection then the diffusion of s.
-            {
-                const double s0v = SS(c0, 0, 0), sxp = SS(c0, 1, 0), sxm = SS(c0, -1, 0), syp = SS(c0, 0, 1);
-                const double sym = SS(c0, 0, -1), szp = SS(cp, 0, 0), szm = SS(cm, 0, 0);
-                const double Cx = a.c50[0], Cy = a.c50[1], Cz = a.c50[2];
-                double x = Cx * u0 * (sxp + s0v) - Cx * uxm * (s0v + sxm);
-                double y = Cy * v0 * (syp + s0v) - Cy * vym * (s0v + sym);
-                double z = Cz * w0 * (szp + s0v) - Cz * wzm * (s0v + szm);
-                double t = -((x + y) + z);
-                double fxp = (0.5 * (n000 + n100)) * (sxp - s0v);
-                double fxm = (0.5 * (nm00 + n000)) * (s0v - sxm);
-                double fyp = (0.5 * (n000 + n010)) * (syp - s0v);
-                double fym = (0.5 * (n0m0 + n000)) * (s0v - sym);
-                double fzp = (0.5 * (n000 + n001)) * (szp - s0v);
-                double fzm = (0.5 * (n00m + n000)) * (s0v - szm);
-                t = t + (((fxp - fxm) * q[6] + (fyp - fym) * q[7]) + (fzp - fzm) * q[8]);
-                if constexpr (STEP) put_with_images(a, a.OS, p, s0v + dt * t, i, j, k, 2);
-                else a.OS[p] = t;
-            }
-#undef VU
-#undef VV
-#undef VW
-#undef SS
-#undef NU
-        }
-        __syncthreads();
-    }
-}
-
-template <bool STEP>
-__global__ __launch_bounds__(NTH, 2) void les_sgs_kernel(const LesArgs a)
-{
-    __shared__ SgsLds L;
-    les_sgs_body<STEP>(a, L, blockIdx.x, blockIdx.y, blockIdx.z);
-}
-
-// ------------------------------------------------------------------------------------------------ v3
+// ------------------------------------------------------------------------------------------------ step kernel
 // put_with_images with the column's x/y image offsets computed once per thread (they do not depend on k)
 // and the z images decided per plane (uniform over the workgroup): a cell off every boundary costs one
 // store. The outputs are not read again by the launch, so they are written nontemporally.

@@ -110,3 +110,20 @@ def test_les_steps_with_graph_replay_are_bit_exact_including_halos(ftc, scope, h
         assert gstats.instantiations == 2
         assert gstats.replays == steps - 3
         assert gstats.parameter_updates == 0
+
+
+@pytest.mark.parametrize("variant", [{"FTC_LES_ROWS": "2"}, {"FTC_LES_XCD_ORDER": "1"}, {"FTC_LES_UNSTAGED": "1"},
+                                     {"FTC_LES_KCHUNK": "5"}])
+@pytest.mark.parametrize("n", [(70, 37, 19), (130, 9, 33)])
+def test_selectable_step_variants_are_bit_exact(ftc, scope, hip, oracle, fields, monkeypatch, variant, n):
+    # The entry reads these switches at every call: two rows per thread, the XCD-contiguous tile order,
+    # the unstaged kernel, and a forced k chunk. Each must give the default kernel's (the oracle's) bits.
+    for key, value in variant.items():
+        monkeypatch.setenv(key, value)
+    U, V, W, S = fields.les_fields(*n)
+    got = tendency(ftc, scope, hip, U, V, W, S, 10.0, 0.17)
+    want = oracle.les_tendency(U, V, W, S, 10.0, 10.0, 10.0, 0.17)
+    nx, ny, nz = n
+    for name, g, w, kz in zip("UVWS", got, want, (nz, nz, nz - 1, nz)):
+        sl = (slice(1, nx + 1), slice(1, ny + 1), slice(1, kz + 1))
+        assert bits_equal(g[sl], w[sl]), f"{name} {variant}: " + describe_mismatch(g[sl], w[sl])

@@ -186,12 +186,7 @@ __device__ __forceinline__ void gemm_f64_db_tile(const GemmCall &g, F64DbLds<BM,
             L.Bs[stage][idx % BK][idx / BK] = rb[e];
         }
     };
-    load(0);
-    put(0);
-    if (BK < g.K) load(BK);
-    __syncthreads();
-    int stage = 0;
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
+    auto multiply = [&](int stage) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             double af[TM], bfr[TN];
@@ -205,13 +200,23 @@ __device__ __forceinline__ void gemm_f64_db_tile(const GemmCall &g, F64DbLds<BM,
                 for (int y = 0; y < TN; ++y)
                     acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bfr[y], af[x], acc[x][y], 0, 0, 0);
         }
-        if (k0 + BK < g.K) {
-            put(stage ^ 1);  // slice k0 + BK; the other stage's readers passed the previous barrier
-            if (k0 + 2 * BK < g.K) load(k0 + 2 * BK);
-        }
+    };
+    load(0);
+    put(0);
+    if (BK < g.K) load(BK);
+    __syncthreads();
+    int stage = 0, k0 = 0;
+    // Every slice but the last: multiply, store the next slice into the other stage (its readers passed
+    // the previous barrier), start the loads of the slice after. The last slice is peeled off, so the
+    // loop body has no store branch.
+    for (; k0 + BK < g.K; k0 += BK) {
+        multiply(stage);
+        put(stage ^ 1);
+        if (k0 + 2 * BK < g.K) load(k0 + 2 * BK);
         __syncthreads();
         stage ^= 1;
     }
+    multiply(stage);
     double *C = static_cast<double *>(g.C) + b * g.sc;
 #pragma unroll
     for (int x = 0; x < TM; ++x)

@@ -573,6 +573,48 @@ struct Bf16TnLds {
     __attribute__((aligned(16))) __bf16 Bl[BN][TnRow<BK>::kWidth];
 };
 
+// The epilogue of a K-contiguous split-bf16 tile: C = alpha*acc (+ beta*C) (- *shift), four adjacent
+// elements per lane (see the operand order in gemm_bf16x3_tn_tile), vector accesses for whole runs.
+template <class TO, int TM, int TN, int WTM, int WTN, bool TOUT>
+__device__ __forceinline__ void tn_store_tile(const GemmCall &g, const f32x4 (&acc)[TM][TN], int m0, int n0, int wm,
+                                              int wn, int lane, long long b)
+{
+    void *C = const_cast<void *>(offset_elems<TO>(g.C, b * g.sc));
+    const bool shifted = g.shift != nullptr;
+    const double shift = shifted ? *g.shift : 0.0;
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y) {
+            // (rr, cc) of element r = 0; element r is r rows (column-major) or r columns (TOUT) further on.
+            const int rr = m0 + wm * WTM + x * 16 + (TOUT ? lane % 16 : 4 * (lane / 16));
+            const int cc = n0 + wn * WTN + y * 16 + (TOUT ? 4 * (lane / 16) : lane % 16);
+            const long long ci = TOUT ? (long long)rr * g.ldc + cc : (long long)rr + (long long)cc * g.ldc;
+            const bool whole = TOUT ? rr < g.M && cc + 3 < g.N : rr + 3 < g.M && cc < g.N;
+            if (whole) {
+                double v[4];
+                if (g.beta != 0.0) load_out4<TO>(C, ci, g.lo_c, v);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    v[r] = g.beta != 0.0 ? g.alpha * (double)acc[x][y][r] + g.beta * v[r] : g.alpha * (double)acc[x][y][r];
+                    if (shifted) v[r] = v[r] - shift;
+                }
+                store_out4<TO>(C, ci, g.lo_c, v);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int er = TOUT ? rr : rr + r, ec = TOUT ? cc + r : cc;
+                    if (er < g.M && ec < g.N) {
+                        double v = g.alpha * (double)acc[x][y][r];
+                        if (g.beta != 0.0) v = v + g.beta * load_out<TO>(C, ci + r, g.lo_c);
+                        if (shifted) v = v - shift;
+                        store_out<TO>(C, ci + r, g.lo_c, v);
+                    }
+                }
+            }
+        }
+}
+
 template <class TA, class TB, class TO, int BM, int BN, bool TOUT, int BK = 32>
 __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16TnLds<BM, BN, BK> &L, int mt, int nt,
                                                     long long b)
@@ -662,40 +704,7 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16TnLds
             }
         }
     }
-    void *C = const_cast<void *>(offset_elems<TO>(g.C, b * g.sc));
-    const bool shifted = g.shift != nullptr;
-    const double shift = shifted ? *g.shift : 0.0;
-#pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TN; ++y) {
-            // (rr, cc) of element r = 0; element r is r rows (column-major) or r columns (TOUT) further on.
-            const int rr = m0 + wm * WTM + x * 16 + (TOUT ? lane % 16 : 4 * (lane / 16));
-            const int cc = n0 + wn * WTN + y * 16 + (TOUT ? 4 * (lane / 16) : lane % 16);
-            const long long ci = TOUT ? (long long)rr * g.ldc + cc : (long long)rr + (long long)cc * g.ldc;
-            const bool whole = TOUT ? rr < g.M && cc + 3 < g.N : rr + 3 < g.M && cc < g.N;
-            if (whole) {
-                double v[4];
-                if (g.beta != 0.0) load_out4<TO>(C, ci, g.lo_c, v);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    v[r] = g.beta != 0.0 ? g.alpha * (double)acc[x][y][r] + g.beta * v[r] : g.alpha * (double)acc[x][y][r];
-                    if (shifted) v[r] = v[r] - shift;
-                }
-                store_out4<TO>(C, ci, g.lo_c, v);
-            } else {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int er = TOUT ? rr : rr + r, ec = TOUT ? cc + r : cc;
-                    if (er < g.M && ec < g.N) {
-                        double v = g.alpha * (double)acc[x][y][r];
-                        if (g.beta != 0.0) v = v + g.beta * load_out<TO>(C, ci + r, g.lo_c);
-                        if (shifted) v = v - shift;
-                        store_out<TO>(C, ci + r, g.lo_c, v);
-                    }
-                }
-            }
-        }
+    tn_store_tile<TO, TM, TN, WTM, WTN, TOUT>(g, acc, m0, n0, wm, wn, lane, b);
 }
 
 template <class TA, class TB, class TO, int BM, int BN, bool TOUT, int BK = 32>

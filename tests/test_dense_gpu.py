@@ -154,7 +154,7 @@ def test_layout_flags_are_checked_before_any_launch(ftc, scope, hip):
 
 
 @pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 2e-4)])
-@pytest.mark.parametrize("n", [96, 301])
+@pytest.mark.parametrize("n", [96, 301, 256])  # 256: whole 128 x 128 tiles (the direct-to-LDS split path)
 def test_chain_contraction_entry(ftc, scope, hip, precision, tol, n):
     rng = np.random.default_rng(n + precision)
     A, B, C = (np.asfortranarray(rng.uniform(-1, 1, (n, n))) for _ in range(3))

@@ -129,34 +129,50 @@ __global__ __launch_bounds__(256) void ens_projection_kernel(const EnsMember *ms
     }
 }
 
-// res = rhs - lap(phi): dense arrays, periodic x and y, Neumann z.
-__global__ __launch_bounds__(256) void ens_residual_kernel(const EnsMember *ms, const TileItem *items, int n)
+// res = rhs - lap(phi): dense arrays, periodic x and y, Neumann z. Also the tile item's sum of phi, in the
+// order of ens_mean_partial_kernel (each column over k, then the 256 columns in thread order), into
+// part[base + it], for the mean the next correction subtracts.
+__global__ __launch_bounds__(256) void ens_residual_kernel(const EnsMember *ms, const TileItem *items, int n,
+                                                           double *part, int base)
 {
+    __shared__ double col[256];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const TileItem t = items[it];
         const EnsMember m = ms[t.member];
         const int i = t.bx * 64 + threadIdx.x, j = t.by * 4 + threadIdx.y;
-        if (i >= m.nx || j >= m.ny) continue;
-        const int nx = m.nx, ny = m.ny, nz = m.nz;
-        const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
-        const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
-        const long long plane = (long long)nx * ny;
-        // phi at (i, j) over k is carried in registers (the k-1, k, k+1 window; Neumann ends repeat the
-        // end plane), so only the x and y neighbours are read per plane.
+        double acc = 0.0;
+        if (i < m.nx && j < m.ny) {
+            const int nx = m.nx, ny = m.ny, nz = m.nz;
+            const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+            const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+            const long long plane = (long long)nx * ny;
+            // phi at (i, j) over k is carried in registers (the k-1, k, k+1 window; Neumann ends repeat the
+            // end plane), so only the x and y neighbours are read per plane.
 #define P(x, y, z) m.phi[(x) + (long long)nx * (y) + plane * (z)]
-        double c0 = P(i, j, 0), zm = c0;
+            double c0 = P(i, j, 0), zm = c0;
 #pragma unroll 4
-        for (int k = 0; k < nz; ++k) {
-            const double zp = k == nz - 1 ? c0 : P(i, j, k + 1);
-            const double lap = (((P(im, j, k) + P(ip, j, k)) - 2.0 * c0) * m.rx +
-                                ((P(i, jm, k) + P(i, jp, k)) - 2.0 * c0) * m.ry) +
-                               ((zm + zp) - 2.0 * c0) * m.rz;
-            const long long q = i + (long long)nx * j + plane * k;
-            m.res[q] = m.rhs[q] - lap;
-            zm = c0;
-            c0 = zp;
-        }
+            for (int k = 0; k < nz; ++k) {
+                const double zp = k == nz - 1 ? c0 : P(i, j, k + 1);
+                const double lap = (((P(im, j, k) + P(ip, j, k)) - 2.0 * c0) * m.rx +
+                                    ((P(i, jm, k) + P(i, jp, k)) - 2.0 * c0) * m.ry) +
+                                   ((zm + zp) - 2.0 * c0) * m.rz;
+                const long long q = i + (long long)nx * j + plane * k;
+                m.res[q] = m.rhs[q] - lap;
+                acc = acc + c0;
+                zm = c0;
+                c0 = zp;
+            }
 #undef P
+        }
+        col[tid] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            double sum = 0.0;
+            for (int c = 0; c < 256; ++c) sum = sum + col[c];
+            part[base + it] = sum;
+        }
+        __syncthreads();
     }
 }
 
@@ -285,7 +301,7 @@ static void periodic_basis(int n, double h, std::vector<double> &Q, std::vector<
     }
 }
 
-enum Stage { kLes, kDiv, kG1, kG2, kTri, kG3, kG4, kRes, kG1c, kG4c, kMean, kSub, kProj, kStageCount };
+enum Stage { kLes, kDiv, kG1, kG2, kTri, kG3, kG4, kRes, kG1c, kG4c, kMean, kSub, kProj, kFold, kStageCount };
 
 struct Range {
     int start, count;
@@ -346,6 +362,7 @@ static void member_calls(const ftc_ensemble &e, const EnsMember &m, const double
         out[3] = GemmCall{QxTf, m.w1, m.phi, f, f, d, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 0.0, ak};
         out[4] = GemmCall{Qxf, m.res, m.w0, f, d, f, nx, ny, nx, nz, nx, nx, ny, 0, plane, plane, 1.0, 0.0, at};
         out[5] = GemmCall{QxTf, m.w1, m.phi, f, f, d, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 1.0, ak};
+        out[5].shift = m.part + 256;  // the member's mean of phi from the preceding residual pass
         return;
     }
     const int d = kDouble;
@@ -481,7 +498,11 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
     case kRes:
         n = pick(e.r_tiles, e.n_tiles, &start);
         if (n) hipLaunchKernelGGL(ens_residual_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
-                                  e.d_members, e.d_tiles + start, n);
+                                  e.d_members, e.d_tiles + start, n, e.d_tile_part, start);
+        break;
+    case kFold:
+        hipLaunchKernelGGL(ens_mean_fold_kernel, dim3(all ? (e.count + 63) / 64 : 1), dim3(64), 0, s, e.d_members,
+                           e.d_tile_ranges, e.d_tile_part, all ? 0 : member, all ? e.count : 1);
         break;
     case kSub:
         n = pick(e.r_tiles, e.n_tiles, &start);
@@ -539,10 +560,16 @@ static std::vector<int> step_stages(const ftc_ensemble &e)
 {
     std::vector<int> st{kLes, kDiv, kG1, kG2, kTri, kG3, kG4};
     if (mixed(e)) {
+        // Each correction's residual pass also sums phi per tile; the fold gives each member's mean, which
+        // the correction's G4 subtracts (the Laplacian does not see a constant, and the split-bf16
+        // transforms leak one). After the last correction only its own, much smaller leak is left. A single
+        // application removes the mean explicitly.
         for (int c = 1; c < e.solves; ++c)
-            for (int x : {kRes, kG1c, kG2, kTri, kG3, kG4c}) st.push_back(x);
-        st.push_back(kMean);
-        st.push_back(kSub);
+            for (int x : {kRes, kFold, kG1c, kG2, kTri, kG3, kG4c}) st.push_back(x);
+        if (e.solves == 1) {
+            st.push_back(kMean);
+            st.push_back(kSub);
+        }
     }
     st.push_back(kProj);
     return st;

@@ -55,7 +55,9 @@ def build_hip(jobs: int = 8) -> Path:
     with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, jobs)) as pool:
         objects = list(pool.map(_compile, sources))
     tmp = HIP_LIBRARY.with_suffix(".so.tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects)])
+    # -Bsymbolic: calls between the library's own exported functions (device_begin -> access_begin_v1)
+    # bind inside it, also when another library exporting the same C ABI is loaded globally first.
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", str(tmp), *map(str, objects)])
     os.replace(tmp, HIP_LIBRARY)
     return HIP_LIBRARY
 
@@ -64,7 +66,7 @@ def build_cpu() -> Path:
     """Builds the CPU reference backend of the runtime."""
     cxx = shutil.which("g++") or "g++"
     tmp = CPU_LIBRARY.with_suffix(".so.tmp")
-    _run([cxx, *CXX_FLAGS, "-shared", "-o", str(tmp), str(CSRC / "runtime.cpp")])
+    _run([cxx, *CXX_FLAGS, "-shared", "-Wl,-Bsymbolic", "-o", str(tmp), str(CSRC / "runtime.cpp")])
     os.replace(tmp, CPU_LIBRARY)
     return CPU_LIBRARY
 

@@ -497,10 +497,16 @@ static Effects effects(const Buffer &b, const fort_scope_access *a)
 
 // Checks that every read is defined; returns the boxes to upload for a device access (reads, and the
 // defined parts of partial writes, that the device does not hold) or to download for a host access.
-static std::vector<Box> plan_transfer(const Buffer &b, const Effects &e, bool device)
+// The sections of b to copy to the device (device) or to the host before an access with effects e: the
+// defined parts of its read and write regions that are not current there (writes too, because a partial
+// write leaves the rest of its region as it was). A strict access must read only defined values; the
+// may-access of original host work (strict false) may cover holes, which are then neither copied nor
+// defined.
+static std::vector<Box> plan_transfer(const Buffer &b, const Effects &e, bool device, bool strict = true)
 {
-    for (const Box &r : e.reads)
-        require(b.init.covers(r), FORT_SCOPE_UNINITIALIZED, "task reads values that were never defined");
+    if (strict)
+        for (const Box &r : e.reads)
+            require(b.init.covers(r), FORT_SCOPE_UNINITIALIZED, "task reads values that were never defined");
     const Region &have = device ? b.dev_cur : b.host_cur;
     std::vector<Box> need;
     Region acc;
@@ -1255,6 +1261,75 @@ int fort_scope_access_begin_v1(fort_scope_t h, const fort_scope_plan_binding *bi
             list[i]->prep_writes = eff[i].writes;
             list[i]->prep_overwrites = eff[i].overwrites;
         }
+    });
+}
+
+int fort_scope_original_host_begin_v1(fort_scope_t h, const fort_scope_plan_binding *bindings, std::size_t count)
+{
+    return with(h, [&](Context &c) {
+        require(count == 0 || bindings != nullptr, FORT_SCOPE_ARGUMENT, "missing bindings");
+        std::vector<Buffer *> list;
+        std::vector<Effects> eff;
+        // Every binding is checked before anything moves, so a later bad binding publishes nothing.
+        for (std::size_t i = 0; i < count; ++i) {
+            Buffer &b = buffer(c, bindings[i].buffer);
+            for (Buffer *o : list) require(o != &b, FORT_SCOPE_ALIAS, "task access bindings repeat a buffer");
+            require(!b.prepared, FORT_SCOPE_STATE, "buffer already has a prepared access");
+            list.push_back(&b);
+            eff.push_back(effects(b, &bindings[i].access));
+        }
+        for (Buffer *b : list) drain_buffer(c, *b);
+        std::vector<std::vector<Box>> downloads;
+        for (std::size_t i = 0; i < count; ++i) downloads.push_back(plan_transfer(*list[i], eff[i], false, false));
+        for (std::size_t i = 0; i < count; ++i) {
+            if (!downloads[i].empty()) ensure_stream(c);
+            download(c, *list[i], downloads[i]);
+            list[i]->prepared = 2;
+            list[i]->prep_writes = eff[i].writes;
+            list[i]->prep_overwrites = eff[i].overwrites;
+        }
+    });
+}
+
+// True when layout l describes buffer b's physical allocation exactly.
+static bool describes(const fort_scope_layout &l, const Buffer &b)
+{
+    if (l.rank != b.rank || l.type != b.type || l.element_bytes != b.elem || l.host != b.host ||
+        l.generation != b.generation || !l.extents)
+        return false;
+    for (uint32_t k = 0; k < b.rank; ++k) {
+        const int64_t lower = l.lower_bounds ? l.lower_bounds[k] : 1;
+        if (l.extents[k] != b.extents[k] || lower != b.lower_bounds[k]) return false;
+    }
+    return true;
+}
+
+int fort_scope_identity_swap_v1(fort_scope_t h, fort_buffer_t first, fort_buffer_t second,
+                                const fort_scope_layout *first_after, const fort_scope_layout *second_after,
+                                fort_buffer_t *first_current, fort_buffer_t *second_current)
+{
+    return with(h, [&](Context &c) {
+        Domain &d = *c.domain;
+        // The exchange already happened in the caller, so a rejection cannot be answered by a replay: it
+        // fails the context and the domain.
+        auto reject = [&](const char *why) {
+            d.failed = true;
+            fail(FORT_SCOPE_EXECUTION, std::string("identity swap: ") + why);
+        };
+        if (!first_after || !second_after || !first_current || !second_current) reject("missing argument");
+        auto fa = d.buffers.find(first), fb = d.buffers.find(second);
+        if (fa == d.buffers.end() || fb == d.buffers.end()) reject("unknown buffer");
+        if (first == second) reject("the two buffers are one");
+        Buffer &a = *fa->second, &b = *fb->second;
+        if (a.identity == b.identity) reject("the two identities are one");
+        if (a.prepared || b.prepared) reject("a buffer has a prepared access");
+        if (!describes(*first_after, b) || !describes(*second_after, a))
+            reject("a post-swap layout does not describe the other allocation");
+        d.by_identity[a.identity] = b.handle;
+        d.by_identity[b.identity] = a.handle;
+        std::swap(a.identity, b.identity);
+        *first_current = b.handle;
+        *second_current = a.handle;
     });
 }
 

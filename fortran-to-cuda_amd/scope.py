@@ -120,6 +120,9 @@ RUNTIME_SIGNATURES = {
     "fort_scope_forget_definition": (c_int, [c_uint64, c_uint64]),
     "fort_scope_layout_get": (c_int, [c_uint64, c_uint64, POINTER(Layout)]),
     "fort_scope_access_begin_v1": (c_int, [c_uint64, POINTER(PlanBinding), c_size_t, c_int, POINTER(c_void_p)]),
+    "fort_scope_original_host_begin_v1": (c_int, [c_uint64, POINTER(PlanBinding), c_size_t]),
+    "fort_scope_identity_swap_v1": (c_int, [c_uint64, c_uint64, c_uint64, POINTER(Layout), POINTER(Layout), U64P,
+                                            U64P]),
     "fort_scope_external_execute_v1": (c_int, [c_uint64, POINTER(ExternalCall), EXTERNAL_WORKER, c_void_p,
                                                POINTER(ExternalReport)]),
     "fort_scope_observe_v1": (c_int, [c_uint64, U64P, c_size_t]),

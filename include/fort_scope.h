@@ -161,6 +161,20 @@ int fort_scope_forget_definition(fort_scope_t context, fort_buffer_t buffer);   
 int fort_scope_layout_get(fort_scope_t context, fort_buffer_t buffer, fort_scope_layout *layout); /* :218 */
 int fort_scope_access_begin_v1(fort_scope_t context, const fort_scope_plan_binding *bindings, size_t count,
                                int device, void **devices);                          /* :232-233 */
+/* Original CPU work (the unchanged source body) with may-access envelopes: the tracked values of the read
+ * and write regions reach the host, holes that were never defined are neither copied nor defined, device
+ * replicas of reads stay; host_end invalidates the device copies of possible writes and only overwrites
+ * add definitions. All bindings are checked before any transfer. (scoped_runtime.h:234-246) */
+int fort_scope_original_host_begin_v1(fort_scope_t context, const fort_scope_plan_binding *bindings,
+                                      size_t count);
+/* Commit a descriptor-only exchange of two registered identities that the caller has already executed
+ * (a MOVE_ALLOC swap): first_after must describe the second physical allocation exactly (host address,
+ * extents, lower bounds, generation) and second_after the first. Only the identity map changes; no data
+ * moves. Any failure poisons the context and its domain and returns EXECUTION, with the outputs
+ * untouched. (scoped_runtime.h:198-211) */
+int fort_scope_identity_swap_v1(fort_scope_t context, fort_buffer_t first, fort_buffer_t second,
+                                const fort_scope_layout *first_after, const fort_scope_layout *second_after,
+                                fort_buffer_t *first_current, fort_buffer_t *second_current);
 int fort_scope_external_execute_v1(fort_scope_t context, const fort_scope_external_call_v1 *call,
                                    fort_scope_external_worker_v1 worker, void *user,
                                    fort_scope_external_report_v1 *report);          /* :294-296 */

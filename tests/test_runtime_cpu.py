@@ -365,3 +365,202 @@ def test_reservation_without_a_stated_budget(scope, cpu_runtime, monkeypatch):
         hs = (c_uint64 * 1)(h)
         scope.check(lib, lib.fort_scope_device_reserve_many_v1(dom.ctx, hs, 1, devs))
         assert devs[0]
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Original host work with may-access envelopes (compiler/tests/test_original_host_access.py) and the
+# descriptor-only identity exchange (compiler/tests/test_scoped_domains.py:318-432).
+
+def original_begin(scope, lib, dom, bindings):
+    arr = (scope.PlanBinding * len(bindings))(*[b for b, _ in bindings])
+    return lib.fort_scope_original_host_begin_v1(dom.ctx, arr, len(bindings))
+
+
+def strict_device_read(scope, lib, dom, handle, reads=(), flags=0):
+    access, _keep = scope.make_access(flags, reads, (), ())
+    ptr = c_void_p()
+    return lib.fort_scope_device_begin(dom.ctx, handle, byref(access), byref(ptr))
+
+
+def device_write(scope, lib, dom, handle, box, values):
+    st, devs = begin_device(scope, lib, dom, [binding(scope, handle, writes=[box], overwrites=[box])])
+    scope.check(lib, st)
+    view = dev_view(devs[0], (8,))
+    view[box[0][0]:box[1][0]] = values
+    scope.check(lib, lib.fort_scope_device_end(dom.ctx, handle))
+
+
+def test_original_host_reads_of_unknown_values_define_nothing(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a, initialized=False)
+        scope.check(lib, original_begin(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL)]))
+        assert a[3] == 3.0
+        scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))
+        # A numerical read still needs definitions the original read did not give.
+        assert strict_device_read(scope, lib, dom, h, flags=scope.READ_ALL) == scope.UNINITIALIZED
+        s = dom.stats()
+        assert s.uploads == s.downloads == 0
+
+
+def test_original_host_reads_publish_only_tracked_cells_and_keep_the_device_copy(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.full(8, -99.0)
+    cells = ((2,), (5,))
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a, initialized=False)
+        device_write(scope, lib, dom, h, cells, [14.0, 21.0, 28.0])
+        scope.check(lib, original_begin(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL)]))
+        assert list(a) == [-99, -99, 14, 21, 28, -99, -99, -99]
+        scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))
+        assert strict_device_read(scope, lib, dom, h, flags=scope.READ_ALL) == scope.UNINITIALIZED
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, reads=[cells])])
+        scope.check(lib, st)
+        assert list(dev_view(devs[0], (8,))[2:5]) == [14, 21, 28]
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        s = dom.stats()
+        assert s.upload_bytes == 0 and s.download_bytes == 24  # the read kept the device replica
+
+
+def test_original_possible_writes_keep_holes_and_invalidate_device_values(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.full(8, -99.0)
+    known = ((2,), (5,))
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a, initialized=False)
+        device_write(scope, lib, dom, h, known, [2.0, 3.0, 4.0])
+        scope.check(lib, original_begin(scope, lib, dom, [binding(scope, h, flags=scope.WRITE_ALL)]))
+        a[3] += 10.0
+        a[0] = 17.0
+        scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))
+        assert list(a) == [17, -99, 2, 13, 4, -99, -99, -99]
+        # A possible write defines nothing new ...
+        assert strict_device_read(scope, lib, dom, h, reads=[((0,), (1,))]) == scope.UNINITIALIZED
+        # ... but it replaces the device copy of what it may have changed.
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, reads=[known])])
+        scope.check(lib, st)
+        assert list(dev_view(devs[0], (8,))[2:5]) == [2, 13, 4]
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        s = dom.stats()
+        assert s.upload_bytes == s.download_bytes == 24
+
+
+def test_only_proved_original_overwrites_define_values(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.full(8, -99.0)
+    cells = ((1,), (3,))
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a, initialized=False)
+        b = binding(scope, h, flags=scope.WRITE_ALL, overwrites=[cells])
+        scope.check(lib, original_begin(scope, lib, dom, [b]))
+        a[1], a[2] = 7.0, 9.0
+        scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))
+        assert strict_device_read(scope, lib, dom, h, flags=scope.READ_ALL) == scope.UNINITIALIZED
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, reads=[cells])])
+        scope.check(lib, st)
+        assert list(dev_view(devs[0], (8,))[1:3]) == [7, 9]
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        assert dom.stats().upload_bytes == 16 and dom.stats().download_bytes == 0
+
+
+def test_a_bad_later_original_binding_moves_nothing(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.full(8, -99.0)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a, initialized=False)
+        device_write(scope, lib, dom, h, ((2,), (4,)), [7.0, 9.0])
+        bad = [binding(scope, h, flags=scope.READ_ALL), binding(scope, 999999, flags=scope.READ_ALL)]
+        assert original_begin(scope, lib, dom, bad) != scope.OK
+        assert list(a) == [-99.0] * 8 and dom.stats().download_bytes == 0
+        scope.check(lib, original_begin(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL)]))
+        scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))
+        assert list(a) == [-99, -99, 7, 9, -99, -99, -99, -99]
+
+
+def register(scope, lib, dom, array, identity, generation, lower, initialized=True):
+    ext = (c_size_t * array.ndim)(*array.shape)
+    low = (ctypes.c_int64 * array.ndim)(*lower)
+    layout = scope.Layout(array.ndim, scope.REAL64, 8, array.ctypes.data, ext, low, generation)
+    h = c_uint64()
+    status = lib.fort_scope_register(dom.ctx, identity, generation, byref(layout), 1 if initialized else 0, byref(h))
+    return status, h.value, (layout, ext, low)
+
+
+def layout_of(scope, array, generation, lower):
+    ext = (c_size_t * array.ndim)(*array.shape)
+    low = (ctypes.c_int64 * array.ndim)(*lower)
+    return scope.Layout(array.ndim, scope.REAL64, 8, array.ctypes.data, ext, low, generation), (ext, low)
+
+
+def test_identity_swap_exchanges_only_the_identity_map(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.arange(1.0, 5.0).reshape((2, 2), order="F")
+    b = np.arange(11.0, 17.0).reshape((3, 2), order="F")
+    with new_domain(scope, lib) as dom:
+        st, ha, _ = register(scope, lib, dom, a, 1, 11, (-2, 3))
+        scope.check(lib, st)
+        st, hb, _ = register(scope, lib, dom, b, 2, 27, (-7, 9))
+        scope.check(lib, st)
+        for h, shape, add in ((ha, a.shape, 100.0), (hb, b.shape, 200.0)):
+            s, devs = begin_device(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL | scope.WRITE_ALL)])
+            scope.check(lib, s)
+            dev_view(devs[0], shape)[...] += add
+            scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        before = bytes(dom.stats())
+        after_a, keep_a = layout_of(scope, b, 27, (-7, 9))
+        after_b, keep_b = layout_of(scope, a, 11, (-2, 3))
+        cur_a, cur_b = c_uint64(), c_uint64()
+        scope.check(lib, lib.fort_scope_identity_swap_v1(dom.ctx, ha, hb, byref(after_a), byref(after_b),
+                                                         byref(cur_a), byref(cur_b)))
+        assert (cur_a.value, cur_b.value) == (hb, ha)
+        assert bytes(dom.stats()) == before  # nothing copied, allocated or waited for
+        # Identity 1 now names b's allocation, identity 2 a's, with their device values.
+        st, again1, _ = register(scope, lib, dom, b, 1, 27, (-7, 9))
+        scope.check(lib, st)
+        st, again2, _ = register(scope, lib, dom, a, 2, 11, (-2, 3))
+        scope.check(lib, st)
+        assert (again1, again2) == (hb, ha)
+        assert list(a.ravel(order="F")) == [1, 2, 3, 4]  # device-current values not yet published
+    assert list(a.ravel(order="F")) == [101, 102, 103, 104]
+    assert list(b.ravel(order="F")) == [211, 212, 213, 214, 215, 216]
+
+
+@pytest.mark.parametrize("invalid", ["host", "generation", "extent", "lower", "same_handle", "prepared"])
+def test_a_bad_identity_swap_fails_the_context_without_publication(scope, cpu_runtime, invalid):
+    lib = cpu_runtime
+    a, b = np.array([1.0, 2.0]), np.array([3.0, 4.0])
+    dom = new_domain(scope, lib)
+    st, ha, _ = register(scope, lib, dom, a, 1, 2, (0,))
+    scope.check(lib, st)
+    st, hb, _ = register(scope, lib, dom, b, 2, 7, (0,))
+    scope.check(lib, st)
+    s, devs = begin_device(scope, lib, dom, [binding(scope, ha, flags=scope.READ_ALL | scope.WRITE_ALL)])
+    scope.check(lib, s)
+    dev_view(devs[0], (2,))[0] = 1000.0
+    scope.check(lib, lib.fort_scope_device_end(dom.ctx, ha))
+    after_a, keep_a = layout_of(scope, b, 7, (0,))
+    after_b, keep_b = layout_of(scope, a, 2, (0,))
+    if invalid == "host":
+        after_a.host = a.ctypes.data
+    elif invalid == "generation":
+        after_a.generation = 2
+    elif invalid == "extent":
+        after_a, keep_a = layout_of(scope, b[:1], 7, (0,))
+        after_a.host = b.ctypes.data
+    elif invalid == "lower":
+        after_a, keep_a = layout_of(scope, b, 7, (-1,))
+    elif invalid == "same_handle":
+        hb = ha
+    elif invalid == "prepared":
+        scope.check(lib, begin_device(scope, lib, dom, [binding(scope, hb, flags=scope.READ_ALL)])[0])
+    cur_a, cur_b = c_uint64(111), c_uint64(222)
+    assert lib.fort_scope_identity_swap_v1(dom.ctx, ha, hb, byref(after_a), byref(after_b), byref(cur_a),
+                                           byref(cur_b)) == scope.EXECUTION
+    assert (cur_a.value, cur_b.value) == (111, 222)
+    assert lib.fort_scope_wait(dom.ctx) == scope.EXECUTION
+    assert lib.fort_scope_close(dom.ctx) == scope.EXECUTION
+    scope.check(lib, lib.fort_scope_abandon(dom.ctx))
+    scope.check(lib, lib.fort_scope_domain_abandon_v1(dom.domain))
+    dom.closed = True
+    assert list(a) == [1.0, 2.0] and list(b) == [3.0, 4.0]

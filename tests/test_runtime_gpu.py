@@ -1,5 +1,6 @@
-"""The runtime's graph queue on the device: a merged member launch (scoped_runtime.h:401-413) runs one
-kernel whose members' effects are committed as their union, and a refused one runs nothing.
+"""The runtime on the device: a merged member launch (scoped_runtime.h:401-413) runs one kernel whose
+members' effects are committed as their union, and a refused one runs nothing; original host work and the
+identity exchange (scoped_runtime.h:198-211, 234-246) with device-resident values.
 
 The task body is the library's iota kernel (x[i] = scale * i), so the expected contents are exact.
 """
@@ -79,3 +80,50 @@ def test_refused_member_launch_runs_nothing(scope, hip):
         scope.check(hip, hip.fort_scope_graph_flush_v1(dom.ctx))
         dom.observe(h)
     assert np.array_equal(a, np.full(n, 3.0))
+
+
+def test_original_host_read_publishes_only_device_defined_cells(scope, hip):
+    # Original CPU work over a partly defined array (compiler/tests/test_original_host_access.py): the
+    # device-defined cells reach the host, the holes stay as they were and stay undefined.
+    n = 8
+    a = np.full(n, -99.0)
+    with scope.Domain(hip) as dom:
+        h = dom.bind(a, initialized=False)
+        dev = c_void_p()
+        scope.check(hip, hip.fort_scope_device_reserve_v1(dom.ctx, h, ctypes.byref(dev)))
+        k, sizes, keep = iota_launch(hip, dev.value + 2 * 8, 4, 7.0)  # cells 2..5 = 0, 7, 14, 21
+        ms, keep2 = members_of(scope, [[(h, {"overwrites": [((2,), (6,))]})]])
+        scope.check(hip, hip.fort_scope_graph_enqueue_members_v1(dom.ctx, 13, ctypes.byref(k), sizes, ms, 1))
+        scope.check(hip, hip.fort_scope_graph_flush_v1(dom.ctx))
+        acc, kp = scope.make_access(scope.READ_ALL)
+        b = (scope.PlanBinding * 1)(scope.PlanBinding(h, acc))
+        scope.check(hip, hip.fort_scope_original_host_begin_v1(dom.ctx, b, 1))
+        assert list(a) == [-99, -99, 0, 7, 14, 21, -99, -99]
+        scope.check(hip, hip.fort_scope_host_end(dom.ctx, h))
+        ptr = c_void_p()
+        assert hip.fort_scope_device_begin(dom.ctx, h, ctypes.byref(acc), ctypes.byref(ptr)) == scope.UNINITIALIZED
+
+
+def test_identity_swap_keeps_the_device_copies_with_their_allocations(scope, hip):
+    n = 1024
+    a, b = np.zeros(n), np.zeros(n)
+    with scope.Domain(hip) as dom:
+        ha, hb = dom.bind(a, identity=1), dom.bind(b, identity=2)
+        for h, scale in ((ha, 1.0), (hb, 2.0)):
+            dev = c_void_p()
+            scope.check(hip, hip.fort_scope_device_reserve_v1(dom.ctx, h, ctypes.byref(dev)))
+            k, sizes, keep = iota_launch(hip, dev.value, n, scale)
+            ms, keep2 = members_of(scope, [[(h, {"overwrites": [((0,), (n,))]})]])
+            scope.check(hip, hip.fort_scope_graph_enqueue_members_v1(dom.ctx, 14, ctypes.byref(k), sizes, ms, 1))
+        scope.check(hip, hip.fort_scope_graph_flush_v1(dom.ctx))
+        layouts = []
+        for arr in (b, a):  # after the exchange identity 1 names b's allocation and identity 2 a's
+            ext, low = (c_size_t * 1)(n), (ctypes.c_int64 * 1)(1)
+            layouts.append((scope.Layout(1, scope.REAL64, 8, arr.ctypes.data, ext, low, 0), ext, low))
+        cur1, cur2 = ctypes.c_uint64(), ctypes.c_uint64()
+        scope.check(hip, hip.fort_scope_identity_swap_v1(dom.ctx, ha, hb, ctypes.byref(layouts[0][0]),
+                                                         ctypes.byref(layouts[1][0]), ctypes.byref(cur1),
+                                                         ctypes.byref(cur2)))
+        assert (cur1.value, cur2.value) == (hb, ha)
+        dom.observe(cur1.value, cur2.value)
+    assert np.array_equal(a, np.arange(n) * 1.0) and np.array_equal(b, np.arange(n) * 2.0)

@@ -228,6 +228,9 @@ struct Context {
     std::size_t scratch_capacity = 0;
     uint64_t scratch_token = 0, next_token = 1;
     bool scratch_active = false;
+    fort_scope_scratch_stats_v1 sstats{};
+    fort_scope_completion_stats_v1 cstats{};
+    fort_scope_trace_state_v1 trace{};
 };
 
 static std::mutex g_registry;
@@ -348,6 +351,7 @@ static void wait(Context &c)
     if (!c.stream_ready) return;
     dev::sync(c.stream);
     c.domain->stats.waits++;
+    c.cstats.stream_waits++;
 }
 
 // ------------------------------------------------------------------------------------------------ queue
@@ -381,6 +385,7 @@ static void evict_one(Context &c, Buffer &victim)
     victim.device = nullptr;
     victim.dev_cur.clear();
     c.domain->device_bytes -= victim.bytes;
+    c.domain->stats.allocated_bytes -= victim.bytes;
     c.domain->residency.evictions++;
     c.domain->residency.evicted_bytes += victim.bytes;
 }
@@ -865,7 +870,10 @@ static int global(F &&body)
 static void close_domain_storage(Domain &d)
 {
     for (auto &kv : d.buffers)
-        if (kv.second->device) dev::release(kv.second->device);
+        if (kv.second->device) {
+            dev::release(kv.second->device);
+            d.stats.allocated_bytes -= kv.second->bytes;
+        }
     d.buffers.clear();
     d.by_identity.clear();
     d.device_bytes = 0;
@@ -1114,6 +1122,7 @@ int fort_scope_scratch_acquire_v1(fort_scope_t h, std::size_t bytes, fort_scope_
             Domain &d = *c.domain;
             require(d.device_bytes + d.workspace_bytes + bytes <= budget_of(d), FORT_SCOPE_RESOURCE,
                     "scratch exceeds the device budget");
+            const bool grow = c.scratch != nullptr;
             if (c.scratch) dev::release(c.scratch);
             c.scratch = nullptr;
             c.scratch_capacity = 0;
@@ -1121,6 +1130,18 @@ int fort_scope_scratch_acquire_v1(fort_scope_t h, std::size_t bytes, fort_scope_
             c.scratch = dev::alloc(bytes);
             require(c.scratch != nullptr, FORT_SCOPE_RESOURCE, "scratch allocation failed");
             c.scratch_capacity = bytes;
+            c.sstats.allocations++;
+            if (grow) c.sstats.grows++;
+        } else if (bytes) {
+            c.sstats.reuses++;
+        }
+        c.sstats.acquisitions++;
+        c.sstats.active_bytes = bytes;
+        if (bytes > c.sstats.peak_scratch_bytes) c.sstats.peak_scratch_bytes = bytes;
+        {
+            const Domain &d = *c.domain;
+            const uint64_t total = d.device_bytes + d.workspace_bytes + c.scratch_capacity;
+            if (total > c.sstats.peak_total_device_bytes) c.sstats.peak_total_device_bytes = total;
         }
         c.scratch_active = true;
         c.scratch_token = c.next_token++;
@@ -1138,7 +1159,63 @@ int fort_scope_scratch_release_v1(fort_scope_t h, uint64_t token)
     return with(h, [&](Context &c) {
         require(c.scratch_active && token == c.scratch_token, FORT_SCOPE_STATE, "no such active scratch lease");
         c.scratch_active = false;
+        c.sstats.releases++;
+        c.sstats.active_bytes = 0;
     });
+}
+
+int fort_scope_scratch_stats_get_v1(fort_scope_t h, fort_scope_scratch_stats_v1 *stats)
+{
+    return with(h, [&](Context &c) {
+        require(stats != nullptr, FORT_SCOPE_ARGUMENT, "missing stats output");
+        *stats = c.sstats;
+        stats->version = 1;
+        stats->active = c.scratch_active ? 1 : 0;
+        stats->capacity_bytes = c.scratch_capacity;
+    }, Mode::Query);
+}
+
+int fort_scope_completion_stats_get_v1(fort_scope_t h, fort_scope_completion_stats_v1 *stats)
+{
+    return with(h, [&](Context &c) {
+        require(stats != nullptr, FORT_SCOPE_ARGUMENT, "missing stats output");
+        *stats = c.cstats;
+        stats->version = 1;
+        stats->reserved = 0;
+    }, Mode::Query);
+}
+
+static bool tracing() { return std::getenv("FORT_RUNTIME_TRACE") != nullptr; }
+
+static void trace_field(char (&dst)[65], const char *src)
+{
+    std::memset(dst, 0, sizeof dst);
+    if (!src) return;
+    const std::size_t n = strnlen(src, sizeof dst);
+    if (n < sizeof dst) std::memcpy(dst, src, n);  // an ID without its NUL within 65 bytes stays unknown
+}
+
+void fort_scope_trace_set_v1(fort_scope_t h, uint32_t mask, const char *owner, const char *procedure,
+                             const char *segment, const char *operation, const char *implementation,
+                             const char *boundary, fort_scope_trace_state_v1 *previous)
+{
+    if (!tracing()) return;
+    (void)with(h, [&](Context &c) {
+        if (previous) *previous = c.trace;
+        c.trace.version = 1;
+        if (mask & FORT_SCOPE_TRACE_OWNER) trace_field(c.trace.owner, owner);
+        if (mask & FORT_SCOPE_TRACE_PROCEDURE) trace_field(c.trace.procedure, procedure);
+        if (mask & FORT_SCOPE_TRACE_SEGMENT) trace_field(c.trace.segment, segment);
+        if (mask & FORT_SCOPE_TRACE_OPERATION) trace_field(c.trace.operation, operation);
+        if (mask & FORT_SCOPE_TRACE_IMPLEMENTATION) trace_field(c.trace.implementation, implementation);
+        if (mask & FORT_SCOPE_TRACE_BOUNDARY) trace_field(c.trace.boundary, boundary);
+    }, Mode::Query);
+}
+
+void fort_scope_trace_restore_v1(fort_scope_t h, const fort_scope_trace_state_v1 *previous)
+{
+    if (!tracing() || !previous) return;
+    (void)with(h, [&](Context &c) { c.trace = *previous; }, Mode::Query);
 }
 
 static int register_impl(fort_scope_t h, uint64_t identity, uint64_t generation, const fort_scope_layout *layout,
@@ -1253,7 +1330,12 @@ int fort_scope_access_begin_v1(fort_scope_t h, const fort_scope_plan_binding *bi
             eff.push_back(effects(b, &bindings[i].access));
         }
         std::vector<std::vector<Box>> downloads;
-        for (std::size_t i = 0; i < count; ++i) downloads.push_back(plan_transfer(*list[i], eff[i], false));
+        bool moves = false;
+        for (std::size_t i = 0; i < count; ++i) {
+            downloads.push_back(plan_transfer(*list[i], eff[i], false));
+            moves = moves || !downloads.back().empty();
+        }
+        if (!moves) c.cstats.independent_host_tasks++;
         for (std::size_t i = 0; i < count; ++i) {
             if (!downloads[i].empty()) ensure_stream(c);
             download(c, *list[i], downloads[i]);
@@ -1280,7 +1362,12 @@ int fort_scope_original_host_begin_v1(fort_scope_t h, const fort_scope_plan_bind
         }
         for (Buffer *b : list) drain_buffer(c, *b);
         std::vector<std::vector<Box>> downloads;
-        for (std::size_t i = 0; i < count; ++i) downloads.push_back(plan_transfer(*list[i], eff[i], false, false));
+        bool moves = false;
+        for (std::size_t i = 0; i < count; ++i) {
+            downloads.push_back(plan_transfer(*list[i], eff[i], false, false));
+            moves = moves || !downloads.back().empty();
+        }
+        if (!moves) c.cstats.independent_host_tasks++;
         for (std::size_t i = 0; i < count; ++i) {
             if (!downloads[i].empty()) ensure_stream(c);
             download(c, *list[i], downloads[i]);
@@ -1449,9 +1536,66 @@ int fort_scope_unregister(fort_scope_t h, fort_buffer_t handle)
         if (b.device) {
             dev::release(b.device);
             c.domain->device_bytes -= b.bytes;
+            c.domain->stats.allocated_bytes -= b.bytes;
         }
         c.domain->by_identity.erase(b.identity);
         c.domain->buffers.erase(handle);
+    });
+}
+
+int fort_scope_recycle_v1(fort_scope_t h, fort_buffer_t handle)
+{
+    return with(h, [&](Context &c) {
+        Buffer &b = buffer(c, handle);
+        require(!b.prepared, FORT_SCOPE_STATE, "buffer has a prepared access");
+        // With queued users the dead values stay until the owner's next definition replaces them, and the
+        // queue is not submitted for it; without, the values are forgotten as by discard.
+        if (b.queued > 0) return;
+        b.init.clear();
+        b.host_cur.clear();
+        b.dev_cur.clear();
+    });
+}
+
+int fort_scope_retire_v1(fort_scope_t h, fort_buffer_t handle)
+{
+    return with(h, [&](Context &c) {
+        Buffer &b = buffer(c, handle);
+        require(!b.prepared, FORT_SCOPE_STATE, "buffer has a prepared access");
+        // Dead values: no publication. Queued and running uses of the storage (device memory and the
+        // borrowed host array) complete first; then the handle is gone.
+        drain_buffer(c, b);
+        wait(c);
+        if (b.device) {
+            dev::release(b.device);
+            c.domain->device_bytes -= b.bytes;
+            c.domain->stats.allocated_bytes -= b.bytes;
+        }
+        c.domain->by_identity.erase(b.identity);
+        c.domain->buffers.erase(handle);
+    });
+}
+
+int fort_scope_admit_source_reads_v1(fort_scope_t h, fort_buffer_t handle, const fort_scope_section *sections,
+                                     std::size_t count)
+{
+    return with(h, [&](Context &c) {
+        Buffer &b = buffer(c, handle);
+        require(count == 0 || sections != nullptr, FORT_SCOPE_ARGUMENT, "missing sections");
+        require(!b.prepared, FORT_SCOPE_STATE, "buffer has a prepared access");
+        // Only values outside the defined coverage gain a definition and a current host copy; whatever is
+        // defined keeps its freshness (a dirty device copy stays the current one). Both regions are built
+        // before either is committed.
+        Region init = b.init, host = b.host_cur;
+        for (std::size_t i = 0; i < count; ++i) {
+            const Box box = section_box(b, sections[i]);
+            for (const Box &fresh : b.init.missing(box)) {
+                require(init.add(fresh) && host.add(fresh), FORT_SCOPE_BOUNDARY,
+                        "admitted reads exceed the region limit");
+            }
+        }
+        b.init = init;
+        b.host_cur = host;
     });
 }
 

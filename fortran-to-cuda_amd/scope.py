@@ -71,6 +71,23 @@ class ScratchLease(ctypes.Structure):
                 ("bytes", c_size_t), ("capacity", c_size_t)]
 
 
+class ScratchStats(ctypes.Structure):
+    _fields_ = [("version", c_uint32), ("active", c_uint32)] + [
+        (n, c_uint64) for n in ("acquisitions", "allocations", "reuses", "releases", "grows", "capacity_bytes",
+                                "active_bytes", "peak_scratch_bytes", "peak_total_device_bytes")]
+
+
+class CompletionStats(ctypes.Structure):
+    _fields_ = [("version", c_uint32), ("reserved", c_uint32)] + [
+        (n, c_uint64) for n in ("event_records", "event_queries", "event_waits", "stream_waits", "created_events",
+                                "reused_events", "independent_host_tasks")]
+
+
+class TraceState(ctypes.Structure):
+    _fields_ = [("version", c_uint32), ("reserved", c_uint32)] + [
+        (n, ctypes.c_char * 65) for n in ("owner", "procedure", "segment", "operation", "implementation", "boundary")]
+
+
 class ExternalBinding(ctypes.Structure):
     _fields_ = [("buffer", c_uint64), ("type", c_uint32), ("rank", c_uint32), ("element_bytes", c_size_t),
                 ("access", Access)]
@@ -121,6 +138,13 @@ RUNTIME_SIGNATURES = {
     "fort_scope_layout_get": (c_int, [c_uint64, c_uint64, POINTER(Layout)]),
     "fort_scope_access_begin_v1": (c_int, [c_uint64, POINTER(PlanBinding), c_size_t, c_int, POINTER(c_void_p)]),
     "fort_scope_original_host_begin_v1": (c_int, [c_uint64, POINTER(PlanBinding), c_size_t]),
+    "fort_scope_recycle_v1": (c_int, [c_uint64, c_uint64]),
+    "fort_scope_retire_v1": (c_int, [c_uint64, c_uint64]),
+    "fort_scope_admit_source_reads_v1": (c_int, [c_uint64, c_uint64, POINTER(Section), c_size_t]),
+    "fort_scope_scratch_stats_get_v1": (c_int, [c_uint64, POINTER(ScratchStats)]),
+    "fort_scope_completion_stats_get_v1": (c_int, [c_uint64, POINTER(CompletionStats)]),
+    "fort_scope_trace_set_v1": (None, [c_uint64, c_uint32] + [ctypes.c_char_p] * 6 + [POINTER(TraceState)]),
+    "fort_scope_trace_restore_v1": (None, [c_uint64, POINTER(TraceState)]),
     "fort_scope_identity_swap_v1": (c_int, [c_uint64, c_uint64, c_uint64, POINTER(Layout), POINTER(Layout), U64P,
                                             U64P]),
     "fort_scope_external_execute_v1": (c_int, [c_uint64, POINTER(ExternalCall), EXTERNAL_WORKER, c_void_p,

@@ -71,6 +71,27 @@ typedef struct fort_scope_scratch_lease_v1 {   /* scoped_runtime.h:105-110 */
     void *device;
     size_t bytes, capacity;
 } fort_scope_scratch_lease_v1;
+/* Scratch and completion counters (scoped_runtime.h:58-63, 111-117). */
+typedef struct fort_scope_scratch_stats_v1 {
+    uint32_t version, active;
+    uint64_t acquisitions, allocations, reuses, releases, grows;
+    uint64_t capacity_bytes, active_bytes, peak_scratch_bytes, peak_total_device_bytes;
+} fort_scope_scratch_stats_v1;
+typedef struct fort_scope_completion_stats_v1 {
+    uint32_t version, reserved;
+    uint64_t event_records, event_queries, event_waits, stream_waits;
+    uint64_t created_events, reused_events, independent_host_tasks;
+} fort_scope_completion_stats_v1;
+/* Submission provenance for diagnostics (scoped_runtime.h:20-41). */
+enum fort_scope_trace_field {
+    FORT_SCOPE_TRACE_OWNER = 1, FORT_SCOPE_TRACE_PROCEDURE = 2, FORT_SCOPE_TRACE_SEGMENT = 4,
+    FORT_SCOPE_TRACE_OPERATION = 8, FORT_SCOPE_TRACE_IMPLEMENTATION = 16, FORT_SCOPE_TRACE_BOUNDARY = 32,
+    FORT_SCOPE_TRACE_ALL = 63
+};
+typedef struct fort_scope_trace_state_v1 {
+    uint32_t version, reserved;
+    char owner[65], procedure[65], segment[65], operation[65], implementation[65], boundary[65];
+} fort_scope_trace_state_v1;
 typedef struct fort_scope_plan_binding {       /* scoped_runtime.h:120-123 */
     fort_buffer_t buffer;
     fort_scope_access access;
@@ -152,6 +173,19 @@ int fort_scope_device_budget_default_v1(int device, size_t *bytes);             
 int fort_scope_domain_budget_get_v1(fort_scope_domain_t domain, size_t *budget, size_t *unclaimed); /* :178 */
 int fort_scope_scratch_acquire_v1(fort_scope_t context, size_t bytes, fort_scope_scratch_lease_v1 *lease); /* :186-187 */
 int fort_scope_scratch_release_v1(fort_scope_t context, uint64_t token);             /* :188 */
+int fort_scope_scratch_stats_get_v1(fort_scope_t context, fort_scope_scratch_stats_v1 *stats); /* :189 */
+/* This runtime orders all work of a context on its one stream and records no events: the event counters
+ * stay zero, stream_waits counts stream synchronisations, independent_host_tasks counts host accesses
+ * that needed no transfer. (:190) */
+int fort_scope_completion_stats_get_v1(fort_scope_t context, fort_scope_completion_stats_v1 *stats);
+/* Attribution of later submissions, kept per context while FORT_RUNTIME_TRACE is set (otherwise both
+ * calls return at once): masked fields are replaced (null or empty clears them), *previous receives the
+ * state before the call; restore reinstates a saved state. Never changes status, coherence or order.
+ * (:38-41) */
+void fort_scope_trace_set_v1(fort_scope_t context, uint32_t update_mask, const char *owner, const char *procedure,
+                             const char *segment, const char *operation, const char *implementation,
+                             const char *boundary, fort_scope_trace_state_v1 *previous);
+void fort_scope_trace_restore_v1(fort_scope_t context, const fort_scope_trace_state_v1 *previous);
 int fort_scope_register(fort_scope_t context, uint64_t identity, uint64_t generation,
                         const fort_scope_layout *layout, int host_initialized, fort_buffer_t *buffer); /* :191-192 */
 int fort_scope_register_sections(fort_scope_t context, uint64_t identity, uint64_t generation,
@@ -181,6 +215,17 @@ int fort_scope_external_execute_v1(fort_scope_t context, const fort_scope_extern
 int fort_scope_observe_v1(fort_scope_t context, const fort_buffer_t *buffers, size_t count); /* :303 */
 int fort_scope_observe_all_v1(fort_scope_t context);                                /* :309 */
 int fort_scope_discard_v1(fort_scope_t context, fort_buffer_t buffer);              /* :310 */
+/* Dead values its owner is about to define again: as discard when no queued task uses the buffer, else
+ * the values stay until the new definition replaces them and the queue is not submitted. (:311-314) */
+int fort_scope_recycle_v1(fort_scope_t context, fort_buffer_t buffer);
+/* Destroys a dead resource without publication, after its queued and running uses complete; the handle
+ * becomes stale. (:298-300, 315) */
+int fort_scope_retire_v1(fort_scope_t context, fort_buffer_t buffer);
+/* A reached source-read precondition: values of the sections outside the defined coverage become defined
+ * with a current host copy; defined values keep their freshness; no transfer or wait. Exceeding the region
+ * limit commits nothing (BOUNDARY). (:85-92) */
+int fort_scope_admit_source_reads_v1(fort_scope_t context, fort_buffer_t buffer, const fort_scope_section *sections,
+                                     size_t count);
 int fort_scope_host_begin(fort_scope_t context, fort_buffer_t buffer, const fort_scope_access *access); /* :316 */
 int fort_scope_host_end(fort_scope_t context, fort_buffer_t buffer);                /* :317 */
 int fort_scope_device_begin(fort_scope_t context, fort_buffer_t buffer, const fort_scope_access *access,

@@ -564,3 +564,184 @@ def test_a_bad_identity_swap_fails_the_context_without_publication(scope, cpu_ru
     scope.check(lib, lib.fort_scope_domain_abandon_v1(dom.domain))
     dom.closed = True
     assert list(a) == [1.0, 2.0] and list(b) == [3.0, 4.0]
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Dead values (discard, recycle, retire: compiler/tests/test_scoped_graph_replay.py:126-171), reached
+# source reads (compiler/tests/test_source_read_runtime.py), scratch and completion counters and the
+# trace attribution state.
+
+def register_defined(scope, lib, dom, array, boxes, lower):
+    ext = (c_size_t * array.ndim)(*array.shape)
+    low = (ctypes.c_int64 * array.ndim)(*lower)
+    layout = scope.Layout(array.ndim, scope.REAL64, 8, array.ctypes.data, ext, low, 0)
+    secs = scope.Sections(boxes)
+    h = c_uint64()
+    scope.check(lib, lib.fort_scope_register_sections(dom.ctx, 77, 0, byref(layout), secs.array, secs.count,
+                                                      byref(h)))
+    return h.value
+
+
+def admit(scope, lib, dom, h, boxes):
+    secs = scope.Sections(boxes)
+    return lib.fort_scope_admit_source_reads_v1(dom.ctx, h, secs.array, secs.count)
+
+
+def host_read(scope, lib, dom, h, flags=0, reads=()):
+    access, _keep = scope.make_access(flags, reads, (), ())
+    return lib.fort_scope_host_begin(dom.ctx, h, byref(access))
+
+
+def test_discard_and_retire_publish_no_dead_values(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL | scope.WRITE_ALL)])
+        scope.check(lib, st)
+        dev_view(devs[0], (8,))[0] = 1000.0
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        before = dom.stats()
+        scope.check(lib, lib.fort_scope_discard_v1(dom.ctx, h))
+        assert dom.stats().downloads == before.downloads and dom.stats().waits == before.waits
+        assert host_read(scope, lib, dom, h, flags=scope.READ_ALL) == scope.UNINITIALIZED
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, flags=scope.WRITE_ALL | scope.OVERWRITE_ALL)])
+        scope.check(lib, st)
+        dev_view(devs[0], (8,))[:] = 2000.0 + np.arange(8)
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        assert dom.stats().allocations == 1
+        scope.check(lib, lib.fort_scope_retire_v1(dom.ctx, h))
+        assert dom.stats().downloads == 0 and dom.stats().allocated_bytes == 0
+        assert host_read(scope, lib, dom, h, flags=scope.READ_ALL) == scope.STALE
+    assert list(a) == list(range(8))
+
+
+def test_recycle_without_queued_users_forgets_like_discard(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        scope.check(lib, lib.fort_scope_recycle_v1(dom.ctx, h))
+        assert host_read(scope, lib, dom, h, flags=scope.READ_ALL) == scope.UNINITIALIZED
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, overwrites=[((0,), (8,))])])
+        scope.check(lib, st)
+        assert lib.fort_scope_recycle_v1(dom.ctx, h) == scope.STATE  # a prepared access
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+
+
+def test_retire_and_discard_refuse_a_prepared_access_and_retire_waits(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL)])
+        scope.check(lib, st)
+        assert lib.fort_scope_discard_v1(dom.ctx, h) == scope.STATE
+        assert lib.fort_scope_retire_v1(dom.ctx, h) == scope.STATE
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        before = dom.stats().waits
+        scope.check(lib, lib.fort_scope_observe_v1(dom.ctx, None, 0))
+        assert dom.stats().waits == before
+        scope.check(lib, lib.fort_scope_retire_v1(dom.ctx, h))
+        assert dom.stats().waits == before + 1
+
+
+def test_admitted_source_reads_keep_dirty_device_values_and_unread_cells(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    dom = new_domain(scope, lib)
+    h = register_defined(scope, lib, dom, a, [((2,), (4,))], (-3,))
+    st, devs = begin_device(scope, lib, dom, [binding(scope, h, reads=[((2,), (4,))], writes=[((2,), (4,))])])
+    scope.check(lib, st)
+    dev_view(devs[0], (8,))[2:4] = [102.0, 103.0]
+    scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+    before = bytes(dom.stats())
+    scope.check(lib, admit(scope, lib, dom, h, [((1,), (6,))]))
+    scope.check(lib, admit(scope, lib, dom, h, [((1,), (6,))]))
+    assert bytes(dom.stats()) == before  # no transfer, no wait
+    # A repeated admission must not mark the stale host copy current over the device writes.
+    st, devs = begin_device(scope, lib, dom, [binding(scope, h, reads=[((1,), (6,))])])
+    scope.check(lib, st)
+    assert list(dev_view(devs[0], (8,))[1:6]) == [1, 102, 103, 4, 5]
+    scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+    assert dom.stats().upload_bytes == 5 * 8
+    assert host_read(scope, lib, dom, h, reads=[((0,), (1,))]) == scope.UNINITIALIZED
+    dom.close()
+    assert list(a) == [0, 1, 102, 103, 4, 5, 6, 7]
+    assert admit(scope, lib, dom, h, [((0,), (1,))]) == scope.STALE
+
+
+def test_source_read_admission_bounds_prepared_access_and_region_limit(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a, initialized=False)
+        assert admit(scope, lib, dom, h, [((0,), (9,))]) == scope.ARGUMENT
+        scope.check(lib, admit(scope, lib, dom, h, [((0,), (1,))]))
+        access, keep = scope.make_access(0, [((0,), (1,))], (), ())
+        scope.check(lib, lib.fort_scope_host_begin(dom.ctx, h, byref(access)))
+        assert admit(scope, lib, dom, h, [((1,), (2,))]) == scope.STATE
+        assert b"prepared" in lib.fort_scope_error()
+        scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))
+        assert host_read(scope, lib, dom, h, reads=[((1,), (2,))]) == scope.UNINITIALIZED
+    limit = 128  # rectangles per region
+    b = np.arange(3.0 * limit + 4)
+    dom = new_domain(scope, lib)
+    h = register_defined(scope, lib, dom, b, [((3 * i,), (3 * i + 1,)) for i in range(limit)], (1,))
+    before = bytes(dom.stats())
+    assert admit(scope, lib, dom, h, [((3 * limit,), (3 * limit + 1,))]) == scope.BOUNDARY
+    assert bytes(dom.stats()) == before
+    assert host_read(scope, lib, dom, h, reads=[((3 * limit,), (3 * limit + 1,))]) == scope.UNINITIALIZED
+    dom.close()
+
+
+def test_scratch_and_completion_counters(scope, cpu_runtime):
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    with new_domain(scope, lib) as dom:
+        lease = scope.ScratchLease()
+        for size in (1024, 512, 4096):
+            scope.check(lib, lib.fort_scope_scratch_acquire_v1(dom.ctx, size, byref(lease)))
+            s = scope.ScratchStats()
+            scope.check(lib, lib.fort_scope_scratch_stats_get_v1(dom.ctx, byref(s)))
+            assert s.active == 1 and s.active_bytes == size
+            scope.check(lib, lib.fort_scope_scratch_release_v1(dom.ctx, lease.token))
+        scope.check(lib, lib.fort_scope_scratch_stats_get_v1(dom.ctx, byref(s)))
+        assert (s.version, s.active, s.acquisitions, s.allocations, s.reuses, s.releases, s.grows) == \
+            (1, 0, 3, 2, 1, 3, 1)
+        assert s.capacity_bytes == 4096 and s.peak_scratch_bytes == 4096 and s.active_bytes == 0
+        h = dom.bind(a)
+        cs = scope.CompletionStats()
+        scope.check(lib, lib.fort_scope_completion_stats_get_v1(dom.ctx, byref(cs)))
+        independent = cs.independent_host_tasks
+        assert host_read(scope, lib, dom, h, flags=scope.READ_ALL) == scope.OK  # current on the host
+        scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))
+        scope.check(lib, lib.fort_scope_completion_stats_get_v1(dom.ctx, byref(cs)))
+        assert cs.version == 1 and cs.independent_host_tasks == independent + 1
+        assert cs.event_records == cs.event_waits == 0
+
+
+def test_trace_attribution_is_saved_and_restored_only_while_tracing(scope, cpu_runtime, monkeypatch):
+    lib = cpu_runtime
+    with new_domain(scope, lib) as dom:
+        monkeypatch.delenv("FORT_RUNTIME_TRACE", raising=False)
+        prev = scope.TraceState()
+        prev.version = 77
+        lib.fort_scope_trace_set_v1(dom.ctx, 63, b"o", b"p", b"s", b"op", b"impl", b"b", byref(prev))
+        assert prev.version == 77  # tracing off: returns before looking at the context
+        monkeypatch.setenv("FORT_RUNTIME_TRACE", "1")
+        lib.fort_scope_trace_set_v1(dom.ctx, 1 | 2, b"owner1", b"proc1", None, None, None, None, byref(prev))
+        assert prev.owner == b"" and prev.procedure == b""
+        saved = scope.TraceState()
+        lib.fort_scope_trace_set_v1(dom.ctx, 2, None, b"proc2", None, None, None, None, byref(saved))
+        assert saved.owner == b"owner1" and saved.procedure == b"proc1"
+        now = scope.TraceState()
+        lib.fort_scope_trace_set_v1(dom.ctx, 0, None, None, None, None, None, None, byref(now))  # mask 0 saves
+        assert now.owner == b"owner1" and now.procedure == b"proc2"
+        lib.fort_scope_trace_restore_v1(dom.ctx, byref(saved))
+        lib.fort_scope_trace_set_v1(dom.ctx, 0, None, None, None, None, None, None, byref(now))
+        assert now.procedure == b"proc1"
+        too_long = b"x" * 65  # no NUL within 65 bytes: unknown
+        lib.fort_scope_trace_set_v1(dom.ctx, 1, too_long, None, None, None, None, None, byref(now))
+        lib.fort_scope_trace_set_v1(dom.ctx, 0, None, None, None, None, None, None, byref(now))
+        assert now.owner == b""

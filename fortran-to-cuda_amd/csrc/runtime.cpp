@@ -223,6 +223,7 @@ struct Context {
     fort_scope_graph_queue_stats_v1 qstats{};
 #ifndef FTC_CPU_BACKEND
     std::unordered_map<uint64_t, GraphFamily> graphs;
+    std::unordered_map<uint64_t, GraphEntry> groups;  // graph_submit_v1 / graph_execute_v1, by group key
 #endif
     void *scratch = nullptr;
     std::size_t scratch_capacity = 0;
@@ -885,6 +886,8 @@ static void release_context(Context &c)
     for (auto &kv : c.graphs)
         for (GraphEntry &g : kv.second.variants) destroy_graph(c, g);
     c.graphs.clear();
+    for (auto &kv : c.groups) destroy_graph(c, kv.second);
+    c.groups.clear();
 #endif
     if (c.scratch) dev::release(c.scratch);
     c.scratch = nullptr;
@@ -1763,6 +1766,193 @@ int fort_scope_graph_enqueue_members_v1(fort_scope_t h, uint64_t code_key, const
             merged.push_back(pb);
         }
         accept_task(c, code_key, *kernel, argument_sizes, merged.data(), merged.size());
+    });
+}
+
+// ------------------------------------------------------------------------------------------------ groups
+static void check_kernel(const fort_scope_kernel_v1 &k)
+{
+    require(k.function != nullptr, FORT_SCOPE_ARGUMENT, "missing kernel function");
+    require(!k.argument_count || k.arguments, FORT_SCOPE_ARGUMENT, "missing kernel arguments");
+    for (int d = 0; d < 3; ++d)
+        require(k.grid[d] && k.block[d], FORT_SCOPE_ARGUMENT, "empty kernel launch dimensions");
+}
+
+#ifndef FTC_CPU_BACKEND
+static hipKernelNodeParams group_params(const fort_scope_kernel_v1 &k)
+{
+    hipKernelNodeParams p{};
+    p.blockDim = dim3(k.block[0], k.block[1], k.block[2]);
+    p.gridDim = dim3(k.grid[0], k.grid[1], k.grid[2]);
+    p.func = const_cast<void *>(k.function);
+    p.kernelParams = const_cast<void **>(k.arguments);  // HIP copies the values during this call
+    p.sharedMemBytes = static_cast<unsigned>(k.shared_bytes);
+    p.extra = nullptr;
+    return p;
+}
+#endif
+
+// Runs an ordered kernel group on the context stream: one chain graph per nonzero group key, rebuilt
+// when the functions or argument counts change and otherwise replayed with every node's parameters,
+// dimensions and shared memory updated; group key 0 launches the kernels directly.
+static void submit_group(Context &c, uint64_t key, const std::vector<const fort_scope_kernel_v1 *> &ks)
+{
+#ifdef FTC_CPU_BACKEND
+    (void)c;
+    (void)key;
+    (void)ks;
+    fail(FORT_SCOPE_RESOURCE, "the CPU reference backend executes no device kernels");
+#else
+    ensure_stream(c);
+    dev::set(c.domain->device);
+    c.gstats.submissions++;
+    c.gstats.logical_kernels += ks.size();
+    c.domain->stats.launches += ks.size();
+    auto poison = [&](const char *what, hipError_t e) {
+        (void)hipGetLastError();
+        c.poisoned = true;
+        fail(FORT_SCOPE_EXECUTION, std::string(what) + hipGetErrorString(e));
+    };
+    if (key == 0 || !c.graphs_enabled) {
+        for (const fort_scope_kernel_v1 *k : ks) {
+            hipError_t e = hipLaunchKernel(k->function, dim3(k->grid[0], k->grid[1], k->grid[2]),
+                                           dim3(k->block[0], k->block[1], k->block[2]),
+                                           const_cast<void **>(k->arguments), k->shared_bytes, c.stream);
+            if (e != hipSuccess) poison("kernel launch failed: ", e);
+        }
+        return;
+    }
+    GraphEntry &g = c.groups[key];
+    // The topology of a group: its functions and argument counts (kept in dims for the comparison).
+    bool same = g.exec && g.nodes.size() == ks.size();
+    for (std::size_t i = 0; same && i < ks.size(); ++i)
+        same = g.dims[i].size() == 2 && g.dims[i][0] == (uint32_t)ks[i]->argument_count &&
+               g.args[i].size() == sizeof(void *) &&
+               std::memcmp(g.args[i].data(), &ks[i]->function, sizeof(void *)) == 0;
+    if (same) {
+        for (std::size_t i = 0; i < ks.size(); ++i) {
+            hipKernelNodeParams p = group_params(*ks[i]);
+            hipError_t e = hipGraphExecKernelNodeSetParams(g.exec, g.nodes[i], &p);
+            if (e != hipSuccess) poison("graph parameter update failed: ", e);
+            c.gstats.parameter_updates++;
+        }
+        c.gstats.replays++;
+    } else {
+        if (g.exec) c.gstats.rebuilds++;
+        destroy_graph(c, g);
+        hipError_t e = hipGraphCreate(&g.graph, 0);
+        if (e != hipSuccess) poison("graph creation failed: ", e);
+        hipGraphNode_t prev = nullptr;
+        g.args.clear();
+        g.dims.clear();
+        for (const fort_scope_kernel_v1 *k : ks) {
+            hipKernelNodeParams p = group_params(*k);
+            hipGraphNode_t node = nullptr;
+            e = hipGraphAddKernelNode(&node, g.graph, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+            if (e != hipSuccess) poison("graph node creation failed: ", e);
+            g.nodes.push_back(node);
+            prev = node;
+            std::vector<unsigned char> fn(sizeof(void *));
+            std::memcpy(fn.data(), &k->function, sizeof(void *));
+            g.args.push_back(fn);
+            g.dims.push_back({(uint32_t)k->argument_count, 0});
+        }
+        e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) poison("graph instantiation failed: ", e);
+        c.gstats.creations++;
+        c.gstats.instantiations++;
+    }
+    hipError_t e = hipGraphLaunch(g.exec, c.stream);
+    if (e != hipSuccess) poison("graph launch failed: ", e);
+#endif
+}
+
+int fort_scope_graph_submit_v1(fort_scope_t h, uint64_t group_key, const fort_scope_kernel_v1 *kernels,
+                               std::size_t count)
+{
+    return with(h, [&](Context &c) {
+        if (!count) return;  // a no-op group initializes nothing
+        require(kernels != nullptr, FORT_SCOPE_ARGUMENT, "missing kernels");
+        std::vector<const fort_scope_kernel_v1 *> ks;
+        for (std::size_t i = 0; i < count; ++i) {
+            check_kernel(kernels[i]);
+            ks.push_back(&kernels[i]);
+        }
+        // Work queued before the group runs before it.
+        flush(c);
+        submit_group(c, group_key, ks);
+    });
+}
+
+int fort_scope_graph_execute_v1(fort_scope_t h, uint64_t group_key, const fort_scope_graph_task_v1 *tasks,
+                                std::size_t count)
+{
+    return with(h, [&](Context &c) {
+        if (!count) return;
+        require(tasks != nullptr, FORT_SCOPE_ARGUMENT, "missing tasks");
+        struct Use {
+            Buffer *b;
+            Effects e;
+        };
+        std::vector<std::vector<Use>> uses(count);
+        std::vector<Buffer *> order;                   // the group's buffers, first use first
+        std::map<Buffer *, Region> defined, produced;  // simulated definitions; written by the group so far
+        std::map<Buffer *, Region> need;               // external requirements to upload
+        for (std::size_t i = 0; i < count; ++i) {
+            const fort_scope_graph_task_v1 &t = tasks[i];
+            check_kernel(t.kernel);
+            require(!t.binding_count || t.bindings, FORT_SCOPE_ARGUMENT, "missing task bindings");
+            for (std::size_t j = 0; j < t.binding_count; ++j) {
+                Buffer &b = buffer(c, t.bindings[j].buffer);
+                for (const Use &u : uses[i]) require(u.b != &b, FORT_SCOPE_ALIAS, "task bindings repeat a buffer");
+                require(!b.prepared, FORT_SCOPE_STATE, "buffer already has a prepared access");
+                if (!defined.count(&b)) {
+                    defined[&b] = b.init;
+                    produced[&b] = Region{};
+                    need[&b] = Region{};
+                    order.push_back(&b);
+                }
+                Effects e = effects(b, &t.bindings[j].access);
+                // Reads must be defined by the state before the group or by an earlier task of it.
+                for (const Box &r : e.reads)
+                    require(defined[&b].covers(r), FORT_SCOPE_UNINITIALIZED,
+                            "a task of the group reads values that are not defined before it");
+                // Values defined before the group that the device lacks and no earlier task produced go up.
+                std::vector<Box> touched = e.reads;
+                touched.insert(touched.end(), e.writes.begin(), e.writes.end());
+                for (const Box &box : touched)
+                    for (const Box &d0 : b.init.within(box))
+                        for (const Box &m : b.dev_cur.missing(d0))
+                            for (const Box &x : produced[&b].missing(m))
+                                for (const Box &fresh : need[&b].missing(x))
+                                    require(need[&b].add(fresh), FORT_SCOPE_RESOURCE, "region limit exceeded");
+                uses[i].push_back({&b, e});
+            }
+            for (const Use &u : uses[i]) {
+                for (const Box &o : u.e.overwrites)
+                    require(defined[u.b].add(o) && produced[u.b].add(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
+                for (const Box &w : u.e.writes)
+                    require(produced[u.b].add(w), FORT_SCOPE_RESOURCE, "region limit exceeded");
+            }
+        }
+#ifdef FTC_CPU_BACKEND
+        fail(FORT_SCOPE_RESOURCE, "the CPU reference backend executes no device kernels");
+#else
+        for (Buffer *b : order) drain_buffer(c, *b);
+        ensure_stream(c);
+        reserve(c, order.data(), order.size());
+        for (Buffer *b : order)
+            for (const Box &box : need[b].boxes) {
+                copy_box(c, *b, box, true);
+                require(b->dev_cur.add(box), FORT_SCOPE_RESOURCE, "region limit exceeded");
+            }
+        std::vector<const fort_scope_kernel_v1 *> ks;
+        for (std::size_t i = 0; i < count; ++i) ks.push_back(&tasks[i].kernel);
+        submit_group(c, group_key, ks);
+        // Definitions and coherence only after a successful submission, task by task in order.
+        for (std::size_t i = 0; i < count; ++i)
+            for (const Use &u : uses[i]) commit(*u.b, u.e.writes, u.e.overwrites, true);
+#endif
     });
 }
 

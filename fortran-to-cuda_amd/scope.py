@@ -138,6 +138,8 @@ RUNTIME_SIGNATURES = {
     "fort_scope_layout_get": (c_int, [c_uint64, c_uint64, POINTER(Layout)]),
     "fort_scope_access_begin_v1": (c_int, [c_uint64, POINTER(PlanBinding), c_size_t, c_int, POINTER(c_void_p)]),
     "fort_scope_original_host_begin_v1": (c_int, [c_uint64, POINTER(PlanBinding), c_size_t]),
+    "fort_scope_graph_submit_v1": (c_int, [c_uint64, c_uint64, c_void_p, c_size_t]),
+    "fort_scope_graph_execute_v1": (c_int, [c_uint64, c_uint64, c_void_p, c_size_t]),
     "fort_scope_recycle_v1": (c_int, [c_uint64, c_uint64]),
     "fort_scope_retire_v1": (c_int, [c_uint64, c_uint64]),
     "fort_scope_admit_source_reads_v1": (c_int, [c_uint64, c_uint64, POINTER(Section), c_size_t]),

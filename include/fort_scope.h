@@ -254,6 +254,19 @@ typedef struct fort_scope_graph_member_v1 {    /* scoped_runtime.h:407-410 */
 int fort_scope_graph_enqueue_members_v1(fort_scope_t context, uint64_t code_key, const fort_scope_kernel_v1 *kernel,
                                         const size_t *argument_sizes, const fort_scope_graph_member_v1 *members,
                                         size_t member_count);                       /* :411-413 */
+/* An ordered kernel group on the context stream (each node after its predecessor), for kernels whose
+ * resources the caller already acquired (device_end follows). A nonzero group key reuses its executable
+ * graph, rebuilt when functions or argument counts change, every replay updating all node parameters;
+ * key 0 launches directly. Argument values are copied during the call. (scoped_runtime.h:349-360, 417) */
+int fort_scope_graph_submit_v1(fort_scope_t context, uint64_t group_key, const fort_scope_kernel_v1 *kernels,
+                               size_t count);
+/* Checked group execution: the tasks' effects are simulated in order, so a producer may define what a
+ * later task reads; only the requirements defined before the group are uploaded, and definitions and
+ * coherence are committed after a successful submission. No binding may be prepared; device storage is
+ * reserved (device_reserve) before the arguments are built. The CPU backend returns RESOURCE after
+ * validation. (scoped_runtime.h:418-426) */
+int fort_scope_graph_execute_v1(fort_scope_t context, uint64_t group_key, const fort_scope_graph_task_v1 *tasks,
+                                size_t count);
 int fort_scope_graph_flush_v1(fort_scope_t context);                                /* :414 */
 int fort_scope_graph_queue_cancel_v1(fort_scope_t context);                         /* :415 */
 int fort_scope_graph_queue_stats_get_v1(fort_scope_t context, fort_scope_graph_queue_stats_v1 *stats); /* :416 */

@@ -745,3 +745,52 @@ def test_trace_attribution_is_saved_and_restored_only_while_tracing(scope, cpu_r
         lib.fort_scope_trace_set_v1(dom.ctx, 1, too_long, None, None, None, None, None, byref(now))
         lib.fort_scope_trace_set_v1(dom.ctx, 0, None, None, None, None, None, None, byref(now))
         assert now.owner == b""
+
+
+class Kernel(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("function", c_void_p),
+                ("grid", ctypes.c_uint32 * 3), ("block", ctypes.c_uint32 * 3), ("shared_bytes", c_size_t),
+                ("arguments", c_void_p), ("argument_count", c_size_t)]
+
+
+class GraphTask(ctypes.Structure):
+    _fields_ = [("kernel", Kernel), ("bindings", c_void_p), ("binding_count", c_size_t)]
+
+
+def group_task(scope, uses, keep):
+    arr = (scope.PlanBinding * len(uses))(*[scope.PlanBinding(h, acc) for h, acc in uses])
+    keep.append(arr)
+    k = Kernel(1, 0, c_void_p(0x1000), (ctypes.c_uint32 * 3)(1, 1, 1), (ctypes.c_uint32 * 3)(64, 1, 1), 0, None, 0)
+    return GraphTask(k, ctypes.cast(arr, c_void_p), len(uses))
+
+
+def test_group_execution_simulates_effects_in_order(scope, cpu_runtime):
+    # compiler/tests/test_scoped_graph_replay.py:172-194: a producer may define what a later task reads;
+    # the CPU backend validates the whole group, then refuses to run it (RESOURCE) and commits nothing.
+    lib = cpu_runtime
+    arrays = [np.arange(8.0) for _ in range(3)]
+    with new_domain(scope, lib) as dom:
+        a = dom.bind(arrays[0])
+        b = dom.bind(arrays[1], initialized=False)
+        out = dom.bind(arrays[2], initialized=False)
+        keep = []
+        ra, _k1 = scope.make_access(scope.READ_ALL)
+        rb, _k2 = scope.make_access(scope.READ_ALL)
+        wb, _k3 = scope.make_access(scope.WRITE_ALL | scope.OVERWRITE_ALL)
+        wo, _k4 = scope.make_access(scope.WRITE_ALL | scope.OVERWRITE_ALL)
+        produce = group_task(scope, [(a, ra), (b, wb)], keep)
+        consume = group_task(scope, [(b, rb), (out, wo)], keep)
+        before = bytes(dom.stats())
+        ordered = (GraphTask * 2)(produce, consume)
+        assert lib.fort_scope_graph_execute_v1(dom.ctx, 8, ordered, 2) == scope.RESOURCE
+        assert bytes(dom.stats()) == before
+        assert host_read(scope, lib, dom, b, flags=scope.READ_ALL) == scope.UNINITIALIZED
+        assert host_read(scope, lib, dom, out, flags=scope.READ_ALL) == scope.UNINITIALIZED
+        reverse = (GraphTask * 2)(consume, produce)
+        assert lib.fort_scope_graph_execute_v1(dom.ctx, 8, reverse, 2) == scope.UNINITIALIZED
+        stale = group_task(scope, [(2**63, ra)], keep)
+        assert lib.fort_scope_graph_execute_v1(dom.ctx, 9, (GraphTask * 2)(produce, stale), 2) == scope.STALE
+        assert bytes(dom.stats()) == before
+        assert lib.fort_scope_graph_execute_v1(dom.ctx, 9, None, 0) == scope.OK  # an empty group
+        assert lib.fort_scope_graph_submit_v1(dom.ctx, 1, None, 0) == scope.OK
+        assert lib.fort_scope_graph_submit_v1(dom.ctx, 1, ctypes.byref(produce.kernel), 1) == scope.RESOURCE

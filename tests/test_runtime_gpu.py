@@ -127,3 +127,61 @@ def test_identity_swap_keeps_the_device_copies_with_their_allocations(scope, hip
         assert (cur1.value, cur2.value) == (hb, ha)
         dom.observe(cur1.value, cur2.value)
     assert np.array_equal(a, np.arange(n) * 1.0) and np.array_equal(b, np.arange(n) * 2.0)
+
+
+class GraphTask(ctypes.Structure):
+    _fields_ = [("kernel", Kernel), ("bindings", c_void_p), ("binding_count", c_size_t)]
+
+
+def test_group_submit_replays_with_updated_arguments(scope, hip):
+    # graph_submit_v1 (scoped_runtime.h:349-360): resources acquired by the caller, one chain graph per
+    # group key, every replay with the new argument values.
+    n = 2048
+    a = np.zeros(n)
+    with scope.Domain(hip) as dom:
+        h = dom.bind(a, initialized=False)
+        for scale in (1.0, 3.0, 5.0):
+            acc, kp = scope.make_access(0, (), (), [((0,), (n,))])
+            b = (scope.PlanBinding * 1)(scope.PlanBinding(h, acc))
+            devs = (c_void_p * 1)()
+            scope.check(hip, hip.fort_scope_access_begin_v1(dom.ctx, b, 1, 1, devs))
+            k, sizes, keep = iota_launch(hip, devs[0], n, scale)
+            scope.check(hip, hip.fort_scope_graph_submit_v1(dom.ctx, 77, ctypes.byref(k), 1))
+            scope.check(hip, hip.fort_scope_device_end(dom.ctx, h))
+            dom.observe(h)
+            assert np.array_equal(a, scale * np.arange(n)), scale
+        g = dom.graph_stats()
+        assert g.instantiations >= 1 and g.replays >= 2
+
+
+def test_group_execution_uploads_only_external_requirements(scope, hip):
+    n = 1024
+    src = np.arange(n, dtype=np.float64)
+    mid, out = np.zeros(n), np.zeros(n)
+    with scope.Domain(hip) as dom:
+        hs, hm, ho = dom.bind(src), dom.bind(mid, initialized=False), dom.bind(out, initialized=False)
+        ptrs = []
+        for h in (hm, ho):
+            dev = c_void_p()
+            scope.check(hip, hip.fort_scope_device_reserve_v1(dom.ctx, h, ctypes.byref(dev)))
+            ptrs.append(dev.value)
+        keep = []
+
+        def task(uses, ptr, scale):
+            arr = (scope.PlanBinding * len(uses))(*[scope.PlanBinding(h, acc) for h, acc in uses])
+            k, sizes, kk = iota_launch(hip, ptr, n, scale)
+            keep.extend([arr, kk, k])
+            return GraphTask(k, ctypes.cast(arr, c_void_p), len(uses))
+
+        r_src, k1 = scope.make_access(scope.READ_ALL)
+        w_mid, k2 = scope.make_access(0, (), (), [((0,), (n,))])
+        r_mid, k3 = scope.make_access(scope.READ_ALL)
+        w_out, k4 = scope.make_access(0, (), (), [((0,), (n,))])
+        tasks = (GraphTask * 2)(task([(hs, r_src), (hm, w_mid)], ptrs[0], 2.0), task([(hm, r_mid), (ho, w_out)],
+                                                                                      ptrs[1], 4.0))
+        before = dom.stats().upload_bytes
+        scope.check(hip, hip.fort_scope_graph_execute_v1(dom.ctx, 5, tasks, 2))
+        # The intermediate is defined inside the group: only the source went up.
+        assert dom.stats().upload_bytes - before == n * 8
+        dom.observe(hm, ho)
+    assert np.array_equal(mid, 2.0 * np.arange(n)) and np.array_equal(out, 4.0 * np.arange(n))

@@ -62,7 +62,11 @@ inline int enqueue_kernel(fort_scope_t ctx, uint64_t code_key, const void *funct
     for (std::size_t i = 0; i < bindings.size(); ++i) {
         for (const Sec3 &s : bindings[i].reads) rs[i].push_back({s.lo, s.hi});
         for (const Sec3 &s : bindings[i].writes) ws[i].push_back({s.lo, s.hi});
-        for (const Sec3 &s : bindings[i].overwrites) os[i].push_back({s.lo, s.hi});
+        // An overwrite is also a write: the runtime takes must-overwrite sections within may-write ones.
+        for (const Sec3 &s : bindings[i].overwrites) {
+            ws[i].push_back({s.lo, s.hi});
+            os[i].push_back({s.lo, s.hi});
+        }
         plan[i].buffer = bindings[i].handle;
         plan[i].access.flags = 0;
         plan[i].access.read_count = rs[i].size();

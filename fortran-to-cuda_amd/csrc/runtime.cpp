@@ -494,20 +494,27 @@ static Effects effects(const Buffer &b, const fort_scope_access *a)
     require(!a->read_count || a->reads, FORT_SCOPE_ARGUMENT, "missing read sections");
     require(!a->write_count || a->writes, FORT_SCOPE_ARGUMENT, "missing write sections");
     require(!a->overwrite_count || a->overwrites, FORT_SCOPE_ARGUMENT, "missing overwrite sections");
-    for (std::size_t i = 0; i < a->read_count; ++i) e.reads.push_back(section_box(b, a->reads[i]));
-    for (std::size_t i = 0; i < a->write_count; ++i) e.writes.push_back(section_box(b, a->writes[i]));
-    for (std::size_t i = 0; i < a->overwrite_count; ++i) e.overwrites.push_back(section_box(b, a->overwrites[i]));
-    for (Box &x : e.reads) require(!x.empty() || true, FORT_SCOPE_ARGUMENT, "");
+    // Empty sections name nothing and are dropped (compiler/runtime/scoped_runtime.cu:804).
+    auto add = [&](std::vector<Box> &list, const fort_scope_section &s) {
+        const Box box = section_box(b, s);
+        if (!box.empty()) list.push_back(box);
+    };
+    for (std::size_t i = 0; i < a->read_count; ++i) add(e.reads, a->reads[i]);
+    for (std::size_t i = 0; i < a->write_count; ++i) add(e.writes, a->writes[i]);
+    for (std::size_t i = 0; i < a->overwrite_count; ++i) add(e.overwrites, a->overwrites[i]);
+    // What an access must overwrite it may write (compiler/runtime/scoped_runtime.cu:816-817).
+    Region may;
+    for (const Box &w : e.writes) require(may.add(w), FORT_SCOPE_RESOURCE, "region limit exceeded");
+    for (const Box &o : e.overwrites)
+        require(may.covers(o), FORT_SCOPE_ARGUMENT, "must-overwrite exceeds may-write sections");
     return e;
 }
 
-// Checks that every read is defined; returns the boxes to upload for a device access (reads, and the
-// defined parts of partial writes, that the device does not hold) or to download for a host access.
 // The sections of b to copy to the device (device) or to the host before an access with effects e: the
-// defined parts of its read and write regions that are not current there (writes too, because a partial
-// write leaves the rest of its region as it was). A strict access must read only defined values; the
-// may-access of original host work (strict false) may cover holes, which are then neither copied nor
-// defined.
+// defined parts of its reads, and of its writes outside its overwrites (a partial write leaves the rest
+// of its region as it was), that are not current there. A strict access must read only defined values;
+// the may-access of original host work (strict false) may cover holes, which are then neither copied
+// nor defined.
 static std::vector<Box> plan_transfer(const Buffer &b, const Effects &e, bool device, bool strict = true)
 {
     if (strict)
@@ -526,7 +533,12 @@ static std::vector<Box> plan_transfer(const Buffer &b, const Effects &e, bool de
             }
     };
     for (const Box &r : e.reads) want(r);
-    for (const Box &w : e.writes) want(w);
+    // A write preserves what it does not overwrite; the overwritten parts need no copy
+    // (compiler/runtime/scoped_runtime.cu:925-928).
+    Region over;
+    for (const Box &o : e.overwrites) require(over.add(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
+    for (const Box &w : e.writes)
+        for (const Box &kept : over.missing(w)) want(kept);
     return need;
 }
 

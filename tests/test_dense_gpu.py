@@ -64,7 +64,8 @@ def device_gemm(ftc, scope, hip, A, B, C, precision, batch, tile=-1, alpha=1.0, 
         hs = [dom.bind(X) for X in (A, B, C)]
         binds = []
         keep = []
-        for h, access in zip(hs, (scope.READ_ALL, scope.READ_ALL, scope.READ_ALL | scope.OVERWRITE_ALL)):
+        output = scope.READ_ALL | scope.WRITE_ALL | scope.OVERWRITE_ALL
+        for h, access in zip(hs, (scope.READ_ALL, scope.READ_ALL, output)):
             acc, kp = scope.make_access(access)
             keep.append(kp)
             binds.append(scope.PlanBinding(h, acc))
@@ -154,7 +155,7 @@ def test_layout_flags_are_checked_before_any_launch(ftc, scope, hip):
 
 
 @pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 2e-4)])
-@pytest.mark.parametrize("n", [96, 301, 256])  # 256: whole 128 x 128 tiles (the direct-to-LDS split path)
+@pytest.mark.parametrize("n", [96, 301, 256])  # 256: whole 128 x 128 tiles (the pre-split K-contiguous path)
 def test_chain_contraction_entry(ftc, scope, hip, precision, tol, n):
     rng = np.random.default_rng(n + precision)
     A, B, C = (np.asfortranarray(rng.uniform(-1, 1, (n, n))) for _ in range(3))

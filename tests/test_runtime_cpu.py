@@ -60,7 +60,7 @@ def test_device_writes_reach_the_host_only_where_written(scope, cpu_runtime):
     with new_domain(scope, lib) as dom:
         h = dom.bind(a)
         box = ((1, 1, 1), (4, 3, 2))
-        b = binding(scope, h, reads=[box], overwrites=[box])
+        b = binding(scope, h, reads=[box], writes=[box], overwrites=[box])
         st, devs = begin_device(scope, lib, dom, [b])
         scope.check(lib, st)
         s = dom.stats()
@@ -99,7 +99,8 @@ def test_reading_undefined_values_is_refused(scope, cpu_runtime):
         st, _ = begin_device(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL)])
         assert st == scope.UNINITIALIZED
         # An overwrite defines the section; a later read of it is accepted.
-        st, _ = begin_device(scope, lib, dom, [binding(scope, h, overwrites=[((0, 0), (4, 2))])])
+        half = ((0, 0), (4, 2))
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, writes=[half], overwrites=[half])])
         scope.check(lib, st)
         scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
         st, _ = begin_device(scope, lib, dom, [binding(scope, h, reads=[((0, 0), (4, 2))])])
@@ -107,6 +108,37 @@ def test_reading_undefined_values_is_refused(scope, cpu_runtime):
         scope.check(lib, lib.fort_scope_cancel_access(dom.ctx, h))
         st, _ = begin_device(scope, lib, dom, [binding(scope, h, reads=[((0, 0), (4, 3))])])
         assert st == scope.UNINITIALIZED
+
+
+def test_overwritten_sections_are_not_copied_and_must_be_writable(scope, cpu_runtime):
+    # compiler/tests/test_scoped_runtime.py:264-291 (an overwrite skips input) and the may-write rule of
+    # compiler/runtime/scoped_runtime.cu:816-817, here on defined values that are not current on the device.
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        # Writes over [1, 7), of which [2, 6) is overwritten: only the preserved cells 1 and 6 go up.
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, writes=[((1,), (7,))], overwrites=[((2,), (6,))])])
+        scope.check(lib, st)
+        assert (dom.stats().uploads, dom.stats().upload_bytes) == (2, 2 * 8)
+        dev_view(devs[0], (8,))[1:7] = -1.0
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        every = scope.WRITE_ALL | scope.OVERWRITE_ALL
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, flags=every)])
+        scope.check(lib, st)
+        assert dom.stats().upload_bytes == 2 * 8  # nothing more
+        dev_view(devs[0], (8,))[:] = 5.0
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        for bad in (dict(overwrites=[((0,), (2,))]), dict(writes=[((0,), (3,))], overwrites=[((2,), (5,))]),
+                    dict(flags=scope.OVERWRITE_ALL)):
+            st, _ = begin_device(scope, lib, dom, [binding(scope, h, **bad)])
+            assert st == scope.ARGUMENT
+        # Empty sections name nothing: an empty overwrite needs no write.
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, reads=[((0,), (8,))], overwrites=[((3,), (3,))])])
+        scope.check(lib, st)
+        scope.check(lib, lib.fort_scope_cancel_access(dom.ctx, h))
+        dom.observe(h)
+    assert np.array_equal(a, np.full(8, 5.0))
 
 
 def test_repeated_buffers_and_bad_sections_are_rejected(scope, cpu_runtime):
@@ -128,7 +160,8 @@ def test_budget_pressure_evicts_the_least_recently_used_and_publishes_it(scope, 
     with new_domain(scope, lib, budget=2 * n * 8 + 16) as dom:
         ha, hb, hc = dom.bind(a), dom.bind(b), dom.bind(c)
         for h in (ha, hb):
-            st, devs = begin_device(scope, lib, dom, [binding(scope, h, flags=scope.READ_ALL | scope.OVERWRITE_ALL)])
+            every = scope.READ_ALL | scope.WRITE_ALL | scope.OVERWRITE_ALL
+            st, devs = begin_device(scope, lib, dom, [binding(scope, h, flags=every)])
             scope.check(lib, st)
             dev_view(devs[0], (n,))[:] += 10.0
             scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
@@ -225,8 +258,8 @@ def test_member_launches_are_checked_on_their_actual_sections(scope, cpu_runtime
         assert launch(member((ha, w(0, 8))), member((ha, w(8, 16)))) == scope.RESOURCE
         assert launch(member((ha, w(0, 9))), member((ha, r(8, 16)))) == scope.BOUNDARY
         assert launch(member((ha, r(0, 16))), member((ha, r(0, 16)), (hb, w(0, 16)))) == scope.RESOURCE
-        assert launch(member((ha, r(0, 4))), member((hb, {"overwrites": [((0,), (16,))]}), (ha, w(4, 16)))) \
-            == scope.RESOURCE
+        whole_b = {"writes": [((0,), (16,))], "overwrites": [((0,), (16,))]}
+        assert launch(member((ha, r(0, 4))), member((hb, whole_b), (ha, w(4, 16)))) == scope.RESOURCE
         assert launch(member((ha, {"flags": scope.WRITE_ALL})), member((ha, r(15, 16)))) == scope.BOUNDARY
         assert launch(member((ha, r(0, 4)), (ha, r(4, 8)))) == scope.ALIAS
         assert dom.stats().allocations == 0
@@ -266,7 +299,7 @@ def test_external_provider_transaction(scope, cpu_runtime):
 
     with new_domain(scope, lib) as dom:
         hp, hr = dom.bind(phi, initialized=False), dom.bind(rhs)
-        a0, k0 = scope.make_access(scope.OVERWRITE_ALL)
+        a0, k0 = scope.make_access(scope.WRITE_ALL | scope.OVERWRITE_ALL)
         a1, k1 = scope.make_access(scope.READ_ALL)
         eb = (scope.ExternalBinding * 2)(scope.ExternalBinding(hp, scope.REAL64, 2, 8, a0),
                                          scope.ExternalBinding(hr, scope.REAL64, 2, 8, a1))
@@ -310,7 +343,7 @@ def test_a_provider_that_declines_before_enqueue_changes_nothing(scope, cpu_runt
 
     with new_domain(scope, lib) as dom:
         h = dom.bind(a)
-        acc, keep = scope.make_access(scope.READ_ALL | scope.OVERWRITE_ALL)
+        acc, keep = scope.make_access(scope.READ_ALL | scope.WRITE_ALL | scope.OVERWRITE_ALL)
         eb = (scope.ExternalBinding * 1)(scope.ExternalBinding(h, scope.REAL64, 1, 8, acc))
         call = scope.ExternalCall(1, 0, eb, 1, 0, 0)
         assert lib.fort_scope_external_execute_v1(dom.ctx, byref(call), worker, None, None) == scope.BOUNDARY
@@ -623,7 +656,7 @@ def test_recycle_without_queued_users_forgets_like_discard(scope, cpu_runtime):
         h = dom.bind(a)
         scope.check(lib, lib.fort_scope_recycle_v1(dom.ctx, h))
         assert host_read(scope, lib, dom, h, flags=scope.READ_ALL) == scope.UNINITIALIZED
-        st, _ = begin_device(scope, lib, dom, [binding(scope, h, overwrites=[((0,), (8,))])])
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, writes=[((0,), (8,))], overwrites=[((0,), (8,))])])
         scope.check(lib, st)
         assert lib.fort_scope_recycle_v1(dom.ctx, h) == scope.STATE  # a prepared access
         scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))

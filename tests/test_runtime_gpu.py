@@ -36,6 +36,11 @@ def iota_launch(hip, device_ptr, n, scale):
     return k, sizes, (keep, argv)
 
 
+def overwrite(lo, hi):
+    """A rank-1 access that writes all of [lo, hi) (a must-overwrite section is also a may-write one)."""
+    return {"writes": [((lo,), (hi,))], "overwrites": [((lo,), (hi,))]}
+
+
 def members_of(scope, groups):
     keep, out = [], []
     for uses in groups:
@@ -59,8 +64,8 @@ def test_member_launch_commits_the_union_of_its_members(scope, hip):
         scope.check(hip, hip.fort_scope_device_reserve_v1(dom.ctx, h, ctypes.byref(dev)))
         k, sizes, keep = iota_launch(hip, dev.value, n, 0.5)
         half = n // 2
-        ms, keep2 = members_of(scope, [[(h, {"overwrites": [((0,), (half,))]})],
-                                       [(h, {"overwrites": [((half,), (n,))]})]])
+        ms, keep2 = members_of(scope, [[(h, overwrite(0, half))],
+                                       [(h, overwrite(half, n))]])
         scope.check(hip, hip.fort_scope_graph_enqueue_members_v1(dom.ctx, 11, ctypes.byref(k), sizes, ms, 2))
         scope.check(hip, hip.fort_scope_graph_flush_v1(dom.ctx))
         dom.observe(h)
@@ -92,7 +97,7 @@ def test_original_host_read_publishes_only_device_defined_cells(scope, hip):
         dev = c_void_p()
         scope.check(hip, hip.fort_scope_device_reserve_v1(dom.ctx, h, ctypes.byref(dev)))
         k, sizes, keep = iota_launch(hip, dev.value + 2 * 8, 4, 7.0)  # cells 2..5 = 0, 7, 14, 21
-        ms, keep2 = members_of(scope, [[(h, {"overwrites": [((2,), (6,))]})]])
+        ms, keep2 = members_of(scope, [[(h, overwrite(2, 6))]])
         scope.check(hip, hip.fort_scope_graph_enqueue_members_v1(dom.ctx, 13, ctypes.byref(k), sizes, ms, 1))
         scope.check(hip, hip.fort_scope_graph_flush_v1(dom.ctx))
         acc, kp = scope.make_access(scope.READ_ALL)
@@ -113,7 +118,7 @@ def test_identity_swap_keeps_the_device_copies_with_their_allocations(scope, hip
             dev = c_void_p()
             scope.check(hip, hip.fort_scope_device_reserve_v1(dom.ctx, h, ctypes.byref(dev)))
             k, sizes, keep = iota_launch(hip, dev.value, n, scale)
-            ms, keep2 = members_of(scope, [[(h, {"overwrites": [((0,), (n,))]})]])
+            ms, keep2 = members_of(scope, [[(h, overwrite(0, n))]])
             scope.check(hip, hip.fort_scope_graph_enqueue_members_v1(dom.ctx, 14, ctypes.byref(k), sizes, ms, 1))
         scope.check(hip, hip.fort_scope_graph_flush_v1(dom.ctx))
         layouts = []

@@ -123,10 +123,11 @@ contains
     close(unit)
   end subroutine
 
-  ! As the dump template writes a field: the values, then storage_size, shape, lbound and ubound.
+  ! As the dump template writes a field: the values, then storage_size, shape, lbound and ubound. The
+  ! dummy is allocatable so that lbound and ubound are the array's own, as where the template runs.
   subroutine dump(name, values)
     character(*), intent(in) :: name
-    real(c_double), intent(in) :: values(..)
+    real(c_double), allocatable, intent(in) :: values(..)
     integer :: unit
     open(newunit=unit, file='out/fields-'//name//'.bin', access='stream', form='unformatted', status='replace')
     select rank (values)

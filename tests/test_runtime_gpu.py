@@ -146,7 +146,7 @@ def test_group_submit_replays_with_updated_arguments(scope, hip):
     with scope.Domain(hip) as dom:
         h = dom.bind(a, initialized=False)
         for scale in (1.0, 3.0, 5.0):
-            acc, kp = scope.make_access(0, (), (), [((0,), (n,))])
+            acc, kp = scope.make_access(0, (), [((0,), (n,))], [((0,), (n,))])
             b = (scope.PlanBinding * 1)(scope.PlanBinding(h, acc))
             devs = (c_void_p * 1)()
             scope.check(hip, hip.fort_scope_access_begin_v1(dom.ctx, b, 1, 1, devs))
@@ -179,9 +179,9 @@ def test_group_execution_uploads_only_external_requirements(scope, hip):
             return GraphTask(k, ctypes.cast(arr, c_void_p), len(uses))
 
         r_src, k1 = scope.make_access(scope.READ_ALL)
-        w_mid, k2 = scope.make_access(0, (), (), [((0,), (n,))])
+        w_mid, k2 = scope.make_access(0, (), [((0,), (n,))], [((0,), (n,))])
         r_mid, k3 = scope.make_access(scope.READ_ALL)
-        w_out, k4 = scope.make_access(0, (), (), [((0,), (n,))])
+        w_out, k4 = scope.make_access(0, (), [((0,), (n,))], [((0,), (n,))])
         tasks = (GraphTask * 2)(task([(hs, r_src), (hm, w_mid)], ptrs[0], 2.0), task([(hm, r_mid), (ho, w_out)],
                                                                                       ptrs[1], 4.0))
         before = dom.stats().upload_bytes

@@ -78,6 +78,8 @@ def build_fortran() -> None:
         return
     out = BUILD / "fortran"
     out.mkdir(parents=True, exist_ok=True)
+    # fortran/app/ holds the domain module of applications built on the reference's Fortran host layer;
+    # resident_app.py compiles it with that layer.
     sources = sorted((PACKAGE / "fortran").glob("*.f90"))
     for src in sources:
         _run([fc, "-c", "-O2", "-J" if "gfortran" in fc else "-module-dir", str(out), str(src),

@@ -60,7 +60,11 @@ static void copy2d(void *dst, std::size_t dpitch, const void *src, std::size_t s
         std::memcpy(static_cast<char *>(dst) + r * dpitch, static_cast<const char *>(src) + r * spitch, width);
 }
 static void sync(Stream) {}
+constexpr bool kStaged = false;  // "device" memory is host memory: strided copies are plain memcpy loops
+static void *host_alloc(std::size_t n) { return std::malloc(n ? n : 1); }
+static void host_release(void *p) { std::free(p); }
 static Stream make_stream() { return nullptr; }
+static void prime(Stream, void *, std::size_t) {}
 static void destroy_stream(Stream) {}
 static std::size_t free_bytes() { return std::size_t(64) << 30; }
 static std::size_t total_bytes() { return std::size_t(64) << 30; }
@@ -98,16 +102,57 @@ static void release(void *p) { (void)hipFree(p); }
 static void copy2d(void *dst, std::size_t dpitch, const void *src, std::size_t spitch, std::size_t width,
                    std::size_t height, bool h2d, Stream s)
 {
-    check(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height,
-                           h2d ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, s),
-          FORT_SCOPE_EXECUTION, "hipMemcpy2DAsync");
+    const hipMemcpyKind kind = h2d ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    // A contiguous block takes the linear copy: the pitched one spent milliseconds per megabyte in the
+    // runtime for it, pinned or not.
+    if (height == 1 || (dpitch == width && spitch == width))
+        check(hipMemcpyAsync(dst, src, width * height, kind, s), FORT_SCOPE_EXECUTION, "hipMemcpyAsync");
+    else
+        check(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, kind, s), FORT_SCOPE_EXECUTION,
+              "hipMemcpy2DAsync");
 }
 static void sync(Stream s) { check(hipStreamSynchronize(s), FORT_SCOPE_EXECUTION, "hipStreamSynchronize"); }
+// Strided copies between the device and pageable host memory (the application's arrays) move row by row
+// inside the HIP runtime, at a few MB/s for rows of a few hundred bytes; between the device and pinned
+// memory they are pitched DMA. So strided boxes travel through a pinned staging buffer.
+constexpr bool kStaged = true;
+static void *host_alloc(std::size_t n)
+{
+    void *p = nullptr;
+    check(hipHostMalloc(&p, n ? n : 1, hipHostMallocDefault), FORT_SCOPE_RESOURCE, "hipHostMalloc");
+    return p;
+}
+static void host_release(void *p)
+{
+    if (p) (void)hipHostFree(p);
+}
 static Stream make_stream()
 {
     hipStream_t s = nullptr;
     check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), FORT_SCOPE_RESOURCE, "hipStreamCreate");
     return s;
+}
+// The first large linear copy of a process and the first pitched one each set up their copy path,
+// which took 6-7 ms apiece on an MI355X (small copies take another path and do not). Copies of `bytes`
+// each way between the staging buffer `pinned` and a device block, linear from aligned and unaligned
+// addresses and pitched, move that cost to the context's creation, so that the first upload and the
+// first observation of an epoch do not pay it.
+static void prime(Stream s, void *pinned, std::size_t bytes)
+{
+    void *d = nullptr;
+    check(hipMalloc(&d, bytes + 64), FORT_SCOPE_RESOURCE, "hipMalloc");
+    for (std::size_t skew : {std::size_t(0), std::size_t(8)}) {
+        char *dd = static_cast<char *>(d) + skew, *hh = static_cast<char *>(pinned) + skew;
+        check(hipMemcpyAsync(dd, hh, bytes, hipMemcpyHostToDevice, s), FORT_SCOPE_RESOURCE, "hipMemcpyAsync");
+        check(hipMemcpyAsync(hh, dd, bytes, hipMemcpyDeviceToHost, s), FORT_SCOPE_RESOURCE, "hipMemcpyAsync");
+    }
+    const std::size_t width = 512, pitch = 1024, rows = bytes / pitch;
+    check(hipMemcpy2DAsync(d, pitch, pinned, width, width, rows, hipMemcpyHostToDevice, s), FORT_SCOPE_RESOURCE,
+          "hipMemcpy2DAsync");
+    check(hipMemcpy2DAsync(pinned, width, d, pitch, width, rows, hipMemcpyDeviceToHost, s), FORT_SCOPE_RESOURCE,
+          "hipMemcpy2DAsync");
+    check(hipStreamSynchronize(s), FORT_SCOPE_RESOURCE, "hipStreamSynchronize");
+    (void)hipFree(d);
 }
 static void destroy_stream(Stream s)
 {
@@ -227,6 +272,10 @@ struct Context {
 #endif
     void *scratch = nullptr;
     std::size_t scratch_capacity = 0;
+    // Pinned staging of strided section copies; `staging_queued` while an upload from it may still run.
+    void *staging = nullptr;
+    std::size_t staging_capacity = 0;
+    bool staging_queued = false;
     uint64_t scratch_token = 0, next_token = 1;
     bool scratch_active = false;
     fort_scope_scratch_stats_v1 sstats{};
@@ -299,6 +348,8 @@ static void strides(const Buffer &b, std::size_t *s)
 // Copies one box between the host array and its mirror: the leading dimensions the box spans completely
 // form one contiguous run, the next dimension is the row count of a pitched 2D copy, and the remaining
 // dimensions are iterated (compiler/runtime/section_copy.hpp:14-80 plans the same decomposition).
+static void ensure_stream(Context &c);
+
 static void copy_box(Context &c, Buffer &b, const Box &box, bool h2d)
 {
     if (box.empty()) return;
@@ -311,33 +362,122 @@ static void copy_box(Context &c, Buffer &b, const Box &box, bool h2d)
     int row_dim = m + 1;
     std::size_t rows = row_dim < r ? box.hi[row_dim] - box.lo[row_dim] : 1;
     std::size_t pitch = row_dim < r ? s[row_dim] * b.elem : run;
+    std::size_t planes = 1;
+    for (int d = row_dim + 1; d < r; ++d) planes *= box.hi[d] - box.lo[d];
+    const std::size_t packed = run * rows;  // one plane of the box, rows back to back
+    const std::size_t bytes = box.volume() * b.elem;
+    auto count = [&]() {
+        if (h2d) {
+            c.domain->stats.uploads++;
+            c.domain->stats.upload_bytes += bytes;
+        } else {
+            c.domain->stats.downloads++;
+            c.domain->stats.download_bytes += bytes;
+        }
+    };
+    auto offset_of = [&](const std::size_t *at) {
+        std::size_t off = 0;
+        for (int d = 0; d < r; ++d) off += at[d] * s[d];
+        return off * b.elem;
+    };
+    auto ensure_staging = [&](std::size_t need) {
+        if (c.staging_capacity >= need) return;
+        ensure_stream(c);
+        if (c.staging_queued) dev::sync(c.stream);
+        c.staging_queued = false;
+        dev::host_release(c.staging);
+        c.staging = nullptr;
+        c.staging_capacity = 0;
+        c.staging = dev::host_alloc(need);
+        c.staging_capacity = need;
+    };
+    // One contiguous run goes directly; anything strided is staged (HIP backend).
+    const bool staged = dev::kStaged && (rows > 1 || planes > 1);
     std::size_t idx[kMaxRank];
     for (int d = 0; d < r; ++d) idx[d] = box.lo[d];
-    for (;;) {
-        std::size_t off = 0;
-        for (int d = 0; d < r; ++d) off += idx[d] * s[d];
-        off *= b.elem;
-        char *h = static_cast<char *>(b.host) + off;
-        char *g = static_cast<char *>(b.device) + off;
-        if (h2d) dev::copy2d(g, pitch, h, pitch, run, rows, true, c.stream);
-        else dev::copy2d(h, pitch, g, pitch, run, rows, false, c.stream);
+    auto next_plane = [&]() {
         int d = row_dim + 1;
         while (d < r) {
-            if (++idx[d] < box.hi[d]) break;
+            if (++idx[d] < box.hi[d]) return true;
             idx[d] = box.lo[d];
             ++d;
         }
-        if (d >= r) break;
+        return false;
+    };
+    if (staged && !h2d) {
+        // A download of a dense box copies its whole span in one transfer and unpacks its rows on the
+        // host; the cells between the rows come along unused.
+        std::size_t last[kMaxRank];
+        for (int d = 0; d < r; ++d) last[d] = box.hi[d] - 1;
+        // The span starts and ends on 256-byte boundaries of the allocation where it can (the copy
+        // engine's fast path), clipped to the allocation.
+        const std::size_t first = offset_of(box.lo) & ~std::size_t(255);
+        const std::size_t end = std::min(b.bytes, (offset_of(last) + b.elem + 255) & ~std::size_t(255));
+        const std::size_t span = end - first;
+        if (span <= 2 * bytes + 512 && span <= c.staging_capacity) {
+            ensure_staging(span);
+            char *stage = static_cast<char *>(c.staging);
+            dev::copy2d(stage, span, static_cast<char *>(b.device) + first, span, span, 1, false, c.stream);
+            dev::sync(c.stream);
+            c.staging_queued = false;
+            do {
+                const std::size_t off = offset_of(idx);
+                for (std::size_t q = 0; q < rows; ++q)
+                    std::memcpy(static_cast<char *>(b.host) + off + q * pitch, stage + off - first + q * pitch, run);
+            } while (next_plane());
+            count();
+            return;
+        }
     }
-    std::size_t bytes = box.volume() * b.elem;
-    if (h2d) {
-        c.domain->stats.uploads++;
-        c.domain->stats.upload_bytes += bytes;
-    } else {
-        c.domain->stats.downloads++;
-        c.domain->stats.download_bytes += bytes;
+    // Otherwise the box moves in batches of planes, each plane one pitched copy.
+    std::size_t batch = planes;
+    if (staged) {
+        batch = std::max<std::size_t>(packed, std::min<std::size_t>(packed * planes, c.staging_capacity)) / packed;
+        ensure_staging(batch * packed);
     }
+    std::vector<std::size_t> offsets;  // byte offsets of the planes of one batch
+    auto flush_batch = [&]() {
+        if (offsets.empty()) return;
+        char *stage = static_cast<char *>(c.staging);
+        if (h2d) {
+            if (c.staging_queued) dev::sync(c.stream);  // an earlier upload may still read the buffer
+            for (std::size_t p = 0; p < offsets.size(); ++p)
+                for (std::size_t q = 0; q < rows; ++q)
+                    std::memcpy(stage + p * packed + q * run, static_cast<char *>(b.host) + offsets[p] + q * pitch, run);
+            for (std::size_t p = 0; p < offsets.size(); ++p)
+                dev::copy2d(static_cast<char *>(b.device) + offsets[p], pitch, stage + p * packed, run, run, rows, true,
+                            c.stream);
+            c.staging_queued = true;
+        } else {
+            for (std::size_t p = 0; p < offsets.size(); ++p)
+                dev::copy2d(stage + p * packed, run, static_cast<char *>(b.device) + offsets[p], pitch, run, rows,
+                            false, c.stream);
+            dev::sync(c.stream);
+            c.staging_queued = false;
+            for (std::size_t p = 0; p < offsets.size(); ++p)
+                for (std::size_t q = 0; q < rows; ++q)
+                    std::memcpy(static_cast<char *>(b.host) + offsets[p] + q * pitch, stage + p * packed + q * run, run);
+        }
+        offsets.clear();
+    };
+    do {
+        const std::size_t off = offset_of(idx);
+        if (staged) {
+            offsets.push_back(off);
+            if (offsets.size() == batch) flush_batch();
+        } else {
+            char *h = static_cast<char *>(b.host) + off;
+            char *g = static_cast<char *>(b.device) + off;
+            if (h2d) dev::copy2d(g, pitch, h, pitch, run, rows, true, c.stream);
+            else dev::copy2d(h, pitch, g, pitch, run, rows, false, c.stream);
+        }
+    } while (next_plane());
+    flush_batch();
+    count();
 }
+
+// Pinned staging every context starts with: strided boxes up to this size move in one batch.
+constexpr std::size_t kStagingBytes = std::size_t(16) << 20;
 
 static void ensure_stream(Context &c)
 {
@@ -345,6 +485,11 @@ static void ensure_stream(Context &c)
     dev::set(c.domain->device);
     c.stream = dev::make_stream();
     c.stream_ready = true;
+    if (dev::kStaged && !c.staging) {
+        c.staging = dev::host_alloc(kStagingBytes);
+        c.staging_capacity = kStagingBytes;
+        dev::prime(c.stream, c.staging, std::size_t(4) << 20);
+    }
 }
 
 static void wait(Context &c)
@@ -903,6 +1048,13 @@ static void release_context(Context &c)
 #endif
     if (c.scratch) dev::release(c.scratch);
     c.scratch = nullptr;
+    if (c.staging) {
+        if (c.staging_queued && c.stream_ready) dev::sync(c.stream);
+        dev::host_release(c.staging);
+    }
+    c.staging = nullptr;
+    c.staging_capacity = 0;
+    c.staging_queued = false;
     if (c.stream_ready) dev::destroy_stream(c.stream);
     c.stream_ready = false;
 }

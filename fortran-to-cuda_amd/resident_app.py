@@ -1,0 +1,380 @@
+"""Builds an unchanged Fortran application from its TOML description on this library's device entries.
+
+The reference's resident flow (compiler/resident/application.py, directives.py, compiler/scopes/
+resident_splice.py) reads an application description, applies its edits, expands its `!$fort`
+residency directives and replaces the loop nests of the selected procedures by blocks that keep the
+arrays on the device and call generated entries. This module does the same for the procedures that
+have hand-written gfx950 entries here: the procedure's executable part becomes a block that binds every
+array argument to the application's resident_domain, calls the entry's `run` and falls back to the
+original statements when the domain is inactive, an array cannot be kept, or the entry declines.
+
+What is read of the description (compiler/resident/application.py:10-75):
+  name, sources, program          the sources in compilation order and the main program
+  [variants.<v>] defines          preprocessor definitions of a variant
+  [[selection]] procedures        `module::procedure` patterns (`*` matches every name)
+  [[edit]] file, at, action, text edits after the entries are in place (before, after, replace with
+                                  {original}; `at` is found exactly once, whitespace-tolerant), which may
+                                  add residency directives
+Directives (compiler/resident/directives.py:85-130): enter [device(d)] [budget(b)] [if(c)], leave,
+close, publish [if(c)], observe(arrays).
+
+A procedure the selection names but no entry here implements stays as it is; `plan` reports both.
+The generated sources are written to a build directory and compiled with the Fortran compiler that
+built the reference's host layer (compiler/runtime/scoped_memory.f90, resident_domain.f90).
+"""
+from __future__ import annotations
+
+import argparse
+import fnmatch
+import os
+import re
+import subprocess
+import sys
+import tempfile
+from dataclasses import dataclass
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+FORTRAN = HERE / "fortran"
+
+
+@dataclass(frozen=True)
+class EntrySpec:
+    """A hand-written entry: its interface module (fortran/ftc_entries.f90) and the dummy arguments of
+    the procedure it replaces, arrays first (real(8), rank 3), then scalars passed by reference."""
+    module: str
+    procedure: str
+    interface: str
+    arrays: int
+    scalars: int
+
+
+ENTRIES = {
+    # benchmarks/cdu/src/cdu.f90:113-133 (also compiler/tests/fixtures/stencils/cdu.f90)
+    ("momentumadvection", "cdu"): EntrySpec("momentumadvection", "cdu", "fort_shared_momentumadvection_cdu", 4, 6),
+    # compiler/tests/fixtures/stencils/cvd.f90:104-125
+    ("momentumadvection", "cdv"): EntrySpec("momentumadvection", "cdv", "fort_shared_momentumadvection_cdv", 4, 6),
+    # compiler/tests/fixtures/stencils/cdw.f90:104-125
+    ("momentumadvection", "cdw"): EntrySpec("momentumadvection", "cdw", "fort_shared_momentumadvection_cdw", 4, 6),
+}
+
+DOMAIN_MODULE, DOMAIN, CHECK = "ftc_resident", "ftc_domain", "ftc_require"
+
+
+def load_description(path: Path) -> dict:
+    import tomli
+
+    with open(path, "rb") as f:
+        return tomli.load(f)
+
+
+# ------------------------------------------------------------------------------------------- edits
+def _unique_span(text: str, piece: str, what: str) -> tuple[int, int]:
+    found = [m.span() for m in re.finditer(re.escape(piece), text)]
+    if not found:
+        # Tolerant of layout: any run of blanks matches any other.
+        pattern = r"\s+".join(re.escape(word) for word in piece.split())
+        if piece[:1].isspace():
+            pattern = r"[ \t]*" + pattern
+        if piece[-1:] == "\n":
+            pattern += r"[ \t]*\n"
+        found = [m.span() for m in re.finditer(pattern, text)]
+    if len(found) != 1:
+        raise ValueError(f"{what}: expected one {piece!r}, found {len(found)}")
+    return found[0]
+
+
+def apply_edit(text: str, edit: dict, what: str) -> str:
+    """One [[edit]]: `at` located exactly once, then before, after or replace ({original} kept)."""
+    action = edit.get("action")
+    if action not in ("before", "after", "replace"):
+        raise ValueError(f"{what}: an edit needs an action: before, after or replace")
+    unsupported = set(edit) - {"file", "at", "action", "text"}
+    if unsupported or "at" not in edit:
+        raise ValueError(f"{what}: only `at` edits are supported here (found {sorted(unsupported) or 'no at'})")
+    start, end = _unique_span(text, edit["at"], what)
+    insert = edit.get("text", "")
+    if action == "before":
+        return text[:start] + insert + text[start:]
+    if action == "after":
+        return text[:end] + insert + text[end:]
+    return text[:start] + insert.replace("{original}", text[start:end]) + text[end:]
+
+
+# -------------------------------------------------------------------------------------- directives
+DIRECTIVE = re.compile(r"^(?P<indent>[ \t]*)!\$fort[ \t]+(?P<body>.*?)[ \t]*$", re.IGNORECASE)
+_CLAUSE = re.compile(r"\s*([a-z_]+)\s*(\((?:[^()]|\([^()]*\))*\))?", re.IGNORECASE)
+
+
+def _directive(body: str, line: int) -> tuple[str, list[str], dict]:
+    words, pos = [], 0
+    while pos < len(body):
+        m = _CLAUSE.match(body, pos)
+        if not m or m.end() == pos:
+            raise ValueError(f"Malformed residency directive on line {line}: {body!r}")
+        words.append((m.group(1).lower(), m.group(2)[1:-1].strip() if m.group(2) else None))
+        pos = m.end()
+    if not words:
+        raise ValueError(f"Empty residency directive on line {line}")
+    (name, arguments), clauses = words[0], dict(words[1:])
+    allowed = {"enter": {"device", "budget", "if"}, "leave": set(), "close": set(), "publish": {"if"},
+               "observe": {"if"}}
+    if name not in allowed:
+        raise ValueError(f"Unknown residency directive on line {line}: {name}")
+    if set(clauses) - allowed[name] or any(v is None for v in clauses.values()):
+        raise ValueError(f"Unsupported clause in the residency directive on line {line}: {body!r}")
+    if (arguments is not None) != (name == "observe"):
+        raise ValueError(f"The {name} directive on line {line} "
+                         + ("needs a list of arrays" if name == "observe" else "takes no arguments"))
+    names = [a.strip() for a in arguments.split(",")] if arguments else []
+    if not all(re.fullmatch(r"[A-Za-z_]\w*", n) for n in names):
+        raise ValueError(f"The observe directive on line {line} takes array names, not expressions")
+    return name, names, clauses
+
+
+def _block(indent: str, body: list[str], binding: bool = False) -> str:
+    lines = ["block"] + (["  use, intrinsic :: iso_c_binding"] if binding else [])
+    lines += [f"  use {DOMAIN_MODULE}, only: {DOMAIN}", *("  " + b for b in body), "end block"]
+    return "".join(indent + line + "\n" for line in lines)
+
+
+def expand_directives(text: str, origin: str = "source") -> tuple[str, list[dict]]:
+    """Every `!$fort` line replaced by the domain calls it stands for; returns the text and records."""
+    out, records = [], []
+    for number, line in enumerate(text.splitlines(keepends=True), 1):
+        m = DIRECTIVE.match(line.rstrip("\r\n"))
+        if not m:
+            out.append(line)
+            continue
+        name, names, clauses = _directive(m.group("body"), number)
+        cond = clauses.get("if")
+        active = f"{DOMAIN}%active" + (f" .and. ({cond})" if cond else "")
+        indent = m.group("indent")
+        if name == "enter":
+            budget = f", int({clauses['budget']}, c_size_t)" if "budget" in clauses else ""
+            start = f"call {DOMAIN}%begin_epoch(int({clauses.get('device', '0')}, c_int){budget})"
+            body = [f"if (.not. {DOMAIN}%active) then", "  " + (f"if ({cond}) {start}" if cond else start), "endif"]
+            out.append(_block(indent, body, binding=True))
+        elif name == "leave":
+            out.append(_block(indent, [f"if ({DOMAIN}%active) call {DOMAIN}%end_epoch()"]))
+        elif name == "close":
+            out.append(_block(indent, [f"call {DOMAIN}%close()"]))
+        elif name == "publish":
+            out.append(_block(indent, [f"if ({active}) call {DOMAIN}%publish()"]))
+        else:
+            handles = "ftc_observed"
+            body = [f"integer(c_int64_t) :: {handles}({len(names)})", f"if ({active}) then"]
+            body += [f"  {handles}({i}) = {DOMAIN}%bind({n}, lbound({n}), .true., origin='{origin}::{n.lower()}')"
+                     for i, n in enumerate(names, 1)]
+            body += [f"  call {DOMAIN}%observe({handles})", "endif"]
+            out.append(_block(indent, body, binding=True))
+        records.append({"line": number, "directive": name, "arrays": names, "clauses": clauses})
+    return "".join(out), records
+
+
+# ---------------------------------------------------------------------------------- procedure splice
+_MODULE = re.compile(r"^\s*module\s+(\w+)\s*(?:!.*)?$", re.IGNORECASE | re.MULTILINE)
+_END_MODULE = re.compile(r"^\s*end\s*module\b", re.IGNORECASE | re.MULTILINE)
+_SPEC = re.compile(r"^\s*(real|integer|logical|character|complex|double\s*precision|type\s*\(|class\s*\(|use\b|"
+                   r"implicit\b|dimension\b|parameter\b|intent\b|external\b|intrinsic\b|save\b|contiguous\b|"
+                   r"target\b|optional\b|pointer\b|allocatable\b|import\b|procedure\s*\()", re.IGNORECASE)
+
+
+def _modules(text: str) -> list[tuple[str, int, int]]:
+    spans = []
+    for m in _MODULE.finditer(text):
+        if m.group(1).lower() == "procedure":
+            continue
+        end = _END_MODULE.search(text, m.end())
+        spans.append((m.group(1).lower(), m.start(), end.start() if end else len(text)))
+    return spans
+
+
+def _logical_lines(lines: list[str], start: int) -> int:
+    """Index of the line after the (possibly continued) statement that begins at `start`."""
+    i = start
+    while i < len(lines) and lines[i].split("!", 1)[0].rstrip().endswith("&"):
+        i += 1
+    return i + 1
+
+
+def find_procedure(text: str, module: str, name: str) -> tuple[int, int, int, list[str]] | None:
+    """(header line, first executable line, END line, dummy names) of `subroutine name` in `module`,
+    as line indices of text.splitlines(keepends=True); None when it is not there."""
+    lines = text.splitlines(keepends=True)
+    offsets = [0]
+    for line in lines:
+        offsets.append(offsets[-1] + len(line))
+    for mod, lo, hi in _modules(text):
+        if mod != module:
+            continue
+        header = re.compile(rf"^\s*(?:(?:pure|elemental|recursive|impure|module)\s+)*subroutine\s+{name}\s*\(",
+                            re.IGNORECASE)
+        for i, line in enumerate(lines):
+            if not (lo <= offsets[i] < hi) or not header.match(line):
+                continue
+            stop = _logical_lines(lines, i)
+            statement = "".join(l.split("!", 1)[0].rstrip().rstrip("&") for l in lines[i:stop])
+            dummies = [d.strip() for d in statement.split("(", 1)[1].rsplit(")", 1)[0].split(",") if d.strip()]
+            j = stop
+            while j < len(lines):
+                code = lines[j].split("!", 1)[0].strip()
+                if code and not _SPEC.match(code):
+                    break
+                j = _logical_lines(lines, j) if code else j + 1
+            end = re.compile(rf"^\s*end\s*(?:subroutine(?:\s+{name})?)?\s*(?:!.*)?$", re.IGNORECASE)
+            k = j
+            while k < len(lines) and not end.match(lines[k]):
+                k += 1
+            if k == len(lines):
+                raise ValueError(f"no END of subroutine {name}")
+            return i, j, k, dummies
+    return None
+
+
+def splice_procedure(text: str, spec: EntrySpec, origin: str) -> str:
+    """The procedure's executable part wrapped in the resident block that calls the entry."""
+    found = find_procedure(text, spec.module, spec.procedure)
+    if found is None:
+        raise ValueError(f"{spec.module}::{spec.procedure} not found")
+    _, first, end, dummies = found
+    if len(dummies) != spec.arrays + spec.scalars:
+        raise ValueError(f"{spec.module}::{spec.procedure} has {len(dummies)} arguments; its entry takes "
+                         f"{spec.arrays} arrays and {spec.scalars} scalars")
+    lines = text.splitlines(keepends=True)
+    arrays, scalars = dummies[:spec.arrays], dummies[spec.arrays:]
+    ind = "    "
+    n = len(arrays)
+    block = [
+        "block",
+        "  use, intrinsic :: iso_c_binding",
+        "  use fort_scoped_memory, only: FORT_SCOPE_GPU, FORT_SCOPE_DECLINED",
+        f"  use {DOMAIN_MODULE}, only: {DOMAIN}, {CHECK}",
+        f"  use {spec.interface}, only: ftc_run => run",
+        f"  integer(c_int64_t) :: ftc_handles({n})",
+        "  integer(c_int) :: ftc_status",
+        "  logical :: ftc_resident_call",
+        f"  ftc_resident_call = {DOMAIN}%active",
+        "  if (ftc_resident_call) then",
+        *(f"    ftc_handles({i}) = {DOMAIN}%bind_real64({a}, lbound({a}), .true., &\n"
+          f"{ind}      origin='{origin}::{a.lower()}', tolerant=.true.)" for i, a in enumerate(arrays, 1)),
+        # An array that cannot be kept as it is bound here has no handle: the statements stay on the host.
+        "    ftc_resident_call = all(ftc_handles /= 0_c_int64_t)",
+        "  endif",
+        "  if (ftc_resident_call) then",
+        f"    ftc_status = ftc_run({DOMAIN}%context, FORT_SCOPE_GPU, &\n"
+        + "".join(f"{ind}      ftc_handles({i}), &\n" for i in range(1, n + 1))
+        + f"{ind}      " + ", ".join(scalars) + ")",
+        "    ftc_resident_call = ftc_status /= FORT_SCOPE_DECLINED",
+        f"    if (ftc_resident_call) call {CHECK}(ftc_status)",
+        "  endif",
+        "  if (.not. ftc_resident_call) then",
+        f"    if ({DOMAIN}%active) call {DOMAIN}%publish()",
+    ]
+    head = "".join(ind + b + "\n" for b in block)
+    tail = "".join(ind + b + "\n" for b in ["  endif", "end block"])
+    return "".join(lines[:first]) + head + "".join(lines[first:end]) + tail + "".join(lines[end:])
+
+
+def selected(description: dict, module: str, procedure: str) -> bool:
+    for table in description.get("selection", ()):
+        for pattern in table.get("procedures", ()):
+            pm, _, pp = pattern.lower().partition("::")
+            if fnmatch.fnmatch(module, pm) and fnmatch.fnmatch(procedure, pp or "*"):
+                return True
+    return False
+
+
+def plan(description: dict, texts: dict[str, str]) -> dict:
+    """Which selected procedures are spliced onto entries and which stay on the host."""
+    spliced, native = [], []
+    for name, text in texts.items():
+        for module, lo, hi in _modules(text):
+            for m in re.finditer(r"^\s*(?:(?:pure|elemental|recursive|impure)\s+)*subroutine\s+(\w+)",
+                                 text[lo:hi], re.IGNORECASE | re.MULTILINE):
+                proc = m.group(1).lower()
+                if not selected(description, module, proc):
+                    continue
+                (spliced if (module, proc) in ENTRIES else native).append((name, f"{module}::{proc}"))
+    return {"spliced": spliced, "native": native}
+
+
+def generate(toml_path: Path, out_dir: Path) -> tuple[list[Path], dict]:
+    """Writes the application's sources, edited, spliced and with directives expanded, into out_dir;
+    returns them in compilation order and the plan."""
+    description = load_description(toml_path)
+    root = toml_path.parent
+    sources = [root / s for s in description["sources"]]
+    texts = {p.name: p.read_text() for p in sources}
+    for number, edit in enumerate(description.get("edit", ()), 1):
+        if edit.get("file") not in texts:
+            raise ValueError(f"edit {number} names a file that is not a source: {edit.get('file')}")
+        texts[edit["file"]] = apply_edit(texts[edit["file"]], edit, f"edit {number} ({edit['file']})")
+    chosen = plan(description, texts)
+    for name, qualified in chosen["spliced"]:
+        module, proc = qualified.split("::")
+        texts[name] = splice_procedure(texts[name], ENTRIES[module, proc], qualified)
+    out_dir.mkdir(parents=True, exist_ok=True)
+    written = []
+    for p in sources:
+        text, records = expand_directives(texts[p.name], origin=p.stem)
+        chosen.setdefault("directives", []).extend({"file": p.name, **r} for r in records)
+        target = out_dir / p.name
+        target.write_text(text)
+        written.append(target)
+    return written, chosen
+
+
+def build(toml_path: Path, executable: Path, layer: Path, library: str = "ftc_hip", variant: str = "default",
+          defines: tuple[str, ...] = (), keep: Path | None = None) -> dict:
+    """Generates the sources and builds `executable` against the host layer in `layer` (objects, modules
+    and the compiler's path in `layer/compiler`) and lib<library>.so of this package."""
+    fc = (layer / "compiler").read_text().strip()
+    description = load_description(toml_path)
+    defs = [*description.get("variants", {}).get(variant, {}).get("defines", ()), *defines]
+    flags = ["-cpp", "-O3", "-fopenmp", "-ffp-contract=off", *(f"-D{d}" for d in defs)]
+    with tempfile.TemporaryDirectory(prefix="ftc_app_") as tmp:
+        work = keep or Path(tmp)
+        sources, chosen = generate(toml_path, work / "src")
+        objects = []
+        run = lambda cmd: subprocess.run(cmd, cwd=work, check=True, capture_output=True, text=True)
+        for src in [FORTRAN / "ftc_entries.f90", FORTRAN / "app" / "ftc_resident.f90", *sources]:
+            obj = work / (src.stem + ".o")
+            run([fc, *flags, "-I", str(layer), "-module-dir", str(work), "-c", str(src), "-o", str(obj)])
+            objects.append(str(obj))
+        executable.parent.mkdir(parents=True, exist_ok=True)
+        # Found relative to the executable, so that a tree that moves keeps finding its library.
+        rpath = "$ORIGIN/" + os.path.relpath(HERE, executable.parent)
+        run([fc, "-fopenmp", *objects, str(layer / "scoped_memory.o"), str(layer / "resident_domain.o"),
+             "-L", str(HERE), f"-l{library}", f"-Wl,-rpath,{rpath}", "-o", str(executable)])
+    return chosen
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n", 1)[0])
+    sub = ap.add_subparsers(dest="command", required=True)
+    b = sub.add_parser("build", help="build the resident executable of an application description")
+    b.add_argument("description", type=Path)
+    b.add_argument("executable", type=Path)
+    b.add_argument("--layer", type=Path, required=True, help="objects and modules of the Fortran host layer")
+    b.add_argument("--library", default="ftc_hip")
+    b.add_argument("--variant", default="default")
+    b.add_argument("-D", dest="defines", action="append", default=[])
+    g = sub.add_parser("generate", help="write the generated sources only")
+    g.add_argument("description", type=Path)
+    g.add_argument("out", type=Path)
+    args = ap.parse_args(argv)
+    if args.command == "build":
+        chosen = build(args.description.resolve(), args.executable.resolve(), args.layer.resolve(), args.library,
+                       args.variant, tuple(args.defines))
+    else:
+        _, chosen = generate(args.description.resolve(), args.out)
+    for name, qualified in chosen["spliced"]:
+        print(f"entry   {qualified} ({name})")
+    for name, qualified in chosen["native"]:
+        print(f"host    {qualified} ({name})")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

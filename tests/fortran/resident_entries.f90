@@ -2,52 +2,15 @@
 ! does (compiler/scopes/resident_splice.py:113-196): inside an epoch of the reference's own
 ! fort_resident_domain (compiler/runtime/resident_domain.f90), every array argument is bound at each
 ! call, tolerant, with its own lower bounds, and the entry's module procedure `run` is called with the
-! domain's context, FORT_SCOPE_GPU, the handles and the scalars by reference. The interface modules
-! below have the form the reference emits for an entry (compiler/emission/cuda/scoped.py:514-521).
+! domain's context, FORT_SCOPE_GPU, the handles and the scalars by reference. The entries' interface
+! modules are the ones this library ships (fortran-to-cuda_amd/fortran/ftc_entries.f90), in the form the
+! reference emits for an entry (compiler/emission/cuda/scoped.py:514-521).
 !
 ! Inputs and outputs are field dumps in the reference's storage-shape-bounds-v1 format
 ! (benchmarks/elmm/templates/field-dumps.f90): fields-<name>.bin in storage order and fields-<name>.shape.
 !   les   nx ny nz steps dx dt cs   reads in/fields-{U,V,W,S}.bin, takes `steps` explicit LES steps
 !                                    alternating between two sets of arrays, writes out/fields-{U,V,W,S}
 !   chain n precision               reads in/fields-{A,B,C}.bin, writes out/fields-D (D = A*B*C)
-module les_step_entry
-  use iso_c_binding
-  implicit none
-  private
-  public :: run
-  interface
-    function run(fort_context, fort_mode, un_handle, vn_handle, wn_handle, sn_handle, u_handle, v_handle, &
-                 w_handle, s_handle, dx, dy, dz, dt, cs, nx, ny, nz, sgs) &
-        bind(C, name='cpp_fort_shared_les_step') result(fort_status)
-      import :: c_int, c_int64_t, c_double
-      integer(c_int) :: fort_status
-      integer(c_int64_t), value :: fort_context
-      integer(c_int), value :: fort_mode
-      integer(c_int64_t), value :: un_handle, vn_handle, wn_handle, sn_handle, u_handle, v_handle, w_handle, s_handle
-      real(c_double), intent(in) :: dx, dy, dz, dt, cs
-      integer(c_int), intent(in) :: nx, ny, nz, sgs
-    end function
-  end interface
-end module
-
-module chain_contraction_entry
-  use iso_c_binding
-  implicit none
-  private
-  public :: run
-  interface
-    function run(fort_context, fort_mode, d_handle, a_handle, b_handle, c_handle, n, precision) &
-        bind(C, name='cpp_fort_shared_chain_contraction') result(fort_status)
-      import :: c_int, c_int64_t
-      integer(c_int) :: fort_status
-      integer(c_int64_t), value :: fort_context
-      integer(c_int), value :: fort_mode
-      integer(c_int64_t), value :: d_handle, a_handle, b_handle, c_handle
-      integer(c_int), intent(in) :: n, precision
-    end function
-  end interface
-end module
-
 module application
   use, intrinsic :: iso_c_binding
   use, intrinsic :: iso_fortran_env, only: error_unit
@@ -75,7 +38,7 @@ contains
   end subroutine
 
   subroutine les_step(un, vn, wn, sn, u, v, w, s, dx, dt, cs)
-    use les_step_entry, only: les_run => run
+    use fort_shared_les_step, only: les_run => run
     real(c_double), dimension(0:,0:,0:), intent(inout), target :: un, vn, wn, sn, u, v, w, s
     real(c_double), intent(in) :: dx, dt, cs
     integer(c_int64_t) :: h(8)
@@ -97,7 +60,7 @@ contains
   end subroutine
 
   subroutine chain(d, a, b, c, precision)
-    use chain_contraction_entry, only: chain_run => run
+    use fort_shared_chain_contraction, only: chain_run => run
     real(c_double), dimension(:,:), intent(inout), target :: d, a, b, c
     integer(c_int), intent(in) :: precision
     integer(c_int64_t) :: h(4)

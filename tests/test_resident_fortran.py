@@ -9,8 +9,9 @@ the layer names must resolve, every structure it passes must have the layout it 
 statuses and messages must come back as the reference's runtime gives them.
 
 tests/fortran/resident_host.f90 runs on the CPU backend; tests/fortran/resident_entries.f90 calls the
-LES step and chain contraction entries as spliced application code calls them and exchanges its arrays
-as field dumps, which are compared with the oracle (bit for bit) and with numpy's FP64 products.
+LES step and chain contraction entries through the interface modules this library ships
+(fortran-to-cuda_amd/fortran/ftc_entries.f90) as spliced application code calls them, and exchanges its
+arrays as field dumps, which are compared with the oracle (bit for bit) and with numpy's FP64 products.
 """
 import subprocess
 from pathlib import Path
@@ -38,11 +39,16 @@ def build(tmp_path, source, library):
         pytest.skip(f"{fc} unavailable")
     if not (PACKAGE / f"lib{library}.so").exists():
         pytest.skip(f"lib{library}.so is not built")
-    obj = tmp_path / (source.stem + ".o")
     run = lambda cmd: subprocess.run(cmd, cwd=tmp_path, check=True, capture_output=True, text=True, timeout=120)
-    run([fc, "-cpp", "-O2", "-fopenmp", "-I", str(LAYER), "-c", str(source), "-o", str(obj)])
+    objects = []
+    # The entries' interface modules this library ships come first.
+    for src in (PACKAGE / "fortran" / "ftc_entries.f90", source):
+        obj = tmp_path / (src.stem + ".o")
+        run([fc, "-cpp", "-O2", "-fopenmp", "-I", str(LAYER), "-module-dir", str(tmp_path), "-c", str(src),
+             "-o", str(obj)])
+        objects.append(str(obj))
     binary = tmp_path / source.stem
-    run([fc, "-fopenmp", str(obj), str(LAYER / "scoped_memory.o"), str(LAYER / "resident_domain.o"),
+    run([fc, "-fopenmp", *objects, str(LAYER / "scoped_memory.o"), str(LAYER / "resident_domain.o"),
          "-L", str(PACKAGE), f"-l{library}", f"-Wl,-rpath,{PACKAGE}", "-o", str(binary)])
     return binary
 

@@ -191,6 +191,7 @@ struct Buffer {
     Region init, host_cur, dev_cur;
     // A prepared access: 0 none, 1 device, 2 host; with the sections it may change.
     int prepared = 0;
+    bool prep_original = false;  // the prepared host access is original host work
     std::vector<Box> prep_writes, prep_overwrites;
     uint64_t last_use = 0;
     int queued = 0;  // accepted tasks not yet submitted that use this buffer
@@ -655,58 +656,68 @@ static Effects effects(const Buffer &b, const fort_scope_access *a)
     return e;
 }
 
-// The sections of b to copy to the device (device) or to the host before an access with effects e: the
-// defined parts of its reads, and of its writes outside its overwrites (a partial write leaves the rest
-// of its region as it was), that are not current there. A strict access must read only defined values;
-// the may-access of original host work (strict false) may cover holes, which are then neither copied
-// nor defined.
+// The values a buffer physically holds: what is defined, and whatever either copy holds current. A
+// definition kill (fort_scope_forget_definition) leaves the currency, so values written before it are
+// still published (compiler/runtime/scoped_regions.hpp:189-196).
+static Region physical(const Buffer &b)
+{
+    Region p = b.init;
+    for (const Region *r : {&b.host_cur, &b.dev_cur})
+        for (const Box &x : r->boxes) require(p.add(x), FORT_SCOPE_RESOURCE, "region limit exceeded");
+    return p;
+}
+
+// The sections of b to copy to the device (device) or to the host before an access with effects e: its
+// reads and the parts of its writes it does not overwrite (a partial write leaves the rest of its region
+// as it was), where they are not current there. A strict access must find all of them defined
+// (compiler/runtime/scoped_runtime.cu:906-929); the may-access of original host work (strict false) may
+// cover holes, and copies only the values the buffer physically holds.
 static std::vector<Box> plan_transfer(const Buffer &b, const Effects &e, bool device, bool strict = true)
 {
+    std::vector<Box> wanted = e.reads;
+    Region over;
+    for (const Box &o : e.overwrites) require(over.add(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
+    for (const Box &w : e.writes)
+        for (const Box &kept : over.missing(w)) wanted.push_back(kept);
     if (strict)
-        for (const Box &r : e.reads)
-            require(b.init.covers(r), FORT_SCOPE_UNINITIALIZED, "task reads values that were never defined");
+        for (const Box &r : wanted)
+            require(b.init.covers(r), FORT_SCOPE_UNINITIALIZED,
+                    "task reads values that were never defined");
+    const Region known = strict ? b.init : physical(b);
     const Region &have = device ? b.dev_cur : b.host_cur;
     std::vector<Box> need;
     Region acc;
-    auto want = [&](const Box &box) {
-        for (const Box &defined : b.init.within(box))
-            for (const Box &m : have.missing(defined)) {
+    for (const Box &box : wanted)
+        for (const Box &held : known.within(box))
+            for (const Box &m : have.missing(held))
                 for (const Box &fresh : acc.missing(m)) {
                     need.push_back(fresh);
                     require(acc.add(fresh), FORT_SCOPE_RESOURCE, "region limit exceeded");
                 }
-            }
-    };
-    for (const Box &r : e.reads) want(r);
-    // A write preserves what it does not overwrite; the overwritten parts need no copy
-    // (compiler/runtime/scoped_runtime.cu:925-928).
-    Region over;
-    for (const Box &o : e.overwrites) require(over.add(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
-    for (const Box &w : e.writes)
-        for (const Box &kept : over.missing(w)) want(kept);
     return need;
 }
 
-// Commits the effects of a finished (or accepted) access to the coherence state.
-static void commit(Buffer &b, const std::vector<Box> &writes, const std::vector<Box> &overwrites, bool device)
+// Commits the effects of a finished (or accepted) access to the coherence state: what it may write is
+// defined (its partial writes were defined before) and current on its side only
+// (compiler/runtime/scoped_regions.hpp:219-225). Original host work defines and makes current only what
+// it proves to overwrite, and takes the device copy of everything it may write as outdated (:230-236).
+static void commit(Buffer &b, const std::vector<Box> &writes, const std::vector<Box> &overwrites, bool device,
+                   bool original = false)
 {
     Region &mine = device ? b.dev_cur : b.host_cur;
     Region &other = device ? b.host_cur : b.dev_cur;
     for (const Box &o : overwrites) require(b.init.add(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
     for (const Box &w : writes) {
-        // A partial write changes only defined values or values it defines itself.
-        require(mine.add(w), FORT_SCOPE_RESOURCE, "region limit exceeded");
+        if (!original) {
+            require(b.init.add(w), FORT_SCOPE_RESOURCE, "region limit exceeded");
+            require(mine.add(w), FORT_SCOPE_RESOURCE, "region limit exceeded");
+        }
         require(other.remove(w), FORT_SCOPE_RESOURCE, "region limit exceeded");
     }
     for (const Box &o : overwrites) {
         require(mine.add(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
         require(other.remove(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
     }
-    // What is current where must stay within what is defined.
-    Region keep;
-    for (const Box &i : b.init.boxes)
-        for (const Box &x : mine.within(i)) require(keep.add(x), FORT_SCOPE_RESOURCE, "region limit exceeded");
-    mine = keep;
 }
 
 static void prepare_device(Context &c, const fort_scope_plan_binding *bindings, std::size_t count, void **devices,
@@ -748,9 +759,10 @@ static void prepare_device(Context &c, const fort_scope_plan_binding *bindings, 
 static void end_access(Context &c, Buffer &b, bool device, bool cancel)
 {
     require(b.prepared == (device ? 1 : 2), FORT_SCOPE_STATE, "no prepared access of this kind on the buffer");
-    if (!cancel) commit(b, b.prep_writes, b.prep_overwrites, device);
+    if (!cancel) commit(b, b.prep_writes, b.prep_overwrites, device, !device && b.prep_original);
     if (!cancel && device) c.domain->stats.launches++;
     b.prepared = 0;
+    b.prep_original = false;
     b.prep_writes.clear();
     b.prep_overwrites.clear();
 }
@@ -764,11 +776,13 @@ static void download(Context &c, Buffer &b, const std::vector<Box> &boxes)
     for (const Box &box : boxes) require(b.host_cur.add(box), FORT_SCOPE_RESOURCE, "region limit exceeded");
 }
 
+// Publication: every value the buffer physically holds and the host does not (compiler/runtime/
+// scoped_runtime.cu:1410-1413).
 static void observe(Context &c, Buffer &b)
 {
     drain_buffer(c, b);
     std::vector<Box> need;
-    for (const Box &i : b.init.boxes)
+    for (const Box &i : physical(b).boxes)
         for (const Box &m : b.host_cur.missing(i)) need.push_back(m);
     download(c, b, need);
 }
@@ -1184,14 +1198,15 @@ int fort_scope_domain_close_v1(fort_scope_domain_t domain)
         Domain &d = domain_of(domain);
         require(d.borrower == nullptr, FORT_SCOPE_STATE, "a context still borrows the domain");
         require(d.workspace.empty(), FORT_SCOPE_STATE, "workspace leases are outstanding");
-        // Publication frontier: every defined value reaches the host before storage is released.
+        // Publication frontier: every value a buffer physically holds reaches the host before storage is
+        // released.
         if (!d.failed && !d.buffers.empty()) {
             Context tmp;
             tmp.domain = &d;
             for (auto &kv : d.buffers) {
                 Buffer &b = *kv.second;
                 std::vector<Box> need;
-                for (const Box &i : b.init.boxes)
+                for (const Box &i : physical(b).boxes)
                     for (const Box &m : b.host_cur.missing(i)) need.push_back(m);
                 if (!need.empty()) {
                     ensure_stream(tmp);
@@ -1454,11 +1469,10 @@ int fort_scope_forget_definition(fort_scope_t h, fort_buffer_t handle)
     return with(h, [&](Context &c) {
         Buffer &b = buffer(c, handle);
         require(!b.prepared, FORT_SCOPE_STATE, "buffer has a prepared access");
-        // Semantic definitions go; queued tasks that write the buffer keep their effects because the
-        // coherence state already reflects them at acceptance.
+        // Semantic definitions go; the physical currency stays, so values already written are still
+        // published (compiler/runtime/scoped_runtime.cu:1901-1909). Queued tasks that write the buffer
+        // keep their effects because the coherence state already reflects them at acceptance.
         b.init.clear();
-        b.host_cur.clear();
-        b.dev_cur.clear();
     });
 }
 
@@ -1539,6 +1553,7 @@ int fort_scope_original_host_begin_v1(fort_scope_t h, const fort_scope_plan_bind
             if (!downloads[i].empty()) ensure_stream(c);
             download(c, *list[i], downloads[i]);
             list[i]->prepared = 2;
+            list[i]->prep_original = true;
             list[i]->prep_writes = eff[i].writes;
             list[i]->prep_overwrites = eff[i].overwrites;
         }
@@ -1757,8 +1772,11 @@ int fort_scope_admit_source_reads_v1(fort_scope_t h, fort_buffer_t handle, const
         for (std::size_t i = 0; i < count; ++i) {
             const Box box = section_box(b, sections[i]);
             for (const Box &fresh : b.init.missing(box)) {
-                require(init.add(fresh) && host.add(fresh), FORT_SCOPE_BOUNDARY,
-                        "admitted reads exceed the region limit");
+                require(init.add(fresh), FORT_SCOPE_BOUNDARY, "admitted reads exceed the region limit");
+                // A definition kill does not erase a newer physical device value: only values with no
+                // tracked copy take the host's (compiler/runtime/scoped_regions.hpp:205-216).
+                for (const Box &untracked : b.dev_cur.missing(fresh))
+                    require(host.add(untracked), FORT_SCOPE_BOUNDARY, "admitted reads exceed the region limit");
             }
         }
         b.init = init;
@@ -2077,13 +2095,17 @@ int fort_scope_graph_execute_v1(fort_scope_t h, uint64_t group_key, const fort_s
                     order.push_back(&b);
                 }
                 Effects e = effects(b, &t.bindings[j].access);
-                // Reads must be defined by the state before the group or by an earlier task of it.
-                for (const Box &r : e.reads)
+                // Reads, and what partial writes preserve, must be defined by the state before the group
+                // or by an earlier task of it.
+                std::vector<Box> touched = e.reads;
+                Region over;
+                for (const Box &o : e.overwrites) require(over.add(o), FORT_SCOPE_RESOURCE, "region limit exceeded");
+                for (const Box &w : e.writes)
+                    for (const Box &kept : over.missing(w)) touched.push_back(kept);
+                for (const Box &r : touched)
                     require(defined[&b].covers(r), FORT_SCOPE_UNINITIALIZED,
                             "a task of the group reads values that are not defined before it");
                 // Values defined before the group that the device lacks and no earlier task produced go up.
-                std::vector<Box> touched = e.reads;
-                touched.insert(touched.end(), e.writes.begin(), e.writes.end());
                 for (const Box &box : touched)
                     for (const Box &d0 : b.init.within(box))
                         for (const Box &m : b.dev_cur.missing(d0))

@@ -827,3 +827,86 @@ def test_group_execution_simulates_effects_in_order(scope, cpu_runtime):
         assert lib.fort_scope_graph_execute_v1(dom.ctx, 9, None, 0) == scope.OK  # an empty group
         assert lib.fort_scope_graph_submit_v1(dom.ctx, 1, None, 0) == scope.OK
         assert lib.fort_scope_graph_submit_v1(dom.ctx, 1, ctypes.byref(produce.kernel), 1) == scope.RESOURCE
+
+
+def test_a_definition_kill_keeps_the_physical_values_for_publication(scope, cpu_runtime):
+    # compiler/tests/test_scoped_runtime.py:509-531: forgetting a definition refuses later reads but keeps
+    # the device's values as the current physical copy, so closing still publishes them.
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        every = scope.WRITE_ALL | scope.OVERWRITE_ALL
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, flags=every)])
+        scope.check(lib, st)
+        dev_view(devs[0], (8,))[:] = 1000.0 + np.arange(8)
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        scope.check(lib, lib.fort_scope_forget_definition(dom.ctx, h))
+        assert dom.stats().downloads == 0
+        assert host_read(scope, lib, dom, h, flags=scope.READ_ALL) == scope.UNINITIALIZED
+        middle = ((2,), (6,))
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, writes=[middle], overwrites=[middle])])
+        scope.check(lib, st)
+        dev_view(devs[0], (8,))[2:6] = 2000.0 + np.arange(2, 6)
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        assert dom.stats().allocations == 1
+    assert list(a) == [1000, 1001, 2002, 2003, 2004, 2005, 1006, 1007]
+
+
+def test_a_definition_kill_cannot_lose_an_executed_device_prefix(scope, cpu_runtime):
+    # compiler/tests/test_scoped_runtime.py:533-560
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        written = ((1,), (7,))
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, writes=[written], overwrites=[written])])
+        scope.check(lib, st)
+        dev_view(devs[0], (8,))[1:7] = 3000.0 + np.arange(1, 7)
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        before = dom.stats()
+        scope.check(lib, lib.fort_scope_forget_definition(dom.ctx, h))
+        after = dom.stats()
+        assert (after.download_bytes, after.waits) == (before.download_bytes, before.waits)
+        assert host_read(scope, lib, dom, h, reads=[((3,), (4,))]) == scope.UNINITIALIZED
+    assert list(a) == [0, 3001, 3002, 3003, 3004, 3005, 3006, 7]
+
+
+def test_a_partial_write_needs_the_values_it_preserves(scope, cpu_runtime):
+    # The cells a write may leave as they were must be defined (compiler/runtime/scoped_runtime.cu:925-928),
+    # on the device and on the host alike; what it overwrites needs nothing.
+    lib = cpu_runtime
+    a = np.zeros(8)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a, initialized=False)
+        partial = binding(scope, h, writes=[((0,), (8,))], overwrites=[((2,), (6,))])
+        st, _ = begin_device(scope, lib, dom, [partial])
+        assert st == scope.UNINITIALIZED
+        access, _keep = scope.make_access(0, (), [((0,), (8,))], [((2,), (6,))])
+        assert lib.fort_scope_host_begin(dom.ctx, h, byref(access)) == scope.UNINITIALIZED
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, writes=[((2,), (6,))], overwrites=[((2,), (6,))])])
+        scope.check(lib, st)
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, writes=[((2,), (6,))])])
+        scope.check(lib, st)  # now defined: a partial write inside it is accepted
+        scope.check(lib, lib.fort_scope_cancel_access(dom.ctx, h))
+
+
+def test_admitted_reads_after_a_kill_keep_the_newer_device_values(scope, cpu_runtime):
+    # compiler/runtime/scoped_regions.hpp:205-216: a value the device physically holds keeps that copy;
+    # only values no copy tracks take the host's.
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        part = ((0,), (4,))
+        st, devs = begin_device(scope, lib, dom, [binding(scope, h, writes=[part], overwrites=[part])])
+        scope.check(lib, st)
+        dev_view(devs[0], (8,))[0:4] = 50.0
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+        scope.check(lib, lib.fort_scope_forget_definition(dom.ctx, h))
+        scope.check(lib, admit(scope, lib, dom, h, [((0,), (8,))]))
+        scope.check(lib, host_read(scope, lib, dom, h, flags=scope.READ_ALL))
+        assert list(a) == [50, 50, 50, 50, 4, 5, 6, 7]
+        assert dom.stats().download_bytes == 4 * 8
+        scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))

@@ -299,11 +299,12 @@ def plan(description: dict, texts: dict[str, str]) -> dict:
     return {"spliced": spliced, "native": native}
 
 
-def generate(toml_path: Path, out_dir: Path) -> tuple[list[Path], dict]:
+def generate(toml_path: Path, out_dir: Path, source_root: Path | None = None) -> tuple[list[Path], dict]:
     """Writes the application's sources, edited, spliced and with directives expanded, into out_dir;
-    returns them in compilation order and the plan."""
+    returns them in compilation order and the plan. Sources are found relative to `source_root`, by
+    default the description's directory."""
     description = load_description(toml_path)
-    root = toml_path.parent
+    root = source_root or toml_path.parent
     sources = [root / s for s in description["sources"]]
     texts = {p.name: p.read_text() for p in sources}
     for number, edit in enumerate(description.get("edit", ()), 1):
@@ -326,7 +327,7 @@ def generate(toml_path: Path, out_dir: Path) -> tuple[list[Path], dict]:
 
 
 def build(toml_path: Path, executable: Path, layer: Path, library: str = "ftc_hip", variant: str = "default",
-          defines: tuple[str, ...] = (), keep: Path | None = None) -> dict:
+          defines: tuple[str, ...] = (), keep: Path | None = None, source_root: Path | None = None) -> dict:
     """Generates the sources and builds `executable` against the host layer in `layer` (objects, modules
     and the compiler's path in `layer/compiler`) and lib<library>.so of this package."""
     fc = (layer / "compiler").read_text().strip()
@@ -335,7 +336,7 @@ def build(toml_path: Path, executable: Path, layer: Path, library: str = "ftc_hi
     flags = ["-cpp", "-O3", "-fopenmp", "-ffp-contract=off", *(f"-D{d}" for d in defs)]
     with tempfile.TemporaryDirectory(prefix="ftc_app_") as tmp:
         work = keep or Path(tmp)
-        sources, chosen = generate(toml_path, work / "src")
+        sources, chosen = generate(toml_path, work / "src", source_root)
         objects = []
         run = lambda cmd: subprocess.run(cmd, cwd=work, check=True, capture_output=True, text=True)
         for src in [FORTRAN / "ftc_entries.f90", FORTRAN / "app" / "ftc_resident.f90", *sources]:
@@ -363,12 +364,16 @@ def main(argv=None) -> int:
     g = sub.add_parser("generate", help="write the generated sources only")
     g.add_argument("description", type=Path)
     g.add_argument("out", type=Path)
+    for p in (b, g):
+        p.add_argument("--source-root", type=Path, default=None,
+                       help="where the description's sources are (default: its own directory)")
     args = ap.parse_args(argv)
+    root = args.source_root.resolve() if args.source_root else None
     if args.command == "build":
         chosen = build(args.description.resolve(), args.executable.resolve(), args.layer.resolve(), args.library,
-                       args.variant, tuple(args.defines))
+                       args.variant, tuple(args.defines), source_root=root)
     else:
-        _, chosen = generate(args.description.resolve(), args.out)
+        _, chosen = generate(args.description.resolve(), args.out, root)
     for name, qualified in chosen["spliced"]:
         print(f"entry   {qualified} ({name})")
     for name, qualified in chosen["native"]:

@@ -6,7 +6,10 @@ On the CPU: the edits, the directive expansion and the procedure splice on a sma
 file's own, compiled and run with the domain left inactive, so that every spliced block takes its
 original statements; and the refusals. On the GPU: the reference's own CDU application
 (benchmarks/cdu/cdu.toml), whose resident executable `make -C oracle apps` builds next to the native
-one, must print the native checksum of U2 digit for digit at 64^3 and at a ragged 70 x 37 x 19.
+one, must print the native checksum of U2 digit for digit at 64^3 and at a ragged 70 x 37 x 19; and the
+reference's three stencil correctness drivers (compiler/tests/fixtures/stencils/*_main.f90, described by
+oracle/apps/stencil_*.toml) must dump the interior of their output exactly as their native builds do, at
+16^3 and at 23 x 9 x 5.
 """
 import subprocess
 from pathlib import Path
@@ -169,3 +172,24 @@ def test_the_reference_cdu_application_prints_the_native_checksum(resident, nati
     for key in ("grid", "iterations", "warmup_iters", "checksum"):
         assert got[key].split() == want[key].split(), key
     print(f"{resident}: ms_per_iter resident {got['ms_per_iter'].split()[0]}, native {want['ms_per_iter'].split()[0]}")
+
+
+def run_output(path):
+    if not path.exists():
+        pytest.skip(f"{path.name} is not built (make -C oracle apps)")
+    out = subprocess.run([str(path)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    return out.stdout.split()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stencil", ["cdu", "cvd", "cdw"])
+@pytest.mark.parametrize("size", ["16", "ragged"])
+def test_the_reference_stencil_drivers_dump_the_native_values(stencil, size):
+    if size == "16":
+        got, want = run_output(APPS / f"{stencil}_main"), run_output(ROOT / "oracle" / "_ref" / f"{stencil}_main")
+        assert len(want) == 16 ** 3
+    else:
+        got, want = run_output(APPS / f"{stencil}_main_ragged"), run_output(APPS / f"{stencil}_main_native_ragged")
+        assert len(want) == 23 * 9 * 5
+    assert got == want  # g0.17, one value a line: the device values equal the native ones to the last digit

@@ -12,6 +12,10 @@ tests/fortran/resident_host.f90 runs on the CPU backend; tests/fortran/resident_
 LES step and chain contraction entries through the interface modules this library ships
 (fortran-to-cuda_amd/fortran/ftc_entries.f90) as spliced application code calls them, and exchanges its
 arrays as field dumps, which are compared with the oracle (bit for bit) and with numpy's FP64 products.
+tests/fortran/resident_pressure.f90 runs ELMM's own resident pressure state
+(benchmarks/elmm/sources/resident_pressure_state.f90 and gpu_pressure.f90, compiled against this library's
+poisson_solver_device module in place of the absent poisson-solver submodule's) on this library's
+pressure provider, against the oracle's direct solve.
 """
 import subprocess
 from pathlib import Path
@@ -29,8 +33,9 @@ PACKAGE = ROOT / "fortran-to-cuda_amd"
 fd = graft._module("fielddumps")
 
 
-def build(tmp_path, source, library):
-    """Compiles `source` against the layer's modules and links it to `library` (ftc_cpu or ftc_hip)."""
+def build(tmp_path, source, library, extra=()):
+    """Compiles `source` against the layer's modules and links it to `library` (ftc_cpu or ftc_hip), with
+    the layer objects `extra` besides the two host-layer modules."""
     compiler_file = LAYER / "compiler"
     if not (LAYER / "resident_domain.o").exists() or not compiler_file.exists():
         pytest.skip("the reference's Fortran host layer is not built (make -C oracle resident)")
@@ -48,7 +53,11 @@ def build(tmp_path, source, library):
              "-o", str(obj)])
         objects.append(str(obj))
     binary = tmp_path / source.stem
-    run([fc, "-fopenmp", *objects, str(LAYER / "scoped_memory.o"), str(LAYER / "resident_domain.o"),
+    for name in extra:
+        if not (LAYER / name).exists():
+            pytest.skip(f"{name} is not built (make -C oracle resident)")
+    run([fc, "-fopenmp", *objects, *(str(LAYER / name) for name in extra), str(LAYER / "scoped_memory.o"),
+         str(LAYER / "resident_domain.o"),
          "-L", str(PACKAGE), f"-l{library}", f"-Wl,-rpath,{PACKAGE}", "-o", str(binary)])
     return binary
 
@@ -152,3 +161,29 @@ def test_chain_contraction_called_from_fortran(entries_program, tmp_path, n, pre
     D = values.reshape((n, n), order="F")
     want = A @ B @ C
     assert np.linalg.norm(D - want) <= tol * np.linalg.norm(want)
+
+
+@pytest.fixture(scope="module")
+def pressure_program(tmp_path_factory):
+    return build(tmp_path_factory.mktemp("resident_pressure"), SOURCES / "resident_pressure.f90", "ftc_hip",
+                 extra=("poisson_solver_device.o", "gpu_pressure.o", "resident_pressure_state.o"))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flags,tol", [(0, 1e-11), (2, 1e-10)])
+def test_elmm_resident_pressure_state_solves_on_this_library(pressure_program, tmp_path, flags, tol):
+    from oracle import poisson as poisson_oracle
+
+    n, h = (37, 29, 17), (10.0, 10.0, 10.0)
+    b = poisson_oracle.divergence_rhs(*n, h, seed=11)
+    rhs = np.zeros(tuple(m + 2 for m in n), order="F")
+    rhs[1:-1, 1:-1, 1:-1] = b
+    fd.write_dump(tmp_path / "in", "RHS", rhs, lower_bounds=(0, 0, 0))
+    run_entries(pressure_program, tmp_path, *n, repr(h[0]), flags)
+    (values, layout), = fd.read_dumps(tmp_path / "out", np.float64, 1).values()
+    assert layout["lower_bounds"] == [-1, -1, -1] and layout["shape"] == [m + 4 for m in n]
+    phi = values.reshape(layout["shape"], order="F")
+    assert phi[0, 0, 0] == 123.0  # the ghost the driver planted is left alone
+    want = poisson_oracle.solve(b, h)
+    err = np.abs(phi[2:-2, 2:-2, 2:-2] - want).max() / np.abs(want).max()
+    assert err <= tol, err

@@ -1,4 +1,4 @@
-! Fortran interfaces of the C-ABI entries in include/ftc_les.h, ftc_dense.h and ftc_reduce.h.
+! Fortran interfaces of the C-ABI entries in include/ftc_les.h, ftc_dense.h, ftc_reduce.h and ftc_poisson.h.
 !
 ! One module per entry, each exporting `run`, in the shape the reference compiler gives its generated
 ! interfaces (compiler/emission/cuda/scoped.py:514-522): the context and the mode by value, array handles
@@ -142,3 +142,41 @@ module fort_shared_fixed_order_reduction
     end function
   end interface
 end module fort_shared_fixed_order_reduction
+
+module fort_shared_pressure_divergence
+  use iso_c_binding
+  implicit none
+  private
+  public :: run
+  interface
+    function run(fort_context, fort_mode, rhs_handle, u_handle, v_handle, w_handle, dx, dy, dz, dt, nx, ny, nz) &
+        bind(C, name='cpp_fort_shared_pressure_divergence') result(fort_status)
+      import :: c_int, c_int64_t, c_double
+      integer(c_int) :: fort_status
+      integer(c_int64_t), value :: fort_context
+      integer(c_int), value :: fort_mode
+      integer(c_int64_t), value :: rhs_handle, u_handle, v_handle, w_handle
+      real(c_double), intent(in) :: dx, dy, dz, dt
+      integer(c_int), intent(in) :: nx, ny, nz
+    end function
+  end interface
+end module fort_shared_pressure_divergence
+
+module fort_shared_pressure_projection
+  use iso_c_binding
+  implicit none
+  private
+  public :: run
+  interface
+    function run(fort_context, fort_mode, u_handle, v_handle, w_handle, phi_handle, dx, dy, dz, dt, nx, ny, nz) &
+        bind(C, name='cpp_fort_shared_pressure_projection') result(fort_status)
+      import :: c_int, c_int64_t, c_double
+      integer(c_int) :: fort_status
+      integer(c_int64_t), value :: fort_context
+      integer(c_int), value :: fort_mode
+      integer(c_int64_t), value :: u_handle, v_handle, w_handle, phi_handle
+      real(c_double), intent(in) :: dx, dy, dz, dt
+      integer(c_int), intent(in) :: nx, ny, nz
+    end function
+  end interface
+end module fort_shared_pressure_projection

@@ -98,6 +98,24 @@ const char *elmm_gpu_pressure_error_v1(void);                               /* g
 /* The provider behind an adapter handle, for refinement controls and reports (no reference counterpart). */
 poisson_solver_handle_v1 *elmm_gpu_pressure_provider_v1(elmm_gpu_pressure_handle_v1 *handle);
 
+/* The nests around the solve of one domain (ELMM's prepoisson and postpoisson, benchmarks/elmm/elmm.toml:212-225,
+ * whose sources are absent; the arithmetic is oracle/step.c's). U, V, W and RHS have extents (n+2), Phi (n+4),
+ * column-major, as benchmarks/elmm/sources/gpu_pressure.h:8-12 lays them out.
+ *   divergence:  RHS = ((dU/dx + dV/dy) + dW/dz) / dt on the interior cells (backward differences); RHS's
+ *                ghosts are not written.
+ *   projection:  U -= dt/dx (phi(i+1) - phi(i)), V likewise, W on the interior faces k < nz, with phi periodic
+ *                in x and y (its own cells, not its ghosts); then the halo images of U, V, W as the LES step
+ *                writes them (periodic x, y; U, V mirrored with a sign change across the z walls; W's walls
+ *                not written). */
+int cpp_fort_shared_pressure_divergence(fort_scope_t ctx, int mode, fort_buffer_t rhs, fort_buffer_t u,
+                                        fort_buffer_t v, fort_buffer_t w, const double *dx, const double *dy,
+                                        const double *dz, const double *dt, const int *nx, const int *ny,
+                                        const int *nz);
+int cpp_fort_shared_pressure_projection(fort_scope_t ctx, int mode, fort_buffer_t u, fort_buffer_t v,
+                                        fort_buffer_t w, fort_buffer_t phi, const double *dx, const double *dy,
+                                        const double *dz, const double *dt, const int *nx, const int *ny,
+                                        const int *nz);
+
 #ifdef __cplusplus
 }
 #endif

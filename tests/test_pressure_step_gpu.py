@@ -1,0 +1,117 @@
+"""The nests around the pressure solve of one domain (include/ftc_poisson.h: cpp_fort_shared_pressure_divergence
+and cpp_fort_shared_pressure_projection, ELMM's prepoisson and postpoisson) against the oracle
+(oracle/step.c orc_divergence, orc_projection), bit for bit, halos included; and a whole resident step
+in ELMM's order (LES step, divergence, pressure solve, projection, all on the device in one epoch)
+against the oracle's projected step (oracle/binding.py projected_step, whose solve is the FP64 direct
+one of oracle/poisson.py): the FP64 provider within 1e-11 and the mixed one within 1e-10 of max|field|.
+"""
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(40, 24, 12), (33, 17, 9), (64, 64, 32)]
+
+
+def with_ghosts(inner, ghosts):
+    out = np.zeros(tuple(n + 2 * ghosts for n in inner.shape), order="F")
+    out[ghosts:-ghosts, ghosts:-ghosts, ghosts:-ghosts] = inner
+    return out
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_divergence_matches_the_oracle_bit_for_bit(ftc, scope, hip, oracle, fields, n):
+    nx, ny, nz = n
+    dx, dt = 10.0, 0.2
+    U, V, W, _ = fields.les_fields(nx, ny, nz)
+    rhs = np.full((nx + 2, ny + 2, nz + 2), 7.0, order="F")  # the ghosts are not written
+    with scope.Domain(hip) as dom:
+        hr = dom.bind(rhs, initialized=False)
+        hu, hv, hw = (dom.bind(x.copy(order="F")) for x in (U, V, W))
+        scope.check(hip, hip.cpp_fort_shared_pressure_divergence(
+            dom.ctx, 1, hr, hu, hv, hw, ftc.dp(dx), ftc.dp(dx), ftc.dp(dx), ftc.dp(dt),
+            ftc.ip(nx), ftc.ip(ny), ftc.ip(nz)))
+        dom.flush()
+        dom.observe(hr)
+    want = oracle.divergence(U, V, W, dx, dx, dx, dt)
+    assert bits_equal(rhs[1:-1, 1:-1, 1:-1], want)
+    ghosts = np.ones(rhs.shape, bool)
+    ghosts[1:-1, 1:-1, 1:-1] = False
+    assert (rhs[ghosts] == 7.0).all()
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_projection_matches_the_oracle_bit_for_bit_including_halos(ftc, scope, hip, oracle, fields, n):
+    nx, ny, nz = n
+    dx, dt = 10.0, 0.2
+    U, V, W, _ = fields.les_fields(nx, ny, nz, seed=5)
+    phi = np.asfortranarray(np.random.default_rng(sum(n)).uniform(-1, 1, n))
+    got = [U.copy(order="F"), V.copy(order="F"), W.copy(order="F")]
+    with scope.Domain(hip) as dom:
+        hs = [dom.bind(x) for x in got]
+        hp = dom.bind(with_ghosts(phi, 2))
+        scope.check(hip, hip.cpp_fort_shared_pressure_projection(
+            dom.ctx, 1, *hs, hp, ftc.dp(dx), ftc.dp(dx), ftc.dp(dx), ftc.dp(dt), ftc.ip(nx), ftc.ip(ny), ftc.ip(nz)))
+        dom.flush()
+        dom.observe(*hs)
+    want = [U.copy(order="F"), V.copy(order="F"), W.copy(order="F")]
+    oracle.projection(*want, phi, dx, dx, dx, dt)
+    for name, g, w in zip("UVW", got, want):
+        assert bits_equal(g, w), name
+
+
+def test_pressure_step_entries_refuse_wrong_layouts(ftc, scope, hip):
+    a = np.zeros((6, 5, 4), order="F")
+    b = np.zeros((7, 5, 4), order="F")
+    with scope.Domain(hip) as dom:
+        h = [dom.bind(x.copy(order="F")) for x in (a, a, a, a)]
+        hb = dom.bind(b)
+        args = (ftc.dp(1.0), ftc.dp(1.0), ftc.dp(1.0), ftc.dp(0.1), ftc.ip(4), ftc.ip(3), ftc.ip(2))
+        assert hip.cpp_fort_shared_pressure_divergence(dom.ctx, 1, hb, h[1], h[2], h[3], *args) == scope.BOUNDARY
+        assert hip.cpp_fort_shared_pressure_projection(dom.ctx, 1, h[0], h[1], h[2], h[3], *args) == scope.BOUNDARY
+        assert hip.cpp_fort_shared_pressure_divergence(dom.ctx, 1, h[0], h[0], h[2], h[3], *args) == scope.ALIAS
+        assert hip.cpp_fort_shared_pressure_divergence(dom.ctx, 0, *h, *args) == scope.DECLINED
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("n", [(48, 40, 24), (37, 29, 17)])
+def test_a_resident_projected_step_matches_the_oracle(ftc, scope, hip, oracle, fields, mixed, n):
+    from __graft_entry__ import _module
+
+    pressure = _module("pressure")
+    nx, ny, nz = n
+    dx, dt, cs = 10.0, 0.2, 0.17
+    U, V, W, S = fields.les_fields(nx, ny, nz, seed=9)
+    src = [U.copy(order="F"), V.copy(order="F"), W.copy(order="F"), S.copy(order="F")]
+    dst = [np.zeros_like(U, order="F") for _ in range(4)]
+    rhs = np.zeros_like(U, order="F")
+    phi = np.zeros((nx + 4, ny + 4, nz + 4), order="F")
+    sc = (ftc.dp(dx), ftc.dp(dx), ftc.dp(dx), ftc.dp(dt))
+    with scope.Domain(hip) as dom:
+        hs = [dom.bind(x) for x in src]
+        hd = [dom.bind(x, initialized=True) for x in dst]
+        hr = dom.bind(rhs, lower=(0, 0, 0), initialized=False)
+        hp = dom.bind(phi, lower=(-1, -1, -1))
+        solver = pressure.PressureSolver(hip, dom, n, (nx * dx, ny * dx, nz * dx), mixed)
+        scope.check(hip, hip.cpp_fort_shared_les_step(dom.ctx, 1, *hd, *hs, *sc, ftc.dp(cs), ftc.ip(nx), ftc.ip(ny),
+                                                       ftc.ip(nz), ftc.ip(1)))
+        scope.check(hip, hip.cpp_fort_shared_pressure_divergence(dom.ctx, 1, hr, *hd[:3], *sc, ftc.ip(nx), ftc.ip(ny),
+                                                                 ftc.ip(nz)))
+        ext = solver.solve(hp, hr)
+        assert ext.started == 1 and ext.committed == 1
+        scope.check(hip, hip.cpp_fort_shared_pressure_projection(dom.ctx, 1, *hd[:3], hp, *sc, ftc.ip(nx),
+                                                                 ftc.ip(ny), ftc.ip(nz)))
+        dom.flush()
+        dom.observe(*hd)
+        stats = dom.stats()
+        solver.destroy()
+    want = oracle.projected_step(U, V, W, S, dx, dx, dx, dt, cs)
+    tol = 1e-10 if mixed else 1e-11
+    for name, g, w in zip("UVW", dst[:3], want[:3]):
+        err = np.abs(g - w).max() / np.abs(w).max()
+        assert err <= tol, (name, err)
+    assert bits_equal(dst[3], want[3])  # the scalar is not projected
+    # Everything stayed on the device between the entries: only the inputs went up.
+    assert stats.upload_bytes <= 8 * (4 * U.size + dst[0].size * 4 + phi.size)

@@ -98,6 +98,64 @@ def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 1
     return out, sols, b
 
 
+def projected_step_section(ftc, scope, lib, fields, n=(512, 512, 128), steps=10, dx=10.0, dt=0.2, cs=0.17):
+    """A whole resident time step in ELMM's order (LES step, divergence into the solver's RHS, pressure
+    solve, projection), all entries on the device in one epoch, ping-ponging two field sets: the FP64
+    solve against the mixed one (three solves, no host synchronisation). Timed with HIP events over
+    `steps` steps after one warm-up step; both modes start from the same fields, and the field difference
+    between them is reported after warm-up + `steps` steps."""
+    pressure = graft._module("pressure")
+    nx, ny, nz = n
+    start = fields.les_fields(nx, ny, nz)
+    out, finals = {}, {}
+    sc = (ftc.dp(dx), ftc.dp(dx), ftc.dp(dx), ftc.dp(dt))
+    ext = (ftc.ip(nx), ftc.ip(ny), ftc.ip(nz))
+    for name, mixed in (("fp64", False), ("mixed", True)):
+        a = [x.copy(order="F") for x in start]
+        b = [np.zeros_like(start[0], order="F") for _ in range(4)]
+        rhs = np.zeros_like(start[0], order="F")
+        phi = np.zeros((nx + 4, ny + 4, nz + 4), order="F")
+        dom = scope.Domain(lib)
+        ha, hb = [dom.bind(x) for x in a], [dom.bind(x, initialized=True) for x in b]
+        hr, hp = dom.bind(rhs, lower=(0, 0, 0), initialized=False), dom.bind(phi, lower=(-1, -1, -1))
+        solver = pressure.PressureSolver(lib, dom, n, (nx * dx, ny * dx, nz * dx), mixed)
+        # A fixed number of solves and no residual norms: no host synchronisation inside the step.
+        solver.refinement(1e-12, 3 if mixed else 1, False)
+        stream = ctypes.c_void_p(dom.stream())
+        ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+        lib.ftc_event_create(ctypes.byref(ev0))
+        lib.ftc_event_create(ctypes.byref(ev1))
+
+        def step(k):
+            src, dst = (ha, hb) if k % 2 == 0 else (hb, ha)
+            scope.check(lib, lib.cpp_fort_shared_les_step(dom.ctx, 1, *dst, *src, *sc, ftc.dp(cs), *ext, ftc.ip(1)))
+            scope.check(lib, lib.cpp_fort_shared_pressure_divergence(dom.ctx, 1, hr, *dst[:3], *sc, *ext))
+            solver.solve(hp, hr)
+            scope.check(lib, lib.cpp_fort_shared_pressure_projection(dom.ctx, 1, *dst[:3], hp, *sc, *ext))
+            return dst
+
+        step(0)
+        dom.flush()
+        dom.wait()
+        lib.ftc_event_record(ev0, stream)
+        for k in range(1, steps + 1):
+            last = step(k)
+        dom.flush()
+        lib.ftc_event_record(ev1, stream)
+        ms = ctypes.c_float()
+        lib.ftc_event_elapsed_ms(ev0, ev1, ctypes.byref(ms))
+        dom.observe(*last[:3])
+        finals[name] = [x.copy() for x in ((b if steps % 2 == 0 else a)[:3])]
+        solver.destroy()
+        dom.close()
+        out[name] = {"ms_per_step": round(ms.value / steps, 4)}
+    out["speedup_mixed_over_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    out["max_rel_field_diff_mixed_vs_fp64"] = max(
+        float(np.abs(m - f).max() / np.abs(f).max()) for m, f in zip(finals["mixed"], finals["fp64"]))
+    out["steps"] = steps + 1
+    return out
+
+
 def chain_section(ftc, scope, lib, n=4096, reps=3):
     """D = A*B*C on n x n FP64 arrays, FP64 MFMA vs split-bf16 products, resident."""
     rng = np.random.default_rng(20261123)
@@ -285,6 +343,8 @@ def main() -> int:
         log("bench: Poisson 512x512x128, FP64 vs mixed")
         extras["poisson"], poisson_sols, poisson_rhs = poisson_section(ftc, scope, lib, fields)
         log("bench: chain contraction 4096")
+        log("projected step (LES step + divergence + solve + projection)")
+        extras["projected_step"] = projected_step_section(ftc, scope, lib, fields)
         extras["chain_contraction"], chain_results, chain_inputs = chain_section(ftc, scope, lib)
         log("bench: 64-member ensemble")
         extras["ensemble"] = ensemble_section(lib, fields)

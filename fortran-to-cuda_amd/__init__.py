@@ -33,6 +33,7 @@ ENTRY_SIGNATURES = {
     "cpp_fort_shared_chain_contraction": [c_uint64, c_int, _H, _H, _H, _H, _IP, _IP],
     "cpp_fort_shared_pressure_divergence": [c_uint64, c_int, _H, _H, _H, _H, _DP, _DP, _DP, _DP, _IP, _IP, _IP],
     "cpp_fort_shared_pressure_projection": [c_uint64, c_int, _H, _H, _H, _H, _DP, _DP, _DP, _DP, _IP, _IP, _IP],
+    "cpp_fort_shared_pressure_fix_reference": [c_uint64, c_int, _H, _DP, _IP, _IP, _IP],
     "cpp_fort_shared_fixed_order_reduction": [c_uint64, c_int, _H, _IP, POINTER(ctypes.c_longlong),
                                               POINTER(ctypes.c_longlong), _DP],
 }

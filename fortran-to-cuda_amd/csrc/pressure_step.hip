@@ -1,6 +1,7 @@
-// The two nests around the pressure solve of one domain, as entries: ELMM's prepoisson (the divergence of
-// the provisional velocity into the solver's right-hand side) and postpoisson (the projection
-// u -= dt grad(phi) with the halos), benchmarks/elmm/elmm.toml:212-225. Their sources are absent from the
+// The nests around the pressure solve of one domain, as entries: ELMM's prepoisson (the divergence of
+// the provisional velocity into the solver's right-hand side), postpoisson (the projection
+// u -= dt grad(phi) with the halos) and fixpressuretoreference (phi shifted by its reference-plane mean,
+// the mean being a fixed-order reduction), benchmarks/elmm/elmm.toml:212-225, 274-293. Their sources are absent from the
 // reference, so the arithmetic is this repository's definition, the one oracle/step.c (orc_divergence,
 // orc_projection) states and the ensemble's batched kernels use: the device results equal the oracle's
 // bit for bit.
@@ -64,6 +65,20 @@ __global__ __launch_bounds__(kPTX *kPTY) void pressure_projection_kernel(const P
         put_column(s.V, p, s.V[p] - s.gy * (s.phi[fy + o] - f), ci, zi, 0);
         if (c < nz) put_column(s.W, p, s.W[p] - s.gz * (s.phi[f0 + o + pk] - f), ci, zi, 1);
     }
+}
+
+// phi -= ref on every cell of the array (the solution is fixed only up to a constant, so its ghosts shift
+// with it).
+struct ShiftArgs {
+    double *phi;
+    long long n;  // cells of the array
+    double ref;
+};
+
+__global__ __launch_bounds__(256) void pressure_shift_kernel(const ShiftArgs a)
+{
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < a.n; i += (long long)gridDim.x * 256)
+        a.phi[i] = a.phi[i] - a.ref;
 }
 
 static Sec3 pbox(std::size_t i0, std::size_t i1, std::size_t j0, std::size_t j1, std::size_t k0, std::size_t k1)
@@ -169,6 +184,27 @@ int cpp_fort_shared_pressure_projection(fort_scope_t ctx, int mode, fort_buffer_
     const dim3 grid((unsigned)((X + kPTX - 1) / kPTX), (unsigned)((Y + kPTY - 1) / kPTY));
     return enqueue_kernel(ctx, 0xd1e0000000000002ull, (const void *)pressure_projection_kernel, grid,
                           dim3(kPTX, kPTY), 0, &s, sizeof s, b);
+}
+
+int cpp_fort_shared_pressure_fix_reference(fort_scope_t ctx, int mode, fort_buffer_t phi, const double *phi_ref,
+                                           const int *nx, const int *ny, const int *nz)
+{
+    FTC_CHECK(entry_preamble(mode));
+    if (!phi_ref || !nx || !ny || !nz) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "null shared scalar argument");
+    if (*nx <= 0 || *ny <= 0 || *nz <= 0) return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "empty pressure domain");
+    std::size_t e[3];
+    FTC_CHECK(layout3(ctx, phi, FORT_SCOPE_REAL64, 8, e));
+    if (e[0] != (std::size_t)*nx + 4 || e[1] != (std::size_t)*ny + 4 || e[2] != (std::size_t)*nz + 4)
+        return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "phi must have two ghost layers");
+    std::vector<void *> dev;
+    FTC_CHECK(reserve(ctx, {phi}, dev));
+    const ShiftArgs args{(double *)dev[0], (long long)(e[0] * e[1] * e[2]), *phi_ref};
+    const Sec3 all = pbox(0, e[0], 0, e[1], 0, e[2]);
+    std::vector<Binding3> b(1);
+    b[0] = {phi, {all}, {}, {all}};
+    const unsigned blocks = (unsigned)std::min<long long>(4096, (args.n + 255) / 256);
+    return enqueue_kernel(ctx, 0xd1e0000000000003ull, (const void *)pressure_shift_kernel, dim3(blocks), dim3(256), 0,
+                          &args, sizeof args, b);
 }
 
 }  // extern "C"

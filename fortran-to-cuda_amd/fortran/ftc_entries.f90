@@ -180,3 +180,22 @@ module fort_shared_pressure_projection
     end function
   end interface
 end module fort_shared_pressure_projection
+
+module fort_shared_pressure_fix_reference
+  use iso_c_binding
+  implicit none
+  private
+  public :: run
+  interface
+    function run(fort_context, fort_mode, phi_handle, phi_ref, nx, ny, nz) &
+        bind(C, name='cpp_fort_shared_pressure_fix_reference') result(fort_status)
+      import :: c_int, c_int64_t, c_double
+      integer(c_int) :: fort_status
+      integer(c_int64_t), value :: fort_context
+      integer(c_int), value :: fort_mode
+      integer(c_int64_t), value :: phi_handle
+      real(c_double), intent(in) :: phi_ref
+      integer(c_int), intent(in) :: nx, ny, nz
+    end function
+  end interface
+end module fort_shared_pressure_fix_reference

@@ -115,6 +115,12 @@ int cpp_fort_shared_pressure_projection(fort_scope_t ctx, int mode, fort_buffer_
                                         fort_buffer_t w, fort_buffer_t phi, const double *dx, const double *dy,
                                         const double *dz, const double *dt, const int *nx, const int *ny,
                                         const int *nz);
+/* ELMM's fixpressuretoreference: phi -= *phi_ref on every cell of Phi (extents n+4; the solution is fixed
+ * only up to a constant, so its ghosts shift with it). The caller forms *phi_ref as the mean of the
+ * reference plane, which templates/serial-pressure-reference.f90 licenses as a free-order sum on the
+ * device: cpp_fort_shared_fixed_order_reduction over the plane, divided by nx*ny. */
+int cpp_fort_shared_pressure_fix_reference(fort_scope_t ctx, int mode, fort_buffer_t phi, const double *phi_ref,
+                                           const int *nx, const int *ny, const int *nz);
 
 #ifdef __cplusplus
 }

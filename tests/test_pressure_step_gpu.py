@@ -1,6 +1,7 @@
-"""The nests around the pressure solve of one domain (include/ftc_poisson.h: cpp_fort_shared_pressure_divergence
-and cpp_fort_shared_pressure_projection, ELMM's prepoisson and postpoisson) against the oracle
-(oracle/step.c orc_divergence, orc_projection), bit for bit, halos included; and a whole resident step
+"""The nests around the pressure solve of one domain (include/ftc_poisson.h: cpp_fort_shared_pressure_divergence,
+cpp_fort_shared_pressure_projection and cpp_fort_shared_pressure_fix_reference, ELMM's prepoisson,
+postpoisson and fixpressuretoreference) against the oracle (oracle/step.c orc_divergence,
+orc_projection; oracle/reduce.c for the reference-plane sum), bit for bit, halos included; and a whole resident step
 in ELMM's order (LES step, divergence, pressure solve, projection, all on the device in one epoch)
 against the oracle's projected step (oracle/binding.py projected_step, whose solve is the FP64 direct
 one of oracle/poisson.py): the FP64 provider within 1e-11 and the mixed one within 1e-10 of max|field|.
@@ -115,3 +116,29 @@ def test_a_resident_projected_step_matches_the_oracle(ftc, scope, hip, oracle, f
     assert bits_equal(dst[3], want[3])  # the scalar is not projected
     # Everything stayed on the device between the entries: only the inputs went up.
     assert stats.upload_bytes <= 8 * (4 * U.size + dst[0].size * 4 + phi.size)
+
+
+@pytest.mark.parametrize("n", [(40, 24, 12), (33, 17, 9)])
+def test_fix_to_the_reference_plane_mean(ftc, scope, hip, oracle, n):
+    # ELMM's fixpressuretoreference (elmm.toml:274-293): the mean of Pr(1:nx, 1:ny, k_ref) as a fixed-order
+    # sum on the device, then phi -= that mean on every cell, bit for bit against the oracle's sum.
+    import ctypes
+
+    nx, ny, nz = n
+    phi = np.asfortranarray(np.random.default_rng(nx).uniform(-1, 1, (nx + 4, ny + 4, nz + 4)))
+    k_ref = nz // 2 + 1  # 1-based interior plane; the array index is k_ref + 1 with two ghosts below
+    lo, hi = (2, 2, k_ref + 1), (nx + 2, ny + 2, k_ref + 2)
+    got = phi.copy(order="F")
+    total = ctypes.c_double(0.0)
+    with scope.Domain(hip) as dom:
+        hp = dom.bind(got, lower=(-1, -1, -1))
+        scope.check(hip, hip.cpp_fort_shared_fixed_order_reduction(
+            dom.ctx, 1, hp, ftc.ip(0), (ctypes.c_longlong * 3)(*lo), (ctypes.c_longlong * 3)(*hi), ctypes.byref(total)))
+        ref = total.value / (nx * ny)
+        scope.check(hip, hip.cpp_fort_shared_pressure_fix_reference(dom.ctx, 1, hp, ftc.dp(ref), ftc.ip(nx),
+                                                                    ftc.ip(ny), ftc.ip(nz)))
+        dom.flush()
+        dom.observe(hp)
+    want_total = oracle.fixed_order_reduce(phi, lo, hi, 0, 0.0)
+    assert bits_equal(np.array([total.value]), np.array([want_total]))
+    assert bits_equal(got, phi - want_total / (nx * ny))

@@ -11,10 +11,12 @@ original statements when the domain is inactive, an array cannot be kept, or the
 What is read of the description (compiler/resident/application.py:10-75):
   name, sources, program          the sources in compilation order and the main program
   [variants.<v>] defines          preprocessor definitions of a variant
+  [[patch]], [[rewrite]]          edits of the sources before the entries are placed, in that order
   [[selection]] procedures        `module::procedure` patterns (`*` matches every name)
-  [[edit]] file, at, action, text edits after the entries are in place (before, after, replace with
-                                  {original}; `at` is found exactly once, whitespace-tolerant), which may
-                                  add residency directives
+  [[edit]]                        edits after the entries are in place, which may add residency
+                                  directives: file, optionally procedure (function = true), one of at,
+                                  between, call, module_end (or the whole procedure); action before, after
+                                  or replace ({original}); text or template
 Directives (compiler/resident/directives.py:85-130): enter [device(d)] [budget(b)] [if(c)], leave,
 close, publish [if(c)], observe(arrays).
 
@@ -84,16 +86,83 @@ def _unique_span(text: str, piece: str, what: str) -> tuple[int, int]:
     return found[0]
 
 
-def apply_edit(text: str, edit: dict, what: str) -> str:
-    """One [[edit]]: `at` located exactly once, then before, after or replace ({original} kept)."""
+def _routine_span(text: str, name: str, function: bool) -> tuple[int, int]:
+    """The span of `subroutine name` (or `function name`), header to END line inclusive."""
+    kind = "function" if function else "subroutine"
+    head = re.compile(rf"^[ \t]*(?=\S)(?!end\b)(?:[\w(),=* \t]*\s)?{kind}\s+{re.escape(name)}\b.*$",
+                      re.IGNORECASE | re.MULTILINE)
+    found = list(head.finditer(text))
+    if len(found) != 1:
+        raise ValueError(f"expected one {kind} {name}, found {len(found)}")
+    end = re.compile(rf"^[ \t]*end[ \t]*(?:{kind}(?:[ \t]+{re.escape(name)})?)?[ \t]*(?:!.*)?$",
+                     re.IGNORECASE | re.MULTILINE).search(text, found[0].end())
+    if end is None:
+        raise ValueError(f"no END of {kind} {name}")
+    stop = text.find("\n", end.end())
+    return found[0].start(), len(text) if stop < 0 else stop + 1
+
+
+def _call_span(text: str, name: str, low: int, high: int, what: str) -> tuple[int, int]:
+    """The span of the one CALL statement of `name` in text[low:high], continuation lines included."""
+    pattern = re.compile(rf"^[ \t]*call[ \t]+{re.escape(name)}\b", re.IGNORECASE | re.MULTILINE)
+    found = [m for m in pattern.finditer(text, low, high)]
+    if len(found) != 1:
+        raise ValueError(f"{what}: expected one call of {name}, found {len(found)}")
+    start = pos = found[0].start()
+    while True:
+        stop = text.find("\n", pos)
+        stop = len(text) if stop < 0 else stop + 1
+        if not text[pos:stop].split("!", 1)[0].rstrip().endswith("&"):
+            return start, stop
+        pos = stop
+
+
+def apply_edit(text: str, edit: dict, what: str, root: Path | None = None) -> str:
+    """One [[edit]] (compiler/resident/application.py:49-58, 262-290): confined to `procedure` (a
+    `function` when `function = true`) if given, the place is `at` (a piece of text found exactly once,
+    whitespace-tolerant), `between` (from the start of the first piece to the end of the second), `call`
+    (the CALL statement of that procedure), `module_end` (before END MODULE of that module) or, with none of
+    them, the whole procedure; `action` is before, after or replace, and `text` or `template` (a file under
+    `root`) is what goes there, a replacement's {original} standing for what it replaces."""
     action = edit.get("action")
     if action not in ("before", "after", "replace"):
         raise ValueError(f"{what}: an edit needs an action: before, after or replace")
-    unsupported = set(edit) - {"file", "at", "action", "text"}
-    if unsupported or "at" not in edit:
-        raise ValueError(f"{what}: only `at` edits are supported here (found {sorted(unsupported) or 'no at'})")
-    start, end = _unique_span(text, edit["at"], what)
-    insert = edit.get("text", "")
+    unknown = set(edit) - {"file", "procedure", "function", "action", "text", "template", "at", "between", "call",
+                           "module_end"}
+    if unknown:
+        raise ValueError(f"{what}: unknown edit keys {sorted(unknown)}")
+    locators = [k for k in ("at", "between", "call", "module_end") if k in edit]
+    if len(locators) > 1:
+        raise ValueError(f"{what}: one of at, between, call, module_end")
+    if "template" in edit:
+        if "text" in edit or root is None:
+            raise ValueError(f"{what}: a template needs the description's directory and no text")
+        insert = (root / edit["template"]).read_text()
+    else:
+        insert = edit.get("text", "")
+    low, high = 0, len(text)
+    if "procedure" in edit:
+        low, high = _routine_span(text, edit["procedure"], bool(edit.get("function", False)))
+    region = text[low:high]
+    if "at" in edit:
+        start, end = _unique_span(region, edit["at"], what)
+        start, end = low + start, low + end
+    elif "between" in edit:
+        first, second = edit["between"]
+        start = _unique_span(region, first, what)[0]
+        tail = _unique_span(region[start:], second, what)[1]
+        start, end = low + start, low + start + tail
+    elif "call" in edit:
+        start, end = _call_span(text, edit["call"], low, high, what)
+    elif "module_end" in edit:
+        m = re.search(rf"(?im)^\s*end\s*module\s+{re.escape(edit['module_end'])}\b", text)
+        if m is None:
+            raise ValueError(f"{what}: no END MODULE {edit['module_end']}")
+        start = end = m.start()
+    elif "procedure" in edit:
+        start, end = low, high
+    else:
+        raise ValueError(f"{what}: an edit needs at, between, call, module_end or procedure")
     if action == "before":
         return text[:start] + insert + text[start:]
     if action == "after":
@@ -307,14 +376,25 @@ def generate(toml_path: Path, out_dir: Path, source_root: Path | None = None) ->
     root = source_root or toml_path.parent
     sources = [root / s for s in description["sources"]]
     texts = {p.name: p.read_text() for p in sources}
-    for number, edit in enumerate(description.get("edit", ()), 1):
-        if edit.get("file") not in texts:
-            raise ValueError(f"edit {number} names a file that is not a source: {edit.get('file')}")
-        texts[edit["file"]] = apply_edit(texts[edit["file"]], edit, f"edit {number} ({edit['file']})")
+
+    def edits(kind):
+        for number, edit in enumerate(description.get(kind, ()), 1):
+            if edit.get("file") not in texts:
+                raise ValueError(f"{kind} {number} names a file that is not a source: {edit.get('file')}")
+            texts[edit["file"]] = apply_edit(texts[edit["file"]], edit, f"{kind} {number} ({edit['file']})",
+                                             toml_path.parent)
+
+    # The order of compiler/resident/application.py:45-47: patches of the snapshot, rewrites of the sources
+    # before the compiler's passes, then, with the entry calls in place, the edits.
+    edits("patch")
+    edits("rewrite")
     chosen = plan(description, texts)
     for name, qualified in chosen["spliced"]:
         module, proc = qualified.split("::")
         texts[name] = splice_procedure(texts[name], ENTRIES[module, proc], qualified)
+    edits("edit")
+    # The compiler's own loop rewrites reshape nests it generates entries for; hand-written entries have none.
+    chosen["ignored_rewrites"] = {f: list(v) for f, v in description.get("rewrites", {}).items()}
     out_dir.mkdir(parents=True, exist_ok=True)
     written = []
     for p in sources:

@@ -149,6 +149,59 @@ def test_malformed_edits_and_directives_are_refused(tmp_path, change, message):
         app.generate(write_app(tmp_path, DESCRIPTION.replace(*change)), tmp_path / "gen")
 
 
+SOURCE = """module m
+contains
+  subroutine a(x)
+    real :: x
+    x = 1
+    call b(x, &
+           2)
+  end subroutine a
+  subroutine b(x, n)
+    real :: x
+    integer :: n
+    x = n
+    call c(x)
+  end subroutine b
+  real function f(x)
+    real :: x
+    f = x
+  end function f
+end module m
+"""
+
+
+def test_edit_locators(tmp_path):
+    # compiler/resident/application.py:49-58, 262-290: procedure confinement, at, between, call (a continued
+    # statement), module_end, the whole procedure, templates and {original}.
+    edit = lambda **e: app.apply_edit(SOURCE, {"file": "m.f90", **e}, "edit", tmp_path)
+    got = edit(procedure="b", at="x = n", action="replace", text="x = 2 * n")
+    assert "x = 2 * n" in got and "x = 1" in got
+    got = edit(call="b", action="before", text="    ! before\n")
+    assert got.index("! before") < got.index("call b(x, &") and got.count("call b") == 1
+    got = edit(call="b", action="after", text="    ! after\n")
+    assert got.index("           2)\n    ! after") > 0
+    got = edit(between=["x = 1", "2)"], action="replace", text="x = 3")
+    assert "call b(x" not in got and "    x = 3\n  end subroutine a" in got
+    got = edit(module_end="m", action="before", text="  ! last\n")
+    assert got.rstrip().endswith("! last\nend module m")
+    got = edit(procedure="f", function=True, action="replace", text="! gone\n")
+    assert "function f" not in got and "! gone" in got
+    (tmp_path / "t.f90").write_text("    ! {template}\n")
+    got = edit(procedure="a", at="x = 1", action="replace", template="t.f90")
+    assert "! {template}" in got
+    got = edit(procedure="a", at="x = 1", action="replace", text="[{original}]")
+    assert "[x = 1]" in got
+    for bad, message in ((dict(call="c", procedure="a", action="after"), "expected one call of c"),
+                         (dict(at="x", between=["a", "b"], action="after"), "one of at"),
+                         (dict(procedure="z", action="after"), "expected one subroutine z"),
+                         (dict(module_end="q", action="after"), "no END MODULE q"),
+                         (dict(action="after"), "needs at, between"),
+                         (dict(at="x = 1", action="after", where="here"), "unknown edit keys")):
+        with pytest.raises(ValueError, match=message):
+            edit(**bad)
+
+
 def test_an_entry_whose_procedure_has_other_arguments_is_refused(tmp_path):
     toml = write_app(tmp_path)
     kernel = tmp_path / "src" / "kernel.f90"

@@ -776,15 +776,25 @@ static void download(Context &c, Buffer &b, const std::vector<Box> &boxes)
     for (const Box &box : boxes) require(b.host_cur.add(box), FORT_SCOPE_RESOURCE, "region limit exceeded");
 }
 
-// Publication: every value the buffer physically holds and the host does not (compiler/runtime/
-// scoped_runtime.cu:1410-1413).
+// The values the buffer physically holds and the host does not: its defined values less what the host
+// holds, then the device's current values outside both (compiler/runtime/scoped_runtime.cu:1410-1413).
+// The pieces are disjoint and come from subtractions only, so no union has to fit the region limit.
+static std::vector<Box> unpublished(const Buffer &b)
+{
+    std::vector<Box> need;
+    for (const Box &i : b.init.boxes)
+        for (const Box &m : b.host_cur.missing(i)) need.push_back(m);
+    for (const Box &d : b.dev_cur.boxes)
+        for (const Box &m : b.host_cur.missing(d))
+            for (const Box &x : b.init.missing(m)) need.push_back(x);
+    return need;
+}
+
+// Publication: every value the buffer physically holds reaches the host.
 static void observe(Context &c, Buffer &b)
 {
     drain_buffer(c, b);
-    std::vector<Box> need;
-    for (const Box &i : physical(b).boxes)
-        for (const Box &m : b.host_cur.missing(i)) need.push_back(m);
-    download(c, b, need);
+    download(c, b, unpublished(b));
 }
 
 // ------------------------------------------------------------------------------------------------ graphs
@@ -1205,9 +1215,7 @@ int fort_scope_domain_close_v1(fort_scope_domain_t domain)
             tmp.domain = &d;
             for (auto &kv : d.buffers) {
                 Buffer &b = *kv.second;
-                std::vector<Box> need;
-                for (const Box &i : physical(b).boxes)
-                    for (const Box &m : b.host_cur.missing(i)) need.push_back(m);
+                const std::vector<Box> need = unpublished(b);
                 if (!need.empty()) {
                     ensure_stream(tmp);
                     download(tmp, b, need);

@@ -720,6 +720,53 @@ static void commit(Buffer &b, const std::vector<Box> &writes, const std::vector<
     }
 }
 
+// Commit with the reference's reconciliation (compiler/runtime/scoped_runtime.cu:930-950) for a buffer
+// whose regions would need more rectangles than the limit: the committing side first takes every value
+// only the other side holds (outside what the access wrote), then holds all physical values alone, and the
+// other copy, now redundant, is dropped. The regions are restored first if even that cannot be written.
+static void commit_or_reconcile(Context &c, Buffer &b, const std::vector<Box> &writes,
+                                const std::vector<Box> &overwrites, bool device, bool original = false)
+{
+    const Region init = b.init, hc = b.host_cur, dc = b.dev_cur;
+    auto restore = [&] {
+        b.init = init;
+        b.host_cur = hc;
+        b.dev_cur = dc;
+    };
+    try {
+        commit(b, writes, overwrites, device, original);
+        return;
+    } catch (const Error &e) {
+        restore();
+        if (e.status != FORT_SCOPE_RESOURCE || original) throw;
+    }
+    Region written;
+    for (const std::vector<Box> *list : {&writes, &overwrites})
+        for (const Box &w : *list) require(written.add(w), FORT_SCOPE_RESOURCE, "region limit exceeded");
+    const Region &mine = device ? dc : hc, &other = device ? hc : dc;
+    std::vector<Box> pieces;
+    for (const Box &o : other.boxes)
+        for (const Box &m : mine.missing(o))
+            for (const Box &x : written.missing(m)) pieces.push_back(x);
+    Region all = init, defined = init;
+    try {
+        for (const Region *r : {&hc, &dc, static_cast<const Region *>(&written)})
+            for (const Box &x : r->boxes) require(all.add(x), FORT_SCOPE_RESOURCE, "region limit exceeded");
+        for (const Box &x : written.boxes) require(defined.add(x), FORT_SCOPE_RESOURCE, "region limit exceeded");
+    } catch (const Error &) {
+        fail(FORT_SCOPE_RESOURCE, "the buffer's regions exceed the rectangle limit even after reconciliation");
+    }
+    if (!pieces.empty()) {
+        ensure_stream(c);
+        for (const Box &x : pieces) copy_box(c, b, x, device);
+        if (!device) wait(c);
+    }
+    b.init = defined;
+    (device ? b.dev_cur : b.host_cur) = all;
+    (device ? b.host_cur : b.dev_cur).clear();
+    c.domain->stats.reconciliations++;
+}
+
 static void prepare_device(Context &c, const fort_scope_plan_binding *bindings, std::size_t count, void **devices,
                            bool commit_now)
 {
@@ -747,7 +794,7 @@ static void prepare_device(Context &c, const fort_scope_plan_binding *bindings, 
         }
         devices[i] = b.device;
         if (commit_now) {
-            commit(b, eff[i].writes, eff[i].overwrites, true);
+            commit_or_reconcile(c, b, eff[i].writes, eff[i].overwrites, true);
         } else {
             b.prepared = 1;
             b.prep_writes = eff[i].writes;
@@ -759,12 +806,19 @@ static void prepare_device(Context &c, const fort_scope_plan_binding *bindings, 
 static void end_access(Context &c, Buffer &b, bool device, bool cancel)
 {
     require(b.prepared == (device ? 1 : 2), FORT_SCOPE_STATE, "no prepared access of this kind on the buffer");
-    if (!cancel) commit(b, b.prep_writes, b.prep_overwrites, device, !device && b.prep_original);
+    // The access is over whatever the commit does: a failure leaves no prepared access behind.
+    struct Clear {
+        Buffer &b;
+        ~Clear()
+        {
+            b.prepared = 0;
+            b.prep_original = false;
+            b.prep_writes.clear();
+            b.prep_overwrites.clear();
+        }
+    } clear{b};
+    if (!cancel) commit_or_reconcile(c, b, b.prep_writes, b.prep_overwrites, device, !device && b.prep_original);
     if (!cancel && device) c.domain->stats.launches++;
-    b.prepared = 0;
-    b.prep_original = false;
-    b.prep_writes.clear();
-    b.prep_overwrites.clear();
 }
 
 static void download(Context &c, Buffer &b, const std::vector<Box> &boxes)

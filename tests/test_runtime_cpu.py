@@ -910,3 +910,30 @@ def test_admitted_reads_after_a_kill_keep_the_newer_device_values(scope, cpu_run
         assert list(a) == [50, 50, 50, 50, 4, 5, 6, 7]
         assert dom.stats().download_bytes == 4 * 8
         scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))
+
+
+def test_fragmentation_reconciles_current_values(scope, cpu_runtime):
+    # compiler/tests/test_scoped_runtime.py:293-315: alternating single-cell host and device writes need
+    # more than 128 rectangles; the runtime reconciles instead of failing, and closing publishes every value.
+    lib = cpu_runtime
+    size = 4 * 128
+    a = np.arange(float(size))
+    expected = a.copy()
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a)
+        for i in range(2 * 128 + 16):
+            box = ((i,), (i + 1,))
+            if i % 2:
+                access, _keep = scope.make_access(0, (), [box], [box])
+                scope.check(lib, lib.fort_scope_host_begin(dom.ctx, h, byref(access)))
+                a[i] = -100.0 - i
+                scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))
+                expected[i] = -100.0 - i
+            else:
+                st, devs = begin_device(scope, lib, dom, [binding(scope, h, writes=[box], overwrites=[box])])
+                scope.check(lib, st)
+                dev_view(devs[0], (size,))[i] = 1000.0 + i
+                scope.check(lib, lib.fort_scope_device_end(dom.ctx, h))
+                expected[i] = 1000.0 + i
+        assert dom.stats().reconciliations > 0
+    assert np.array_equal(a, expected)

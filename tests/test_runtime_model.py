@@ -249,3 +249,32 @@ def test_random_access_sequences_follow_the_coherence_model_on_the_device(scope,
     # The same sequences on libftc_hip.so: staged pitched copies, dense-span downloads and linear runs
     # must move exactly the model's cells.
     run_sequence(scope, hip, 100 + seed, steps=150, memory=DeviceMemory())
+
+
+@pytest.mark.gpu
+def test_fragmentation_reconciles_on_the_device(scope, hip):
+    # compiler/tests/test_scoped_runtime.py:293-315 on libftc_hip.so: the reconciliation's copies go through
+    # the device's copy paths (staged and linear), and closing publishes every value.
+    size = 4 * 128
+    memory = DeviceMemory()
+    a = np.arange(float(size))
+    expected = a.copy()
+    with scope.Domain(hip) as dom:
+        h = dom.bind(a)
+        for i in range(2 * 128 + 16):
+            box = ((i,), (i + 1,))
+            access, _keep = scope.make_access(0, (), [box], [box])
+            if i % 2:
+                assert hip.fort_scope_host_begin(dom.ctx, h, byref(access)) == 0
+                a[i] = -100.0 - i
+                assert hip.fort_scope_host_end(dom.ctx, h) == 0
+            else:
+                ptr = c_void_p()
+                assert hip.fort_scope_device_begin(dom.ctx, h, byref(access), byref(ptr)) == 0
+                one = np.array([1000.0 + i])
+                assert memory.hip.hipMemcpy(ptr.value + 8 * i, one.ctypes.data, 8, 1) == 0
+                assert memory.hip.hipDeviceSynchronize() == 0
+                assert hip.fort_scope_device_end(dom.ctx, h) == 0
+            expected[i] = a[i] if i % 2 else 1000.0 + i
+        assert dom.stats().reconciliations > 0
+    assert np.array_equal(a, expected)

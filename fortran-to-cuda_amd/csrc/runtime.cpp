@@ -767,6 +767,28 @@ static void commit_or_reconcile(Context &c, Buffer &b, const std::vector<Box> &w
     c.domain->stats.reconciliations++;
 }
 
+// Before an access begins: its commit must be representable, directly or after a reconciliation, or the
+// access is refused with BOUNDARY while nothing has changed, so the caller can still run its original
+// statements (compiler/runtime/scoped_runtime.cu:899-954, the preparation simulated on private state;
+// test_scoped_runtime.py:341-357).
+static void check_commit(const Buffer &b, const Effects &e, bool device, bool original = false)
+{
+    Buffer t = b;
+    try {
+        commit(t, e.writes, e.overwrites, device, original);
+        return;
+    } catch (const Error &err) {
+        if (err.status != FORT_SCOPE_RESOURCE) throw;
+    }
+    bool fits = !original;
+    Region all = b.init, defined = b.init;
+    for (const Region *r : {&b.host_cur, &b.dev_cur})
+        for (const Box &x : r->boxes) fits = fits && all.add(x);
+    for (const std::vector<Box> *list : {&e.writes, &e.overwrites})
+        for (const Box &x : *list) fits = fits && all.add(x) && defined.add(x);
+    require(fits, FORT_SCOPE_BOUNDARY, "section fragmentation requires a scope boundary");
+}
+
 static void prepare_device(Context &c, const fort_scope_plan_binding *bindings, std::size_t count, void **devices,
                            bool commit_now)
 {
@@ -783,6 +805,7 @@ static void prepare_device(Context &c, const fort_scope_plan_binding *bindings, 
     }
     std::vector<std::vector<Box>> uploads;
     for (std::size_t i = 0; i < count; ++i) uploads.push_back(plan_transfer(*list[i], eff[i], true));
+    for (std::size_t i = 0; i < count; ++i) check_commit(*list[i], eff[i], true);
     if (!count) return;
     ensure_stream(c);
     reserve(c, list.data(), list.size());
@@ -1576,6 +1599,7 @@ int fort_scope_access_begin_v1(fort_scope_t h, const fort_scope_plan_binding *bi
         bool moves = false;
         for (std::size_t i = 0; i < count; ++i) {
             downloads.push_back(plan_transfer(*list[i], eff[i], false));
+            check_commit(*list[i], eff[i], false);
             moves = moves || !downloads.back().empty();
         }
         if (!moves) c.cstats.independent_host_tasks++;
@@ -1608,6 +1632,7 @@ int fort_scope_original_host_begin_v1(fort_scope_t h, const fort_scope_plan_bind
         bool moves = false;
         for (std::size_t i = 0; i < count; ++i) {
             downloads.push_back(plan_transfer(*list[i], eff[i], false, false));
+            check_commit(*list[i], eff[i], false, true);
             moves = moves || !downloads.back().empty();
         }
         if (!moves) c.cstats.independent_host_tasks++;

@@ -937,3 +937,25 @@ def test_fragmentation_reconciles_current_values(scope, cpu_runtime):
                 expected[i] = 1000.0 + i
         assert dom.stats().reconciliations > 0
     assert np.array_equal(a, expected)
+
+
+def test_a_definition_beyond_the_rectangle_limit_is_a_boundary_before_execution(scope, cpu_runtime):
+    # compiler/tests/test_scoped_runtime.py:341-357: 128 disjoint defined cells, and the 129th host
+    # overwrite is refused with BOUNDARY at preparation: nothing committed, no access left open.
+    lib = cpu_runtime
+    size = 2 * 128 + 16
+    a = np.zeros(size)
+    with new_domain(scope, lib) as dom:
+        h = dom.bind(a, initialized=False)
+        for i in range(128):
+            box = ((2 * i,), (2 * i + 1,))
+            access, _keep = scope.make_access(0, (), [box], [box])
+            scope.check(lib, lib.fort_scope_host_begin(dom.ctx, h, byref(access)))
+            a[2 * i] = i + 10
+            scope.check(lib, lib.fort_scope_host_end(dom.ctx, h))
+        box = ((256,), (257,))
+        access, _keep = scope.make_access(0, (), [box], [box])
+        assert lib.fort_scope_host_begin(dom.ctx, h, byref(access)) == scope.BOUNDARY
+        st, _ = begin_device(scope, lib, dom, [binding(scope, h, writes=[box], overwrites=[box])])
+        assert st == scope.BOUNDARY
+    assert list(a[0:256:2]) == [i + 10 for i in range(128)] and a[256] == 0

@@ -257,6 +257,9 @@ def main() -> int:
                    help="only time a plain 2 GiB device copy (HBM ceiling and byte-counter calibration)")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the Poisson (FP64 vs mixed) and chain-contraction sections")
+    p.add_argument("--projected-step-only", action="store_true",
+                   help="only time the whole projected step (LES step, divergence, FP64 and mixed solve, "
+                        "projection) and print its section")
     args = p.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the LES domain would shard by "
@@ -271,6 +274,9 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
 
+    if args.projected_step_only:
+        print(json.dumps({"projected_step": projected_step_section(ftc, scope, lib, fields, steps=3)}))
+        return 0
     if args.copy_probe:
         n = 1 << 28
         ms = ctypes.c_float()

@@ -100,7 +100,8 @@ def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 1
 
 def projected_step_section(ftc, scope, lib, fields, n=(512, 512, 128), steps=10, dx=10.0, dt=0.2, cs=0.17):
     """A whole resident time step in ELMM's order (LES step, divergence into the solver's RHS, pressure
-    solve, projection), all entries on the device in one epoch, ping-ponging two field sets: the FP64
+    solve, projection, the shift of phi by its reference-plane mean), all entries on the device in one
+    epoch, ping-ponging two field sets: the FP64
     solve against the mixed one (three solves, no host synchronisation). Timed with HIP events over
     `steps` steps after one warm-up step; both modes start from the same fields, and the field difference
     between them is reported after warm-up + `steps` steps."""
@@ -110,6 +111,9 @@ def projected_step_section(ftc, scope, lib, fields, n=(512, 512, 128), steps=10,
     out, finals = {}, {}
     sc = (ftc.dp(dx), ftc.dp(dx), ftc.dp(dx), ftc.dp(dt))
     ext = (ftc.ip(nx), ftc.ip(ny), ftc.ip(nz))
+    k_ref = nz // 2 + 1  # the reference plane (1-based), at array index k_ref + 1 of phi(-1:)
+    plane_lo = (ctypes.c_longlong * 3)(2, 2, k_ref + 1)
+    plane_hi = (ctypes.c_longlong * 3)(nx + 2, ny + 2, k_ref + 2)
     for name, mixed in (("fp64", False), ("mixed", True)):
         a = [x.copy(order="F") for x in start]
         b = [np.zeros_like(start[0], order="F") for _ in range(4)]
@@ -132,6 +136,13 @@ def projected_step_section(ftc, scope, lib, fields, n=(512, 512, 128), steps=10,
             scope.check(lib, lib.cpp_fort_shared_pressure_divergence(dom.ctx, 1, hr, *dst[:3], *sc, *ext))
             solver.solve(hp, hr)
             scope.check(lib, lib.cpp_fort_shared_pressure_projection(dom.ctx, 1, *dst[:3], hp, *sc, *ext))
+            # fixpressuretoreference: phi less the mean of its reference plane, a fixed-order device sum
+            # whose value comes back to the host, as ELMM's step has it.
+            total = ctypes.c_double(0.0)
+            scope.check(lib, lib.cpp_fort_shared_fixed_order_reduction(dom.ctx, 1, hp, ftc.ip(0), plane_lo, plane_hi,
+                                                                         ctypes.byref(total)))
+            scope.check(lib, lib.cpp_fort_shared_pressure_fix_reference(dom.ctx, 1, hp, ftc.dp(total.value / (nx * ny)),
+                                                                         *ext))
             return dst
 
         step(0)

@@ -117,128 +117,206 @@ def _call_span(text: str, name: str, low: int, high: int, what: str) -> tuple[in
         pos = stop
 
 
+# An edit is resolved in two stages: a locator turns the description's place (at, between, call,
+# module_end, or the whole confining procedure) into a half-open character span [start, end) of the
+# file; an action then says what that span becomes. Both stages are tables, so a new locator or
+# action is one entry.
+_EDIT_KEYS = frozenset({"file", "procedure", "function", "action", "text", "template", "at", "between",
+                        "call", "module_end"})
+_PLACES = ("at", "between", "call", "module_end")
+
+# What the located piece `piece` becomes, given the edit's inserted text `new`.
+_ACTIONS = {
+    "before": lambda piece, new: new + piece,
+    "after": lambda piece, new: piece + new,
+    "replace": lambda piece, new: new.replace("{original}", piece),
+}
+
+
+def _place_at(text, window, value, what):
+    lo, _ = window
+    a, b = _unique_span(text[window[0]:window[1]], value, what)
+    return lo + a, lo + b
+
+
+def _place_between(text, window, value, what):
+    opening, closing = value
+    lo, hi = window
+    first = lo + _unique_span(text[lo:hi], opening, what)[0]
+    return first, first + _unique_span(text[first:hi], closing, what)[1]
+
+
+def _place_call(text, window, value, what):
+    return _call_span(text, value, window[0], window[1], what)
+
+
+def _place_module_end(text, window, value, what):
+    # An empty span just before the END MODULE line of the named module, anywhere in the file.
+    hit = re.compile(rf"^[ \t]*end[ \t]*module[ \t]+{re.escape(value)}\b", re.IGNORECASE | re.MULTILINE).search(text)
+    if hit is None:
+        raise ValueError(f"{what}: module {value} has no end module line to insert at")
+    return hit.start(), hit.start()
+
+
+_LOCATORS = {"at": _place_at, "between": _place_between, "call": _place_call, "module_end": _place_module_end}
+
+
+def _edit_insert(edit: dict, what: str, root: Path | None) -> str:
+    if "template" not in edit:
+        return edit.get("text", "")
+    if "text" in edit or root is None:
+        raise ValueError(f"{what}: give text or a template file, not both (a template is read relative to the "
+                         f"description)")
+    return (root / edit["template"]).read_text()
+
+
 def apply_edit(text: str, edit: dict, what: str, root: Path | None = None) -> str:
-    """One [[edit]] (compiler/resident/application.py:49-58, 262-290): confined to `procedure` (a
-    `function` when `function = true`) if given, the place is `at` (a piece of text found exactly once,
-    whitespace-tolerant), `between` (from the start of the first piece to the end of the second), `call`
-    (the CALL statement of that procedure), `module_end` (before END MODULE of that module) or, with none of
-    them, the whole procedure; `action` is before, after or replace, and `text` or `template` (a file under
-    `root`) is what goes there, a replacement's {original} standing for what it replaces."""
-    action = edit.get("action")
-    if action not in ("before", "after", "replace"):
-        raise ValueError(f"{what}: an edit needs an action: before, after or replace")
-    unknown = set(edit) - {"file", "procedure", "function", "action", "text", "template", "at", "between", "call",
-                           "module_end"}
-    if unknown:
-        raise ValueError(f"{what}: unknown edit keys {sorted(unknown)}")
-    locators = [k for k in ("at", "between", "call", "module_end") if k in edit]
-    if len(locators) > 1:
-        raise ValueError(f"{what}: one of at, between, call, module_end")
-    if "template" in edit:
-        if "text" in edit or root is None:
-            raise ValueError(f"{what}: a template needs the description's directory and no text")
-        insert = (root / edit["template"]).read_text()
-    else:
-        insert = edit.get("text", "")
-    low, high = 0, len(text)
+    """One [[edit]] of the description (compiler/resident/application.py:49-58 define its keys).
+
+    `procedure` (a function when `function = true`) confines the search; the place is one of `at` (a piece
+    found exactly once, blanks matched loosely), `between` (from the first piece through the second),
+    `call` (the CALL statement of that name, continuation lines included) or `module_end` (just before END
+    MODULE of that module); with none of them the place is the whole confining procedure. `action` puts
+    `text` (or the file `template` under `root`) before or after the place, or replaces it, `{original}`
+    standing for the replaced text."""
+    stray = sorted(set(edit) - _EDIT_KEYS)
+    if stray:
+        raise ValueError(f"{what}: unknown edit keys {stray}")
+    action = _ACTIONS.get(edit.get("action"))
+    if action is None:
+        raise ValueError(f"{what}: action {edit.get('action')!r} is none of before, after or replace")
+    places = [key for key in _PLACES if key in edit]
+    if len(places) > 1:
+        raise ValueError(f"{what}: {' and '.join(places)} given; an edit takes one of at, between, call, "
+                         f"module_end")
+    new = _edit_insert(edit, what, root)
+    window = (0, len(text))
     if "procedure" in edit:
-        low, high = _routine_span(text, edit["procedure"], bool(edit.get("function", False)))
-    region = text[low:high]
-    if "at" in edit:
-        start, end = _unique_span(region, edit["at"], what)
-        start, end = low + start, low + end
-    elif "between" in edit:
-        first, second = edit["between"]
-        start = _unique_span(region, first, what)[0]
-        tail = _unique_span(region[start:], second, what)[1]
-        start, end = low + start, low + start + tail
-    elif "call" in edit:
-        start, end = _call_span(text, edit["call"], low, high, what)
-    elif "module_end" in edit:
-        m = re.search(rf"(?im)^\s*end\s*module\s+{re.escape(edit['module_end'])}\b", text)
-        if m is None:
-            raise ValueError(f"{what}: no END MODULE {edit['module_end']}")
-        start = end = m.start()
+        window = _routine_span(text, edit["procedure"], bool(edit.get("function", False)))
+    if places:
+        start, end = _LOCATORS[places[0]](text, window, edit[places[0]], what)
     elif "procedure" in edit:
-        start, end = low, high
+        start, end = window
     else:
-        raise ValueError(f"{what}: an edit needs at, between, call, module_end or procedure")
-    if action == "before":
-        return text[:start] + insert + text[start:]
-    if action == "after":
-        return text[:end] + insert + text[end:]
-    return text[:start] + insert.replace("{original}", text[start:end]) + text[end:]
+        raise ValueError(f"{what}: nothing to place the edit at; it needs at, between, call, module_end or "
+                         f"procedure")
+    return text[:start] + action(text[start:end], new) + text[end:]
 
 
 # -------------------------------------------------------------------------------------- directives
+# A `!$fort` comment line is read as a sequence of items `word` or `word(argument text)`; the first item
+# is the directive, the rest are its clauses. Each directive kind is one row of _KINDS: the clauses it
+# accepts, whether it takes an argument list, and the Fortran block it stands for. The block texts are the
+# protocol the reference's host layer (compiler/runtime/resident_domain.f90) expects.
 DIRECTIVE = re.compile(r"^(?P<indent>[ \t]*)!\$fort[ \t]+(?P<body>.*?)[ \t]*$", re.IGNORECASE)
-_CLAUSE = re.compile(r"\s*([a-z_]+)\s*(\((?:[^()]|\([^()]*\))*\))?", re.IGNORECASE)
+_WORD = re.compile(r"[ \t]*([A-Za-z_]\w*)[ \t]*")
+_ARRAY_NAME = re.compile(r"[A-Za-z_]\w*")
 
 
-def _directive(body: str, line: int) -> tuple[str, list[str], dict]:
-    words, pos = [], 0
-    while pos < len(body):
-        m = _CLAUSE.match(body, pos)
-        if not m or m.end() == pos:
-            raise ValueError(f"Malformed residency directive on line {line}: {body!r}")
-        words.append((m.group(1).lower(), m.group(2)[1:-1].strip() if m.group(2) else None))
-        pos = m.end()
-    if not words:
-        raise ValueError(f"Empty residency directive on line {line}")
-    (name, arguments), clauses = words[0], dict(words[1:])
-    allowed = {"enter": {"device", "budget", "if"}, "leave": set(), "close": set(), "publish": {"if"},
-               "observe": {"if"}}
-    if name not in allowed:
-        raise ValueError(f"Unknown residency directive on line {line}: {name}")
-    if set(clauses) - allowed[name] or any(v is None for v in clauses.values()):
-        raise ValueError(f"Unsupported clause in the residency directive on line {line}: {body!r}")
-    if (arguments is not None) != (name == "observe"):
-        raise ValueError(f"The {name} directive on line {line} "
-                         + ("needs a list of arrays" if name == "observe" else "takes no arguments"))
-    names = [a.strip() for a in arguments.split(",")] if arguments else []
-    if not all(re.fullmatch(r"[A-Za-z_]\w*", n) for n in names):
-        raise ValueError(f"The observe directive on line {line} takes array names, not expressions")
-    return name, names, clauses
+@dataclass(frozen=True)
+class Directive:
+    kind: str
+    arrays: tuple[str, ...]
+    clauses: dict
+    line: int
 
 
-def _block(indent: str, body: list[str], binding: bool = False) -> str:
-    lines = ["block"] + (["  use, intrinsic :: iso_c_binding"] if binding else [])
-    lines += [f"  use {DOMAIN_MODULE}, only: {DOMAIN}", *("  " + b for b in body), "end block"]
-    return "".join(indent + line + "\n" for line in lines)
+def _items(body: str, line: int) -> list[tuple[str, str | None]]:
+    """`word` / `word(...)` items of a directive body; parentheses may nest."""
+    items, at = [], 0
+    while at < len(body):
+        word = _WORD.match(body, at)
+        if word is None:
+            raise ValueError(f"line {line}: cannot read the residency directive {body!r} from column {at + 1}")
+        at, argument = word.end(), None
+        if body.startswith("(", at):
+            depth, close = 0, at
+            for close in range(at, len(body)):
+                depth += {"(": 1, ")": -1}.get(body[close], 0)
+                if depth == 0:
+                    break
+            if depth:
+                raise ValueError(f"line {line}: the residency directive {body!r} has an unclosed parenthesis")
+            argument, at = body[at + 1:close].strip(), close + 1
+            while at < len(body) and body[at] in " \t":
+                at += 1
+        items.append((word.group(1).lower(), argument))
+    return items
+
+
+def _enter_lines(d: Directive) -> list[str]:
+    c = d.clauses
+    budget = f", int({c['budget']}, c_size_t)" if "budget" in c else ""
+    start = f"call {DOMAIN}%begin_epoch(int({c.get('device', '0')}, c_int){budget})"
+    return [f"if (.not. {DOMAIN}%active) then", f"  if ({c['if']}) {start}" if "if" in c else f"  {start}", "endif"]
+
+
+def _guard(d: Directive) -> str:
+    return f"{DOMAIN}%active" + (f" .and. ({d.clauses['if']})" if "if" in d.clauses else "")
+
+
+def _observe_lines(d: Directive, origin: str) -> list[str]:
+    binds = [f"  ftc_observed({n}) = {DOMAIN}%bind({a}, lbound({a}), .true., origin='{origin}::{a.lower()}')"
+             for n, a in enumerate(d.arrays, 1)]
+    return [f"integer(c_int64_t) :: ftc_observed({len(d.arrays)})", f"if ({_guard(d)}) then", *binds,
+            f"  call {DOMAIN}%observe(ftc_observed)", "endif"]
+
+
+# kind: (accepted clauses, takes an argument list, uses iso_c_binding, body lines)
+_KINDS = {
+    "enter": ({"device", "budget", "if"}, False, True, lambda d, o: _enter_lines(d)),
+    "leave": (set(), False, False, lambda d, o: [f"if ({DOMAIN}%active) call {DOMAIN}%end_epoch()"]),
+    "close": (set(), False, False, lambda d, o: [f"call {DOMAIN}%close()"]),
+    "publish": ({"if"}, False, False, lambda d, o: [f"if ({_guard(d)}) call {DOMAIN}%publish()"]),
+    "observe": ({"if"}, True, True, _observe_lines),
+}
+
+
+def parse_directive(body: str, line: int) -> Directive:
+    """The directive a `!$fort` body states (compiler/resident/directives.py:1-36 list its grammar)."""
+    items = _items(body, line)
+    if not items:
+        raise ValueError(f"line {line}: the residency directive is empty")
+    (kind, argument), rest = items[0], items[1:]
+    if kind not in _KINDS:
+        raise ValueError(f"line {line}: {kind!r} is not a residency directive here (one of {', '.join(_KINDS)})")
+    accepted, listed, _, _ = _KINDS[kind]
+    clauses = dict(rest)
+    bad = [w for w, v in rest if w not in accepted or v is None] + ([] if len(clauses) == len(rest) else ["repeated"])
+    if bad:
+        raise ValueError(f"line {line}: {kind} accepts {sorted(accepted) or 'no clauses'}; "
+                         f"unsupported clause {bad[0]!r} in {body!r}")
+    if listed and argument is None:
+        raise ValueError(f"line {line}: {kind} needs the list of arrays it brings back")
+    if not listed and argument is not None:
+        raise ValueError(f"line {line}: {kind} takes no argument list")
+    arrays = tuple(a.strip() for a in argument.split(",")) if argument else ()
+    malformed = [a for a in arrays if not _ARRAY_NAME.fullmatch(a)]
+    if malformed:
+        raise ValueError(f"line {line}: {kind} lists whole arrays by name; {malformed[0]!r} is not an array name")
+    return Directive(kind, arrays, clauses, line)
+
+
+def directive_block(d: Directive, indent: str, origin: str) -> str:
+    """The Fortran block that replaces the directive line: a BLOCK that uses the domain object."""
+    _, _, c_binding, body = _KINDS[d.kind]
+    head = ["block", *(["  use, intrinsic :: iso_c_binding"] if c_binding else []),
+            f"  use {DOMAIN_MODULE}, only: {DOMAIN}"]
+    return "".join(f"{indent}{text}\n" for text in (*head, *(f"  {b}" for b in body(d, origin)), "end block"))
 
 
 def expand_directives(text: str, origin: str = "source") -> tuple[str, list[dict]]:
-    """Every `!$fort` line replaced by the domain calls it stands for; returns the text and records."""
-    out, records = [], []
+    """`text` with every `!$fort` line replaced by its block, and one record per directive."""
+    pieces, records = [], []
     for number, line in enumerate(text.splitlines(keepends=True), 1):
-        m = DIRECTIVE.match(line.rstrip("\r\n"))
-        if not m:
-            out.append(line)
+        found = DIRECTIVE.match(line.rstrip("\r\n"))
+        if found is None:
+            pieces.append(line)
             continue
-        name, names, clauses = _directive(m.group("body"), number)
-        cond = clauses.get("if")
-        active = f"{DOMAIN}%active" + (f" .and. ({cond})" if cond else "")
-        indent = m.group("indent")
-        if name == "enter":
-            budget = f", int({clauses['budget']}, c_size_t)" if "budget" in clauses else ""
-            start = f"call {DOMAIN}%begin_epoch(int({clauses.get('device', '0')}, c_int){budget})"
-            body = [f"if (.not. {DOMAIN}%active) then", "  " + (f"if ({cond}) {start}" if cond else start), "endif"]
-            out.append(_block(indent, body, binding=True))
-        elif name == "leave":
-            out.append(_block(indent, [f"if ({DOMAIN}%active) call {DOMAIN}%end_epoch()"]))
-        elif name == "close":
-            out.append(_block(indent, [f"call {DOMAIN}%close()"]))
-        elif name == "publish":
-            out.append(_block(indent, [f"if ({active}) call {DOMAIN}%publish()"]))
-        else:
-            handles = "ftc_observed"
-            body = [f"integer(c_int64_t) :: {handles}({len(names)})", f"if ({active}) then"]
-            body += [f"  {handles}({i}) = {DOMAIN}%bind({n}, lbound({n}), .true., origin='{origin}::{n.lower()}')"
-                     for i, n in enumerate(names, 1)]
-            body += [f"  call {DOMAIN}%observe({handles})", "endif"]
-            out.append(_block(indent, body, binding=True))
-        records.append({"line": number, "directive": name, "arrays": names, "clauses": clauses})
-    return "".join(out), records
+        d = parse_directive(found.group("body"), number)
+        pieces.append(directive_block(d, found.group("indent"), origin))
+        records.append({"line": number, "directive": d.kind, "arrays": list(d.arrays), "clauses": d.clauses})
+    return "".join(pieces), records
 
 
 # ---------------------------------------------------------------------------------- procedure splice

@@ -139,10 +139,10 @@ def test_an_inactive_domain_runs_the_original_statements(tmp_path):
 @pytest.mark.parametrize("change,message", [
     (("at = \"  ! ---- steps ----\\n\"", "at = \"  ! ---- nowhere ----\\n\""), "expected one"),
     (("action = \"before\"", "action = \"around\""), "before, after or replace"),
-    (("!$fort enter if(.false.) budget(1048576)", "!$fort enter(U)"), "takes no arguments"),
-    (("!$fort observe(U2, U)", "!$fort observe(U2(1,1,1))"), "array names"),
-    (("!$fort leave", "!$fort leave now"), "Unsupported clause"),
-    (("!$fort close", "!$fort vanish"), "Unknown residency directive"),
+    (("!$fort enter if(.false.) budget(1048576)", "!$fort enter(U)"), "takes no argument list"),
+    (("!$fort observe(U2, U)", "!$fort observe(U2(1,1,1))"), "not an array name"),
+    (("!$fort leave", "!$fort leave now"), "unsupported clause 'now'"),
+    (("!$fort close", "!$fort vanish"), "not a residency directive"),
 ])
 def test_malformed_edits_and_directives_are_refused(tmp_path, change, message):
     with pytest.raises(ValueError, match=message):
@@ -195,7 +195,7 @@ def test_edit_locators(tmp_path):
     for bad, message in ((dict(call="c", procedure="a", action="after"), "expected one call of c"),
                          (dict(at="x", between=["a", "b"], action="after"), "one of at"),
                          (dict(procedure="z", action="after"), "expected one subroutine z"),
-                         (dict(module_end="q", action="after"), "no END MODULE q"),
+                         (dict(module_end="q", action="after"), "module q has no end module"),
                          (dict(action="after"), "needs at, between"),
                          (dict(at="x = 1", action="after", where="here"), "unknown edit keys")):
         with pytest.raises(ValueError, match=message):

@@ -789,6 +789,71 @@ static void check_commit(const Buffer &b, const Effects &e, bool device, bool or
     require(fits, FORT_SCOPE_BOUNDARY, "section fragmentation requires a scope boundary");
 }
 
+// An access's whole acquisition, planned on private state before anything moves
+// (compiler/runtime/scoped_runtime.cu:899-954): the boxes to copy to the acquiring side and that side's
+// currency once they are there, checked so that the access's commit then fits, directly or after a
+// reconciliation. When the transfers themselves would split the side's currency past the rectangle limit,
+// the side takes every value only the other side holds instead and then holds all physical values. What
+// cannot be represented even so refuses the access with BOUNDARY while nothing has changed.
+struct Acquisition {
+    std::vector<Box> copies;
+    Region side;
+    bool reconciled = false;
+};
+
+static Acquisition plan_acquisition(const Buffer &b, const Effects &e, bool device, bool original)
+{
+    const Region &mine = device ? b.dev_cur : b.host_cur;
+    const Region &other = device ? b.host_cur : b.dev_cur;
+    Acquisition a;
+    bool fragmented = false;
+    try {
+        a.copies = plan_transfer(b, e, device, !original);
+        a.side = mine;
+        for (const Box &x : a.copies)
+            if (!a.side.add(x)) {
+                fragmented = true;
+                break;
+            }
+    } catch (const Error &err) {
+        if (err.status != FORT_SCOPE_RESOURCE) throw;
+        fragmented = true;
+    }
+    if (fragmented) {
+        if (!original)
+            for (const Box &r : e.reads)
+                require(b.init.covers(r), FORT_SCOPE_UNINITIALIZED, "task reads values that were never defined");
+        a.copies.clear();
+        for (const Box &o : other.boxes)
+            for (const Box &m : mine.missing(o)) a.copies.push_back(m);
+        try {
+            a.side = physical(b);
+        } catch (const Error &err) {
+            if (err.status != FORT_SCOPE_RESOURCE) throw;
+            fail(FORT_SCOPE_BOUNDARY, "section fragmentation requires a scope boundary");
+        }
+        a.reconciled = true;
+    }
+    Buffer t = b;
+    (device ? t.dev_cur : t.host_cur) = a.side;
+    check_commit(t, e, device, original);
+    return a;
+}
+
+// Carries out a planned acquisition: the copies, then the side's currency as planned. Host copies are
+// complete on return.
+static void acquire(Context &c, Buffer &b, const Acquisition &a, bool device)
+{
+    if (!a.copies.empty()) {
+        require(b.device != nullptr || device, FORT_SCOPE_STATE, "defined values are on no copy");
+        ensure_stream(c);
+        for (const Box &box : a.copies) copy_box(c, b, box, device);
+        if (!device) wait(c);
+    }
+    (device ? b.dev_cur : b.host_cur) = a.side;
+    if (a.reconciled) c.domain->stats.reconciliations++;
+}
+
 static void prepare_device(Context &c, const fort_scope_plan_binding *bindings, std::size_t count, void **devices,
                            bool commit_now)
 {
@@ -803,18 +868,16 @@ static void prepare_device(Context &c, const fort_scope_plan_binding *bindings, 
         list.push_back(&b);
         eff.push_back(effects(b, &bindings[i].access));
     }
-    std::vector<std::vector<Box>> uploads;
-    for (std::size_t i = 0; i < count; ++i) uploads.push_back(plan_transfer(*list[i], eff[i], true));
-    for (std::size_t i = 0; i < count; ++i) check_commit(*list[i], eff[i], true);
+    // Every binding is planned before any buffer changes, so a later refusal leaves the earlier ones as
+    // they were.
+    std::vector<Acquisition> plans;
+    for (std::size_t i = 0; i < count; ++i) plans.push_back(plan_acquisition(*list[i], eff[i], true, false));
     if (!count) return;
     ensure_stream(c);
     reserve(c, list.data(), list.size());
     for (std::size_t i = 0; i < count; ++i) {
         Buffer &b = *list[i];
-        for (const Box &box : uploads[i]) {
-            copy_box(c, b, box, true);
-            require(b.dev_cur.add(box), FORT_SCOPE_RESOURCE, "region limit exceeded");
-        }
+        acquire(c, b, plans[i], true);
         devices[i] = b.device;
         if (commit_now) {
             commit_or_reconcile(c, b, eff[i].writes, eff[i].overwrites, true);
@@ -1595,17 +1658,15 @@ int fort_scope_access_begin_v1(fort_scope_t h, const fort_scope_plan_binding *bi
             list.push_back(&b);
             eff.push_back(effects(b, &bindings[i].access));
         }
-        std::vector<std::vector<Box>> downloads;
+        std::vector<Acquisition> plans;
         bool moves = false;
         for (std::size_t i = 0; i < count; ++i) {
-            downloads.push_back(plan_transfer(*list[i], eff[i], false));
-            check_commit(*list[i], eff[i], false);
-            moves = moves || !downloads.back().empty();
+            plans.push_back(plan_acquisition(*list[i], eff[i], false, false));
+            moves = moves || !plans.back().copies.empty();
         }
         if (!moves) c.cstats.independent_host_tasks++;
         for (std::size_t i = 0; i < count; ++i) {
-            if (!downloads[i].empty()) ensure_stream(c);
-            download(c, *list[i], downloads[i]);
+            acquire(c, *list[i], plans[i], false);
             list[i]->prepared = 2;
             list[i]->prep_writes = eff[i].writes;
             list[i]->prep_overwrites = eff[i].overwrites;
@@ -1628,17 +1689,15 @@ int fort_scope_original_host_begin_v1(fort_scope_t h, const fort_scope_plan_bind
             eff.push_back(effects(b, &bindings[i].access));
         }
         for (Buffer *b : list) drain_buffer(c, *b);
-        std::vector<std::vector<Box>> downloads;
+        std::vector<Acquisition> plans;
         bool moves = false;
         for (std::size_t i = 0; i < count; ++i) {
-            downloads.push_back(plan_transfer(*list[i], eff[i], false, false));
-            check_commit(*list[i], eff[i], false, true);
-            moves = moves || !downloads.back().empty();
+            plans.push_back(plan_acquisition(*list[i], eff[i], false, true));
+            moves = moves || !plans.back().copies.empty();
         }
         if (!moves) c.cstats.independent_host_tasks++;
         for (std::size_t i = 0; i < count; ++i) {
-            if (!downloads[i].empty()) ensure_stream(c);
-            download(c, *list[i], downloads[i]);
+            acquire(c, *list[i], plans[i], false);
             list[i]->prepared = 2;
             list[i]->prep_original = true;
             list[i]->prep_writes = eff[i].writes;

@@ -959,3 +959,58 @@ def test_a_definition_beyond_the_rectangle_limit_is_a_boundary_before_execution(
         st, _ = begin_device(scope, lib, dom, [binding(scope, h, writes=[box], overwrites=[box])])
         assert st == scope.BOUNDARY
     assert list(a[0:256:2]) == [i + 10 for i in range(128)] and a[256] == 0
+
+
+def test_a_read_upload_past_the_rectangle_limit_reconciles_before_any_binding_changes(scope, cpu_runtime):
+    # compiler/runtime/scoped_runtime.cu:899-954: the acquisition of every binding, transfers included, is
+    # simulated before anything moves, and a read transfer that would split the device currency past 128
+    # rectangles reconciles (the device takes every host value) instead of failing half-way with the first
+    # binding already prepared.
+    lib = cpu_runtime
+    a = np.arange(16.0)
+    b = np.arange(4.0 * 200) + 0.5
+    with new_domain(scope, lib) as dom:
+        ha, hb = dom.bind(a), dom.bind(b)
+        for i in range(127):
+            box = ((4 * i,), (4 * i + 1,))
+            st, _ = begin_device(scope, lib, dom, [binding(scope, hb, reads=[box])])
+            scope.check(lib, st)
+            scope.check(lib, lib.fort_scope_device_end(dom.ctx, hb))
+        before = dom.stats().reconciliations
+        st, devs = begin_device(scope, lib, dom, [binding(scope, ha, flags=scope.WRITE_ALL),
+                                                  binding(scope, hb, reads=[((2,), (3,)), ((6,), (7,))])])
+        scope.check(lib, st)
+        assert dom.stats().reconciliations == before + 1
+        # The device copy of b holds every host value now, and a is writable.
+        assert np.array_equal(dev_view(devs[1], b.shape), b)
+        dev_view(devs[0], a.shape)[:] = -a
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, ha))
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, hb))
+    assert np.array_equal(a, -np.arange(16.0)) and np.array_equal(b, np.arange(4.0 * 200) + 0.5)
+
+
+def test_a_refused_later_binding_leaves_the_earlier_buffer_unprepared(scope, cpu_runtime):
+    # A two-binding task whose second binding cannot be committed (a 129th disjoint definition) is refused
+    # with BOUNDARY before the first binding changes: the first buffer has no prepared access afterwards and
+    # its host values stand.
+    lib = cpu_runtime
+    a = np.arange(8.0)
+    size = 2 * 128 + 16
+    b = np.zeros(size)
+    with new_domain(scope, lib) as dom:
+        ha, hb = dom.bind(a), dom.bind(b, initialized=False)
+        for i in range(128):
+            box = ((2 * i,), (2 * i + 1,))
+            access, _keep = scope.make_access(0, (), [box], [box])
+            scope.check(lib, lib.fort_scope_host_begin(dom.ctx, hb, byref(access)))
+            b[2 * i] = i + 10
+            scope.check(lib, lib.fort_scope_host_end(dom.ctx, hb))
+        box = ((256,), (257,))
+        st, _ = begin_device(scope, lib, dom, [binding(scope, ha, flags=scope.WRITE_ALL),
+                                               binding(scope, hb, writes=[box], overwrites=[box])])
+        assert st == scope.BOUNDARY
+        st, devs = begin_device(scope, lib, dom, [binding(scope, ha, flags=scope.READ_ALL)])
+        scope.check(lib, st)
+        assert np.array_equal(dev_view(devs[0], a.shape), np.arange(8.0))
+        scope.check(lib, lib.fort_scope_device_end(dom.ctx, ha))
+    assert np.array_equal(a, np.arange(8.0)) and list(b[0:256:2]) == [i + 10 for i in range(128)]

@@ -96,6 +96,16 @@ def fixed_order_reduce(x: np.ndarray, lo, hi, op: int, initial: float) -> float:
     return fn(base, hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], e1, e1 * e2, op, initial)
 
 
+def box_step(a: np.ndarray, b: np.ndarray, dx: float, dt: float, nu: float) -> None:
+    """One step of the box application without sources, in place: the two-layer periodic halo of `a`
+    (boundaries.f90:6-41), then b = a + dt*nu*laplacian(a) and the interior of a := b (stepper.f90:8-37).
+    Arrays are a(-1:nx+2, -1:ny+2, -1:nz+2), Fortran order."""
+    nx, ny, nz = (d - 4 for d in a.shape)
+    assert a.flags.f_contiguous and b.flags.f_contiguous and a.shape == b.shape
+    lib().orc_box_periodic(a.ctypes.data, nx, ny, nz)
+    lib().orc_box_diffuse_store(a.ctypes.data, b.ctypes.data, nx, ny, nz, dx, dt, nu)
+
+
 def ptr(a: np.ndarray | None) -> int | None:
     return None if a is None else a.ctypes.data
 

@@ -4,8 +4,10 @@
  *
  * orc_box_periodic restates the box application's periodic halo (compiler/tests/apps/box/boundaries.f90:6-41):
  * two ghost layers, x faces first, then y faces over the whole x range, then z faces over the whole x-y
- * range, so edges and corners follow from the order. Pinned by bit equality with the reference box
- * application built into oracle/_ref (field.bin) and by brute force (every ghost equals its periodic image).
+ * range, so edges and corners follow from the order. Pinned, with orc_box_diffuse_store, by bit equality
+ * with the reference box application built natively into oracle/_ref/box: its field.bin after N steps, halos
+ * included, equals N oracle steps applied to its field.bin after none
+ * (tests/test_oracle.py::test_box_oracle_reproduces_the_reference_box_field_bit_for_bit).
  *
  * orc_halo_les fills the one-cell halo of the LES fields in the same x, y, z order: periodic in x and y;
  * at the z walls (ELMM's NOSLIP Btype, benchmarks/elmm/elmm.toml:194-200, whose source is not part of the
@@ -73,7 +75,7 @@ void orc_box_periodic(double *a, int nx, int ny, int nz)
 
 /* Box diffusion and store: compiler/tests/apps/box/stepper.f90:8-37.
  * b = a + dt*nu*laplacian, laplacian = (a(i-1)+a(i+1)+a(j-1)+a(j+1)+a(k-1)+a(k+1) - 6*a) / (dx*dx);
- * then the interior of a takes b. Pinned by the reference box application (oracle/_ref). */
+ * then the interior of a takes b. Pinned by the reference box application (oracle/_ref/box), as above. */
 void orc_box_diffuse_store(double *a, double *b, int nx, int ny, int nz, double dx, double dt, double nu)
 {
 #pragma omp parallel for

@@ -5,8 +5,11 @@
  * in order), the slots are folded into 256 values (f takes f, f+256, ...), and the 256 values are folded
  * in index order into the initial value; n <= DIRECT goes straight to 256 values. max/min fold as the
  * reference's ordered maximum/minimum (compiler/runtime/numeric.hpp:48-58).
- * Pinned by: max, min and max-magnitude equal the serial result (order-free operations), and an integer-
- * valued sum below 2^53 equals the exact serial sum (tests/test_reduce.py).
+ * Pinned by: max, min and max-magnitude equal the serial result (order-free operations); an integer-
+ * valued sum below 2^53 equals the exact serial sum; and two hand-derived known answers of real sums whose
+ * ties round differently in every other order (serial, contiguous slots or folds, reversed folds, the
+ * initial value last) fix the slot, fold and host orders of both the direct and the two-level path
+ * (tests/test_reduce.py ORDER_KATS).
  */
 #include <math.h>
 #include <stdlib.h>

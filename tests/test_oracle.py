@@ -10,6 +10,8 @@
   (compiler/tests/apps/box/stepper.f90:8-15) to rounding;
 - halos: every ghost equals its image (brute force).
 """
+import os
+import subprocess
 from pathlib import Path
 
 import numpy as np
@@ -161,3 +163,40 @@ def test_generated_fields_have_consistent_halos(oracle, fields):
         G = F.copy(order="F")
         oracle.halo_les(G, kind)
         assert bits_equal(F, G)
+
+
+BOX = Path(__file__).resolve().parents[1] / "oracle" / "_ref" / "box"
+
+
+def run_box(directory, nx, ny, nz, steps):
+    # compiler/tests/apps/box/main.f90: box NX NY NZ STEPS REPORT_EVERY SOURCES writes u(-1:nx+2,...)
+    # and then what each source saw to field.bin; no sources here.
+    directory.mkdir()
+    env = {**os.environ, "OMP_NUM_THREADS": "4", "OMP_DYNAMIC": "FALSE"}
+    subprocess.run([str(BOX), str(nx), str(ny), str(nz), str(steps), "1000000", "0"], cwd=directory, env=env,
+                   check=True, capture_output=True, timeout=120)
+    raw = np.fromfile(directory / "field.bin", dtype=np.float64)
+    assert raw.size == (nx + 4) * (ny + 4) * (nz + 4)
+    return np.asfortranarray(raw.reshape((nx + 4, ny + 4, nz + 4), order="F"))
+
+
+@pytest.mark.parametrize("nx,ny,nz,steps", [(12, 10, 8, 5), (7, 5, 3, 4), (48, 40, 32, 3)])
+def test_box_oracle_reproduces_the_reference_box_field_bit_for_bit(oracle, tmp_path, nx, ny, nz, steps):
+    # The reference's explicit diffusion with its two-layer periodic halo (compiler/tests/apps/box/
+    # stepper.f90:8-37, boundaries.f90:6-41), built natively from its own sources into oracle/_ref/box: the
+    # field after `steps` steps equals the oracle's steps applied to the field the same program writes
+    # after none, every cell including the halos.
+    if not BOX.exists():
+        pytest.skip("oracle/_ref/box is not built (make -C oracle ref)")
+    start = run_box(tmp_path / "start", nx, ny, nz, 0)
+    want = run_box(tmp_path / "end", nx, ny, nz, steps)
+    dx = 1.0 / nx
+    nu = 0.1
+    dt = 0.1 * dx * dx / nu
+    a, b = start.copy(order="F"), np.zeros_like(start, order="F")
+    for _ in range(steps):
+        oracle.box_step(a, b, dx, dt, nu)
+    assert bits_equal(a, want), describe_mismatch(a, want)
+    # The halo after the last step holds the images taken at its start, so it differs from the interior's
+    # images now: the comparison above covers the halo as the program leaves it.
+    assert not bits_equal(a[2:-2, 2:-2, 2:-2], start[2:-2, 2:-2, 2:-2])

@@ -59,6 +59,9 @@ HELPER_SIGNATURES = {
     "ftc_mfma_layout_probe": (c_int, [c_int, c_int, c_int, c_int, c_int, _DP]),
     "ftc_divide_probe": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "ftc_intrinsic_probe": (c_int, [_IP]),
+    "ftc_fixed_order_reduction_static_v1": (c_int, [c_void_p, ctypes.c_longlong, ctypes.c_longlong,
+                                                    ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
+                                                    c_int, c_double, c_void_p, c_void_p]),
 }
 
 _loaded: dict[str, ctypes.CDLL] = {}

@@ -7,25 +7,25 @@
 // f, f + 256, ...). The order never depends on the launch, so real sums repeat exactly from run to run,
 // and max/min/maxabs equal the serial result.
 #include <cmath>
+#include <mutex>
 
 #include "entry.hpp"
+#include "numeric.hpp"
 #include "reduce.hpp"
 
 namespace ftc {
 
 // max and min fold as the reference's ordered maximum/minimum (compiler/runtime/numeric.hpp:48-58):
-// max(acc, x) = acc > x ? acc : x, so a NaN operand decides by position exactly as on the host.
+// max(acc, x) = acc > x ? acc : x, so a NaN operand or a signed zero decides by position exactly as on
+// the host (the helpers keep the select from becoming v_max_f64 / v_min_f64).
 template <int OP>
 __device__ __forceinline__ double combine(double acc, double x)
 {
     if constexpr (OP == kReduceSum) return acc + x;
     else if constexpr (OP == kReduceSumSquares) return acc + x * x;
-    else if constexpr (OP == kReduceMax) return acc > x ? acc : x;
-    else if constexpr (OP == kReduceMin) return acc < x ? acc : x;
-    else {
-        const double m = x < 0.0 ? -x : x;
-        return acc > m ? acc : m;
-    }
+    else if constexpr (OP == kReduceMax) return numeric::maximum(acc, x);
+    else if constexpr (OP == kReduceMin) return numeric::minimum(acc, x);
+    else return numeric::maximum(acc, numeric::chosen(x < 0.0) ? -x : x);
 }
 
 template <int OP>
@@ -150,6 +150,47 @@ int reduce_launch(int op, const ReduceBox &b, double *slots, double *final256, h
     }
 }
 
+// The host's fold of the 256 values in index order into the initial value, done by one device thread so
+// that the result can stay on the device.
+template <int OP>
+__global__ void reduce_final_fold(const double *final256, double initial, double *result)
+{
+    constexpr int HOP = OP == kReduceSumSquares ? kReduceSum : (OP == kReduceMaxAbs ? kReduceMax : OP);
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double acc = initial;
+    for (int f = 0; f < kReduceFinal; ++f) acc = combine<HOP>(acc, final256[f]);
+    *result = acc;
+}
+
+__global__ void reduce_store_initial(double initial, double *result)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *result = initial;
+}
+
+template <int OP>
+static int launch_to_device(const ReduceBox &b, double *slots, double *final256, double initial, double *result,
+                            hipStream_t stream)
+{
+    FTC_CHECK(launch<OP>(b, slots, final256, stream));
+    hipLaunchKernelGGL(reduce_final_fold<OP>, dim3(1), dim3(64), 0, stream, final256, initial, result);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "reduction launch failed");
+    return FORT_SCOPE_OK;
+}
+
+int reduce_launch_to_device(int op, const ReduceBox &b, double *slots, double *final256, double initial,
+                            double *result, hipStream_t stream)
+{
+    if (b.n1 < 0 || b.n2 < 0 || b.n3 < 0) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "negative extent");
+    switch (op) {
+    case kReduceSum: return launch_to_device<kReduceSum>(b, slots, final256, initial, result, stream);
+    case kReduceSumSquares: return launch_to_device<kReduceSumSquares>(b, slots, final256, initial, result, stream);
+    case kReduceMax: return launch_to_device<kReduceMax>(b, slots, final256, initial, result, stream);
+    case kReduceMin: return launch_to_device<kReduceMin>(b, slots, final256, initial, result, stream);
+    case kReduceMaxAbs: return launch_to_device<kReduceMaxAbs>(b, slots, final256, initial, result, stream);
+    default: return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unknown reduction");
+    }
+}
+
 double reduce_host_fold(int op, double initial, const double *final256)
 {
     double acc = initial;
@@ -164,6 +205,35 @@ double reduce_host_fold(int op, double initial, const double *final256)
         }
     }
     return acc;
+}
+
+// ------------------------------------------------------------------------ the entry that allocates nothing
+// Its slot and fold values live in one file-scope device array, `scratch`, that every call on a device
+// shares: 65,536 slot values followed by 256 fold values, 526,336 bytes, held by each device the library
+// runs on until the process ends. Host code never uses the array's name as an address: the address is
+// looked up once per device with hipGetSymbolAddress, with that device current, and kept in
+// g_scratch_address.
+constexpr std::size_t kStaticScratchDoubles = kReduceSlots + kReduceFinal;
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static double *g_scratch_address[kMaxDevices];
+
+static int static_scratch(double **address)
+{
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices)
+        return fort_scope_report_error(FORT_SCOPE_RESOURCE, "no usable current device");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+            return fort_scope_report_error(FORT_SCOPE_RESOURCE, "the reduction scratch has no device address");
+        g_scratch_address[device] = static_cast<double *>(p);
+    }
+    *address = g_scratch_address[device];
+    return FORT_SCOPE_OK;
 }
 
 }  // namespace ftc
@@ -226,6 +296,37 @@ int cpp_fort_shared_fixed_order_reduction(fort_scope_t ctx, int mode, fort_buffe
     }
     int released = fort_scope_scratch_release_v1(ctx, lease.token);
     return status ? status : released;
+}
+
+// The fixed-order reduction on device pointers, with its partial results in the library's static scratch:
+// no allocation, no host synchronisation, the result left in device memory. See ftc_reduce.h.
+int ftc_fixed_order_reduction_static_v1(const double *x, long long n1, long long n2, long long n3, long long ld1,
+                                        long long ld2, int op, double initial, double *result, void *stream)
+{
+    if (!x || !result) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "null device pointer");
+    if (op < kReduceSum || op > kReduceMaxAbs) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unknown reduction");
+    if (n1 < 0 || n2 < 0 || n3 < 0) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "negative extent");
+    // The partial results need 65,792 values whatever the box, so the scratch bounds no box; the iteration
+    // count and the element offsets must fit 64-bit indices.
+    const long long limit = 1ll << 62;
+    auto beyond = [limit](long long a, long long b) { return a > 0 && b > limit / a; };
+    if (beyond(n1, n2) || beyond(n1 * n2, n3) || beyond(n2, ld1) || beyond(n3, ld2))
+        return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "reduction box beyond 2^62 elements");
+    if (ld1 < n1 || ld2 < ld1 * n2)
+        return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "leading dimensions below the box extents");
+    double *slots = nullptr;
+    FTC_CHECK(static_scratch(&slots));
+    double *final256 = slots + kReduceSlots;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!n1 || !n2 || !n3) {
+        // No iterations: the variable keeps its value, bit for bit (a fold would turn -0.0 into +0.0).
+        hipLaunchKernelGGL(reduce_store_initial, dim3(1), dim3(64), 0, s, initial, result);
+        if (hipGetLastError() != hipSuccess)
+            return fort_scope_report_error(FORT_SCOPE_EXECUTION, "reduction launch failed");
+        return FORT_SCOPE_OK;
+    }
+    ReduceBox b{x, n1, n2, n3, ld1, ld2};
+    return reduce_launch_to_device(op, b, slots, final256, initial, result, s);
 }
 
 }  // extern "C"

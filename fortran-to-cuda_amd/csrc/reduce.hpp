@@ -17,6 +17,10 @@ struct ReduceBox {
 
 // Enqueues the slot and fold kernels; final256 receives the 256 values the host folds.
 int reduce_launch(int op, const ReduceBox &b, double *slots, double *final256, hipStream_t stream);
+// As reduce_launch, then one thread folds the 256 values in index order into `initial` and stores the
+// result in device memory.
+int reduce_launch_to_device(int op, const ReduceBox &b, double *slots, double *final256, double initial,
+                            double *result, hipStream_t stream);
 // Folds the 256 values in index order into `initial`.
 double reduce_host_fold(int op, double initial, const double *final256);
 

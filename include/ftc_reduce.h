@@ -22,6 +22,20 @@ enum ftc_reduction { FTC_REDUCE_SUM = 0, FTC_REDUCE_SUM_SQUARES = 1, FTC_REDUCE_
 int cpp_fort_shared_fixed_order_reduction(fort_scope_t ctx, int mode, fort_buffer_t x, const int *op,
                                           const long long *lo3, const long long *hi3, double *value);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under stream or
+ * graph capture, serving loops, harnesses that forbid allocation): the same fixed-order reduction of the
+ * box x[i + j*ld1 + k*ld2], 0 <= i < n1, 0 <= j < n2, 0 <= k < n3, of a double array the caller has in
+ * device memory, folded into `initial` and stored in *result (device memory), enqueued on `stream` (a
+ * hipStream_t; NULL is the null stream) with no host synchronisation. The call allocates nothing: its
+ * 65,536 slot and 256 fold values live in one static device array of 526,336 bytes per device, which every
+ * call on that device shares and which keeps the last call's values. Calls that overlap on one device (on
+ * different streams, or from different threads) race on it and must be serialised by the caller; every
+ * call overwrites all it later reads, so nothing has to be cleared. Bit-identical to
+ * cpp_fort_shared_fixed_order_reduction on the same box. Boxes beyond 2^62 elements are refused with
+ * BOUNDARY. */
+int ftc_fixed_order_reduction_static_v1(const double *x, long long n1, long long n2, long long n3, long long ld1,
+                                        long long ld2, int op, double initial, double *result, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

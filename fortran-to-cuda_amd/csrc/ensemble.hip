@@ -313,6 +313,7 @@ using namespace ftc;
 
 struct ftc_ensemble {
     int mode = 0, count = 0, solves = 3;
+    bool projection = true;  // false: the step is the LES stage alone
     double dx = 0, dy = 0, dz = 0, dt = 0, cs = 0;
     std::vector<int> nx, ny, nz;
     // tuning
@@ -558,6 +559,7 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
 // The stage sequence of one step.
 static std::vector<int> step_stages(const ftc_ensemble &e)
 {
+    if (!e.projection) return {kLes};
     std::vector<int> st{kLes, kDiv, kG1, kG2, kTri, kG3, kG4};
     if (mixed(e)) {
         // Each correction's residual pass also sums phi per tile; the fold gives each member's mean, which
@@ -827,6 +829,14 @@ int ftc_ensemble_set_solves_v1(ftc_ensemble *e, int solves)
 {
     if (!e || solves < 1 || solves > 16) return eerr(FORT_SCOPE_ARGUMENT, "solves must be 1..16");
     e->solves = solves;
+    if (e->mode != FTC_ENSEMBLE_FP64_PER_MEMBER) return capture(*e);
+    return FORT_SCOPE_OK;
+}
+
+int ftc_ensemble_set_projection_v1(ftc_ensemble *e, int on)
+{
+    if (!e || (on != 0 && on != 1)) return eerr(FORT_SCOPE_ARGUMENT, "projection is 0 or 1");
+    e->projection = on == 1;
     if (e->mode != FTC_ENSEMBLE_FP64_PER_MEMBER) return capture(*e);
     return FORT_SCOPE_OK;
 }

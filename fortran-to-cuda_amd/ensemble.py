@@ -24,6 +24,7 @@ SIGNATURES = {
     "ftc_ensemble_download_v1": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ftc_ensemble_download_pressure_v1": (c_int, [c_void_p, c_int, c_void_p]),
     "ftc_ensemble_set_solves_v1": (c_int, [c_void_p, c_int]),
+    "ftc_ensemble_set_projection_v1": (c_int, [c_void_p, c_int]),
     "ftc_ensemble_autotune_v1": (c_int, [c_void_p, c_char_p]),
     "ftc_ensemble_step_v1": (c_int, [c_void_p, c_int]),
     "ftc_ensemble_synchronize_v1": (c_int, [c_void_p]),
@@ -73,6 +74,9 @@ class Ensemble:
 
     def solves(self, n):
         self._check(self.lib.ftc_ensemble_set_solves_v1(self.handle, n))
+
+    def projection(self, on: bool):
+        self._check(self.lib.ftc_ensemble_set_projection_v1(self.handle, 1 if on else 0))
 
     def autotune(self, cache=None):
         self._check(self.lib.ftc_ensemble_autotune_v1(self.handle, cache.encode() if cache else None))

@@ -52,6 +52,9 @@ int ftc_ensemble_download_v1(ftc_ensemble *e, int member, double *u, double *v, 
 int ftc_ensemble_download_pressure_v1(ftc_ensemble *e, int member, double *phi);
 /* Number of direct solves per pressure solve in the mixed mode (>= 1). */
 int ftc_ensemble_set_solves_v1(ftc_ensemble *e, int solves);
+/* on = 1 (the default): a step is the projected step above. on = 0: a step is the LES stage alone
+ * (F* = F + dt * tendency, with halos), so that stage can be checked and timed by itself. */
+int ftc_ensemble_set_projection_v1(ftc_ensemble *e, int on);
 /* Picks the LES k chunk, the contraction tile and the persistent grid by timing candidates on this
  * ensemble (fixed candidate order, minimum of the median of 3, ties to the first). With a cache path,
  * a stored choice for the same sizes and mode is reused and new choices are stored. */

@@ -11,7 +11,7 @@ the per-member one (same kernels, same order); the mixed batched ensemble within
 import numpy as np
 import pytest
 
-from conftest import bits_equal
+from conftest import bits_equal, describe_mismatch
 
 SIZES = [(40, 36, 12), (33, 47, 9), (64, 32, 16), (8, 6, 4)]
 D, DT, CS = (10.0, 10.0, 10.0), 0.2, 0.17
@@ -30,18 +30,20 @@ def test_projected_oracle_step_is_divergence_free(oracle, fields):
     assert abs(phi.mean()) <= 1e-12 * np.abs(phi).max()
 
 
-def run(hip, ens_mod, fields, mode, steps, tune=None, solves=None):
-    e = ens_mod.Ensemble(hip, SIZES, D, DT, CS, mode)
+def run(hip, ens_mod, fields, mode, steps, tune=None, solves=None, sizes=SIZES, seed=100, projection=True):
+    e = ens_mod.Ensemble(hip, sizes, D, DT, CS, mode)
     try:
         if solves:
             e.solves(solves)
+        if not projection:
+            e.projection(False)
         if tune is not None:
             e.autotune(tune)
-        for m, n in enumerate(SIZES):
-            e.upload(m, *member_fields(fields, n, 100 + m))
+        for m, n in enumerate(sizes):
+            e.upload(m, *member_fields(fields, n, seed + m))
         e.step(steps)
         e.synchronize()
-        out = [e.download(m) + [e.pressure(m)] for m in range(len(SIZES))]
+        out = [e.download(m) + ([e.pressure(m)] if projection else []) for m in range(len(sizes))]
         info = e.info()
     finally:
         e.destroy()
@@ -104,3 +106,56 @@ def test_autotuned_ensemble_reuses_its_cache_and_keeps_fp64_results(hip, ens_mod
     for m in range(len(SIZES)):
         for g, w, a in zip(got[m], ref[m], again[m]):
             assert bits_equal(g, w) and bits_equal(a, w)
+
+
+# The seeded 64-member ensemble of the batched workload: nx, ny uniform in [32, 128], nz in [32, 96], so
+# every member spans several k chunks of the LES kernel. Both batched modes are compared directly with the
+# oracle after two steps: the LES stage alone bit for bit (it is FP64 in both modes), and the projected
+# step within the FP64 direct solve's rounding (1e-12 of each velocity and scalar field's scale, 1e-11 for
+# the pressure, as for the per-member runs above) or the mixed solve's bound (1e-10).
+SEEDED_STEPS = 2
+
+
+@pytest.fixture(scope="module")
+def seeded64(oracle, fields):
+    sizes = fields.ensemble_sizes(64)
+    les, projected = [], []
+    for m, n in enumerate(sizes):
+        start = member_fields(fields, n, 500 + m)
+        state = list(start)
+        for _ in range(SEEDED_STEPS):
+            state = list(oracle.les_step(*state, *D, DT, CS))
+        les.append(state)
+        state = list(start)
+        for _ in range(SEEDED_STEPS):
+            *state, phi = oracle.projected_step(*state, *D, DT, CS)
+        projected.append(state + [phi])
+    return sizes, les, projected
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["FP64_BATCHED", "MIXED_BATCHED"])
+def test_seeded_64_members_les_stage_is_bit_exact_against_the_oracle(hip, ens_mod, fields, seeded64, mode):
+    sizes, les, _ = seeded64
+    print(f"seeded ensemble LES stage, {mode}", flush=True)
+    got, info = run(hip, ens_mod, fields, getattr(ens_mod, mode), SEEDED_STEPS, sizes=sizes, seed=500,
+                    projection=False)
+    assert info.launches_per_step == 1
+    for m, n in enumerate(sizes):
+        for name, g, w in zip("UVWS", got[m], les[m]):
+            assert bits_equal(g, w), (m, n, name, describe_mismatch(g, w))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,bound,phi_bound", [("FP64_BATCHED", 1e-12, 1e-11), ("MIXED_BATCHED", 1e-10, 1e-10)])
+def test_seeded_64_members_projected_step_matches_the_oracle(hip, ens_mod, fields, seeded64, mode, bound, phi_bound):
+    sizes, _, projected = seeded64
+    print(f"seeded ensemble projected step, {mode}", flush=True)
+    got, info = run(hip, ens_mod, fields, getattr(ens_mod, mode), SEEDED_STEPS, sizes=sizes, seed=500)
+    worst = 0.0
+    for m, n in enumerate(sizes):
+        for name, g, w in zip("UVWSP", got[m], projected[m]):
+            err = np.abs(g - w).max() / np.abs(w).max()
+            worst = max(worst, err)
+            assert err <= (phi_bound if name == "P" else bound), (m, n, name, err)
+    print(f"worst field error {worst:.2e}")

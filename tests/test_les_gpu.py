@@ -79,9 +79,10 @@ def test_les_tendency_is_bit_exact(ftc, scope, hip, oracle, fields, cs, n):
         assert bits_equal(g[sl], w[sl]), f"{name}: " + describe_mismatch(g[sl], w[sl])
 
 
-@pytest.mark.parametrize("n,steps", [((48, 40, 24), 6), ((33, 17, 9), 3), ((1, 2, 1), 2)])
+@pytest.mark.parametrize("n,steps", [((48, 40, 24), 6), ((33, 17, 9), 3), ((1, 2, 1), 2), ((512, 512, 128), 4)])
 def test_les_steps_with_graph_replay_are_bit_exact_including_halos(ftc, scope, hip, oracle, fields, n, steps):
     nx, ny, nz = n
+    print(f"LES steps {n} x {steps}", flush=True)
     dx, dt, cs = 10.0, 0.2, 0.17
     U, V, W, S = fields.les_fields(nx, ny, nz)
     a = [U.copy(order="F"), V.copy(order="F"), W.copy(order="F"), S.copy(order="F")]

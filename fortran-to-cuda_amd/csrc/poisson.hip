@@ -8,11 +8,15 @@
 //   B2 = B1 Qy           (per z plane: nx x ny by ny x ny)
 //   (T_z + (lx_p + ly_q) I) Psi_pq = B2_pq   for every mode (p, q): FP64 Thomas along z
 //   B3 = Psi Qy^T,  phi = Qx B3.
-// The four contractions run on the MFMA engine (gemm.hpp): FP64 MFMA for the FP64 solve, split-bf16
+// For periodic x and y with power-of-two sizes the FP64 solve takes the orthonormal Hartley basis
+// instead (cas(2 pi j k / n) / sqrt(n), its own inverse): the four transforms are O(n log n) FFT passes
+// through LDS (fft_kernels.hpp), each one read and one write of the workspace. Otherwise the four
+// contractions run on the MFMA engine (gemm.hpp): FP64 MFMA for the FP64 solve, split-bf16
 // products with FP32 accumulation on FP32 workspaces for the mixed solve, whose FP64 residual
 // r = b - A phi (fixed-order norms) drives FP64 iterative refinement phi += M^-1 r.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -23,6 +27,7 @@
 
 #include "../../include/ftc_poisson.h"
 #include "entry.hpp"
+#include "fft_kernels.hpp"
 #include "gemm.hpp"
 #include "reduce.hpp"
 #include "poisson_kernels.hpp"
@@ -86,6 +91,30 @@ static std::vector<double> transpose(const std::vector<double> &Q, int n)
     return T;
 }
 
+// The Hartley path: n a power of two in [4, 4096], periodic.
+static bool hartley_size(int n) { return n >= 4 && n <= 4096 && (n & (n - 1)) == 0; }
+
+static int log2_exact(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+// Eigenvalues of the periodic second difference for the Hartley index k, (2 cos(2 pi k / n) - 2) / h^2,
+// and the FFT twiddles (cos, -sin)(2 pi k / n) for k < n/2.
+static void hartley_tables(int n, double h, std::vector<double> &lam, std::vector<double> &tw)
+{
+    const double pi = 3.14159265358979323846;
+    lam.assign(n, 0.0);
+    for (int k = 1; k < n; ++k) lam[k] = (2.0 * std::cos(2.0 * pi * k / n) - 2.0) / (h * h);
+    tw.assign(n, 0.0);
+    for (int k = 0; k < n / 2; ++k) {
+        tw[2 * k] = std::cos(2.0 * pi * k / n);
+        tw[2 * k + 1] = -std::sin(2.0 * pi * k / n);
+    }
+}
+
 }  // namespace ftc
 
 using namespace ftc;
@@ -95,6 +124,7 @@ struct poisson_solver_handle_v1 {
     int nx = 0, ny = 0, nz = 0;
     double hx = 0, hy = 0, hz = 0;
     bool perx = true, pery = true, mixed = false;
+    bool hartley = false;  // FP64 solve through the FFT (Hartley) transforms
     double tol = 1e-12;
     int max_solves = 6, measure = 1;
     int device = 0;
@@ -103,6 +133,7 @@ struct poisson_solver_handle_v1 {
     void *arena = nullptr;
     std::size_t arena_bytes = 0;
     double *Qx = nullptr, *QxT = nullptr, *Qy = nullptr, *QyT = nullptr, *lx = nullptr, *ly = nullptr;
+    double2 *twx = nullptr, *twy = nullptr;  // Hartley path: FFT twiddles of x and y
     void *Qxs = nullptr, *QxTs = nullptr, *Qys = nullptr, *QyTs = nullptr;  // split bases (bf16 hi, lo planes)
     void *w0 = nullptr, *w1 = nullptr;
     double *res = nullptr, *slots = nullptr, *final256 = nullptr;
@@ -125,11 +156,59 @@ static int map_status(int scope_status)
     }
 }
 
+template <class TI, class TO>
+static int hartley_pass(const void *in, long long ie, long long ia, long long ic, void *out, long long oe, long long oa,
+                        long long oc, int n, int A, int C, double beta, const double2 *tw, hipStream_t stream)
+{
+    // 4096 complex points per workgroup: G = 8192 / n real lines.
+    HartleyPass h{in, ie, ia, ic, out, oe, oa, oc, n, log2_exact(n), 2 * kFftPoints / n, 0, A, C,
+                  1.0 / std::sqrt((double)n), beta, tw};
+    h.logG = log2_exact(h.G);
+    const std::size_t lds = (std::size_t)hartley_lds_bytes(n);
+    const dim3 grid((unsigned)((A + h.G - 1) / h.G), (unsigned)C);
+    if (ie == 1) hipLaunchKernelGGL((hartley_kernel<TI, TO, true>), grid, dim3(kFftThreads), lds, stream, h);
+    else hipLaunchKernelGGL((hartley_kernel<TI, TO, false>), grid, dim3(kFftThreads), lds, stream, h);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "FFT launch failed");
+    return FORT_SCOPE_OK;
+}
+
+// Raises the dynamic LDS limit of the Hartley kernels to what the largest pass needs.
+static bool hartley_attributes()
+{
+    int bytes = 0;
+    for (int n = 4; n <= 4096; n *= 2) bytes = std::max(bytes, hartley_lds_bytes(n));
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(&hartley_kernel<double, double, true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess &&
+           hipFuncSetAttribute(reinterpret_cast<const void *>(&hartley_kernel<double, double, false>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
+}
+
+// One FP64 solve through the Hartley transforms: phi_out = beta * phi_out + Hx Hy T^-1 Hy Hx in, with
+// the workspaces W0, W1 holding the x index p fastest (W[p + q*nx + k*nx*ny]).
+static int apply_inverse_hartley(poisson_solver_handle_v1 &s, const double *in, long long in1, long long in2,
+                                 View3 out, double beta, hipStream_t stream)
+{
+    const int nx = s.nx, ny = s.ny, nz = s.nz;
+    const long long plane = (long long)nx * ny;
+    double *w0 = static_cast<double *>(s.w0), *w1 = static_cast<double *>(s.w1);
+    FTC_CHECK((hartley_pass<double, double>(in, 1, in1, in2, w0, 1, nx, plane, nx, ny, nz, 0.0, s.twx, stream)));
+    FTC_CHECK((hartley_pass<double, double>(w0, nx, 1, plane, w1, nx, 1, plane, ny, nx, nz, 0.0, s.twy, stream)));
+    const unsigned blocks = (unsigned)((plane + s.cols - 1) / s.cols);
+    const std::size_t lds = 2 * (std::size_t)tridiag_chunks(nz) * s.cols * sizeof(double);
+    hipLaunchKernelGGL((tridiag_z_kernel<double, double>), dim3(blocks), dim3(s.cols), lds, stream,
+                       static_cast<const double *>(w1), TriOut<double>{w0, 0}, nx, ny, nz, s.lx, s.ly,
+                       1.0 / (s.hz * s.hz), s.cols, 0);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "tridiagonal launch failed");
+    FTC_CHECK((hartley_pass<double, double>(w0, nx, 1, plane, w1, nx, 1, plane, ny, nx, nz, 0.0, s.twy, stream)));
+    return hartley_pass<double, double>(w1, 1, nx, plane, out.x, 1, out.ld1, out.ld2, nx, ny, nz, beta, s.twx, stream);
+}
+
 // The contraction calls of one application of the approximate (mixed) or exact (FP64) inverse:
 // phi_out = beta * phi_out + Qx [T^-1 ((Qx^T in) Qy)] Qy^T.
 static int apply_inverse(poisson_solver_handle_v1 &s, const void *in, long long in1, long long in2, View3 out,
                          double beta, hipStream_t stream, bool in_split = false, const double *shift = nullptr)
 {
+    if (s.hartley) return apply_inverse_hartley(s, static_cast<const double *>(in), in1, in2, out, beta, stream);
     const int nx = s.nx, ny = s.ny, nz = s.nz;
     const long long plane = (long long)nx * ny;
     const int prec = s.mixed ? kBf16x3 : kF64;
@@ -279,6 +358,7 @@ static std::size_t arena_size(const poisson_solver_handle_v1 &s)
     b += 2 * align_up((std::size_t)s.nx * s.nx * 8) + 2 * align_up((std::size_t)s.ny * s.ny * 8);
     b += 2 * align_up((std::size_t)s.nx * s.nx * 4) + 2 * align_up((std::size_t)s.ny * s.ny * 4);
     b += align_up((std::size_t)s.nx * 8) + align_up((std::size_t)s.ny * 8);
+    b += align_up((std::size_t)s.nx * 8) + align_up((std::size_t)s.ny * 8);  // Hartley twiddles
     b += 2 * align_up(N * wbytes) + align_up(N * 8);
     b += align_up((kReduceSlots + kReduceFinal + 1) * 8);  // slots, 256 partials, the mean
     b += 2 * align_up((std::size_t)residual_blocks(s) * 8);  // per-block residual sums of squares, of phi
@@ -320,6 +400,8 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
     s->perx = c->boundary_conditions[0] == 0;
     s->pery = c->boundary_conditions[2] == 0;
     s->mixed = (c->flags & POISSON_SOLVER_MIXED) != 0;
+    s->hartley = !s->mixed && s->perx && s->pery && hartley_size(s->nx) && hartley_size(s->ny) &&
+                 !std::getenv("FTC_POISSON_DENSE");
     s->max_solves = s->mixed ? 6 : 1;
     s->cols = 64;
     if (s->nz > kPoissonMaxNz) return perr(POISSON_SOLVER_UNSUPPORTED, "nz above 1024 exceeds the LDS of the z solve");
@@ -354,6 +436,8 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
     s->QyTs = carve((std::size_t)ny * ny * 4);
     s->lx = (double *)carve((std::size_t)nx * 8);
     s->ly = (double *)carve((std::size_t)ny * 8);
+    s->twx = (double2 *)carve((std::size_t)nx * 8);
+    s->twy = (double2 *)carve((std::size_t)ny * 8);
     s->w0 = carve(N * wbytes);
     s->w1 = carve(N * wbytes);
     s->res = (double *)carve(N * 8);
@@ -361,9 +445,20 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
     s->phisum = (double *)carve((std::size_t)residual_blocks(*s) * 8);
     s->slots = (double *)carve((kReduceSlots + kReduceFinal + 1) * 8);
     s->final256 = s->slots + kReduceSlots;
-    std::vector<double> qx, lx, qy, ly;
+    std::vector<double> qx, lx, qy, ly, twx(2, 0.0), twy(2, 0.0);
     basis(nx, s->hx, s->perx, qx, lx);
     basis(ny, s->hy, s->pery, qy, ly);
+    if (s->hartley) {
+        // The z solve indexes the eigenvalues by the transforms' mode index: Hartley order here.
+        hartley_tables(nx, s->hx, lx, twx);
+        hartley_tables(ny, s->hy, ly, twy);
+        if (!hartley_attributes()) {
+            (void)hipGetLastError();
+            (void)hipFree(s->arena);
+            (void)hipHostFree(s->host_final);
+            return perr(POISSON_SOLVER_RESOURCE, "FFT kernel attributes could not be set");
+        }
+    }
     std::vector<double> qxt = transpose(qx, nx), qyt = transpose(qy, ny);
     // The split-bf16 solve takes the bases pre-split (bf16 hi plane, then lo plane: the FP32 footprint).
     std::vector<uint16_t> qxf = split_planes(qx), qxtf = split_planes(qxt), qyf = split_planes(qy),
@@ -377,7 +472,9 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
               hipMemcpy(s->Qys, qyf.data(), qyf.size() * 2, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(s->QyTs, qytf.data(), qytf.size() * 2, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(s->lx, lx.data(), lx.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(s->ly, ly.data(), ly.size() * 8, hipMemcpyHostToDevice) == hipSuccess;
+              hipMemcpy(s->ly, ly.data(), ly.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              (!s->hartley || (hipMemcpy(s->twx, twx.data(), (std::size_t)nx * 8, hipMemcpyHostToDevice) == hipSuccess &&
+                               hipMemcpy(s->twy, twy.data(), (std::size_t)ny * 8, hipMemcpyHostToDevice) == hipSuccess));
     if (!ok) {
         (void)hipGetLastError();
         (void)hipFree(s->arena);

@@ -91,3 +91,21 @@ def test_unsupported_configurations_are_rejected(ftc, scope, hip):
         with pytest.raises(scope.ScopeError) as e:
             pressure.PressureSolver(hip, dom, (16, 16, 1), (1.0, 1.0, 1.0))
         assert e.value.status == scope.ARGUMENT
+
+
+@pytest.mark.parametrize("n", [(256, 64, 16), (8, 16, 5), (4, 4, 3), (512, 512, 8)])
+def test_fft_solve_matches_the_oracle_and_the_dense_solve(ftc, scope, hip, monkeypatch, n):
+    # Periodic x and y of power-of-two sizes take the FFT (Hartley) transforms in the FP64 solve; with
+    # FTC_POISSON_DENSE set at creation the same solver takes the dense FP64 MFMA transforms. Both are
+    # direct solves of the same system, so they agree with the oracle and with each other to rounding.
+    print(f"FFT solve {n}", flush=True)
+    h = (10.0, 7.0, 5.0)
+    b = poisson_oracle.divergence_rhs(*n, h, seed=sum(n) + 7)
+    want = poisson_oracle.solve(b, h)
+    fft, report = run(ftc, scope, hip, b, h, False)
+    assert report.relative_residual <= 1e-12
+    monkeypatch.setenv("FTC_POISSON_DENSE", "1")
+    dense, _ = run(ftc, scope, hip, b, h, False)
+    scale = np.abs(want).max()
+    assert np.abs(fft - want).max() <= 1e-11 * scale
+    assert np.abs(fft - dense).max() <= 1e-11 * scale

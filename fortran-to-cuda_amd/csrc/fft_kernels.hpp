@@ -18,8 +18,8 @@
 // The FFT is a Stockham autosort transform in LDS with radix-8 stages (one radix-4 or radix-2 stage
 // takes the rest of log2 n): each thread holds 16 complex points in registers, applies the stage's
 // twiddles and R-point DFTs there, and writes the results to their sorted places; the first stage reads
-// its operands straight from global memory and needs no twiddles. A workgroup of 256 threads takes 4096
-// complex points: G = 8192 / n real lines of one plane. The lines of plane c are a = 0 .. A-1 and element
+// its operands straight from global memory and needs no twiddles. A workgroup of 256 threads takes POINTS
+// complex points (2048 or 4096): G = 2 * POINTS / n real lines of one plane. The lines of plane c are a = 0 .. A-1 and element
 // e of line (a, c) is at in[a*ia + c*ic + e*ie]. ECONT says that a line's elements are contiguous
 // (ie == 1, the x transform): threads then walk the elements of a line; otherwise the lines are
 // contiguous (ia == 1, the y transform) and threads walk across lines, so both read and write whole
@@ -32,15 +32,14 @@
 namespace ftc {
 
 constexpr int kFftThreads = 256;
-constexpr int kFftPoints = 4096;                        // complex points per workgroup
-constexpr int kFftPerThread = kFftPoints / kFftThreads;  // 16
+constexpr int kFftMaxPoints = 4096;  // complex points per workgroup: 2048 or 4096 (8 or 16 per thread)
 
 struct HartleyPass {
     const void *in;
     long long ie, ia, ic;
     void *out;
     long long oe, oa, oc;
-    int n, logn, G, logG;  // line length and real lines per workgroup (G = 8192 / n), powers of two
+    int n, logn, G, logG;  // line length and real lines per workgroup (2 * POINTS / n), powers of two
     int A, C;              // lines per plane, planes
     double scale, beta;    // out = beta * out + scale * H(in); beta == 0 reads nothing of out
     const double2 *tw;     // tw[k] = (cos, -sin)(2 pi k / n), k < n/2
@@ -48,9 +47,9 @@ struct HartleyPass {
 
 // LDS bytes of a pass over lines of n points: the complex lines (one padding point each, so that the
 // lines of a y transform fall on different banks) and the n/2 twiddles.
-__host__ __device__ constexpr int hartley_lds_bytes(int n)
+__host__ __device__ constexpr int hartley_lds_bytes(int n, int points)
 {
-    return (kFftPoints / n) * (n + 1) * 16 + (n / 2) * 16;
+    return (points / n > 0 ? points / n : 1) * (n + 1) * 16 + (n / 2) * 16;
 }
 
 template <class T>
@@ -129,14 +128,14 @@ __device__ __forceinline__ double2 twiddle(const double2 *tw, int m, int n)
 }
 
 // One Stockham stage of radix R on the LDS lines (not the first: reads LDS, applies twiddles).
-template <int R, bool ECONT>
+template <int R, bool ECONT, int PT>
 __device__ __forceinline__ void stockham_stage(double2 *z, const double2 *tw, int n, int logn, int logl, int stride,
-                                               int ns, int logns, double2 (&v)[kFftPerThread])
+                                               int ns, int logns, double2 (&v)[PT])
 {
     constexpr int LOGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
     const int lognr = logn - LOGR, nr = 1 << lognr;
 #pragma unroll
-    for (int t = 0; t < kFftPerThread / R; ++t) {
+    for (int t = 0; t < PT / R; ++t) {
         int line, j;
         butterfly_at<ECONT>(threadIdx.x + t * kFftThreads, lognr, logl, line, j);
         const double2 *zl = z + line * stride;
@@ -145,7 +144,7 @@ __device__ __forceinline__ void stockham_stage(double2 *z, const double2 *tw, in
     }
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < kFftPerThread / R; ++t) {
+    for (int t = 0; t < PT / R; ++t) {
         int line, j;
         butterfly_at<ECONT>(threadIdx.x + t * kFftThreads, lognr, logl, line, j);
         const int k = j & (ns - 1);
@@ -163,14 +162,14 @@ __device__ __forceinline__ void stockham_stage(double2 *z, const double2 *tw, in
 
 // The first Stockham stage (ns = 1: no twiddles, results at j*R + r): operands straight from global
 // memory, two real lines (a, a+1) per complex line; lines past the plane's last read as zero.
-template <int R, class TI, bool ECONT>
+template <int R, class TI, bool ECONT, int PT>
 __device__ __forceinline__ void first_stage(const HartleyPass &h, const TI *in, double2 *z, int a0, int logl,
-                                            int stride, double2 (&v)[kFftPerThread])
+                                            int stride, double2 (&v)[PT])
 {
     constexpr int LOGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
     const int lognr = h.logn - LOGR, nr = 1 << lognr;
 #pragma unroll
-    for (int t = 0; t < kFftPerThread / R; ++t) {
+    for (int t = 0; t < PT / R; ++t) {
         int line, j;
         butterfly_at<ECONT>(threadIdx.x + t * kFftThreads, lognr, logl, line, j);
         const int a = a0 + 2 * line;
@@ -189,7 +188,7 @@ __device__ __forceinline__ void first_stage(const HartleyPass &h, const TI *in, 
     }
 }
 
-template <class TI, class TO, bool ECONT>
+template <class TI, class TO, bool ECONT, int POINTS>
 __global__ __launch_bounds__(kFftThreads) void hartley_kernel(const HartleyPass h)
 {
     extern __shared__ double2 fft_lds[];
@@ -206,20 +205,21 @@ __global__ __launch_bounds__(kFftThreads) void hartley_kernel(const HartleyPass 
 
     // The radices: radix 8 while at least three factors of two remain, then one radix-4 or radix-2 stage.
     const int eights = logn / 3, rest = logn - 3 * eights;
-    double2 v[kFftPerThread];
+    constexpr int PT = POINTS / kFftThreads;
+    double2 v[PT];
     // First stage (ns = 1, no twiddles), operands from global memory.
-    if (eights) first_stage<8, TI, ECONT>(h, in, z, a0, logl, stride, v);
-    else if (rest == 2) first_stage<4, TI, ECONT>(h, in, z, a0, logl, stride, v);
-    else first_stage<2, TI, ECONT>(h, in, z, a0, logl, stride, v);
+    if (eights) first_stage<8, TI, ECONT, PT>(h, in, z, a0, logl, stride, v);
+    else if (rest == 2) first_stage<4, TI, ECONT, PT>(h, in, z, a0, logl, stride, v);
+    else first_stage<2, TI, ECONT, PT>(h, in, z, a0, logl, stride, v);
     __syncthreads();
     int ns = eights ? 8 : (1 << rest), logns = eights ? 3 : rest;
     for (int s = 1; s < eights; ++s) {
-        stockham_stage<8, ECONT>(z, tw, n, logn, logl, stride, ns, logns, v);
+        stockham_stage<8, ECONT, PT>(z, tw, n, logn, logl, stride, ns, logns, v);
         ns <<= 3;
         logns += 3;
     }
-    if (eights && rest == 2) stockham_stage<4, ECONT>(z, tw, n, logn, logl, stride, ns, logns, v);
-    if (eights && rest == 1) stockham_stage<2, ECONT>(z, tw, n, logn, logl, stride, ns, logns, v);
+    if (eights && rest == 2) stockham_stage<4, ECONT, PT>(z, tw, n, logn, logl, stride, ns, logns, v);
+    if (eights && rest == 1) stockham_stage<2, ECONT, PT>(z, tw, n, logn, logl, stride, ns, logns, v);
 
     const double sc = 0.5 * h.scale, beta = h.beta;
     for (int idx = tid; idx < G * n; idx += kFftThreads) {

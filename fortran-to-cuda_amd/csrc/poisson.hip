@@ -156,18 +156,36 @@ static int map_status(int scope_status)
     }
 }
 
+// Complex points per workgroup of a pass: 16 per thread where the lines run across threads (the y
+// transform: G = 2 * points / n lines make the 128-byte runs), 8 where a line's own elements do.
+static int hartley_points(bool econt, int n)
+{
+    static const int x = std::getenv("FTC_FFT_POINTS_X") ? std::atoi(std::getenv("FTC_FFT_POINTS_X")) : 2048;
+    static const int y = std::getenv("FTC_FFT_POINTS_Y") ? std::atoi(std::getenv("FTC_FFT_POINTS_Y")) : 4096;
+    const int p = econt ? x : y;
+    return std::max(p == 2048 ? 2048 : 4096, n);  // at least two real lines (n <= 4096)
+}
+
+template <class TI, class TO, int POINTS>
+static void hartley_launch(bool econt, dim3 grid, std::size_t lds, hipStream_t stream, const HartleyPass &h)
+{
+    if (econt) hipLaunchKernelGGL((hartley_kernel<TI, TO, true, POINTS>), grid, dim3(kFftThreads), lds, stream, h);
+    else hipLaunchKernelGGL((hartley_kernel<TI, TO, false, POINTS>), grid, dim3(kFftThreads), lds, stream, h);
+}
+
 template <class TI, class TO>
 static int hartley_pass(const void *in, long long ie, long long ia, long long ic, void *out, long long oe, long long oa,
                         long long oc, int n, int A, int C, double beta, const double2 *tw, hipStream_t stream)
 {
-    // 4096 complex points per workgroup: G = 8192 / n real lines.
-    HartleyPass h{in, ie, ia, ic, out, oe, oa, oc, n, log2_exact(n), 2 * kFftPoints / n, 0, A, C,
+    const bool econt = ie == 1;
+    const int points = hartley_points(econt, n);
+    HartleyPass h{in, ie, ia, ic, out, oe, oa, oc, n, log2_exact(n), 2 * points / n, 0, A, C,
                   1.0 / std::sqrt((double)n), beta, tw};
     h.logG = log2_exact(h.G);
-    const std::size_t lds = (std::size_t)hartley_lds_bytes(n);
+    const std::size_t lds = (std::size_t)hartley_lds_bytes(n, points);
     const dim3 grid((unsigned)((A + h.G - 1) / h.G), (unsigned)C);
-    if (ie == 1) hipLaunchKernelGGL((hartley_kernel<TI, TO, true>), grid, dim3(kFftThreads), lds, stream, h);
-    else hipLaunchKernelGGL((hartley_kernel<TI, TO, false>), grid, dim3(kFftThreads), lds, stream, h);
+    if (points == 2048) hartley_launch<TI, TO, 2048>(econt, grid, lds, stream, h);
+    else hartley_launch<TI, TO, 4096>(econt, grid, lds, stream, h);
     if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "FFT launch failed");
     return FORT_SCOPE_OK;
 }
@@ -176,11 +194,15 @@ static int hartley_pass(const void *in, long long ie, long long ia, long long ic
 static bool hartley_attributes()
 {
     int bytes = 0;
-    for (int n = 4; n <= 4096; n *= 2) bytes = std::max(bytes, hartley_lds_bytes(n));
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(&hartley_kernel<double, double, true>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess &&
-           hipFuncSetAttribute(reinterpret_cast<const void *>(&hartley_kernel<double, double, false>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
+    for (int n = 4; n <= 4096; n *= 2)
+        for (int points : {2048, 4096}) bytes = std::max(bytes, hartley_lds_bytes(n, std::max(points, n)));
+    bool ok = true;
+    for (const void *f : {reinterpret_cast<const void *>(&hartley_kernel<double, double, true, 2048>),
+                          reinterpret_cast<const void *>(&hartley_kernel<double, double, false, 2048>),
+                          reinterpret_cast<const void *>(&hartley_kernel<double, double, true, 4096>),
+                          reinterpret_cast<const void *>(&hartley_kernel<double, double, false, 4096>)})
+        ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
+    return ok;
 }
 
 // One FP64 solve through the Hartley transforms: phi_out = beta * phi_out + Hx Hy T^-1 Hy Hx in, with

@@ -36,6 +36,11 @@ ENTRY_SIGNATURES = {
     "cpp_fort_shared_pressure_fix_reference": [c_uint64, c_int, _H, _DP, _IP, _IP, _IP],
     "cpp_fort_shared_fixed_order_reduction": [c_uint64, c_int, _H, _IP, POINTER(ctypes.c_longlong),
                                               POINTER(ctypes.c_longlong), _DP],
+    "cpp_fort_shared_fixed_order_reduction_device": [c_uint64, c_int, _H, _H, _H, _IP, POINTER(ctypes.c_longlong),
+                                                     POINTER(ctypes.c_longlong), _DP],
+    "cpp_fort_shared_pressure_fix_reference_sum": [c_uint64, c_int, _H, _H, _IP, _IP, _IP],
+    "cpp_fort_shared_pressure_projection_bc": [c_uint64, c_int, _H, _H, _H, _H, _DP, _DP, _DP, _DP, _IP, _IP, _IP,
+                                               _IP, _IP],
 }
 
 

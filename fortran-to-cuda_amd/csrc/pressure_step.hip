@@ -67,18 +67,28 @@ __global__ __launch_bounds__(kPTX *kPTY) void pressure_projection_kernel(const P
     }
 }
 
-// phi -= ref on every cell of the array (the solution is fixed only up to a constant, so its ghosts shift
-// with it).
-struct ShiftArgs {
+// phi -= ref on the interior cells, the ones the solve defines (rank-3 array with two ghost layers:
+// cells 2 .. n+1 of each dimension, zero-based). The ghost layers keep what they held; nothing on the
+// step reads them (the projection takes its x and y neighbours from the interior by periodicity), and
+// a halo fill of phi refills them from the shifted interior. With ref_sum set, ref is ref_sum[0] / divisor,
+// read on the device (the reference-plane sum a queued reduction left there).
+struct ShiftArgs {  // no padding bytes: the runtime compares argument bytes to pick a graph variant
     double *phi;
-    long long n;  // cells of the array
-    double ref;
+    const double *ref_sum;
+    long long pj, pk;  // strides of phi
+    double ref, divisor;
+    int nx, ny, nz, zero;
 };
 
 __global__ __launch_bounds__(256) void pressure_shift_kernel(const ShiftArgs a)
 {
-    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < a.n; i += (long long)gridDim.x * 256)
-        a.phi[i] = a.phi[i] - a.ref;
+    const double ref = a.ref_sum ? a.ref_sum[0] / a.divisor : a.ref;
+    const long long cells = (long long)a.nx * a.ny * a.nz;
+    for (long long t = blockIdx.x * 256ll + threadIdx.x; t < cells; t += (long long)gridDim.x * 256) {
+        const long long i = t % a.nx, r = t / a.nx;
+        const long long p = (i + 2) + (r % a.ny + 2) * a.pj + (r / a.ny + 2) * a.pk;
+        a.phi[p] = a.phi[p] - ref;
+    }
 }
 
 static Sec3 pbox(std::size_t i0, std::size_t i1, std::size_t j0, std::size_t j1, std::size_t k0, std::size_t k1)
@@ -117,6 +127,37 @@ static int domain_args(fort_scope_t ctx, const fort_buffer_t *fields, int count,
     s.gy = *dt / *dy;
     s.gz = *dt / *dz;
     return FORT_SCOPE_OK;
+}
+
+static int fix_reference(fort_scope_t ctx, int mode, fort_buffer_t phi, fort_buffer_t ref_sum, const double *phi_ref,
+                         const int *nx, const int *ny, const int *nz)
+{
+    FTC_CHECK(entry_preamble(mode));
+    if ((!ref_sum && !phi_ref) || !nx || !ny || !nz)
+        return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "null shared scalar argument");
+    if (*nx <= 0 || *ny <= 0 || *nz <= 0) return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "empty pressure domain");
+    std::size_t e[3];
+    FTC_CHECK(layout3(ctx, phi, FORT_SCOPE_REAL64, 8, e));
+    if (e[0] != (std::size_t)*nx + 4 || e[1] != (std::size_t)*ny + 4 || e[2] != (std::size_t)*nz + 4)
+        return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "phi must have two ghost layers");
+    if (ref_sum) {
+        fort_scope_layout l{};
+        FTC_CHECK(fort_scope_layout_get(ctx, ref_sum, &l));
+        if (l.type != FORT_SCOPE_REAL64 || l.element_bytes != 8 || l.rank < 1 || l.extents[0] < 1)
+            return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "the reference sum is a REAL64 array");
+        FTC_CHECK(distinct_outputs({phi}, {ref_sum}));
+    }
+    std::vector<void *> dev;
+    FTC_CHECK(ref_sum ? reserve(ctx, {phi, ref_sum}, dev) : reserve(ctx, {phi}, dev));
+    ShiftArgs args{(double *)dev[0], ref_sum ? (const double *)dev[1] : nullptr, (long long)e[0],
+                   (long long)(e[0] * e[1]), phi_ref ? *phi_ref : 0.0, (double)*nx * (double)*ny, *nx, *ny, *nz, 0};
+    const Sec3 inner = pbox(2, (std::size_t)*nx + 2, 2, (std::size_t)*ny + 2, 2, (std::size_t)*nz + 2);
+    std::vector<Binding3> b{{phi, {inner}, {}, {inner}}};
+    if (ref_sum) b.push_back({ref_sum, {Sec3{{0, 0, 0}, {1, 1, 1}}}, {}, {}});
+    const long long cells = (long long)*nx * *ny * *nz;
+    const unsigned blocks = (unsigned)std::min<long long>(4096, (cells + 255) / 256);
+    return enqueue_kernel(ctx, ref_sum ? 0xd1e0000000000004ull : 0xd1e0000000000003ull,
+                          (const void *)pressure_shift_kernel, dim3(blocks), dim3(256), 0, &args, sizeof args, b);
 }
 
 }  // namespace ftc
@@ -186,25 +227,28 @@ int cpp_fort_shared_pressure_projection(fort_scope_t ctx, int mode, fort_buffer_
                           dim3(kPTX, kPTY), 0, &s, sizeof s, b);
 }
 
+int cpp_fort_shared_pressure_projection_bc(fort_scope_t ctx, int mode, fort_buffer_t u, fort_buffer_t v,
+                                           fort_buffer_t w, fort_buffer_t phi, const double *dx, const double *dy,
+                                           const double *dz, const double *dt, const int *nx, const int *ny,
+                                           const int *nz, const int *bc_x, const int *bc_y)
+{
+    FTC_CHECK(entry_preamble(mode));
+    if (!bc_x || !bc_y) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "null shared scalar argument");
+    if (*bc_x != 0 || *bc_y != 0)
+        return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "the projection implements periodic x and y only");
+    return cpp_fort_shared_pressure_projection(ctx, mode, u, v, w, phi, dx, dy, dz, dt, nx, ny, nz);
+}
+
 int cpp_fort_shared_pressure_fix_reference(fort_scope_t ctx, int mode, fort_buffer_t phi, const double *phi_ref,
                                            const int *nx, const int *ny, const int *nz)
 {
-    FTC_CHECK(entry_preamble(mode));
-    if (!phi_ref || !nx || !ny || !nz) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "null shared scalar argument");
-    if (*nx <= 0 || *ny <= 0 || *nz <= 0) return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "empty pressure domain");
-    std::size_t e[3];
-    FTC_CHECK(layout3(ctx, phi, FORT_SCOPE_REAL64, 8, e));
-    if (e[0] != (std::size_t)*nx + 4 || e[1] != (std::size_t)*ny + 4 || e[2] != (std::size_t)*nz + 4)
-        return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "phi must have two ghost layers");
-    std::vector<void *> dev;
-    FTC_CHECK(reserve(ctx, {phi}, dev));
-    const ShiftArgs args{(double *)dev[0], (long long)(e[0] * e[1] * e[2]), *phi_ref};
-    const Sec3 all = pbox(0, e[0], 0, e[1], 0, e[2]);
-    std::vector<Binding3> b(1);
-    b[0] = {phi, {all}, {}, {all}};
-    const unsigned blocks = (unsigned)std::min<long long>(4096, (args.n + 255) / 256);
-    return enqueue_kernel(ctx, 0xd1e0000000000003ull, (const void *)pressure_shift_kernel, dim3(blocks), dim3(256), 0,
-                          &args, sizeof args, b);
+    return fix_reference(ctx, mode, phi, 0, phi_ref, nx, ny, nz);
+}
+
+int cpp_fort_shared_pressure_fix_reference_sum(fort_scope_t ctx, int mode, fort_buffer_t phi, fort_buffer_t ref_sum,
+                                               const int *nx, const int *ny, const int *nz)
+{
+    return fix_reference(ctx, mode, phi, ref_sum, nullptr, nx, ny, nz);
 }
 
 }  // extern "C"

@@ -60,7 +60,8 @@ __device__ __forceinline__ void advance(const ReduceBox &b, const BoxStep &st, l
 // Slot s folds iterations s, s + slots, ... in order. The loads of kUnroll consecutive iterations are
 // issued together; the combines stay in iteration order.
 template <int OP>
-__global__ void reduce_slots(const ReduceBox b, long long n, int slots, BoxStep st, double *out)
+__device__ __forceinline__ void reduce_slots_body(const ReduceBox &b, long long n, int slots, const BoxStep &st,
+                                                  double *out)
 {
     constexpr int kUnroll = 8;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,11 +91,17 @@ __global__ void reduce_slots(const ReduceBox b, long long n, int slots, BoxStep 
     out[s] = acc;
 }
 
+template <int OP>
+__global__ void reduce_slots(const ReduceBox b, long long n, int slots, BoxStep st, double *out)
+{
+    reduce_slots_body<OP>(b, n, slots, st, out);
+}
+
 // Folds `count` partial results into 256 (thread f takes f, f + 256, ... in order). For sums of squares
 // the partial results are already squared, so the fold is a plain sum. The loads of a batch of 32 are all
 // issued before the batch is combined, in order, so one memory latency covers 32 partials.
 template <int OP>
-__global__ void reduce_fold(const double *partial, int count, double *out)
+__device__ __forceinline__ void reduce_fold_body(const double *partial, int count, double *out)
 {
     constexpr int kBatch = 32;
     const int f = threadIdx.x;
@@ -112,6 +119,12 @@ __global__ void reduce_fold(const double *partial, int count, double *out)
             if (t0 + q * kReduceFinal < count) acc = combine<FOP>(acc, v[q]);
     }
     out[f] = acc;
+}
+
+template <int OP>
+__global__ void reduce_fold(const double *partial, int count, double *out)
+{
+    reduce_fold_body<OP>(partial, count, out);
 }
 
 static BoxStep box_step(const ReduceBox &b, long long n, long long slots)
@@ -205,6 +218,105 @@ double reduce_host_fold(int op, double initial, const double *final256)
         }
     }
     return acc;
+}
+
+// ------------------------------------------------------------------------ the queued form
+// The three nests of a lowered reduction as tasks of the context's graph queue, the partial results in a
+// registered work array as the reference keeps them (compiler/resident/reductions.py:18-31), and the final
+// fold into the variable done on the device, so the value stays there until it is observed.
+// No padding bytes: the runtime compares argument bytes to find a graph variant to replay.
+struct SlotsTask {
+    ReduceBox b;
+    long long n;
+    BoxStep st;
+    double *out;
+    int slots, zero;
+};
+struct FoldTask {
+    const double *partial;
+    double *out;
+    int count, zero;
+};
+struct FinalTask {
+    const double *final256;
+    double initial;
+    double *result;
+};
+
+template <int OP>
+__global__ void reduce_slots_task(const SlotsTask a)
+{
+    reduce_slots_body<OP>(a.b, a.n, a.slots, a.st, a.out);
+}
+
+template <int OP>
+__global__ void reduce_fold_task(const FoldTask a)
+{
+    reduce_fold_body<OP>(a.partial, a.count, a.out);
+}
+
+template <int OP>
+__global__ void reduce_final_task(const FinalTask a)
+{
+    constexpr int HOP = OP == kReduceSumSquares ? kReduceSum : (OP == kReduceMaxAbs ? kReduceMax : OP);
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double acc = a.initial;
+    for (int f = 0; f < kReduceFinal; ++f) acc = combine<HOP>(acc, a.final256[f]);
+    *a.result = acc;
+}
+
+// A registered REAL64 work array: at least `count` elements along its first dimension, every other
+// dimension of extent 1, so that its first `count` elements are one contiguous section.
+static int work_array(fort_scope_t ctx, fort_buffer_t h, std::size_t count, uint32_t *rank)
+{
+    fort_scope_layout l{};
+    FTC_CHECK(fort_scope_layout_get(ctx, h, &l));
+    if (l.type != FORT_SCOPE_REAL64 || l.element_bytes != 8 || l.rank < 1 || l.rank > 3)
+        return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "reduction work arrays must be REAL64 of rank 1 to 3");
+    for (uint32_t d = 1; d < l.rank; ++d)
+        if (l.extents[d] != 1)
+            return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "reduction work arrays extend along their first dimension");
+    if (l.extents[0] < count) return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "reduction work array too small");
+    *rank = l.rank;
+    return FORT_SCOPE_OK;
+}
+
+// Elements [lo, hi) of a work array (its later dimensions have extent 1).
+static Sec3 linear(std::size_t lo, std::size_t hi, uint32_t rank)
+{
+    (void)rank;
+    return Sec3{{lo, 0, 0}, {hi, 1, 1}};
+}
+
+__global__ void reduce_store_task(const FinalTask a)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *a.result = a.initial;
+}
+
+template <int OP>
+static int enqueue_reduction(fort_scope_t ctx, fort_buffer_t x, fort_buffer_t partial, fort_buffer_t result,
+                             const ReduceBox &b, const Sec3 &box, uint32_t prank, uint32_t rrank, double initial,
+                             double *pdev, double *rdev)
+{
+    const long long n = b.n1 * b.n2 * b.n3;
+    const uint64_t key = 0xfd00000000000000ull | (uint64_t)OP;
+    const Sec3 slots_sec = linear(0, kReduceSlots, prank), final_sec = linear(kReduceSlots, kReduceSlots + kReduceFinal, prank);
+    double *final256 = pdev + kReduceSlots;
+    if (n <= kReduceDirect) {
+        SlotsTask a{b, n, box_step(b, n, kReduceFinal), final256, kReduceFinal, 0};
+        FTC_CHECK(enqueue_kernel(ctx, key | 0x100, (const void *)reduce_slots_task<OP>, dim3(1), dim3(kReduceFinal), 0,
+                                 &a, sizeof a, {{x, {box}, {}, {}}, {partial, {}, {}, {final_sec}}}));
+    } else {
+        SlotsTask a{b, n, box_step(b, n, kReduceSlots), pdev, kReduceSlots, 0};
+        FTC_CHECK(enqueue_kernel(ctx, key | 0x200, (const void *)reduce_slots_task<OP>, dim3(kReduceSlots / 256),
+                                 dim3(256), 0, &a, sizeof a, {{x, {box}, {}, {}}, {partial, {}, {}, {slots_sec}}}));
+        FoldTask f{pdev, final256, kReduceSlots, 0};
+        FTC_CHECK(enqueue_kernel(ctx, key | 0x300, (const void *)reduce_fold_task<OP>, dim3(1), dim3(kReduceFinal), 0,
+                                 &f, sizeof f, {{partial, {slots_sec}, {}, {final_sec}}}));
+    }
+    FinalTask fin{final256, initial, rdev};
+    return enqueue_kernel(ctx, key | 0x400, (const void *)reduce_final_task<OP>, dim3(1), dim3(64), 0, &fin, sizeof fin,
+                          {{partial, {final_sec}, {}, {}}, {result, {}, {}, {linear(0, 1, rrank)}}});
 }
 
 // ------------------------------------------------------------------------ the entry that allocates nothing
@@ -327,6 +439,49 @@ int ftc_fixed_order_reduction_static_v1(const double *x, long long n1, long long
     }
     ReduceBox b{x, n1, n2, n3, ld1, ld2};
     return reduce_launch_to_device(op, b, slots, final256, initial, result, s);
+}
+
+// The reduction as queued tasks: partial results in `partial` (at least 65,792 elements), the variable's
+// new value in result(1) on the device. See ftc_reduce.h.
+int cpp_fort_shared_fixed_order_reduction_device(fort_scope_t ctx, int mode, fort_buffer_t x, fort_buffer_t partial,
+                                                 fort_buffer_t result, const int *op, const long long *lo3,
+                                                 const long long *hi3, const double *initial)
+{
+    FTC_CHECK(entry_preamble(mode));
+    if (!op || !lo3 || !hi3 || !initial) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "null scalar argument");
+    if (*op < kReduceSum || *op > kReduceMaxAbs) return fort_scope_report_error(FORT_SCOPE_ARGUMENT, "unknown reduction");
+    FTC_CHECK(distinct_outputs({partial, result}, {x}));
+    std::size_t e[3];
+    FTC_CHECK(layout3(ctx, x, FORT_SCOPE_REAL64, 8, e));
+    for (int d = 0; d < 3; ++d)
+        if (lo3[d] < 0 || hi3[d] < lo3[d] || (std::size_t)hi3[d] > e[d])
+            return fort_scope_report_error(FORT_SCOPE_BOUNDARY, "reduction box outside the array");
+    uint32_t prank = 0, rrank = 0;
+    FTC_CHECK(work_array(ctx, partial, kReduceSlots + kReduceFinal, &prank));
+    FTC_CHECK(work_array(ctx, result, 1, &rrank));
+    const long long n1 = hi3[0] - lo3[0], n2 = hi3[1] - lo3[1], n3 = hi3[2] - lo3[2];
+    std::vector<void *> dev;
+    FTC_CHECK(reserve(ctx, {x, partial, result}, dev));
+    const double *base = static_cast<const double *>(dev[0]);
+    ReduceBox b{base + lo3[0] + lo3[1] * (long long)e[0] + lo3[2] * (long long)e[0] * (long long)e[1],
+                n1 ? n1 : 1, n2 ? n2 : 1, (n1 && n2) ? n3 : 0, (long long)e[0], (long long)e[0] * (long long)e[1]};
+    const Sec3 box{{(std::size_t)lo3[0], (std::size_t)lo3[1], (std::size_t)lo3[2]},
+                   {(std::size_t)hi3[0], (std::size_t)hi3[1], (std::size_t)hi3[2]}};
+    double *pdev = static_cast<double *>(dev[1]), *rdev = static_cast<double *>(dev[2]);
+    if (!n1 || !n2 || !n3) {
+        // No iterations: the variable keeps its value, bit for bit.
+        FinalTask keep{nullptr, *initial, rdev};
+        return enqueue_kernel(ctx, 0xfd00000000000500ull, (const void *)reduce_store_task, dim3(1), dim3(64), 0, &keep,
+                              sizeof keep, {{result, {}, {}, {linear(0, 1, rrank)}}});
+    }
+    switch (*op) {
+    case kReduceSum: return enqueue_reduction<kReduceSum>(ctx, x, partial, result, b, box, prank, rrank, *initial, pdev, rdev);
+    case kReduceSumSquares:
+        return enqueue_reduction<kReduceSumSquares>(ctx, x, partial, result, b, box, prank, rrank, *initial, pdev, rdev);
+    case kReduceMax: return enqueue_reduction<kReduceMax>(ctx, x, partial, result, b, box, prank, rrank, *initial, pdev, rdev);
+    case kReduceMin: return enqueue_reduction<kReduceMin>(ctx, x, partial, result, b, box, prank, rrank, *initial, pdev, rdev);
+    default: return enqueue_reduction<kReduceMaxAbs>(ctx, x, partial, result, b, box, prank, rrank, *initial, pdev, rdev);
+    }
 }
 
 }  // extern "C"

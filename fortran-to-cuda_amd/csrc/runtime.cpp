@@ -224,6 +224,20 @@ struct Domain {
     std::mutex mu;
 };
 
+#ifndef FTC_CPU_BACKEND
+// External work captured from the stream its worker was given: one child-graph node of the chain.
+struct CapturedWork {
+    hipGraph_t graph = nullptr;
+    CapturedWork() = default;
+    CapturedWork(const CapturedWork &) = delete;
+    CapturedWork &operator=(const CapturedWork &) = delete;
+    ~CapturedWork()
+    {
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+#endif
+
 struct Task {
     uint64_t code_key = 0;
     const void *function = nullptr;
@@ -233,6 +247,9 @@ struct Task {
     std::vector<std::size_t> offsets, sizes;
     std::vector<Buffer *> buffers;
     uint64_t threads = 0;
+#ifndef FTC_CPU_BACKEND
+    std::shared_ptr<CapturedWork> captured;  // set: the task is this captured graph, not a kernel
+#endif
 };
 
 #ifndef FTC_CPU_BACKEND
@@ -270,6 +287,10 @@ struct Context {
 #ifndef FTC_CPU_BACKEND
     std::unordered_map<uint64_t, GraphFamily> graphs;
     std::unordered_map<uint64_t, GraphEntry> groups;  // graph_submit_v1 / graph_execute_v1, by group key
+    // External workers run under stream capture on this stream when work is held for whole-step graphs.
+    hipStream_t capture_stream = nullptr;
+    // Executables of captured work launched on their own, destroyed once the stream has drained.
+    std::vector<hipGraphExec_t> retired;
 #endif
     void *scratch = nullptr;
     std::size_t scratch_capacity = 0;
@@ -499,6 +520,10 @@ static void wait(Context &c)
     dev::sync(c.stream);
     c.domain->stats.waits++;
     c.cstats.stream_waits++;
+#ifndef FTC_CPU_BACKEND
+    for (hipGraphExec_t e : c.retired) (void)hipGraphExecDestroy(e);
+    c.retired.clear();
+#endif
 }
 
 // ------------------------------------------------------------------------------------------------ queue
@@ -955,6 +980,18 @@ static void launch_direct(Context &c, Task &t)
     (void)t;
     fail(FORT_SCOPE_RESOURCE, "the CPU reference backend executes no device kernels");
 #else
+    if (t.captured) {
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipGraphInstantiate(&exec, t.captured->graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphLaunch(exec, c.stream);
+        if (exec) c.retired.push_back(exec);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            c.poisoned = true;
+            fail(FORT_SCOPE_EXECUTION, std::string("captured work launch failed: ") + hipGetErrorString(e));
+        }
+        return;
+    }
     std::vector<void *> params(t.offsets.size());
     for (std::size_t i = 0; i < t.offsets.size(); ++i) params[i] = t.args.data() + t.offsets[i];
     hipError_t e = hipLaunchKernel(t.function, dim3(t.grid[0], t.grid[1], t.grid[2]),
@@ -1004,9 +1041,15 @@ static bool build_graph(Context &c, GraphEntry &g, std::vector<Task> &tasks)
     hipGraphNode_t prev = nullptr;
     for (Task &t : tasks) {
         std::vector<void *> params;
-        hipKernelNodeParams p = node_params(t, params);
         hipGraphNode_t node = nullptr;
-        if (hipGraphAddKernelNode(&node, g.graph, prev ? &prev : nullptr, prev ? 1 : 0, &p) != hipSuccess) {
+        hipError_t e;
+        if (t.captured) {
+            e = hipGraphAddChildGraphNode(&node, g.graph, prev ? &prev : nullptr, prev ? 1 : 0, t.captured->graph);
+        } else {
+            hipKernelNodeParams p = node_params(t, params);
+            e = hipGraphAddKernelNode(&node, g.graph, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+        }
+        if (e != hipSuccess) {
             (void)hipGetLastError();
             destroy_graph(c, g);
             return false;
@@ -1080,6 +1123,7 @@ static void flush(Context &c)
                                      static_cast<uint32_t>(t.shared)};
     };
     GraphEntry *g = nullptr;
+    bool fresh = false;
     for (GraphEntry &v : family.variants) {
         bool same = true;
         for (std::size_t i = 0; same && i < tasks.size(); ++i)
@@ -1102,6 +1146,7 @@ static void flush(Context &c)
             return;
         }
         c.gstats.live_graphs++;
+        fresh = true;
     } else {
         // Every variant slot is taken: update the least recently used one in place.
         g = &family.variants.front();
@@ -1110,7 +1155,7 @@ static void flush(Context &c)
         for (std::size_t i = 0; i < tasks.size(); ++i) {
             Task &t = tasks[i];
             std::vector<uint32_t> dims = dims_of(t);
-            if (g->args[i] == t.args && g->dims[i] == dims) continue;
+            if (t.captured || (g->args[i] == t.args && g->dims[i] == dims)) continue;
             std::vector<void *> params;
             hipKernelNodeParams p = node_params(t, params);
             hipError_t e = hipGraphExecKernelNodeSetParams(g->exec, g->nodes[i], &p);
@@ -1124,6 +1169,19 @@ static void flush(Context &c)
             c.gstats.parameter_updates++;
         }
         c.gstats.replays++;
+    }
+    if (!fresh) {
+        // Captured work is captured anew each time; its node takes the new capture's parameters.
+        for (std::size_t i = 0; i < tasks.size(); ++i) {
+            if (!tasks[i].captured) continue;
+            hipError_t e = hipGraphExecChildGraphNodeSetParams(g->exec, g->nodes[i], tasks[i].captured->graph);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                c.poisoned = true;
+                fail(FORT_SCOPE_EXECUTION, std::string("captured work update failed: ") + hipGetErrorString(e));
+            }
+            c.gstats.parameter_updates++;
+        }
     }
     g->last_use = ++family.clock;
     hipError_t e = hipGraphLaunch(g->exec, c.stream);
@@ -1209,6 +1267,11 @@ static void release_context(Context &c)
     c.graphs.clear();
     for (auto &kv : c.groups) destroy_graph(c, kv.second);
     c.groups.clear();
+    if (!c.retired.empty() && c.stream_ready) dev::sync(c.stream);
+    for (hipGraphExec_t e : c.retired) (void)hipGraphExecDestroy(e);
+    c.retired.clear();
+    if (c.capture_stream) (void)hipStreamDestroy(c.capture_stream);
+    c.capture_stream = nullptr;
 #endif
     if (c.scratch) dev::release(c.scratch);
     c.scratch = nullptr;
@@ -2330,6 +2393,114 @@ int fort_scope_graph_stats_get_v1(fort_scope_t h, fort_scope_graph_stats_v1 *sta
     }, Mode::Query);
 }
 
+#ifndef FTC_CPU_BACKEND
+// The views an external worker receives of its bound buffers (scoped_runtime.h:247-263).
+static void external_views(Context &c, const std::vector<fort_scope_plan_binding> &plan,
+                           std::vector<fort_scope_external_view_v1> &views, std::vector<std::vector<std::size_t>> &held)
+{
+    views.resize(plan.size());
+    held.assign(plan.size(), {});
+    for (std::size_t i = 0; i < plan.size(); ++i) {
+        Buffer &b = buffer(c, plan[i].buffer);
+        std::size_t s[kMaxRank];
+        strides(b, s);
+        for (uint32_t d = 0; d < b.rank; ++d) held[i].push_back(s[d] * b.elem);
+        views[i] = fort_scope_external_view_v1{b.handle, b.generation, b.type, b.rank, b.elem, b.bytes,
+                                               b.extents, held[i].data(), b.lower_bounds, b.device};
+    }
+}
+
+// When the context holds its work for whole-step graphs (FORT_SCOPE_GRAPH_QUEUE_WORK=0), an external
+// worker runs under stream capture instead of on the context stream: nothing executes, and what it
+// enqueued becomes one task of the queue (a child-graph node of the step's chain graph), committed like a
+// kernel task. The key of the task is the captured topology (node types, kernels and launch shapes), so a
+// step that repeats replays its graph and only refreshes the captured node's parameters. Returns false,
+// with nothing changed, where the work cannot be captured (a worker that synchronises, reads back or
+// declines, an empty capture): the caller then runs the worker directly on the context stream.
+static bool capture_external(Context &c, const std::vector<fort_scope_plan_binding> &plan,
+                             fort_scope_external_worker_v1 worker, void *user, fort_scope_external_report_v1 *report)
+{
+    if (!c.graphs_enabled || !c.queue_enabled || c.queue_work != 0 || c.poisoned) return false;
+    ensure_stream(c);
+    dev::set(c.domain->device);
+    if (!c.capture_stream &&
+        hipStreamCreateWithFlags(&c.capture_stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        c.capture_stream = nullptr;
+        return false;
+    }
+    std::vector<void *> devices(plan.size());
+    prepare_device(c, plan.data(), plan.size(), devices.data(), false);
+    auto cancel = [&] {
+        for (auto &p : plan) end_access(c, buffer(c, p.buffer), true, true);
+    };
+    std::vector<fort_scope_external_view_v1> views;
+    std::vector<std::vector<std::size_t>> held;
+    external_views(c, plan, views, held);
+    auto work = std::make_shared<CapturedWork>();
+    if (hipStreamBeginCapture(c.capture_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        cancel();
+        return false;
+    }
+    int started = -1, rc = FORT_SCOPE_EXECUTION;
+    try {
+        rc = worker(user, views.data(), views.size(), reinterpret_cast<void *>(c.capture_stream), &started);
+    } catch (...) {
+        rc = FORT_SCOPE_EXECUTION;
+    }
+    const hipError_t ended = hipStreamEndCapture(c.capture_stream, &work->graph);
+    (void)hipGetLastError();
+    std::size_t count = 0;
+    bool usable = rc == FORT_SCOPE_OK && started == 1 && ended == hipSuccess && work->graph &&
+                  hipGraphGetNodes(work->graph, nullptr, &count) == hipSuccess && count > 0;
+    Task task;
+    if (usable) {
+        std::vector<hipGraphNode_t> nodes(count);
+        usable = hipGraphGetNodes(work->graph, nodes.data(), &count) == hipSuccess;
+        uint64_t key = 0x6a09e667f3bcc909ull;
+        for (std::size_t i = 0; usable && i < count; ++i) {
+            hipGraphNodeType type;
+            usable = hipGraphNodeGetType(nodes[i], &type) == hipSuccess;
+            key = fnv(key, &type, sizeof type);
+            if (usable && type == hipGraphNodeTypeKernel) {
+                hipKernelNodeParams kp{};
+                usable = hipGraphKernelNodeGetParams(nodes[i], &kp) == hipSuccess;
+                const uint32_t shape[7] = {kp.gridDim.x, kp.gridDim.y, kp.gridDim.z, kp.blockDim.x, kp.blockDim.y,
+                                           kp.blockDim.z, kp.sharedMemBytes};
+                key = fnv(key, &kp.func, sizeof kp.func);
+                key = fnv(key, shape, sizeof shape);
+            }
+        }
+        (void)hipGetLastError();
+        task.code_key = key;
+    }
+    if (!usable) {
+        // Nothing ran under capture: the accesses are withdrawn and the worker runs directly.
+        cancel();
+        return false;
+    }
+    task.captured = work;
+    task.threads = 1;
+    for (const auto &p : plan) {
+        Buffer &b = buffer(c, p.buffer);
+        b.queued++;
+        task.buffers.push_back(&b);
+    }
+    for (const auto &p : plan) end_access(c, buffer(c, p.buffer), true, false);
+    c.qstats.accepted_tasks++;
+    c.queue.push_back(std::move(task));
+    c.queued_threads += 1;
+    c.qstats.pending_tasks = c.queue.size();
+    if (c.queue.size() > c.qstats.peak_pending_tasks) c.qstats.peak_pending_tasks = c.queue.size();
+    if (report) {
+        report->started = 1;
+        report->committed = 1;
+    }
+    return true;
+}
+#endif
+
 int fort_scope_external_execute_v1(fort_scope_t h, const fort_scope_external_call_v1 *call,
                                    fort_scope_external_worker_v1 worker, void *user,
                                    fort_scope_external_report_v1 *report)
@@ -2353,8 +2524,11 @@ int fort_scope_external_execute_v1(fort_scope_t h, const fort_scope_external_cal
                     "external binding type, rank or element size mismatch");
             plan[i].buffer = eb.buffer;
             plan[i].access = eb.access;
-            drain_buffer(c, b);
         }
+#ifndef FTC_CPU_BACKEND
+        if (capture_external(c, plan, worker, user, report)) return;
+#endif
+        for (auto &pb : plan) drain_buffer(c, buffer(c, pb.buffer));
         std::vector<void *> devices(plan.size());
         prepare_device(c, plan.data(), plan.size(), devices.data(), false);
         std::vector<fort_scope_external_view_v1> views(plan.size());

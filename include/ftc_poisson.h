@@ -115,12 +115,26 @@ int cpp_fort_shared_pressure_projection(fort_scope_t ctx, int mode, fort_buffer_
                                         fort_buffer_t w, fort_buffer_t phi, const double *dx, const double *dy,
                                         const double *dz, const double *dt, const int *nx, const int *ny,
                                         const int *nz);
-/* ELMM's fixpressuretoreference: phi -= *phi_ref on every cell of Phi (extents n+4; the solution is fixed
- * only up to a constant, so its ghosts shift with it). The caller forms *phi_ref as the mean of the
- * reference plane, which templates/serial-pressure-reference.f90 licenses as a free-order sum on the
- * device: cpp_fort_shared_fixed_order_reduction over the plane, divided by nx*ny. */
+/* The projection with the solve's x and y boundary codes (poisson_solver_config_v1.boundary_conditions[0]
+ * and [2]: 0 periodic, 4 Neumann). The projection implements periodic x and y only: a Neumann wall in x or
+ * y is refused with BOUNDARY before any work, so a walled solve is never paired with periodic face
+ * differences. No counterpart in the reference beyond the projection it checks. */
+int cpp_fort_shared_pressure_projection_bc(fort_scope_t ctx, int mode, fort_buffer_t u, fort_buffer_t v,
+                                           fort_buffer_t w, fort_buffer_t phi, const double *dx, const double *dy,
+                                           const double *dz, const double *dt, const int *nx, const int *ny,
+                                           const int *nz, const int *bc_x, const int *bc_y);
+/* ELMM's fixpressuretoreference: phi -= *phi_ref on the interior cells of Phi (extents n+4), the cells the
+ * solve defines; the ghost layers keep their values (the projection reads none of them, and a halo fill of
+ * phi refills them from the shifted interior). The caller forms *phi_ref as the mean of the reference
+ * plane, which templates/serial-pressure-reference.f90 licenses as a free-order sum on the device:
+ * cpp_fort_shared_fixed_order_reduction over the plane, divided by nx*ny. */
 int cpp_fort_shared_pressure_fix_reference(fort_scope_t ctx, int mode, fort_buffer_t phi, const double *phi_ref,
                                            const int *nx, const int *ny, const int *nz);
+/* The same shift with the reference-plane sum read on the device: phi -= ref_sum(1) / (nx*ny), where
+ * ref_sum is a registered REAL64 array that cpp_fort_shared_fixed_order_reduction_device filled. With
+ * both queued, no value crosses to the host inside the step, so the step can replay as one graph. */
+int cpp_fort_shared_pressure_fix_reference_sum(fort_scope_t ctx, int mode, fort_buffer_t phi, fort_buffer_t ref_sum,
+                                               const int *nx, const int *ny, const int *nz);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,17 @@ enum ftc_reduction { FTC_REDUCE_SUM = 0, FTC_REDUCE_SUM_SQUARES = 1, FTC_REDUCE_
 int cpp_fort_shared_fixed_order_reduction(fort_scope_t ctx, int mode, fort_buffer_t x, const int *op,
                                           const long long *lo3, const long long *hi3, double *value);
 
+/* The same reduction as tasks of the context's graph queue, as the reference lowers it
+ * (compiler/resident/reductions.py:18-31): the 65,536 slot and 256 fold values go to the registered REAL64
+ * work array `partial` (at least 65,792 elements along its first dimension, later dimensions of extent 1),
+ * and the fold of the 256 values into *initial, in index order, to result(1) on the device. Nothing is
+ * synchronised and nothing returns to the host: the value is observed like any array, or read by a later
+ * entry (cpp_fort_shared_pressure_fix_reference_sum). Bit-identical to
+ * cpp_fort_shared_fixed_order_reduction with *value = *initial. */
+int cpp_fort_shared_fixed_order_reduction_device(fort_scope_t ctx, int mode, fort_buffer_t x, fort_buffer_t partial,
+                                                 fort_buffer_t result, const int *op, const long long *lo3,
+                                                 const long long *hi3, const double *initial);
+
 /* No counterpart in the reference. For callers that cannot allocate device memory (code under stream or
  * graph capture, serving loops, harnesses that forbid allocation): the same fixed-order reduction of the
  * box x[i + j*ld1 + k*ld2], 0 <= i < n1, 0 <= j < n2, 0 <= k < n3, of a double array the caller has in

@@ -47,9 +47,10 @@ def traffic_from_profiles(name: str):
 
 
 def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 10.0), reps=10, check=None):
-    """Pressure Poisson projection solve: FP64 direct solve vs the mixed solve (split-bf16 transforms,
-    FP64 residuals and refinement, three solves), both resident, timed with HIP events over `reps` solves.
-    Returns the section dict and the two solutions (interiors)."""
+    """Pressure Poisson projection solve: the FP64 direct solve (FFT/Hartley transforms in x and y for
+    periodic power-of-two sizes, FP64 z solves) vs the mixed solve (split-bf16 transforms, FP64 residuals
+    and refinement, three solves), both resident, timed with HIP events over `reps` solves. Returns the
+    section dict and the two solutions (interiors)."""
     pressure = graft._module("pressure")
     nx, ny, nz = n
     b = fields.divergence_rhs(nx, ny, nz, h)
@@ -90,10 +91,17 @@ def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 1
     out["speedup_mixed_over_fp64"] = round(out["fp64"]["ms_per_solve"] / out["mixed"]["ms_per_solve"], 3)
     scale = np.abs(sols["fp64"]).max()
     out["max_rel_diff_mixed_vs_fp64"] = float(np.abs(sols["mixed"] - sols["fp64"]).max() / scale)
-    # Four transforms per solve, each 2*N*K flops with K = nx (x transforms) or ny (y transforms).
-    out["transform_flops_fp64"] = 2 * N * (2 * nx + 2 * ny)
-    out["fp64_mfma_tflops"] = round(out["transform_flops_fp64"] / (out["fp64"]["ms_per_solve"] * 1e-3) / 1e12, 2)
+    pow2 = all(d >= 4 and d & (d - 1) == 0 for d in (nx, ny))
+    out["fp64"]["method"] = "fft (Hartley) x/y passes + FP64 z Thomas" if pow2 else "dense FP64 MFMA transforms"
+    # The FP64 FFT solve streams the field five times (x, y forward, z, y, x inverse), each a read and a
+    # write of N doubles; its floor is one read of b and one write of phi.
+    pass_bytes = 5 * 2 * 8 * N
+    t = out["fp64"]["ms_per_solve"] * 1e-3
+    out["fp64"]["pass_bytes"] = pass_bytes
+    out["fp64"]["pass_GB_per_s"] = round(pass_bytes / t / 1e9, 1)
+    out["fp64"]["hbm_frac_of_pass_bytes"] = round(pass_bytes / t / 1e9 / HBM_PEAK_GBS, 4)
     out["lower_bound_bytes"] = 8 * 2 * N
+    out["fp64"]["hbm_frac_of_lower_bound"] = round(out["lower_bound_bytes"] / t / 1e9 / HBM_PEAK_GBS, 4)
     out["rhs_padded_cells"] = P
     return out, sols, b
 
@@ -101,10 +109,13 @@ def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 1
 def projected_step_section(ftc, scope, lib, fields, n=(512, 512, 128), steps=10, dx=10.0, dt=0.2, cs=0.17):
     """A whole resident time step in ELMM's order (LES step, divergence into the solver's RHS, pressure
     solve, projection, the shift of phi by its reference-plane mean), all entries on the device in one
-    epoch, ping-ponging two field sets: the FP64
-    solve against the mixed one (three solves, no host synchronisation). Timed with HIP events over
-    `steps` steps after one warm-up step; both modes start from the same fields, and the field difference
-    between them is reported after warm-up + `steps` steps."""
+    epoch, ping-ponging two field sets: the FP64 solve against the mixed one (three solves, no host
+    synchronisation). The context holds each step's work until the step's flush
+    (FORT_SCOPE_GRAPH_QUEUE_WORK=0): the solver's launches are captured into one node, the plane sum stays
+    on the device (a queued fixed-order reduction into a one-value array that the shift reads), so each
+    step is one graph replay. Timed with HIP events over `steps` steps after three warm-up steps (the first
+    sighting and the two ping-pong instantiations); both modes start from the same fields, and the field
+    difference between them is reported after warm-up + `steps` steps."""
     pressure = graft._module("pressure")
     nx, ny, nz = n
     start = fields.les_fields(nx, ny, nz)
@@ -114,14 +125,25 @@ def projected_step_section(ftc, scope, lib, fields, n=(512, 512, 128), steps=10,
     k_ref = nz // 2 + 1  # the reference plane (1-based), at array index k_ref + 1 of phi(-1:)
     plane_lo = (ctypes.c_longlong * 3)(2, 2, k_ref + 1)
     plane_hi = (ctypes.c_longlong * 3)(nx + 2, ny + 2, k_ref + 2)
+    warm = 3
     for name, mixed in (("fp64", False), ("mixed", True)):
         a = [x.copy(order="F") for x in start]
         b = [np.zeros_like(start[0], order="F") for _ in range(4)]
         rhs = np.zeros_like(start[0], order="F")
         phi = np.zeros((nx + 4, ny + 4, nz + 4), order="F")
-        dom = scope.Domain(lib)
+        partial, ref = np.zeros(65536 + 256), np.zeros(1)
+        held = os.environ.get("FORT_SCOPE_GRAPH_QUEUE_WORK")
+        os.environ["FORT_SCOPE_GRAPH_QUEUE_WORK"] = "0"
+        try:
+            dom = scope.Domain(lib)
+        finally:
+            if held is None:
+                del os.environ["FORT_SCOPE_GRAPH_QUEUE_WORK"]
+            else:
+                os.environ["FORT_SCOPE_GRAPH_QUEUE_WORK"] = held
         ha, hb = [dom.bind(x) for x in a], [dom.bind(x, initialized=True) for x in b]
         hr, hp = dom.bind(rhs, lower=(0, 0, 0), initialized=False), dom.bind(phi, lower=(-1, -1, -1))
+        hpart, href = dom.bind(partial, initialized=False), dom.bind(ref, initialized=False)
         solver = pressure.PressureSolver(lib, dom, n, (nx * dx, ny * dx, nz * dx), mixed)
         # A fixed number of solves and no residual norms: no host synchronisation inside the step.
         solver.refinement(1e-12, 3 if mixed else 1, False)
@@ -135,35 +157,38 @@ def projected_step_section(ftc, scope, lib, fields, n=(512, 512, 128), steps=10,
             scope.check(lib, lib.cpp_fort_shared_les_step(dom.ctx, 1, *dst, *src, *sc, ftc.dp(cs), *ext, ftc.ip(1)))
             scope.check(lib, lib.cpp_fort_shared_pressure_divergence(dom.ctx, 1, hr, *dst[:3], *sc, *ext))
             solver.solve(hp, hr)
-            scope.check(lib, lib.cpp_fort_shared_pressure_projection(dom.ctx, 1, *dst[:3], hp, *sc, *ext))
+            scope.check(lib, lib.cpp_fort_shared_pressure_projection_bc(dom.ctx, 1, *dst[:3], hp, *sc, *ext,
+                                                                        ftc.ip(0), ftc.ip(0)))
             # fixpressuretoreference: phi less the mean of its reference plane, a fixed-order device sum
-            # whose value comes back to the host, as ELMM's step has it.
-            total = ctypes.c_double(0.0)
-            scope.check(lib, lib.cpp_fort_shared_fixed_order_reduction(dom.ctx, 1, hp, ftc.ip(0), plane_lo, plane_hi,
-                                                                         ctypes.byref(total)))
-            scope.check(lib, lib.cpp_fort_shared_pressure_fix_reference(dom.ctx, 1, hp, ftc.dp(total.value / (nx * ny)),
-                                                                         *ext))
+            # left on the device for the shift.
+            scope.check(lib, lib.cpp_fort_shared_fixed_order_reduction_device(dom.ctx, 1, hp, hpart, href, ftc.ip(0),
+                                                                              plane_lo, plane_hi, ftc.dp(0.0)))
+            scope.check(lib, lib.cpp_fort_shared_pressure_fix_reference_sum(dom.ctx, 1, hp, href, *ext))
+            dom.flush()
             return dst
 
-        step(0)
-        dom.flush()
+        for k in range(warm):
+            step(k)
         dom.wait()
+        before = dom.graph_stats()
         lib.ftc_event_record(ev0, stream)
-        for k in range(1, steps + 1):
+        for k in range(warm, warm + steps):
             last = step(k)
-        dom.flush()
         lib.ftc_event_record(ev1, stream)
         ms = ctypes.c_float()
         lib.ftc_event_elapsed_ms(ev0, ev1, ctypes.byref(ms))
+        after = dom.graph_stats()
         dom.observe(*last[:3])
-        finals[name] = [x.copy() for x in ((b if steps % 2 == 0 else a)[:3])]
+        finals[name] = [x.copy() for x in ((b if (warm + steps) % 2 == 1 else a)[:3])]
         solver.destroy()
         dom.close()
-        out[name] = {"ms_per_step": round(ms.value / steps, 4)}
+        out[name] = {"ms_per_step": round(ms.value / steps, 4),
+                     "graph_submissions_per_step": (after.submissions - before.submissions) / steps,
+                     "graph_replays_per_step": (after.replays - before.replays) / steps}
     out["speedup_mixed_over_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
     out["max_rel_field_diff_mixed_vs_fp64"] = max(
         float(np.abs(m - f).max() / np.abs(f).max()) for m, f in zip(finals["mixed"], finals["fp64"]))
-    out["steps"] = steps + 1
+    out["steps"] = warm + steps
     return out
 
 

@@ -7,9 +7,9 @@ buffer sets. Prints one JSON line:
   value      ms per LES step (lower is better), timed over --steps steps between two device syncs
   roofline   algorithmic HBM bytes of the step kernel, 8*(4P + 4I) with P padded and I interior cells,
              divided by its average duration from HIP events recorded on the context stream
-  cpu_baseline  the oracle's OpenMP C restatement of the same step on the host cores, on a bounded sample
-             (one warm-up and --cpu-steps steps of the full grid), also bit-compared with the first
-             device step
+  cpu_baseline  the repository's native Fortran build of the same step (oracle/_build/les_native) on the
+             host cores: one warm-up step, 7 repeats of --cpu-steps steps, the median; also with 4
+             threads; its warm-up step is bit-compared with the first device step
 """
 from __future__ import annotations
 
@@ -280,6 +280,66 @@ def ensemble_section(lib, fields, count=64, steps=10, tune_cache=None):
     return out
 
 
+NATIVE = ROOT / "oracle" / "_build" / "les_native"
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_native(path_in, path_out, nx, ny, nz, steps, repeats, threads):
+    """Runs the native Fortran step (oracle/_build/les_native) and returns its per-repeat ms per step."""
+    import subprocess
+
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores",
+               OMP_DYNAMIC="FALSE")
+    cmd = [str(NATIVE), str(nx), str(ny), str(nz), str(steps), str(repeats), str(path_in)]
+    if path_out:
+        cmd.append(str(path_out))
+    out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600, check=True).stdout
+    reps = [float(line.split()[-1]) for line in out.splitlines() if line.startswith("repeat ")]
+    median = float(next(line.split()[-1] for line in out.splitlines() if line.startswith("median_ms_per_step")))
+    return reps, median
+
+
+def native_baseline(initial, first, nx, ny, nz, steps):
+    """The CPU baseline: the repository's native Fortran build of the same LES step (amdflang or gfortran,
+    -O3 -fopenmp -ffp-contract=off), on the same seeded fields, timed with system_clock around its step
+    loop: one warm-up step, then 7 repeats of `steps` steps, the median; once with every core this process
+    may use and once with 4 threads (the reference harness's default). Its warm-up step is also the
+    full-size bit check of the first device step."""
+    import tempfile
+
+    stamp = NATIVE.with_name("les_native.compiler")
+    compiler = stamp.read_text().strip() if stamp.exists() else "unknown compiler"
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or cores
+    threads = min(threads, cores)
+    with tempfile.TemporaryDirectory(prefix="ftc_native_") as tmp:
+        path_in, path_out = Path(tmp) / "in.bin", Path(tmp) / "out.bin"
+        np.concatenate([a.ravel(order="F") for a in initial]).tofile(path_in)
+        log(f"bench: native Fortran CPU baseline, {threads} threads")
+        reps, median = run_native(path_in, path_out, nx, ny, nz, steps, 7, threads)
+        raw = np.fromfile(path_out, dtype=np.float64)
+        size = (nx + 2) * (ny + 2) * (nz + 2)
+        parity = all(np.array_equal(raw[f * size:(f + 1) * size], g.ravel(order="F")) for f, g in enumerate(first))
+        log("bench: native Fortran CPU baseline, 4 threads")
+        reps4, median4 = run_native(path_in, None, nx, ny, nz, 1, 7, min(4, cores))
+    cpu = {"value": round(median, 3), "unit": "ms/step", "cores": threads, "kind": "port",
+           "build": "native Fortran of the same step (oracle/native/les_native.f90): " + compiler,
+           "sample": f"full {nx}x{ny}x{nz} grid: 1 warm-up step, then 7 repeats of {steps} steps, median of the "
+                     f"repeats; OMP_PROC_BIND=close OMP_PLACES=cores",
+           "repeats_ms_per_step": reps, "cpu_model": cpu_model(),
+           "threads_4": {"value": round(median4, 3), "repeats_ms_per_step": reps4}}
+    return cpu, parity
+
+
 def main() -> int:
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -288,7 +348,8 @@ def main() -> int:
     p.add_argument("--nx", type=int, default=512)
     p.add_argument("--ny", type=int, default=512)
     p.add_argument("--nz", type=int, default=128)
-    p.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps for the CPU baseline (0: skip)")
+    p.add_argument("--cpu-steps", type=int, default=2,
+                   help="steps per timed repeat of the native CPU baseline (0: skip)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only time a plain 2 GiB device copy (HBM ceiling and byte-counter calibration)")
     p.add_argument("--no-extras", action="store_true",
@@ -409,18 +470,7 @@ def main() -> int:
                 extras["chain_contraction"][name]["max_rel_error_vs_fp64_cpu"] = float(
                     np.abs(D - ref).max() / np.abs(ref).max())
 
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-        log(f"bench: CPU baseline with {threads} threads")
-        want = oracle.les_step(*initial, dx, dy, dz, dt, cs)  # warm-up, and the parity reference
-        parity = all(np.array_equal(g.view(np.uint64), w.view(np.uint64)) for g, w in zip(first, want))
-        t0 = time.perf_counter()
-        cur = want
-        for _ in range(args.cpu_steps):
-            cur = oracle.les_step(*cur, dx, dy, dz, dt, cs)
-        cpu_ms = (time.perf_counter() - t0) * 1e3 / args.cpu_steps
-        cpu = {"value": round(cpu_ms, 3), "unit": "ms/step", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} full {nx}x{ny}x{nz} steps after one warm-up step, OpenMP C oracle "
-                         f"(gcc -O2 -ffp-contract=off)"}
+        cpu, parity = native_baseline(initial, first, nx, ny, nz, args.cpu_steps)
 
     line = {
         "metric": METRIC,

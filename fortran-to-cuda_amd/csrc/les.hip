@@ -62,6 +62,19 @@ static int choose_kchunk(int nx, int ny, int kplanes, int rows = 1, int resident
     return std::max(std::min(kchunk, kplanes), std::min(4, kplanes));
 }
 
+// Compute units of the current device (256 on an MI355X), looked up once per device.
+static int compute_units()
+{
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int n = 0;
+        cached[dev] = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0 ? n : 256;
+    }
+    return cached[dev];
+}
+
 template <int MASK, bool SGS, bool STEP>
 static const void *kernel_of()
 {
@@ -197,6 +210,8 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     // ghost rows share its L2. The XCD-contiguous order (FTC_LES_XCD_ORDER) also shares the x neighbours'
     // lines, but only between tiles resident at the same time; over four rounds of workgroups it read more.
     a.tile_order = std::getenv("FTC_LES_XCD_ORDER") ? 1 : 0;
+    const bool persistent = staged && std::getenv("FTC_LES_PERSISTENT");
+    if (persistent) a.tile_order = 2;
     const std::size_t X = nx, Y = ny, Z = nz;
     const Sec3 full = box(0, X + 2, 0, Y + 2, 0, Z + 2);
     std::vector<Binding3> b(8);
@@ -224,7 +239,15 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     else if (step) fn = sgs ? kernel_of<15, true, true>() : kernel_of<15, false, true>();
     else fn = sgs ? kernel_of<15, true, false>() : kernel_of<15, false, false>();
     const uint64_t key = 0x1e5000000000000full ^ ((uint64_t)step << 8) ^ ((uint64_t)sgs << 9);
-    return enqueue_kernel(ctx, key, fn, grid_of(a, nz, rows), dim3(TX, TY, 1), 0, &a, sizeof a, b);
+    dim3 grid = grid_of(a, nz, rows);
+    if (persistent) {
+        // One resident round: the workgroups the device holds at once (three, or two with two rows, per
+        // CU), a multiple of eight, and no more than there are tiles (rounded up to eight).
+        const unsigned tiles = grid.x * grid.y * grid.z;
+        const unsigned resident = (unsigned)compute_units() * (rows == 2 ? 2u : 3u) / kXcds * kXcds;
+        grid = dim3(std::max(kXcds, std::min(resident, (tiles + kXcds - 1) / kXcds * kXcds)), 1, 1);
+    }
+    return enqueue_kernel(ctx, key, fn, grid, dim3(TX, TY, 1), 0, &a, sizeof a, b);
 }
 
 }  // namespace ftc

@@ -19,7 +19,8 @@ struct LesArgs {
     int nx, ny, nz;  // i loop 2..nx+1, j loop 2..ny+1, U/V/S k loop 2..nz+1
     int kw;          // W k loop 2..kw
     int kchunk;
-    int tile_order;  // 0: tiles in launch order; 1: each XCD takes a contiguous run of tiles (xcd.hpp)
+    int tile_order;  // 0: tiles in launch order; 1: each XCD takes a contiguous run of tiles (xcd.hpp);
+                     // 2: persistent banded walk (one resident round, les_sgs3_kernel)
     long long sj, sk;                   // strides of j and k in elements
     double c25[3], c50[3], c125[3];     // 0.25/d, 0.5/d, 0.125/d for d = dx, dy, dz
     double sgs[10];                     // 1/d, 0.25/d, 1/d^2 (x, y, z each), (Cs*Delta)^2
@@ -723,6 +724,21 @@ template <bool STEP, int ROWS>
 __global__ __launch_bounds__(NTH, ROWS == 1 ? 3 : 2) void les_sgs3_kernel(const LesArgs a)
 {
     __shared__ Sgs3Lds<ROWS> L;
+    if (a.tile_order == 2) {
+        // Persistent banded walk: one resident round of workgroups (a multiple of eight). XCD x owns the
+        // contiguous tile range [T x / 8, T (x+1) / 8) of the T tiles in x-fastest order, and its
+        // workgroups take consecutive slots of that range in step, so the tiles resident on one XCD at a
+        // time form a band whose x and y neighbours re-read each other's ring rows from the same L2.
+        // Every tile belongs to one range and one slot of it: each is computed exactly once.
+        const unsigned gx = (a.nx + TX - 1) / TX, gy = (a.ny + TY * ROWS - 1) / (TY * ROWS);
+        const unsigned gz = (a.nz + a.kchunk - 1) / a.kchunk;
+        const unsigned total = gx * gy * gz, per = gridDim.x / kXcds;
+        const unsigned x = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+        const unsigned lo = (unsigned)((unsigned long long)total * x / kXcds);
+        const unsigned hi = (unsigned)((unsigned long long)total * (x + 1) / kXcds);
+        for (unsigned t = lo + slot; t < hi; t += per) les_sgs3_body<STEP, ROWS>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
+        return;
+    }
     const unsigned gx = gridDim.x, gy = gridDim.y;
     unsigned t = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     // Tiles x fastest, then y, then the k chunk: with the XCD-contiguous order the x and y neighbours

@@ -114,11 +114,13 @@ def test_les_steps_with_graph_replay_are_bit_exact_including_halos(ftc, scope, h
 
 
 @pytest.mark.parametrize("variant", [{"FTC_LES_ROWS": "2"}, {"FTC_LES_XCD_ORDER": "1"}, {"FTC_LES_UNSTAGED": "1"},
-                                     {"FTC_LES_KCHUNK": "5"}])
+                                     {"FTC_LES_KCHUNK": "5"}, {"FTC_LES_PERSISTENT": "1"},
+                                     {"FTC_LES_PERSISTENT": "1", "FTC_LES_ROWS": "2"}])
 @pytest.mark.parametrize("n", [(70, 37, 19), (130, 9, 33)])
 def test_selectable_step_variants_are_bit_exact(ftc, scope, hip, oracle, fields, monkeypatch, variant, n):
     # The entry reads these switches at every call: two rows per thread, the XCD-contiguous tile order,
-    # the unstaged kernel, and a forced k chunk. Each must give the default kernel's (the oracle's) bits.
+    # the unstaged kernel, a forced k chunk, and the persistent banded walk (one resident round of
+    # workgroups). Each must give the default kernel's (the oracle's) bits.
     for key, value in variant.items():
         monkeypatch.setenv(key, value)
     U, V, W, S = fields.les_fields(*n)

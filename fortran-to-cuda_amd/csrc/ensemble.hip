@@ -64,18 +64,29 @@ struct ColItem {
 constexpr int kCols = 64;
 
 // ------------------------------------------------------------------------------------------------ kernels
-template <bool STEP>
-// args is __restrict__: no store of the step writes a member's argument block, so its fields can stay in
-// registers across the body's stores.
+// One persistent launch per tile width (32 x 8 or 64 x 4 columns) over the LES tiles of the members
+// that take that width. args is __restrict__: no store of the step writes a member's argument block, so
+// its fields can stay in registers across the body's stores.
+template <bool STEP, int WX>
 __global__ __launch_bounds__(NTH, 3) void ens_les_kernel(const LesArgs *__restrict__ args,
                                                          const TileItem *__restrict__ items, int n)
 {
-    __shared__ Sgs3Lds<1> L;
+    __shared__ Sgs3Lds<1, WX> L;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const TileItem t = items[it];
-        les_sgs3_body<STEP>(args[t.member], L, t.bx, t.by, t.bz);
+        les_sgs3_body<STEP, 1, WX>(args[t.member], L, t.bx, t.by, t.bz);
         __syncthreads();
     }
+}
+
+// The LES tile width of a member under a tile policy: 0 every member 64 wide, 1 every member 32 wide,
+// 2 per member the width that leaves fewer idle lanes in the last tile column (64 on a tie).
+static int les_width(int policy, int nx)
+{
+    if (policy == 0) return 64;
+    if (policy == 1) return 32;
+    const int idle64 = (nx + 63) / 64 * 64 - nx, idle32 = (nx + 31) / 32 * 32 - nx;
+    return idle32 < idle64 ? 32 : 64;
 }
 
 // rhs = div(U*) / dt on every cell of the member (U* = buffer set `src`).
@@ -318,6 +329,9 @@ struct ftc_ensemble {
     std::vector<int> nx, ny, nz;
     // tuning
     int les_kchunk = 16, gemm_tile = 1, blocks_per_cu = 4, cus = 256;
+    int les_policy = 0;          // LES tile widths: see les_width
+    std::vector<int> les_w;      // the width each member takes
+    int n_les32 = 0;             // LES items of 32-wide members, after the n_les - n_les32 of 64-wide ones
     // device
     void *arena = nullptr;
     std::size_t arena_bytes = 0;
@@ -383,20 +397,30 @@ static int build_items(ftc_ensemble &e)
     std::vector<TileItem> les, tiles;
     std::vector<GemmItem> gemm[6];
     std::vector<ColItem> cols;
-    e.r_les.clear();
+    e.n_les32 = 0;
     e.r_tiles.clear();
     e.r_cols.clear();
     for (auto &r : e.r_gemm) r.clear();
     const int bm = tile_bm(e.gemm_tile);
+    // LES tiles: the members of width 64 first, then those of width 32, each member's tiles contiguous.
+    e.les_w.assign(e.count, 64);
+    e.r_les.assign(e.count, Range{0, 0});
+    for (int width : {64, 32})
+        for (int m = 0; m < e.count; ++m) {
+            if (les_width(e.les_policy, e.nx[m]) != width) continue;
+            e.les_w[m] = width;
+            const int nx = e.nx[m], ny = e.ny[m], nz = e.nz[m], rows = NTH / width;
+            const int kchunk = std::min(e.les_kchunk, nz);
+            Range r{(int)les.size(), 0};
+            for (int bz = 0; bz * kchunk < nz; ++bz)
+                for (int by = 0; by < (ny + rows - 1) / rows; ++by)
+                    for (int bx = 0; bx < (nx + width - 1) / width; ++bx) les.push_back({m, bx, by, bz});
+            r.count = (int)les.size() - r.start;
+            e.r_les[m] = r;
+            if (width == 32) e.n_les32 += r.count;
+        }
     for (int m = 0; m < e.count; ++m) {
         const int nx = e.nx[m], ny = e.ny[m], nz = e.nz[m];
-        const int kchunk = std::min(e.les_kchunk, nz);
-        Range r{(int)les.size(), 0};
-        for (int bz = 0; bz * kchunk < nz; ++bz)
-            for (int by = 0; by < (ny + TY - 1) / TY; ++by)
-                for (int bx = 0; bx < (nx + TX - 1) / TX; ++bx) les.push_back({m, bx, by, bz});
-        r.count = (int)les.size() - r.start;
-        e.r_les.push_back(r);
         Range rt{(int)tiles.size(), 0};
         for (int by = 0; by < (ny + 3) / 4; ++by)
             for (int bx = 0; bx < (nx + 63) / 64; ++bx) tiles.push_back({m, bx, by, 0});
@@ -481,11 +505,26 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
     int start = 0, n = 0;
     hipStream_t s = e.stream;
     switch (stage) {
-    case kLes:
-        n = pick(e.r_les, e.n_les, &start);
-        if (n) hipLaunchKernelGGL(ens_les_kernel<true>, dim3(all ? grid_for(e, n) : n), dim3(TX, TY), 0, s,
-                                  e.d_args[parity], e.d_les + start, n);
+    case kLes: {
+        // One launch per width: the 64-wide members' tiles, then the 32-wide members'.
+        auto launch = [&](int width, int first, int count) {
+            if (!count) return;
+            const dim3 grid(all ? grid_for(e, count) : count);
+            if (width == 32)
+                hipLaunchKernelGGL((ens_les_kernel<true, 32>), grid, dim3(32, NTH / 32), 0, s, e.d_args[parity],
+                                   e.d_les + first, count);
+            else
+                hipLaunchKernelGGL((ens_les_kernel<true, 64>), grid, dim3(64, NTH / 64), 0, s, e.d_args[parity],
+                                   e.d_les + first, count);
+        };
+        if (all) {
+            launch(64, 0, e.n_les - e.n_les32);
+            launch(32, e.n_les - e.n_les32, e.n_les32);
+        } else {
+            launch(e.les_w[member], e.r_les[member].start, e.r_les[member].count);
+        }
         break;
+    }
     case kDiv:
         n = pick(e.r_tiles, e.n_tiles, &start);
         if (n) hipLaunchKernelGGL(ens_divergence_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
@@ -586,7 +625,8 @@ static int enqueue_step(ftc_ensemble &e, int parity)
         e.launches = (int64_t)st.size() * e.count;
     } else {
         for (int stage : st) launch_stage(e, stage, -1, parity);
-        e.launches = (int64_t)st.size();
+        const bool two_widths = e.n_les32 > 0 && e.n_les32 < e.n_les;
+        e.launches = (int64_t)st.size() + (two_widths ? 1 : 0);
     }
     if (hipGetLastError() != hipSuccess) return eerr(FORT_SCOPE_EXECUTION, "stage launch failed");
     return FORT_SCOPE_OK;
@@ -833,6 +873,15 @@ int ftc_ensemble_set_solves_v1(ftc_ensemble *e, int solves)
     return FORT_SCOPE_OK;
 }
 
+int ftc_ensemble_set_les_policy_v1(ftc_ensemble *e, int policy)
+{
+    if (!e || policy < 0 || policy > 2) return eerr(FORT_SCOPE_ARGUMENT, "LES tile policy is 0, 1 or 2");
+    e->les_policy = policy;
+    FTC_CHECK(build_items(*e));
+    if (e->mode != FTC_ENSEMBLE_FP64_PER_MEMBER) return capture(*e);
+    return FORT_SCOPE_OK;
+}
+
 int ftc_ensemble_set_projection_v1(ftc_ensemble *e, int on)
 {
     if (!e || (on != 0 && on != 1)) return eerr(FORT_SCOPE_ARGUMENT, "projection is 0 or 1");
@@ -884,8 +933,8 @@ int ftc_ensemble_autotune_v1(ftc_ensemble *e, const char *cache_path)
             if (tab != std::string::npos) cache[line.substr(0, tab)] = line.substr(tab + 1);
         }
         auto hit = cache.find(key.str());
-        if (hit != cache.end() && std::sscanf(hit->second.c_str(), "%d %d %d", &e->les_kchunk, &e->gemm_tile,
-                                              &e->blocks_per_cu) == 3) {
+        if (hit != cache.end() && std::sscanf(hit->second.c_str(), "%d %d %d %d", &e->les_kchunk, &e->gemm_tile,
+                                              &e->blocks_per_cu, &e->les_policy) == 4) {
             FTC_CHECK(build_items(*e));
             return e->mode != FTC_ENSEMBLE_FP64_PER_MEMBER ? capture(*e) : FORT_SCOPE_OK;
         }
@@ -901,15 +950,19 @@ int ftc_ensemble_autotune_v1(ftc_ensemble *e, const char *cache_path)
             else launch_stage(*e, st, -1, 0);
         }
     };
+    // The LES stage: tile-width policy (all 64 wide, widths per member by idle lanes, all 32 wide) and k chunk.
     float best = 1e30f;
-    int best_k = e->les_kchunk;
-    for (int k : {8, 16, 32, 64, 160}) {
-        e->les_kchunk = k;
-        FTC_CHECK(build_items(*e));
-        const float t = time_ms(*e, [&] { run_stages({kLes}); }, 3);
-        if (t < best) best = t, best_k = k;
-    }
+    int best_k = e->les_kchunk, best_p = e->les_policy;
+    for (int policy : {0, 2, 1})
+        for (int k : {8, 16, 32, 64, 160}) {
+            e->les_policy = policy;
+            e->les_kchunk = k;
+            FTC_CHECK(build_items(*e));
+            const float t = time_ms(*e, [&] { run_stages({kLes}); }, 3);
+            if (t < best) best = t, best_k = k, best_p = policy;
+        }
     e->les_kchunk = best_k;
+    e->les_policy = best_p;
     best = 1e30f;
     int best_t = e->gemm_tile;
     for (int tile : {0, 1}) {
@@ -932,8 +985,11 @@ int ftc_ensemble_autotune_v1(ftc_ensemble *e, const char *cache_path)
     }
     if (hipStreamSynchronize(e->stream) != hipSuccess) return eerr(FORT_SCOPE_EXECUTION, "tuning run failed");
     if (cache_path) {
+        // k chunk, contraction tile, workgroups per CU, LES tile policy, then the width each member takes.
+        std::string widths = " widths=";
+        for (int m = 0; m < e->count; ++m) widths += std::to_string(e->les_w[m]) + (m + 1 < e->count ? "," : "");
         cache[key.str()] = std::to_string(e->les_kchunk) + " " + std::to_string(e->gemm_tile) + " " +
-                           std::to_string(e->blocks_per_cu);
+                           std::to_string(e->blocks_per_cu) + " " + std::to_string(e->les_policy) + widths;
         std::ofstream out(cache_path, std::ios::trunc);
         for (auto &kv : cache) out << kv.first << '\t' << kv.second << '\n';
     }
@@ -950,8 +1006,12 @@ int ftc_ensemble_get_info_v1(ftc_ensemble *e, ftc_ensemble_info_v1 *info)
     info->les_kchunk = e->les_kchunk;
     info->gemm_tile = e->gemm_tile;
     info->blocks_per_cu = e->blocks_per_cu;
-    info->reserved = 0;
-    info->launches_per_step = (int64_t)step_stages(*e).size() * (e->mode == FTC_ENSEMBLE_FP64_PER_MEMBER ? e->count : 1);
+    info->les_policy = e->les_policy;
+    // Batched: one launch per stage, two for the LES stage when both tile widths occur.
+    const bool two_widths = e->n_les32 > 0 && e->n_les32 < e->n_les;
+    info->launches_per_step = e->mode == FTC_ENSEMBLE_FP64_PER_MEMBER
+                                  ? (int64_t)step_stages(*e).size() * e->count
+                                  : (int64_t)step_stages(*e).size() + (two_widths ? 1 : 0);
     double bytes = 0;
     int64_t cells = 0;
     for (int m = 0; m < e->count; ++m) {

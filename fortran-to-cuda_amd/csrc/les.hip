@@ -81,9 +81,19 @@ static const void *kernel_of()
     return reinterpret_cast<const void *>(&les_kernel<MASK, SGS, STEP>);
 }
 
-static dim3 grid_of(const LesArgs &a, int kplanes, int rows = 1)
+static dim3 grid_of(const LesArgs &a, int kplanes, int rows = 1, int width = TX)
 {
-    return dim3((a.nx + TX - 1) / TX, (a.ny + TY * rows - 1) / (TY * rows), (kplanes + a.kchunk - 1) / a.kchunk);
+    const int ty = NTH / width * rows;
+    return dim3((a.nx + width - 1) / width, (a.ny + ty - 1) / ty, (kplanes + a.kchunk - 1) / a.kchunk);
+}
+
+template <bool STEP>
+static const void *staged_kernel(int rows, int width)
+{
+    if (rows == 2) return reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 2>);
+    if (width == 32) return reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 32>);
+    if (width == 128) return reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 128>);
+    return reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1>);
 }
 
 static Sec3 box(std::size_t i0, std::size_t i1, std::size_t j0, std::size_t j1, std::size_t k0, std::size_t k1)
@@ -205,7 +215,11 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     const bool staged = sgs && !std::getenv("FTC_LES_UNSTAGED");
     const char *rows_env = std::getenv("FTC_LES_ROWS");
     const int rows = staged && rows_env && std::atoi(rows_env) == 2 ? 2 : 1;
-    a.kchunk = choose_kchunk(nx, ny, nz, rows, !staged ? 2 : (rows == 2 ? 2 : 3));
+    // FTC_LES_WIDTH selects the 32 x 8 or 128 x 2 tile of the staged kernel (one row per thread).
+    const char *width_env = std::getenv("FTC_LES_WIDTH");
+    int width = staged && rows == 1 && width_env ? std::atoi(width_env) : TX;
+    if (width != 32 && width != 128) width = TX;
+    a.kchunk = choose_kchunk(nx, ny, nz, rows, !staged ? 2 : (rows == 2 || width == 128 ? 2 : 3));
     // Tiles in launch order put each 64-column strip on one XCD, so the y neighbours that re-read a tile's
     // ghost rows share its L2. The XCD-contiguous order (FTC_LES_XCD_ORDER) also shares the x neighbours'
     // lines, but only between tiles resident at the same time; over four rounds of workgroups it read more.
@@ -232,22 +246,21 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     b[6] = {w, {box(0, X + 2, 0, Y + 2, 0, Z + 1)}, {}, {}};
     b[7] = {s, {full}, {}, {}};
     const void *fn;
-    if (staged && rows == 2) fn = step ? reinterpret_cast<const void *>(&les_sgs3_kernel<true, 2>)
-                                            : reinterpret_cast<const void *>(&les_sgs3_kernel<false, 2>);
-    else if (staged) fn = step ? reinterpret_cast<const void *>(&les_sgs3_kernel<true, 1>)
-                               : reinterpret_cast<const void *>(&les_sgs3_kernel<false, 1>);
+    if (staged) fn = step ? staged_kernel<true>(rows, width) : staged_kernel<false>(rows, width);
     else if (step) fn = sgs ? kernel_of<15, true, true>() : kernel_of<15, false, true>();
     else fn = sgs ? kernel_of<15, true, false>() : kernel_of<15, false, false>();
     const uint64_t key = 0x1e5000000000000full ^ ((uint64_t)step << 8) ^ ((uint64_t)sgs << 9);
-    dim3 grid = grid_of(a, nz, rows);
+    dim3 grid = grid_of(a, nz, rows, width);
     if (persistent) {
-        // One resident round: the workgroups the device holds at once (three, or two with two rows, per
-        // CU), a multiple of eight, and no more than there are tiles (rounded up to eight).
+        // One resident round: the workgroups the device holds at once (three, or two with two rows or the
+        // wide tile, per CU), a multiple of eight, and no more than there are tiles (rounded up to eight).
         const unsigned tiles = grid.x * grid.y * grid.z;
-        const unsigned resident = (unsigned)compute_units() * (rows == 2 ? 2u : 3u) / kXcds * kXcds;
+        const unsigned per_cu = rows == 2 || width == 128 ? 2u : 3u;
+        const unsigned resident = (unsigned)compute_units() * per_cu / kXcds * kXcds;
         grid = dim3(std::max(kXcds, std::min(resident, (tiles + kXcds - 1) / kXcds * kXcds)), 1, 1);
     }
-    return enqueue_kernel(ctx, key, fn, grid, dim3(TX, TY, 1), 0, &a, sizeof a, b);
+    const dim3 block = staged ? dim3(width, NTH / width, 1) : dim3(TX, TY, 1);
+    return enqueue_kernel(ctx, key, fn, grid, block, 0, &a, sizeof a, b);
 }
 
 }  // namespace ftc

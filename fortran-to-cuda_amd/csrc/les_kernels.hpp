@@ -383,19 +383,26 @@ __device__ __forceinline__ void put_column(double *F, long long p, double val, c
 // ROWS = 2 gives each thread two adjacent rows of a 64 x 8 tile (80 KB, two workgroups per CU): less halo
 // per cell (viscosity ring 1.29x instead of 1.55x, velocity ring 1.59x instead of 2.13x) and the per-step
 // bookkeeping shared by two cells.
-template <int ROWS>
+// WX is the tile width: the 256 threads form WX x (256 / WX) (32 x 8, 64 x 4 or 128 x 2), each thread
+// ROWS rows. Narrow tiles waste fewer lanes on narrow domains and recompute less ring (32 x 8: viscosity
+// ring 1.33x, velocity ring 1.69x); wide tiles read longer rows.
+template <int ROWS, int WX = TX>
 struct Sgs3Tile {
-    static constexpr int TYT = TY * ROWS;            // rows of the tile
+    static_assert(WX == 32 || WX == 64 || WX == 128, "tile widths 32, 64 or 128");
+    static constexpr int TXT = WX, TYB = NTH / WX;   // thread block
+    static constexpr int TYT = TYB * ROWS;           // rows of the tile
+    static constexpr int VXT = WX + 4, CXT = WX + 2;
     static constexpr int VYT = TYT + 4, CYT = TYT + 2;
-    static constexpr int VC = VX * VYT, CC = CX * CYT;
+    static constexpr int VC = VXT * VYT, CC = CXT * CYT;
     static constexpr int VRT = (VC + NTH - 1) / NTH, CRT = (CC + NTH - 1) / NTH;
 };
 
-template <int ROWS>
+template <int ROWS, int WX = TX>
 struct Sgs3Lds {
-    double vel[3][3][Sgs3Tile<ROWS>::VYT][VX];  // slot = plane % 3
-    double sca[2][Sgs3Tile<ROWS>::CYT][CX];     // slot = plane & 1
-    double nu[2][Sgs3Tile<ROWS>::CYT][CX];      // slot = plane & 1
+    using T = Sgs3Tile<ROWS, WX>;
+    double vel[3][3][T::VYT][T::VXT];  // slot = plane % 3
+    double sca[2][T::CYT][T::CXT];     // slot = plane & 1
+    double nu[2][T::CYT][T::CXT];      // slot = plane & 1
 };
 
 // The upper z-face terms of a cell (advection and diffusion of u, v, w and s), which are the lower terms
@@ -405,17 +412,18 @@ struct ZCarry {
     double u, v, w, wxp, wyp;               // U, V, W at (i, j), W at (i+1, j) and (i, j+1)
 };
 
-template <bool STEP, int ROWS = 1>
-__device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS> &L, int bx, int by, int bz)
+template <bool STEP, int ROWS = 1, int WX = TX>
+__device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX> &L, int bx, int by, int bz)
 {
-    using T = Sgs3Tile<ROWS>;
+    using T = Sgs3Tile<ROWS, WX>;
     constexpr int VR = T::VRT, CR = T::CRT, VCELLS = T::VC, CCELLS = T::CC;
+    constexpr int VX = T::VXT, CX = T::CXT;
     const double *__restrict__ gU = a.U;
     const double *__restrict__ gV = a.V;
     const double *__restrict__ gW = a.W;
     const double *__restrict__ gS = a.S;
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int i0 = 2 + bx * TX, j0 = 2 + by * T::TYT;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * WX + tx;
+    const int i0 = 2 + bx * WX, j0 = 2 + by * T::TYT;
     const int i = i0 + tx;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int k0 = 2 + bz * a.kchunk;
@@ -720,23 +728,33 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS> &L
 #undef NV
 }
 
-template <bool STEP, int ROWS>
-__global__ __launch_bounds__(NTH, ROWS == 1 ? 3 : 2) void les_sgs3_kernel(const LesArgs a)
+// Workgroups per CU the staged kernel is built for: its LDS (52 KB at 64 x 4, 42 KB at 32 x 8, 74 KB at
+// 128 x 2 or with two rows per thread) leaves room for three, or two.
+template <int ROWS, int WX>
+constexpr int sgs3_occupancy()
 {
-    __shared__ Sgs3Lds<ROWS> L;
+    return sizeof(Sgs3Lds<ROWS, WX>) <= 160 * 1024 / 3 ? 3 : 2;
+}
+
+template <bool STEP, int ROWS, int WX = TX>
+__global__ __launch_bounds__(NTH, (sgs3_occupancy<ROWS, WX>())) void les_sgs3_kernel(const LesArgs a)
+{
+    using T = Sgs3Tile<ROWS, WX>;
+    __shared__ Sgs3Lds<ROWS, WX> L;
     if (a.tile_order == 2) {
         // Persistent banded walk: one resident round of workgroups (a multiple of eight). XCD x owns the
         // contiguous tile range [T x / 8, T (x+1) / 8) of the T tiles in x-fastest order, and its
         // workgroups take consecutive slots of that range in step, so the tiles resident on one XCD at a
         // time form a band whose x and y neighbours re-read each other's ring rows from the same L2.
         // Every tile belongs to one range and one slot of it: each is computed exactly once.
-        const unsigned gx = (a.nx + TX - 1) / TX, gy = (a.ny + TY * ROWS - 1) / (TY * ROWS);
+        const unsigned gx = (a.nx + WX - 1) / WX, gy = (a.ny + T::TYT - 1) / T::TYT;
         const unsigned gz = (a.nz + a.kchunk - 1) / a.kchunk;
         const unsigned total = gx * gy * gz, per = gridDim.x / kXcds;
         const unsigned x = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
         const unsigned lo = (unsigned)((unsigned long long)total * x / kXcds);
         const unsigned hi = (unsigned)((unsigned long long)total * (x + 1) / kXcds);
-        for (unsigned t = lo + slot; t < hi; t += per) les_sgs3_body<STEP, ROWS>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
+        for (unsigned t = lo + slot; t < hi; t += per)
+            les_sgs3_body<STEP, ROWS, WX>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
         return;
     }
     const unsigned gx = gridDim.x, gy = gridDim.y;
@@ -744,7 +762,7 @@ __global__ __launch_bounds__(NTH, ROWS == 1 ? 3 : 2) void les_sgs3_kernel(const 
     // Tiles x fastest, then y, then the k chunk: with the XCD-contiguous order the x and y neighbours
     // whose ghost rows and columns a tile re-reads run under the same L2 at about the same time.
     if (a.tile_order) t = xcd_contiguous(t, gx * gy * gridDim.z);
-    les_sgs3_body<STEP, ROWS>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
+    les_sgs3_body<STEP, ROWS, WX>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
 }
 
 }  // namespace ftc

@@ -11,7 +11,7 @@ FP64_PER_MEMBER, FP64_BATCHED, MIXED_BATCHED = 0, 1, 2
 
 class Info(ctypes.Structure):
     _fields_ = [("version", c_uint32), ("mode", c_uint32), ("members", c_int32), ("solves", c_int32),
-                ("les_kchunk", c_int32), ("gemm_tile", c_int32), ("blocks_per_cu", c_int32), ("reserved", c_int32),
+                ("les_kchunk", c_int32), ("gemm_tile", c_int32), ("blocks_per_cu", c_int32), ("les_policy", c_int32),
                 ("launches_per_step", c_int64), ("cells", c_int64), ("algorithmic_bytes", c_double),
                 ("device_bytes", c_uint64)]
 
@@ -25,6 +25,7 @@ SIGNATURES = {
     "ftc_ensemble_download_pressure_v1": (c_int, [c_void_p, c_int, c_void_p]),
     "ftc_ensemble_set_solves_v1": (c_int, [c_void_p, c_int]),
     "ftc_ensemble_set_projection_v1": (c_int, [c_void_p, c_int]),
+    "ftc_ensemble_set_les_policy_v1": (c_int, [c_void_p, c_int]),
     "ftc_ensemble_autotune_v1": (c_int, [c_void_p, c_char_p]),
     "ftc_ensemble_step_v1": (c_int, [c_void_p, c_int]),
     "ftc_ensemble_synchronize_v1": (c_int, [c_void_p]),
@@ -77,6 +78,9 @@ class Ensemble:
 
     def projection(self, on: bool):
         self._check(self.lib.ftc_ensemble_set_projection_v1(self.handle, 1 if on else 0))
+
+    def les_policy(self, policy: int):
+        self._check(self.lib.ftc_ensemble_set_les_policy_v1(self.handle, policy))
 
     def autotune(self, cache=None):
         self._check(self.lib.ftc_ensemble_autotune_v1(self.handle, cache.encode() if cache else None))

@@ -34,7 +34,8 @@ typedef struct ftc_ensemble ftc_ensemble;
 typedef struct ftc_ensemble_info_v1 {
     uint32_t version, mode;
     int32_t members, solves;
-    int32_t les_kchunk, gemm_tile, blocks_per_cu, reserved;
+    int32_t les_kchunk, gemm_tile, blocks_per_cu;
+    int32_t les_policy;            /* LES tile widths: 0 all 64, 1 all 32, 2 per member by idle lanes */
     int64_t launches_per_step;     /* kernel launches one step issues (graph nodes when batched) */
     int64_t cells;                 /* interior cells of all members */
     double algorithmic_bytes;      /* FP64 step floor: sum over members of 8*(4P + 4I) + pressure */
@@ -55,9 +56,14 @@ int ftc_ensemble_set_solves_v1(ftc_ensemble *e, int solves);
 /* on = 1 (the default): a step is the projected step above. on = 0: a step is the LES stage alone
  * (F* = F + dt * tendency, with halos), so that stage can be checked and timed by itself. */
 int ftc_ensemble_set_projection_v1(ftc_ensemble *e, int on);
-/* Picks the LES k chunk, the contraction tile and the persistent grid by timing candidates on this
- * ensemble (fixed candidate order, minimum of the median of 3, ties to the first). With a cache path,
- * a stored choice for the same sizes and mode is reused and new choices are stored. */
+/* Sets the LES tile-width policy (see ftc_ensemble_info_v1.les_policy) without tuning. */
+int ftc_ensemble_set_les_policy_v1(ftc_ensemble *e, int policy);
+/* Picks the LES tile-width policy and k chunk (together, on the LES stage), the contraction tile and the
+ * persistent grid by timing candidates on this ensemble (fixed candidate order, minimum of the median of
+ * 3, ties to the first). The LES tiles are 64 x 4 or 32 x 8 columns; the policies give every member 64,
+ * every member 32, or each member the width that leaves fewer idle lanes in its last tile column. With a
+ * cache path, a stored choice for the same sizes and mode is reused and new choices are stored, with the
+ * width each member takes. */
 int ftc_ensemble_autotune_v1(ftc_ensemble *e, const char *cache_path);
 /* Enqueues `steps` steps on the ensemble's stream (graph replays when batched). */
 int ftc_ensemble_step_v1(ftc_ensemble *e, int steps);

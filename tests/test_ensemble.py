@@ -30,11 +30,14 @@ def test_projected_oracle_step_is_divergence_free(oracle, fields):
     assert abs(phi.mean()) <= 1e-12 * np.abs(phi).max()
 
 
-def run(hip, ens_mod, fields, mode, steps, tune=None, solves=None, sizes=SIZES, seed=100, projection=True):
+def run(hip, ens_mod, fields, mode, steps, tune=None, solves=None, sizes=SIZES, seed=100, projection=True,
+        policy=None):
     e = ens_mod.Ensemble(hip, sizes, D, DT, CS, mode)
     try:
         if solves:
             e.solves(solves)
+        if policy is not None:
+            e.les_policy(policy)
         if not projection:
             e.projection(False)
         if tune is not None:
@@ -101,8 +104,9 @@ def test_autotuned_ensemble_reuses_its_cache_and_keeps_fp64_results(hip, ens_mod
     text = open(cache).read()
     assert "mode=1" in text
     again, info2 = run(hip, ens_mod, fields, ens_mod.FP64_BATCHED, 2, tune=cache)
-    assert (info2.les_kchunk, info2.gemm_tile, info2.blocks_per_cu) == (info.les_kchunk, info.gemm_tile,
-                                                                         info.blocks_per_cu)
+    assert (info2.les_kchunk, info2.gemm_tile, info2.blocks_per_cu, info2.les_policy) == (
+        info.les_kchunk, info.gemm_tile, info.blocks_per_cu, info.les_policy)
+    assert "widths=" in text
     for m in range(len(SIZES)):
         for g, w, a in zip(got[m], ref[m], again[m]):
             assert bits_equal(g, w) and bits_equal(a, w)
@@ -134,13 +138,18 @@ def seeded64(oracle, fields):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["FP64_BATCHED", "MIXED_BATCHED"])
-def test_seeded_64_members_les_stage_is_bit_exact_against_the_oracle(hip, ens_mod, fields, seeded64, mode):
+@pytest.mark.parametrize("mode,policy", [("FP64_BATCHED", 0), ("MIXED_BATCHED", 0), ("FP64_BATCHED", 1),
+                                         ("FP64_BATCHED", 2), ("FP64_PER_MEMBER", 2)])
+def test_seeded_64_members_les_stage_is_bit_exact_against_the_oracle(hip, ens_mod, fields, seeded64, mode, policy):
+    # Tile policies: 0 every member in 64 x 4 tiles, 1 every member in 32 x 8 tiles, 2 each member in the
+    # width that leaves fewer idle lanes (both widths in one step, one persistent launch per width).
     sizes, les, _ = seeded64
-    print(f"seeded ensemble LES stage, {mode}", flush=True)
+    print(f"seeded ensemble LES stage, {mode}, tile policy {policy}", flush=True)
     got, info = run(hip, ens_mod, fields, getattr(ens_mod, mode), SEEDED_STEPS, sizes=sizes, seed=500,
-                    projection=False)
-    assert info.launches_per_step == 1
+                    projection=False, policy=policy)
+    assert info.les_policy == policy
+    if mode != "FP64_PER_MEMBER":
+        assert info.launches_per_step == (2 if policy == 2 else 1)
     for m, n in enumerate(sizes):
         for name, g, w in zip("UVWS", got[m], les[m]):
             assert bits_equal(g, w), (m, n, name, describe_mismatch(g, w))

@@ -210,6 +210,7 @@ struct Domain {
     int device = 0;
     std::size_t budget = 0;
     bool budget_known = false;
+    bool budget_warned = false;  // the notice of a task that does not fit was printed
     std::map<fort_buffer_t, std::unique_ptr<Buffer>> buffers;
     std::map<uint64_t, fort_buffer_t> by_identity;
     Context *borrower = nullptr;
@@ -586,6 +587,21 @@ static std::size_t budget_of(Domain &d)
     return d.budget;
 }
 
+// A device task whose arrays cannot be on the device together under the budget is refused with RESOURCE;
+// an application's spliced block then runs the procedure's own loops on the host. The run is told once
+// per domain that the budget costs it time (scoped_runtime.cu:534-543 says the same).
+[[noreturn]] static void refuse_over_budget(Domain &d, std::size_t budget, const char *message)
+{
+    if (!d.budget_warned) {
+        d.budget_warned = true;
+        std::fprintf(stderr,
+                     "Resident domain: the arrays in use exceed the device budget of %zu bytes; loop nests whose "
+                     "arrays do not fit together run on the host.\n",
+                     budget);
+    }
+    fail(FORT_SCOPE_RESOURCE, message);
+}
+
 // Gives every requested buffer a device allocation, evicting least-recently-used unrequested mirrors
 // that no prepared access or queued task holds.
 static void reserve(Context &c, Buffer *const *list, std::size_t n)
@@ -602,7 +618,7 @@ static void reserve(Context &c, Buffer *const *list, std::size_t n)
     }
     std::size_t budget = budget_of(d);
     std::size_t pinned = d.workspace_bytes + c.scratch_capacity;
-    require(need <= budget, FORT_SCOPE_RESOURCE, "task footprint exceeds the device budget");
+    if (need > budget) refuse_over_budget(d, budget, "task footprint exceeds the device budget");
     if (d.device_bytes + pinned + need > budget && !c.queue.empty()) {
         d.residency.pressure_flushes++;
         flush(c);
@@ -617,7 +633,7 @@ static void reserve(Context &c, Buffer *const *list, std::size_t n)
             if (requested) continue;
             if (!victim || b->last_use < victim->last_use) victim = b;
         }
-        require(victim != nullptr, FORT_SCOPE_RESOURCE, "device budget exhausted by pinned buffers");
+        if (!victim) refuse_over_budget(d, budget, "device budget exhausted by pinned buffers");
         evict_one(c, *victim);
     }
     for (std::size_t i = 0; i < n; ++i) {
@@ -2601,6 +2617,16 @@ int fort_scope_close(fort_scope_t h)
             require(!kv.second->prepared, FORT_SCOPE_STATE, "an access is still prepared");
         flush(c);
         wait(c);
+        if (tracing()) {
+            // What the domain moved and ran, for a run traced with FORT_RUNTIME_TRACE.
+            const fort_scope_stats &s = c.domain->stats;
+            std::fprintf(stderr,
+                         "FORT_SCOPED close launches=%llu uploads=%llu downloads=%llu upload_bytes=%llu "
+                         "download_bytes=%llu\n",
+                         (unsigned long long)s.launches, (unsigned long long)s.uploads,
+                         (unsigned long long)s.downloads, (unsigned long long)s.upload_bytes,
+                         (unsigned long long)s.download_bytes);
+        }
         release_context(c);
         c.domain->borrower = nullptr;
         if (c.owns_domain) owned = c.domain->handle;

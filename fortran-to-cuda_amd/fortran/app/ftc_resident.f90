@@ -7,6 +7,15 @@ module ftc_resident
   use fort_resident_domain, only: resident_domain, ftc_require => resident_require
   implicit none
   private
-  public :: ftc_domain, ftc_require
+  public :: ftc_domain, ftc_require, ftc_address
   type(resident_domain), save :: ftc_domain
+contains
+  ! The address of a variable that need not have the TARGET attribute (a field of an element of an array
+  ! of derived type): the domain compares it with the addresses of the fields it mirrors and never
+  ! dereferences it.
+  function ftc_address(x) result(address)
+    type(*), target, intent(in) :: x(..)
+    type(c_ptr) :: address
+    address = c_loc(x)
+  end function ftc_address
 end module ftc_resident

@@ -1,4 +1,4 @@
-! Fortran interfaces of the C-ABI entries in include/ftc_les.h, ftc_dense.h, ftc_reduce.h and ftc_poisson.h.
+! Fortran interfaces of the C-ABI entries in include/ftc_les.h, ftc_dense.h, ftc_reduce.h, ftc_poisson.h and ftc_box.h.
 !
 ! One module per entry, each exporting `run`, in the shape the reference compiler gives its generated
 ! interfaces (compiler/emission/cuda/scoped.py:514-522): the context and the mode by value, array handles
@@ -199,3 +199,56 @@ module fort_shared_pressure_fix_reference
     end function
   end interface
 end module fort_shared_pressure_fix_reference
+
+! The box application's device procedures (include/ftc_box.h).
+module fort_shared_boundaries_periodic
+  use iso_c_binding
+  implicit none
+  private
+  public :: run
+  interface
+    function run(fort_context, fort_mode, a_handle) bind(C, name='cpp_fort_shared_boundaries_periodic') &
+        result(fort_status)
+      import :: c_int, c_int64_t
+      integer(c_int) :: fort_status
+      integer(c_int64_t), value :: fort_context
+      integer(c_int), value :: fort_mode
+      integer(c_int64_t), value :: a_handle
+    end function
+  end interface
+end module fort_shared_boundaries_periodic
+
+module fort_shared_stepper_diffuse
+  use iso_c_binding
+  implicit none
+  private
+  public :: run
+  interface
+    function run(fort_context, fort_mode, a_handle, b_handle, dx, dt, nu) &
+        bind(C, name='cpp_fort_shared_stepper_diffuse') result(fort_status)
+      import :: c_int, c_int64_t, c_double
+      integer(c_int) :: fort_status
+      integer(c_int64_t), value :: fort_context
+      integer(c_int), value :: fort_mode
+      integer(c_int64_t), value :: a_handle, b_handle
+      real(c_double), intent(in) :: dx, dt, nu
+    end function
+  end interface
+end module fort_shared_stepper_diffuse
+
+module fort_shared_stepper_store
+  use iso_c_binding
+  implicit none
+  private
+  public :: run
+  interface
+    function run(fort_context, fort_mode, a_handle, b_handle) bind(C, name='cpp_fort_shared_stepper_store') &
+        result(fort_status)
+      import :: c_int, c_int64_t
+      integer(c_int) :: fort_status
+      integer(c_int64_t), value :: fort_context
+      integer(c_int), value :: fort_mode
+      integer(c_int64_t), value :: a_handle, b_handle
+    end function
+  end interface
+end module fort_shared_stepper_store

@@ -43,12 +43,14 @@ FORTRAN = HERE / "fortran"
 @dataclass(frozen=True)
 class EntrySpec:
     """A hand-written entry: its interface module (fortran/ftc_entries.f90) and the dummy arguments of
-    the procedure it replaces, arrays first (real(8), rank 3), then scalars passed by reference."""
+    the procedure it replaces, arrays first (real(8), rank 3), then scalars passed by reference; `extra`
+    are further scalars the procedure takes from its module's scope (names the call passes as they are)."""
     module: str
     procedure: str
     interface: str
     arrays: int
     scalars: int
+    extra: tuple[str, ...] = ()
 
 
 ENTRIES = {
@@ -58,6 +60,10 @@ ENTRIES = {
     ("momentumadvection", "cdv"): EntrySpec("momentumadvection", "cdv", "fort_shared_momentumadvection_cdv", 4, 6),
     # compiler/tests/fixtures/stencils/cdw.f90:104-125
     ("momentumadvection", "cdw"): EntrySpec("momentumadvection", "cdw", "fort_shared_momentumadvection_cdw", 4, 6),
+    # compiler/tests/apps/box/boundaries.f90:6-41 and stepper.f90:8-37 (dx, dt, nu are module params)
+    ("boundaries", "periodic"): EntrySpec("boundaries", "periodic", "fort_shared_boundaries_periodic", 1, 0),
+    ("stepper", "diffuse"): EntrySpec("stepper", "diffuse", "fort_shared_stepper_diffuse", 2, 0, ("dx", "dt", "nu")),
+    ("stepper", "store"): EntrySpec("stepper", "store", "fort_shared_stepper_store", 2, 0),
 }
 
 DOMAIN_MODULE, DOMAIN, CHECK = "ftc_resident", "ftc_domain", "ftc_require"
@@ -211,6 +217,8 @@ def apply_edit(text: str, edit: dict, what: str, root: Path | None = None) -> st
 DIRECTIVE = re.compile(r"^(?P<indent>[ \t]*)!\$fort[ \t]+(?P<body>.*?)[ \t]*$", re.IGNORECASE)
 _WORD = re.compile(r"[ \t]*([A-Za-z_]\w*)[ \t]*")
 _ARRAY_NAME = re.compile(r"[A-Za-z_]\w*")
+# One field of an array of derived type, `objects(:)%field` (observe only).
+_FIELD_OF = re.compile(r"([A-Za-z_]\w*)\s*\(\s*:\s*\)\s*%\s*([A-Za-z_]\w*)")
 
 
 @dataclass(frozen=True)
@@ -256,10 +264,17 @@ def _guard(d: Directive) -> str:
 
 
 def _observe_lines(d: Directive, origin: str) -> list[str]:
-    binds = [f"  ftc_observed({n}) = {DOMAIN}%bind({a}, lbound({a}), .true., origin='{origin}::{a.lower()}')"
-             for n, a in enumerate(d.arrays, 1)]
-    return [f"integer(c_int64_t) :: ftc_observed({len(d.arrays)})", f"if ({_guard(d)}) then", *binds,
-            f"  call {DOMAIN}%observe(ftc_observed)", "endif"]
+    # Whole arrays are bound and observed together; a field of an array of derived type reaches its
+    # structures through the domain's mirror of that field, when there is one.
+    plain = [a for a in d.arrays if _ARRAY_NAME.fullmatch(a)]
+    fields = [_FIELD_OF.fullmatch(a).groups() for a in d.arrays if not _ARRAY_NAME.fullmatch(a)]
+    body = [f"  ftc_observed({n}) = {DOMAIN}%bind({a}, lbound({a}), .true., origin='{origin}::{a.lower()}')"
+            for n, a in enumerate(plain, 1)]
+    if plain:
+        body.append(f"  call {DOMAIN}%observe(ftc_observed)")
+    body += [f"  if (size({o}) > 0) call {DOMAIN}%observe_field(ftc_address({o}(lbound({o},1))%{m}))" for o, m in fields]
+    head = [f"integer(c_int64_t) :: ftc_observed({len(plain)})"] if plain else []
+    return [*head, f"if ({_guard(d)}) then", *body, "endif"]
 
 
 # kind: (accepted clauses, takes an argument list, uses iso_c_binding, body lines)
@@ -291,7 +306,7 @@ def parse_directive(body: str, line: int) -> Directive:
     if not listed and argument is not None:
         raise ValueError(f"line {line}: {kind} takes no argument list")
     arrays = tuple(a.strip() for a in argument.split(",")) if argument else ()
-    malformed = [a for a in arrays if not _ARRAY_NAME.fullmatch(a)]
+    malformed = [a for a in arrays if not _ARRAY_NAME.fullmatch(a) and not (kind == "observe" and _FIELD_OF.fullmatch(a))]
     if malformed:
         raise ValueError(f"line {line}: {kind} lists whole arrays by name; {malformed[0]!r} is not an array name")
     return Directive(kind, arrays, clauses, line)
@@ -301,7 +316,7 @@ def directive_block(d: Directive, indent: str, origin: str) -> str:
     """The Fortran block that replaces the directive line: a BLOCK that uses the domain object."""
     _, _, c_binding, body = _KINDS[d.kind]
     head = ["block", *(["  use, intrinsic :: iso_c_binding"] if c_binding else []),
-            f"  use {DOMAIN_MODULE}, only: {DOMAIN}"]
+            f"  use {DOMAIN_MODULE}, only: {DOMAIN}" + (", ftc_address" if any("%" in a for a in d.arrays) else "")]
     return "".join(f"{indent}{text}\n" for text in (*head, *(f"  {b}" for b in body(d, origin)), "end block"))
 
 
@@ -345,9 +360,12 @@ def _logical_lines(lines: list[str], start: int) -> int:
     return i + 1
 
 
+_HEADER = r"^\s*(?!end\b)(?:[\w(),=*]+\s+)*?(subroutine|function)\s+{name}\s*\("
+
+
 def find_procedure(text: str, module: str, name: str) -> tuple[int, int, int, list[str]] | None:
-    """(header line, first executable line, END line, dummy names) of `subroutine name` in `module`,
-    as line indices of text.splitlines(keepends=True); None when it is not there."""
+    """(header line, first executable line, END line, dummy names) of `subroutine name` (or `function
+    name`) in `module`, as line indices of text.splitlines(keepends=True); None when it is not there."""
     lines = text.splitlines(keepends=True)
     offsets = [0]
     for line in lines:
@@ -355,26 +373,31 @@ def find_procedure(text: str, module: str, name: str) -> tuple[int, int, int, li
     for mod, lo, hi in _modules(text):
         if mod != module:
             continue
-        header = re.compile(rf"^\s*(?:(?:pure|elemental|recursive|impure|module)\s+)*subroutine\s+{name}\s*\(",
-                            re.IGNORECASE)
+        header = re.compile(_HEADER.format(name=re.escape(name)), re.IGNORECASE)
         for i, line in enumerate(lines):
             if not (lo <= offsets[i] < hi) or not header.match(line):
                 continue
             stop = _logical_lines(lines, i)
             statement = "".join(l.split("!", 1)[0].rstrip().rstrip("&") for l in lines[i:stop])
-            dummies = [d.strip() for d in statement.split("(", 1)[1].rsplit(")", 1)[0].split(",") if d.strip()]
+            opened = header.match(statement).end()
+            closed = statement.index(")", opened)
+            dummies = [d.strip() for d in statement[opened:closed].split(",") if d.strip()]
             j = stop
             while j < len(lines):
+                # An OpenMP directive belongs to the statement after it: the executable part starts there.
+                if re.match(r"\s*!\$omp\b", lines[j], re.IGNORECASE):
+                    break
                 code = lines[j].split("!", 1)[0].strip()
                 if code and not _SPEC.match(code):
                     break
                 j = _logical_lines(lines, j) if code else j + 1
-            end = re.compile(rf"^\s*end\s*(?:subroutine(?:\s+{name})?)?\s*(?:!.*)?$", re.IGNORECASE)
+            kind = header.match(line).group(1).lower()
+            end = re.compile(rf"^\s*end\s*(?:{kind}(?:\s+{name})?)?\s*(?:!.*)?$", re.IGNORECASE)
             k = j
             while k < len(lines) and not end.match(lines[k]):
                 k += 1
             if k == len(lines):
-                raise ValueError(f"no END of subroutine {name}")
+                raise ValueError(f"no END of {kind} {name}")
             return i, j, k, dummies
     return None
 
@@ -389,13 +412,14 @@ def splice_procedure(text: str, spec: EntrySpec, origin: str) -> str:
         raise ValueError(f"{spec.module}::{spec.procedure} has {len(dummies)} arguments; its entry takes "
                          f"{spec.arrays} arrays and {spec.scalars} scalars")
     lines = text.splitlines(keepends=True)
-    arrays, scalars = dummies[:spec.arrays], dummies[spec.arrays:]
+    arrays, scalars = dummies[:spec.arrays], dummies[spec.arrays:] + list(spec.extra)
     ind = "    "
     n = len(arrays)
+    arguments = [f"ftc_handles({i})" for i in range(1, n + 1)] + ([", ".join(scalars)] if scalars else [])
     block = [
         "block",
         "  use, intrinsic :: iso_c_binding",
-        "  use fort_scoped_memory, only: FORT_SCOPE_GPU, FORT_SCOPE_DECLINED",
+        "  use fort_scoped_memory, only: FORT_SCOPE_GPU, FORT_SCOPE_DECLINED, FORT_SCOPE_RESOURCE",
         f"  use {DOMAIN_MODULE}, only: {DOMAIN}, {CHECK}",
         f"  use {spec.interface}, only: ftc_run => run",
         f"  integer(c_int64_t) :: ftc_handles({n})",
@@ -410,9 +434,11 @@ def splice_procedure(text: str, spec: EntrySpec, origin: str) -> str:
         "  endif",
         "  if (ftc_resident_call) then",
         f"    ftc_status = ftc_run({DOMAIN}%context, FORT_SCOPE_GPU, &\n"
-        + "".join(f"{ind}      ftc_handles({i}), &\n" for i in range(1, n + 1))
-        + f"{ind}      " + ", ".join(scalars) + ")",
-        "    ftc_resident_call = ftc_status /= FORT_SCOPE_DECLINED",
+        + f", &\n".join(f"{ind}      {a}" for a in arguments) + ")",
+        # An entry declines what it does not take; one whose arrays do not fit the device budget together
+        # is refused (RESOURCE): both run the procedure's own statements on the host
+        # (compiler/emission/cuda/scoped.py:472-478, scoped_runtime.cu:1936-1943).
+        "    ftc_resident_call = ftc_status /= FORT_SCOPE_DECLINED .and. ftc_status /= FORT_SCOPE_RESOURCE",
         f"    if (ftc_resident_call) call {CHECK}(ftc_status)",
         "  endif",
         "  if (.not. ftc_resident_call) then",
@@ -421,6 +447,68 @@ def splice_procedure(text: str, spec: EntrySpec, origin: str) -> str:
     head = "".join(ind + b + "\n" for b in block)
     tail = "".join(ind + b + "\n" for b in ["  endif", "end block"])
     return "".join(lines[:first]) + head + "".join(lines[first:end]) + tail + "".join(lines[end:])
+
+
+_INTENT = re.compile(r"intent\s*\(\s*(in|out|inout|in\s+out)\s*\)", re.IGNORECASE)
+
+
+def _dummy_arrays(lines: list[str], header: int, first: int, dummies: list[str]) -> dict[str, str]:
+    """The array dummies of a procedure with their intent ('in', 'out', 'inout'; 'inout' when none is
+    declared), from its specification statements."""
+    names = {d.lower() for d in dummies}
+    found: dict[str, str] = {}
+    i = _logical_lines(lines, header)
+    while i < first:
+        stop = _logical_lines(lines, i)
+        statement = " ".join(l.split("!", 1)[0].strip().rstrip("&") for l in lines[i:stop])
+        i = stop
+        if "::" not in statement:
+            continue
+        attrs, entities = statement.split("::", 1)
+        intent = _INTENT.search(attrs)
+        mode = intent.group(1).lower().replace(" ", "") if intent else "inout"
+        dimensioned = re.search(r"\bdimension\s*\(", attrs, re.IGNORECASE) is not None
+        depth, start, parts = 0, 0, []
+        for pos, ch in enumerate(entities + ","):
+            depth += (ch == "(") - (ch == ")")
+            if ch == "," and depth == 0:
+                parts.append(entities[start:pos].strip())
+                start = pos + 1
+        for part in parts:
+            m = re.match(r"([A-Za-z_]\w*)\s*(\()?", part)
+            if m and m.group(1).lower() in names and (m.group(2) or dimensioned):
+                found[m.group(1)] = mode
+    return found
+
+
+def host_access(text: str, module: str, name: str) -> str:
+    """A selected procedure without an entry stays on the host: its executable part is wrapped in the
+    block that hands it the arrays its caller may keep on the device (compiler/scopes/resident_splice.py:
+    228-313 generates the same protocol): the arrays it only reads are brought up to date and stay on
+    the device, the ones it may change are handed to the host until it is through. Statements that can
+    leave early (RETURN) make the block publish everything instead."""
+    found = find_procedure(text, module, name)
+    if found is None:
+        raise ValueError(f"{module}::{name} not found")
+    header, first, end, dummies = found
+    lines = text.splitlines(keepends=True)
+    arrays = _dummy_arrays(lines, header, first, dummies)
+    if not arrays:
+        return text
+    reads = [a for a, mode in arrays.items() if mode == "in"]
+    writes = [a for a, mode in arrays.items() if mode != "in"]
+    ind = "    "
+    held = lambda items: "[" + ", ".join(f"{DOMAIN}%held({a})" for a in items) + "]"
+    if any(re.match(r"^\s*return\b", l, re.IGNORECASE) for l in lines[first:end]):
+        head = ["block", f"  use {DOMAIN_MODULE}, only: {DOMAIN}", f"  if ({DOMAIN}%active) call {DOMAIN}%publish()"]
+        tail = ["end block"]
+    else:
+        head = ["block", f"  use {DOMAIN_MODULE}, only: {DOMAIN}", "  logical :: ftc_host", f"  ftc_host = {DOMAIN}%active"]
+        head += [f"  if (ftc_host) call {DOMAIN}%observe({held(reads)})"] if reads else []
+        head += [f"  if (ftc_host) call {DOMAIN}%host_modify({held(writes)})"] if writes else []
+        tail = ([f"  if (ftc_host) call {DOMAIN}%host_done({held(writes)})"] if writes else []) + ["end block"]
+    return ("".join(lines[:first]) + "".join(ind + h + "\n" for h in head) + "".join(lines[first:end])
+            + "".join(ind + h + "\n" for h in tail) + "".join(lines[end:]))
 
 
 def selected(description: dict, module: str, procedure: str) -> bool:
@@ -437,7 +525,7 @@ def plan(description: dict, texts: dict[str, str]) -> dict:
     spliced, native = [], []
     for name, text in texts.items():
         for module, lo, hi in _modules(text):
-            for m in re.finditer(r"^\s*(?:(?:pure|elemental|recursive|impure)\s+)*subroutine\s+(\w+)",
+            for m in re.finditer(r"^[ \t]*(?!end\b)(?:[\w(),=*]+[ \t]+)*?(?:subroutine|function)[ \t]+(\w+)[ \t]*\(",
                                  text[lo:hi], re.IGNORECASE | re.MULTILINE):
                 proc = m.group(1).lower()
                 if not selected(description, module, proc):
@@ -470,6 +558,9 @@ def generate(toml_path: Path, out_dir: Path, source_root: Path | None = None) ->
     for name, qualified in chosen["spliced"]:
         module, proc = qualified.split("::")
         texts[name] = splice_procedure(texts[name], ENTRIES[module, proc], qualified)
+    for name, qualified in chosen["native"]:
+        module, proc = qualified.split("::")
+        texts[name] = host_access(texts[name], module, proc)
     edits("edit")
     # The compiler's own loop rewrites reshape nests it generates entries for; hand-written entries have none.
     chosen["ignored_rewrites"] = {f: list(v) for f, v in description.get("rewrites", {}).items()}

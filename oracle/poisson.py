@@ -66,7 +66,9 @@ def solve(rhs: np.ndarray, h, bc=("periodic", "periodic")) -> np.ndarray:
     for k in range(nz):
         diag[k] = -2.0 * r + lam + (r if k == 0 else 0.0) + (r if k == nz - 1 else 0.0)
     zero = lam == 0.0
-    safe = np.where(zero, 1.0, 0.0)  # keeps the singular column finite; it is replaced below
+    # The singular column (lam == 0) is replaced below; until then its diagonal is shifted by -2r, which
+    # makes the column strictly diagonally dominant at every grid spacing, so no pivot is zero.
+    safe = np.where(zero, -2.0 * r, 0.0)
     cp = np.zeros((nz,) + lam.shape)
     dp = np.zeros((nz,) + lam.shape, dtype=np.complex128)
     denom = diag[0] + safe

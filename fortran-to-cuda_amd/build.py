@@ -26,8 +26,8 @@ BUILD = PACKAGE / "_build"
 HIP_LIBRARY = PACKAGE / "libftc_hip.so"
 CPU_LIBRARY = PACKAGE / "libftc_cpu.so"
 
-HIP_SOURCES = ("runtime.cpp", "les.hip", "poisson.hip", "gemm.hip", "reduce.hip", "ensemble.hip", "timing.hip", "box.hip",
-               "numeric.hip", "pressure_step.hip")
+HIP_SOURCES = ("runtime.cpp", "les.hip", "poisson.hip", "gemm.hip", "reduce.hip", "ensemble.hip", "timing.hip",
+               "box.hip", "relax.hip", "numeric.hip", "pressure_step.hip")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]

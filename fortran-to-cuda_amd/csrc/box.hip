@@ -1,6 +1,6 @@
 // Entries for the box application's device procedures (compiler/tests/apps/box): the two-layer periodic
 // halo (boundaries.f90:6-41), the explicit diffusion step (stepper.f90:8-30) and the store of the new
-// interior (stepper.f90:32-37). The arrays are a(-1:n+2, -1:n+2, -1:n+2) of doubles with two ghost layers,
+// interior (stepper.f90:32-36). The arrays are a(-1:n+2, -1:n+2, -1:n+2) of doubles with two ghost layers,
 // so the entries take n from the array layout; the step's dx, dt and nu are module variables of the
 // application that the spliced call passes.
 //

@@ -252,3 +252,58 @@ module fort_shared_stepper_store
     end function
   end interface
 end module fort_shared_stepper_store
+
+module fort_shared_relax_sweep
+  use iso_c_binding
+  implicit none
+  private
+  public :: run
+  interface
+    function run(fort_context, fort_mode, v_handle, u_handle, n) &
+        bind(C, name='cpp_fort_shared_relax_sweep') result(fort_status)
+      import :: c_int, c_int64_t
+      integer(c_int) :: fort_status
+      integer(c_int64_t), value :: fort_context
+      integer(c_int), value :: fort_mode
+      integer(c_int64_t), value :: v_handle, u_handle
+      integer(c_int), intent(in) :: n
+    end function
+  end interface
+end module fort_shared_relax_sweep
+
+module fort_shared_relax_change
+  use iso_c_binding
+  implicit none
+  private
+  public :: run
+  interface
+    function run(fort_context, fort_mode, v_handle, u_handle, n, change) &
+        bind(C, name='cpp_fort_shared_relax_change') result(fort_status)
+      import :: c_int, c_int64_t, c_double
+      integer(c_int) :: fort_status
+      integer(c_int64_t), value :: fort_context
+      integer(c_int), value :: fort_mode
+      integer(c_int64_t), value :: v_handle, u_handle
+      integer(c_int), intent(in) :: n
+      real(c_double), intent(inout) :: change
+    end function
+  end interface
+end module fort_shared_relax_change
+
+module fort_shared_relax_copy
+  use iso_c_binding
+  implicit none
+  private
+  public :: run
+  interface
+    function run(fort_context, fort_mode, u_handle, v_handle, n) &
+        bind(C, name='cpp_fort_shared_relax_copy') result(fort_status)
+      import :: c_int, c_int64_t
+      integer(c_int) :: fort_status
+      integer(c_int64_t), value :: fort_context
+      integer(c_int), value :: fort_mode
+      integer(c_int64_t), value :: u_handle, v_handle
+      integer(c_int), intent(in) :: n
+    end function
+  end interface
+end module fort_shared_relax_copy

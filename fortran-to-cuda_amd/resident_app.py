@@ -28,12 +28,13 @@ from __future__ import annotations
 
 import argparse
 import fnmatch
+import hashlib
 import os
 import re
 import subprocess
 import sys
 import tempfile
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -449,6 +450,327 @@ def splice_procedure(text: str, spec: EntrySpec, origin: str) -> str:
     return "".join(lines[:first]) + head + "".join(lines[first:end]) + tail + "".join(lines[end:])
 
 
+# ------------------------------------------------------------------------------------------ loop nests
+# The reference's compiler cuts the loop nests of a selected main program into generated entries named by a
+# hash of the nest (compiler/emission/cuda/scoped.py:61). Here a nest is spliced onto a hand-written entry
+# when the hash of its normalised text is one this table lists; the program's other statements that touch
+# the nests' arrays run on the host inside host-access blocks.
+@dataclass(frozen=True)
+class NestEntry:
+    """A hand-written entry for one loop nest or array assignment: its interface module (fortran/
+    ftc_entries.f90), the nest's arrays in the entry's order and its scalars, passed by reference under the
+    names the nest uses."""
+    interface: str
+    arrays: tuple[str, ...]
+    scalars: tuple[str, ...]
+
+
+NEST_ENTRIES = {
+    # compiler/tests/apps/relax/relax.f90:29-35, 36-43 and 44
+    "e86ab74e21c687e9": NestEntry("fort_shared_relax_sweep", ("v", "u"), ("n",)),
+    "7f1afb921f136343": NestEntry("fort_shared_relax_change", ("v", "u"), ("n", "change")),
+    "7c2c9121a204dae3": NestEntry("fort_shared_relax_copy", ("u", "v"), ("n",)),
+}
+
+
+def _code(line: str) -> str:
+    """The line without its comment (a `!` outside character constants)."""
+    quote = None
+    for pos, ch in enumerate(line):
+        if quote:
+            quote = None if ch == quote else quote
+        elif ch in "'\"":
+            quote = ch
+        elif ch == "!":
+            return line[:pos]
+    return line
+
+
+def _statement(lines: list[str], i: int) -> tuple[str, int]:
+    """The code of the statement that begins at line i, continuations joined, and the line after it."""
+    parts = []
+    while True:
+        code = _code(lines[i]).strip()
+        more = code.endswith("&")
+        parts.append(code.rstrip("&").lstrip("&").strip())
+        i += 1
+        if not more or i == len(lines):
+            return " ".join(parts), i
+
+
+def nest_key(lines: list[str]) -> str:
+    """The key of a loop nest or a statement: a hash of its code with comments, directives, blanks and
+    letter case taken out, so that layout and OpenMP clauses do not change it."""
+    parts, i = [], 0
+    while i < len(lines):
+        if not _code(lines[i]).strip():
+            i += 1
+            continue
+        text, i = _statement(lines, i)
+        parts.append(re.sub(r"\s+", "", text.lower()))
+    return hashlib.sha256(";".join(parts).encode()).hexdigest()[:16]
+
+
+@dataclass
+class _Item:
+    """A statement or construct of an executable part. start..stop spans it with the OpenMP directives
+    that belong to it (the ones before it, and a closing one after it); first is its own first line."""
+    start: int
+    stop: int
+    first: int
+    kind: str  # "statement", "construct", "middle" (ELSE, CASE ...) or "directive" (!$fort)
+    body: list = field(default_factory=list)
+
+
+def _balanced_end(text: str, open_at: int) -> int:
+    depth = 0
+    for pos in range(open_at, len(text)):
+        depth += (text[pos] == "(") - (text[pos] == ")")
+        if depth == 0:
+            return pos
+    return len(text)
+
+
+def _opens(code: str) -> bool:
+    s = re.sub(r"^\w+\s*:\s*(?=\w)", "", code)  # a construct name
+    if re.match(r"do\b", s) or re.match(r"(select\s*(case|type|rank)|associate|critical)\b", s):
+        return True
+    if re.fullmatch(r"block", s):
+        return True
+    for word in ("if", "where", "forall"):
+        m = re.match(rf"{word}\s*\(", s)
+        if m:
+            rest = s[_balanced_end(s, m.end() - 1) + 1:].strip()
+            return rest == "then" if word == "if" else rest == ""
+    return False
+
+
+_CLOSES = re.compile(r"end\s*(do|if|select|block|where|forall|associate|critical)\b")
+_MIDDLE = re.compile(r"(else\b|elseif\b|elsewhere\b|case\b|class\s+(is|default)\b|type\s+is\b|rank\b)")
+
+
+def _parse(lines: list[str], lo: int, hi: int) -> list[_Item]:
+    """The statements and constructs of lines[lo:hi], constructs holding their bodies."""
+    root: list[_Item] = []
+    stack: list[list[_Item]] = [root]
+    opened: list[_Item] = []
+    pending = None  # first line of the OpenMP directives that precede the next statement
+    i = lo
+    while i < hi:
+        s = lines[i].strip().lower()
+        if s.startswith("!$omp"):
+            j = i + 1
+            while lines[j - 1].rstrip().endswith("&") and j < hi:
+                j += 1
+            if re.match(r"!\$omp\s+end\b", s):
+                if stack[-1]:
+                    stack[-1][-1].stop = j
+            elif pending is None:
+                pending = i
+            i = j
+            continue
+        if s.startswith("!$fort"):
+            stack[-1].append(_Item(i, i + 1, i, "directive"))
+            i += 1
+            continue
+        if not _code(lines[i]).strip():
+            i += 1
+            continue
+        text, j = _statement(lines, i)
+        code = text.lower()
+        start, pending = (i if pending is None else pending), None
+        if _CLOSES.match(code) or re.fullmatch(r"end\s*(do|if)", code):
+            if not opened:
+                raise ValueError(f"line {i + 1}: END without its construct")
+            opened.pop().stop = j
+            stack.pop()
+        elif _MIDDLE.match(code) and _assignment_at(code) < 0:
+            stack[-1].append(_Item(i, j, i, "middle"))
+        else:
+            item = _Item(start, j, i, "construct" if _opens(code) else "statement")
+            stack[-1].append(item)
+            if item.kind == "construct":
+                opened.append(item)
+                stack.append(item.body)
+        i = j
+    if opened:
+        raise ValueError(f"line {opened[-1].first + 1}: construct without its END")
+    return root
+
+
+def _assignment_at(code: str) -> int:
+    """The position of the statement's assignment `=` (outside parentheses, not part of ==, /=, <=, >= or
+    =>), or -1."""
+    depth = 0
+    for pos, ch in enumerate(code):
+        depth += (ch == "(") - (ch == ")")
+        if (ch == "=" and depth == 0 and code[pos - 1:pos] not in ("=", "/", "<", ">")
+                and code[pos + 1:pos + 2] not in ("=", ">")):
+            return pos
+    return -1
+
+
+def _written(code: str, names: set[str]) -> set[str]:
+    """The arrays of `names` a statement may change: on the left of its assignment, in the argument list of
+    a CALL, or in a READ, ALLOCATE or DEALLOCATE."""
+    if re.match(r"(call|read|allocate|deallocate)\b", code):
+        return {w for w in re.findall(r"[a-z_]\w*", code) if w in names}
+    pos = _assignment_at(code)
+    return {w for w in re.findall(r"[a-z_]\w*", code[:pos]) if w in names} if pos >= 0 else set()
+
+
+def _accesses(lines: list[str], item: _Item, names: set[str]) -> tuple[list[str], list[str]]:
+    """The arrays of `names` an item reads only, and the ones it may change, in order of appearance."""
+    seen: list[str] = []
+    written: set[str] = set()
+    i = item.first
+    while i < item.stop:
+        if lines[i].lstrip().lower().startswith("!$") or not _code(lines[i]).strip():
+            i += 1
+            continue
+        text, i = _statement(lines, i)
+        code = text.lower()
+        for w in re.findall(r"[a-z_]\w*", code):
+            if w in names and w not in seen:
+                seen.append(w)
+        written |= _written(code, names)
+    return [a for a in seen if a not in written], [a for a in seen if a in written]
+
+
+def _entry_block(interface: str, arrays: list[str], scalars: list[str], origin: str) -> list[str]:
+    """The head of the resident block that calls an entry; the original statements follow it, then
+    `endif` and `end block`."""
+    ind = "    "
+    n = len(arrays)
+    arguments = [f"ftc_handles({i})" for i in range(1, n + 1)] + ([", ".join(scalars)] if scalars else [])
+    return [
+        "block",
+        "  use, intrinsic :: iso_c_binding",
+        "  use fort_scoped_memory, only: FORT_SCOPE_GPU, FORT_SCOPE_DECLINED, FORT_SCOPE_RESOURCE",
+        f"  use {DOMAIN_MODULE}, only: {DOMAIN}, {CHECK}",
+        f"  use {interface}, only: ftc_run => run",
+        f"  integer(c_int64_t) :: ftc_handles({n})",
+        "  integer(c_int) :: ftc_status",
+        "  logical :: ftc_resident_call",
+        f"  ftc_resident_call = {DOMAIN}%active",
+        "  if (ftc_resident_call) then",
+        *(f"    ftc_handles({i}) = {DOMAIN}%bind_real64({a}, lbound({a}), .true., &\n"
+          f"{ind}      origin='{origin}::{a.lower()}', tolerant=.true.)" for i, a in enumerate(arrays, 1)),
+        # An array that cannot be kept as it is bound here has no handle: the statements stay on the host.
+        "    ftc_resident_call = all(ftc_handles /= 0_c_int64_t)",
+        "  endif",
+        "  if (ftc_resident_call) then",
+        f"    ftc_status = ftc_run({DOMAIN}%context, FORT_SCOPE_GPU, &\n"
+        + ", &\n".join(f"{ind}      {a}" for a in arguments) + ")",
+        # An entry declines what it does not take; one whose arrays do not fit the device budget together
+        # is refused (RESOURCE): both run the procedure's own statements on the host
+        # (compiler/emission/cuda/scoped.py:472-478, scoped_runtime.cu:1936-1943).
+        "    ftc_resident_call = ftc_status /= FORT_SCOPE_DECLINED .and. ftc_status /= FORT_SCOPE_RESOURCE",
+        f"    if (ftc_resident_call) call {CHECK}(ftc_status)",
+        "  endif",
+        "  if (.not. ftc_resident_call) then",
+        f"    if ({DOMAIN}%active) call {DOMAIN}%publish()",
+    ]
+
+
+def _host_block(reads: list[str], writes: list[str]) -> tuple[list[str], list[str]]:
+    """Head and tail of a host-access block: the arrays read are brought up to date and stay valid on the
+    device; the ones changed are the host's until the tail."""
+    held = lambda items: "[" + ", ".join(f"{DOMAIN}%held({a})" for a in items) + "]"
+    head = ["block", f"  use {DOMAIN_MODULE}, only: {DOMAIN}", "  logical :: ftc_host", f"  ftc_host = {DOMAIN}%active"]
+    head += [f"  if (ftc_host) call {DOMAIN}%observe({held(reads)})"] if reads else []
+    head += [f"  if (ftc_host) call {DOMAIN}%host_modify({held(writes)})"] if writes else []
+    tail = ([f"  if (ftc_host) call {DOMAIN}%host_done({held(writes)})"] if writes else []) + ["end block"]
+    return head, tail
+
+
+_PROGRAM = re.compile(r"^[ \t]*program[ \t]+(\w+)", re.IGNORECASE | re.MULTILINE)
+
+
+def find_program(text: str, name: str) -> tuple[int, int] | None:
+    """(first executable line, END PROGRAM line) of `program name`, as line indices; None when absent."""
+    lines = text.splitlines(keepends=True)
+    for i, line in enumerate(lines):
+        m = _PROGRAM.match(line)
+        if not m or m.group(1).lower() != name:
+            continue
+        j = _logical_lines(lines, i)
+        while j < len(lines):
+            code = _code(lines[j]).strip()
+            if code and not _SPEC.match(code) and not lines[j].lstrip().lower().startswith("!$"):
+                break
+            j = _logical_lines(lines, j) if code else j + 1
+        end = re.compile(rf"^\s*end\s*(?:program(?:\s+{name})?)?\s*(?:!.*)?$", re.IGNORECASE)
+        k = j
+        while k < len(lines) and not end.match(lines[k]):
+            k += 1
+        if k == len(lines):
+            raise ValueError(f"no END of program {name}")
+        return j, k
+    return None
+
+
+def splice_nests(text: str, program: str) -> tuple[str, list[dict]]:
+    """`text` with the nests of `program` that have entries spliced onto them, and the other statements
+    between its `!$fort enter` and `!$fort leave` that touch their arrays wrapped in host-access blocks;
+    and one record per spliced nest."""
+    found = find_program(text, program)
+    if found is None:
+        raise ValueError(f"program {program} not found")
+    first, end = found
+    lines = text.splitlines(keepends=True)
+    items = _parse(lines, first, end)
+    spliced: list[tuple[_Item, NestEntry]] = []
+
+    def walk(body):
+        for item in body:
+            if item.kind in ("statement", "construct"):
+                entry = NEST_ENTRIES.get(nest_key(lines[item.first:item.stop]))
+                if entry:
+                    spliced.append((item, entry))
+                    continue
+            walk(item.body)
+
+    walk(items)
+    device = {a for _, e in spliced for a in e.arrays}
+    inside = {id(item) for item, _ in spliced}
+    contains = lambda item: id(item) in inside or any(contains(c) for c in item.body)
+    marks = [i for i, l in enumerate(lines[first:end], first) if re.match(r"\s*!\$fort\s+(enter|leave)\b", l, re.I)]
+    lo, hi = (marks[0], marks[-1]) if len(marks) >= 2 else (first, end)
+    edits: list[tuple[int, int, list[str], list[str]]] = []
+    for item, entry in spliced:
+        edits.append((item.start, item.stop,
+                      _entry_block(entry.interface, list(entry.arrays), list(entry.scalars), program),
+                      ["  endif", "end block"]))
+
+    def touches(i: int) -> bool:
+        return any(w in device for w in re.findall(r"[a-z_]\w*", _statement(lines, i)[0].lower()))
+
+    def guard(body):
+        for item in body:
+            if item.kind not in ("statement", "construct") or id(item) in inside or not lo < item.first < hi:
+                continue
+            # A construct with a spliced nest inside, or a branch whose conditions do not name the arrays,
+            # is guarded statement by statement, so that the arrays move only where the host uses them; a
+            # loop is guarded whole, so that they move once and not per iteration.
+            branch = item.kind == "construct" and not re.match(r"(\w+\s*:\s*)?(do|forall|where)\b",
+                                                               _statement(lines, item.first)[0].lower())
+            heads = [item.first] + [m.first for m in item.body if m.kind == "middle"]
+            if contains(item) or (branch and not any(touches(i) for i in heads)):
+                guard(item.body)
+                continue
+            reads, writes = _accesses(lines, item, device)
+            if reads or writes:
+                edits.append((item.start, item.stop, *_host_block(reads, writes)))
+
+    guard(items)
+    for start, stop, head, tail in sorted(edits, key=lambda e: e[0], reverse=True):
+        ind = re.match(r"[ \t]*", lines[start]).group(0)
+        lines[start:stop] = ([ind + h + "\n" for h in head] + lines[start:stop] + [ind + t + "\n" for t in tail])
+    records = [{"program": program, "line": item.first + 1, "entry": entry.interface} for item, entry in spliced]
+    return "".join(lines), records
+
+
 _INTENT = re.compile(r"intent\s*\(\s*(in|out|inout|in\s+out)\s*\)", re.IGNORECASE)
 
 
@@ -531,7 +853,10 @@ def plan(description: dict, texts: dict[str, str]) -> dict:
                 if not selected(description, module, proc):
                     continue
                 (spliced if (module, proc) in ENTRIES else native).append((name, f"{module}::{proc}"))
-    return {"spliced": spliced, "native": native}
+    # A selected main program ("$program::name") has its loop nests spliced onto the entries listed for them.
+    programs = [(name, m.group(1).lower()) for name, text in texts.items() for m in _PROGRAM.finditer(text)
+                if selected(description, "$program", m.group(1).lower())]
+    return {"spliced": spliced, "native": native, "programs": programs}
 
 
 def generate(toml_path: Path, out_dir: Path, source_root: Path | None = None) -> tuple[list[Path], dict]:
@@ -561,6 +886,10 @@ def generate(toml_path: Path, out_dir: Path, source_root: Path | None = None) ->
     for name, qualified in chosen["native"]:
         module, proc = qualified.split("::")
         texts[name] = host_access(texts[name], module, proc)
+    chosen["nests"] = []
+    for name, program in chosen["programs"]:
+        texts[name], records = splice_nests(texts[name], program)
+        chosen["nests"] += [{"file": name, **r} for r in records]
     edits("edit")
     # The compiler's own loop rewrites reshape nests it generates entries for; hand-written entries have none.
     chosen["ignored_rewrites"] = {f: list(v) for f, v in description.get("rewrites", {}).items()}

@@ -20,7 +20,7 @@ int cpp_fort_shared_boundaries_periodic(fort_scope_t ctx, int mode, fort_buffer_
 int cpp_fort_shared_stepper_diffuse(fort_scope_t ctx, int mode, fort_buffer_t a, fort_buffer_t b, const double *dx,
                                     const double *dt, const double *nu);
 
-/* stepper::store(a, b), stepper.f90:32-37: a(1:n, 1:n, 1:n) = b(1:n, 1:n, 1:n). */
+/* stepper::store(a, b), stepper.f90:32-36: a(1:n, 1:n, 1:n) = b(1:n, 1:n, 1:n). */
 int cpp_fort_shared_stepper_store(fort_scope_t ctx, int mode, fort_buffer_t a, fort_buffer_t b);
 
 #ifdef __cplusplus

@@ -24,20 +24,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kGemmThreads = 256;
 
-// The tile of this workgroup in a one-call launch over grid (mtiles, ntiles, batch). Workgroups are handed
-// to the eight XCDs round-robin in launch order, so consecutive tiles would land in eight different L2s;
-// the remap gives each XCD one contiguous run of the tile order (mt fastest), so the tiles that re-read one
-// operand panel (same nt and b, or same mt and b) run under one L2.
-__device__ __forceinline__ void gemm_block_tile(int &mt, int &nt, long long &b)
-{
-    const unsigned mtiles = gridDim.x, ntiles = gridDim.y, total = mtiles * ntiles * gridDim.z;
-    unsigned L = blockIdx.x + mtiles * (blockIdx.y + ntiles * blockIdx.z);
-    L = xcd_contiguous(L, total);
-    mt = (int)(L % mtiles);
-    nt = (int)((L / mtiles) % ntiles);
-    b = (long long)(L / (mtiles * ntiles));
-}
-
 // ------------------------------------------------------------------------------------------------ FP64
 template <int BM, int BN>
 struct F64Lds {

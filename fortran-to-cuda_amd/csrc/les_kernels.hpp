@@ -742,26 +742,20 @@ __global__ __launch_bounds__(NTH, (sgs3_occupancy<ROWS, WX>())) void les_sgs3_ke
     using T = Sgs3Tile<ROWS, WX>;
     __shared__ Sgs3Lds<ROWS, WX> L;
     if (a.tile_order == 2) {
-        // Persistent banded walk: one resident round of workgroups (a multiple of eight). XCD x owns the
-        // contiguous tile range [T x / 8, T (x+1) / 8) of the T tiles in x-fastest order, and its
-        // workgroups take consecutive slots of that range in step, so the tiles resident on one XCD at a
-        // time form a band whose x and y neighbours re-read each other's ring rows from the same L2.
-        // Every tile belongs to one range and one slot of it: each is computed exactly once.
+        // Persistent banded walk (xcd.hpp) over the tiles in x-fastest order, by one resident round of
+        // workgroups (a multiple of eight): the band resident on one XCD re-reads its x and y neighbours'
+        // ring rows from the same L2.
         const unsigned gx = (a.nx + WX - 1) / WX, gy = (a.ny + T::TYT - 1) / T::TYT;
         const unsigned gz = (a.nz + a.kchunk - 1) / a.kchunk;
-        const unsigned total = gx * gy * gz, per = gridDim.x / kXcds;
-        const unsigned x = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
-        const unsigned lo = (unsigned)((unsigned long long)total * x / kXcds);
-        const unsigned hi = (unsigned)((unsigned long long)total * (x + 1) / kXcds);
-        for (unsigned t = lo + slot; t < hi; t += per)
+        const TileWalk walk = banded_walk(gx * gy * gz, gridDim.x, blockIdx.x);
+        for (unsigned t = walk.first; t < walk.end; t += walk.step)
             les_sgs3_body<STEP, ROWS, WX>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
         return;
     }
-    const unsigned gx = gridDim.x, gy = gridDim.y;
-    unsigned t = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     // Tiles x fastest, then y, then the k chunk: with the XCD-contiguous order the x and y neighbours
     // whose ghost rows and columns a tile re-reads run under the same L2 at about the same time.
-    if (a.tile_order) t = xcd_contiguous(t, gx * gy * gridDim.z);
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned t = launch_tile(a.tile_order);
     les_sgs3_body<STEP, ROWS, WX>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
 }
 

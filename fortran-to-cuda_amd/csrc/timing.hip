@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/fort_scope.h"
+#include "xcd.hpp"
 
 // Grid-stride copy of n doubles as 16-byte pairs, four pairs in flight per thread; the odd last double
 // (if any) is copied by thread 0.
@@ -76,6 +77,26 @@ __global__ void ftc_iota(double *__restrict__ x, long long n, double scale)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] = scale * (double)i;
+}
+
+// Counts the visits of every tile under one of the tile orders of xcd.hpp, one thread per workgroup: scheme
+// 0 launch order, 1 XCD-contiguous, 2 the banded walk of `total` tiles by gridDim.x workgroups, 3 the
+// contraction order. A tile number past the last (which a correct order never gives) is counted in the
+// extra slot counts[total], so every store stays in the array.
+__global__ void ftc_tile_visit_kernel(int scheme, unsigned total, unsigned *counts)
+{
+    auto visit = [&](unsigned long long t) { atomicAdd(&counts[t < total ? t : total], 1u); };
+    if (scheme == 2) {
+        const ftc::TileWalk w = ftc::banded_walk(total, gridDim.x, blockIdx.x);
+        for (unsigned t = w.first; t < w.end; t += w.step) visit(t);
+    } else if (scheme == 3) {
+        int mt, nt;
+        long long b;
+        ftc::gemm_block_tile(mt, nt, b);
+        visit((unsigned long long)mt + gridDim.x * ((unsigned long long)nt + (unsigned long long)gridDim.y * b));
+    } else {
+        visit(ftc::launch_tile(scheme));
+    }
 }
 
 extern "C" {
@@ -190,6 +211,35 @@ int ftc_event_elapsed_ms(void *start, void *stop, float *ms)
 int ftc_event_destroy(void *event)
 {
     return hipEventDestroy(static_cast<hipEvent_t>(event)) == hipSuccess ? FORT_SCOPE_OK : FORT_SCOPE_ARGUMENT;
+}
+
+int ftc_tile_visit_probe_v1(int scheme, unsigned gx, unsigned gy, unsigned gz, unsigned workgroups, unsigned *counts)
+{
+    if (scheme < 0 || scheme > 3 || !counts || !gx || !gy || !gz || gy > 65535 || gz > 65535)
+        return FORT_SCOPE_ARGUMENT;
+    const unsigned long long total = (unsigned long long)gx * gy * gz;
+    if (total > (1ull << 24)) return FORT_SCOPE_ARGUMENT;
+    // The walk's step is workgroups / 8: fewer than eight workgroups, or a count that is not a multiple of
+    // eight, is not a walk the kernels launch.
+    if (scheme == 2 && (workgroups == 0 || workgroups % ftc::kXcds != 0 || workgroups > (1u << 20)))
+        return FORT_SCOPE_ARGUMENT;
+    unsigned *d = nullptr;
+    const std::size_t bytes = (total + 1) * sizeof(unsigned);
+    if (hipMalloc(&d, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return FORT_SCOPE_RESOURCE;
+    }
+    int status = FORT_SCOPE_OK;
+    if (hipMemset(d, 0, bytes) != hipSuccess) status = FORT_SCOPE_EXECUTION;
+    if (status == FORT_SCOPE_OK) {
+        const dim3 grid = scheme == 2 ? dim3(workgroups) : dim3(gx, gy, gz);
+        hipLaunchKernelGGL(ftc_tile_visit_kernel, grid, dim3(1), 0, nullptr, scheme, (unsigned)total, d);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(counts, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            status = FORT_SCOPE_EXECUTION;
+    }
+    (void)hipFree(d);
+    return status;
 }
 
 // The number of visible devices; 0 without a GPU.

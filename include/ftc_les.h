@@ -71,6 +71,13 @@ int ftc_copy_probe(size_t n, int reps, float *ms_per_launch);
 /* The HBM ceilings of this part: kind 0 copy (the probe above), 1 copy with nontemporal loads and stores,
  * 2 read only, 3 write only, over n doubles (n even). No reference counterpart. */
 int ftc_bandwidth_probe(int kind, size_t n, int reps, float *ms_per_launch);
+/* Visit counts of the tile orders the kernels use (csrc/xcd.hpp), for a check that each covers every tile
+ * exactly once (as compiler/tests/test_cuda_coordinate_quotient.py counts the visits of the reference's
+ * grid-stride decode): scheme 0 the launch order and 1 the XCD-contiguous order of a gx x gy x gz grid, 2
+ * the persistent banded walk of gx*gy*gz tiles by `workgroups` (a positive multiple of 8), 3 the
+ * contraction order of a gx x gy x gz grid. counts has gx*gy*gz + 1 entries: the visits of each tile, then
+ * of any tile number past the last (0 for a correct order). No reference counterpart. */
+int ftc_tile_visit_probe_v1(int scheme, unsigned gx, unsigned gy, unsigned gz, unsigned workgroups, unsigned *counts);
 /* The device function of a kernel (double *x, long long n, double scale) that writes x[i] = scale * i for
  * i < n with one thread per element (256-thread blocks): a task body for exercising the graph queue and
  * merged member launches from a caller that has no kernels of its own. No reference counterpart. */

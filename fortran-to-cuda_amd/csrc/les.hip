@@ -88,12 +88,17 @@ static dim3 grid_of(const LesArgs &a, int kplanes, int rows = 1, int width = TX)
 }
 
 template <bool STEP>
-static const void *staged_kernel(int rows, int width)
+static const void *staged_kernel(int rows, int width, bool iso)
 {
     if (rows == 2) return reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 2>);
-    if (width == 32) return reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 32>);
-    if (width == 128) return reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 128>);
-    return reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1>);
+    if (width == 32)
+        return iso ? reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 32, true>)
+                   : reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 32>);
+    if (width == 128)
+        return iso ? reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 128, true>)
+                   : reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 128>);
+    return iso ? reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, TX, true>)
+               : reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1>);
 }
 
 static Sec3 box(std::size_t i0, std::size_t i1, std::size_t j0, std::size_t j1, std::size_t k0, std::size_t k1)
@@ -246,10 +251,13 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     b[6] = {w, {box(0, X + 2, 0, Y + 2, 0, Z + 1)}, {}, {}};
     b[7] = {s, {full}, {}, {}};
     const void *fn;
-    if (staged) fn = step ? staged_kernel<true>(rows, width) : staged_kernel<false>(rows, width);
+    // Equal spacings select the kernel that holds each per-direction coefficient once (same values, same
+    // arithmetic); FTC_LES_ANISO keeps the general one for comparison.
+    const bool iso = *dx == *dy && *dy == *dz && !std::getenv("FTC_LES_ANISO");
+    if (staged) fn = step ? staged_kernel<true>(rows, width, iso) : staged_kernel<false>(rows, width, iso);
     else if (step) fn = sgs ? kernel_of<15, true, true>() : kernel_of<15, false, true>();
     else fn = sgs ? kernel_of<15, true, false>() : kernel_of<15, false, false>();
-    const uint64_t key = 0x1e5000000000000full ^ ((uint64_t)step << 8) ^ ((uint64_t)sgs << 9);
+    const uint64_t key = 0x1e5000000000000full ^ ((uint64_t)step << 8) ^ ((uint64_t)sgs << 9) ^ ((uint64_t)iso << 10);
     dim3 grid = grid_of(a, nz, rows, width);
     if (persistent) {
         // One resident round: the workgroups the device holds at once (three, or two with two rows or the

@@ -412,7 +412,11 @@ struct ZCarry {
     double u, v, w, wxp, wyp;               // U, V, W at (i, j), W at (i+1, j) and (i, j+1)
 };
 
-template <bool STEP, int ROWS = 1, int WX = TX>
+// ISO: the grid spacings are equal (dx == dy == dz), so each per-direction coefficient is one value. The
+// arithmetic is the same (the same values in the same order); the kernel only holds 7 coefficients instead of
+// 20, which keeps them in scalar registers instead of spilling them. 0.25/d serves as both the advection's
+// c25 and the viscosity's quarter-difference factor (the host computes both as 0.25/d).
+template <bool STEP, int ROWS = 1, int WX = TX, bool ISO = false>
 __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX> &L, int bx, int by, int bz)
 {
     using T = Sgs3Tile<ROWS, WX>;
@@ -449,6 +453,12 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
         const int ii = periodic_image(i0 - 1 + cc % CX, nx), jj = periodic_image(j0 - 1 + cc / CX, ny);
         coff[r] = (long long)(ii - 1) + (long long)(jj - 1) * sj;
     }
+    const double c25x = a.c25[0], c25y = ISO ? c25x : a.c25[1], c25z = ISO ? c25x : a.c25[2];
+    const double c50x = a.c50[0], c50y = ISO ? c50x : a.c50[1], c50z = ISO ? c50x : a.c50[2];
+    const double c125x = a.c125[0], c125y = ISO ? c125x : a.c125[1], c125z = ISO ? c125x : a.c125[2];
+    const double ix = a.sgs[0], iy = ISO ? ix : a.sgs[1], iz = ISO ? ix : a.sgs[2];  // 1/d
+    const double hx = a.sgs[6], hy = ISO ? hx : a.sgs[7], hz = ISO ? hx : a.sgs[8];  // 1/d^2
+    const double cs2 = a.sgs[9];                                                    // (Cs Delta)^2
     double rv[VR][3], rs[CR];
     auto fetch_vel = [&](int kp) {
         if (kp < 1 || kp > nz + 2) return;
@@ -497,15 +507,15 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
 #define QU(s, di, dj) L.vel[s][0][vy + (dj)][vx + (di)]
 #define QV(s, di, dj) L.vel[s][1][vy + (dj)][vx + (di)]
 #define QW(s, di, dj) L.vel[s][2][vy + (dj)][vx + (di)]
-            double s11 = (QU(s0, 0, 0) - QU(s0, -1, 0)) * a.sgs[0];
-            double s22 = (QV(s0, 0, 0) - QV(s0, 0, -1)) * a.sgs[1];
-            double s33 = (QW(s0, 0, 0) - QW(sm, 0, 0)) * a.sgs[2];
-            double dudy = ((QU(s0, 0, 1) + QU(s0, -1, 1)) - (QU(s0, 0, -1) + QU(s0, -1, -1))) * a.sgs[4];
-            double dvdx = ((QV(s0, 1, 0) + QV(s0, 1, -1)) - (QV(s0, -1, 0) + QV(s0, -1, -1))) * a.sgs[3];
-            double dudz = ((QU(sp, 0, 0) + QU(sp, -1, 0)) - (QU(sm, 0, 0) + QU(sm, -1, 0))) * a.sgs[5];
-            double dwdx = ((QW(s0, 1, 0) + QW(sm, 1, 0)) - (QW(s0, -1, 0) + QW(sm, -1, 0))) * a.sgs[3];
-            double dvdz = ((QV(sp, 0, 0) + QV(sp, 0, -1)) - (QV(sm, 0, 0) + QV(sm, 0, -1))) * a.sgs[5];
-            double dwdy = ((QW(s0, 0, 1) + QW(sm, 0, 1)) - (QW(s0, 0, -1) + QW(sm, 0, -1))) * a.sgs[4];
+            double s11 = (QU(s0, 0, 0) - QU(s0, -1, 0)) * ix;
+            double s22 = (QV(s0, 0, 0) - QV(s0, 0, -1)) * iy;
+            double s33 = (QW(s0, 0, 0) - QW(sm, 0, 0)) * iz;
+            double dudy = ((QU(s0, 0, 1) + QU(s0, -1, 1)) - (QU(s0, 0, -1) + QU(s0, -1, -1))) * c25y;
+            double dvdx = ((QV(s0, 1, 0) + QV(s0, 1, -1)) - (QV(s0, -1, 0) + QV(s0, -1, -1))) * c25x;
+            double dudz = ((QU(sp, 0, 0) + QU(sp, -1, 0)) - (QU(sm, 0, 0) + QU(sm, -1, 0))) * c25z;
+            double dwdx = ((QW(s0, 1, 0) + QW(sm, 1, 0)) - (QW(s0, -1, 0) + QW(sm, -1, 0))) * c25x;
+            double dvdz = ((QV(sp, 0, 0) + QV(sp, 0, -1)) - (QV(sm, 0, 0) + QV(sm, 0, -1))) * c25z;
+            double dwdy = ((QW(s0, 0, 1) + QW(sm, 0, 1)) - (QW(s0, 0, -1) + QW(sm, 0, -1))) * c25y;
 #undef QU
 #undef QV
 #undef QW
@@ -513,13 +523,12 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
             double s13 = 0.5 * (dudz + dwdx);
             double s23 = 0.5 * (dvdz + dwdy);
             double ss = 2.0 * ((s11 * s11 + s22 * s22) + s33 * s33) + 4.0 * ((s12 * s12 + s13 * s13) + s23 * s23);
-            nuo[lj][li] = a.sgs[9] * sqrt(ss);
+            nuo[lj][li] = cs2 * sqrt(ss);
         }
     };
 
     const double dt = a.dt;
-    const double c25x = a.c25[0], c25y = a.c25[1], c25z = a.c25[2];
-    const double Cz = a.c50[2];
+    const double Cz = c50z;
 #define VU(s, di, dj) L.vel[s][0][ly + 2 + (dj)][tx + 2 + (di)]
 #define VV(s, di, dj) L.vel[s][1][ly + 2 + (dj)][tx + 2 + (di)]
 #define VW(s, di, dj) L.vel[s][2][ly + 2 + (dj)][tx + 2 + (di)]
@@ -616,8 +625,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
                 double t = -((x + y) + z);
                 double Vadv = ((v0 + vxp) + vym) + vxpym;
                 double Wadv = ((w0 + wxp) + wzm) + wxpzm;
-                t = t - ((a.c50[0] * (uxp - uxm) * u0 + a.c125[1] * (uyp - uym) * Vadv) +
-                         a.c125[2] * (uzp - uzm) * Wadv);
+                t = t - ((c50x * (uxp - uxm) * u0 + c125y * (uyp - uym) * Vadv) + c125z * (uzp - uzm) * Wadv);
                 t = t * 0.5;
                 double fxp = n100 * (uxp - u0);
                 double fxm = n000 * (u0 - uxm);
@@ -629,7 +637,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
                 up.du = exz0 * (uzp - u0);
                 double fzp = up.du;
                 double fzm = zc.du;
-                t = t + (((fxp - fxm) * a.sgs[6] + (fyp - fym) * a.sgs[7]) + (fzp - fzm) * a.sgs[8]);
+                t = t + (((fxp - fxm) * hx + (fyp - fym) * hy) + (fzp - fzm) * hz);
                 if constexpr (STEP) put_column(a.OU, p, u0 + dt * t, ci, zi, 0);
                 else a.OU[p] = t;
             }
@@ -643,8 +651,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
                 double t = -((y + x) + z);
                 double Uadv = ((u0 + uyp) + uxm) + uxmyp;
                 double Wadv = ((w0 + wyp) + wzm) + wypzm;
-                t = t - ((a.c125[0] * (vxp - vxm) * Uadv + a.c50[1] * (vyp - vym) * v0) +
-                         a.c125[2] * (vzp - vzm) * Wadv);
+                t = t - ((c125x * (vxp - vxm) * Uadv + c50y * (vyp - vym) * v0) + c125z * (vzp - vzm) * Wadv);
                 t = t * 0.5;
                 double exy0 = 0.25 * (((n000 + n100) + n010) + NV(c0, 1, 1));
                 double exyxm = 0.25 * (((nm00 + n000) + NV(c0, -1, 1)) + n010);
@@ -656,7 +663,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
                 up.dv = eyz0 * (vzp - v0);
                 double fzp = up.dv;
                 double fzm = zc.dv;
-                t = t + (((fxp - fxm) * a.sgs[6] + (fyp - fym) * a.sgs[7]) + (fzp - fzm) * a.sgs[8]);
+                t = t + (((fxp - fxm) * hx + (fyp - fym) * hy) + (fzp - fzm) * hz);
                 if constexpr (STEP) put_column(a.OV, p, v0 + dt * t, ci, zi, 0);
                 else a.OV[p] = t;
             }
@@ -672,8 +679,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
                 double t = -((z + y) + x);
                 double Uadv = ((u0 + uzp) + uxm) + uxmzp;
                 double Vadv = ((v0 + vzp) + vym) + vymzp;
-                t = t - ((a.c125[0] * (wxp - wxm) * Uadv + a.c125[1] * (wyp - wym) * Vadv) +
-                         a.c50[2] * (wzp - wzm) * w0);
+                t = t - ((c125x * (wxp - wxm) * Uadv + c125y * (wyp - wym) * Vadv) + c50z * (wzp - wzm) * w0);
                 t = t * 0.5;
                 double exz0 = 0.25 * (((n000 + n100) + n001) + NV(cp, 1, 0));
                 double exzxm = 0.25 * (((nm00 + n000) + NV(cp, -1, 0)) + n001);
@@ -686,7 +692,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
                 up.dw = n001 * (wzp - w0);
                 double fzp = up.dw;
                 double fzm = zc.dw;
-                t = t + (((fxp - fxm) * a.sgs[6] + (fyp - fym) * a.sgs[7]) + (fzp - fzm) * a.sgs[8]);
+                t = t + (((fxp - fxm) * hx + (fyp - fym) * hy) + (fzp - fzm) * hz);
                 if constexpr (STEP) put_column(a.OW, p, w0 + dt * t, ci, zi, 1);
                 else a.OW[p] = t;
             }
@@ -694,7 +700,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
             {
                 const double s0v = SS(c0, 0, 0), sxp = SS(c0, 1, 0), sxm = SS(c0, -1, 0), syp = SS(c0, 0, 1);
                 const double sym = SS(c0, 0, -1), szp = SS(cp, 0, 0);
-                const double Cx = a.c50[0], Cy = a.c50[1];
+                const double Cx = c50x, Cy = c50y;
                 double x = Cx * u0 * (sxp + s0v) - Cx * uxm * (s0v + sxm);
                 double y = Cy * v0 * (syp + s0v) - Cy * vym * (s0v + sym);
                 up.cs = Cz * w0 * (szp + s0v);
@@ -707,7 +713,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
                 up.ds = (0.5 * (n000 + n001)) * (szp - s0v);
                 double fzp = up.ds;
                 double fzm = zc.ds;
-                t = t + (((fxp - fxm) * a.sgs[6] + (fyp - fym) * a.sgs[7]) + (fzp - fzm) * a.sgs[8]);
+                t = t + (((fxp - fxm) * hx + (fyp - fym) * hy) + (fzp - fzm) * hz);
                 if constexpr (STEP) put_column(a.OS, p, s0v + dt * t, ci, zi, 2);
                 else a.OS[p] = t;
             }
@@ -736,7 +742,7 @@ constexpr int sgs3_occupancy()
     return sizeof(Sgs3Lds<ROWS, WX>) <= 160 * 1024 / 3 ? 3 : 2;
 }
 
-template <bool STEP, int ROWS, int WX = TX>
+template <bool STEP, int ROWS, int WX = TX, bool ISO = false>
 __global__ __launch_bounds__(NTH, (sgs3_occupancy<ROWS, WX>())) void les_sgs3_kernel(const LesArgs a)
 {
     using T = Sgs3Tile<ROWS, WX>;
@@ -749,14 +755,14 @@ __global__ __launch_bounds__(NTH, (sgs3_occupancy<ROWS, WX>())) void les_sgs3_ke
         const unsigned gz = (a.nz + a.kchunk - 1) / a.kchunk;
         const TileWalk walk = banded_walk(gx * gy * gz, gridDim.x, blockIdx.x);
         for (unsigned t = walk.first; t < walk.end; t += walk.step)
-            les_sgs3_body<STEP, ROWS, WX>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
+            les_sgs3_body<STEP, ROWS, WX, ISO>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
         return;
     }
     // Tiles x fastest, then y, then the k chunk: with the XCD-contiguous order the x and y neighbours
     // whose ghost rows and columns a tile re-reads run under the same L2 at about the same time.
     const unsigned gx = gridDim.x, gy = gridDim.y;
     const unsigned t = launch_tile(a.tile_order);
-    les_sgs3_body<STEP, ROWS, WX>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
+    les_sgs3_body<STEP, ROWS, WX, ISO>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
 }
 
 }  // namespace ftc

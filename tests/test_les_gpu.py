@@ -53,12 +53,14 @@ def test_golden_stencil_dump_on_the_device(ftc, scope, hip):
 
 
 def tendency(ftc, scope, hip, U, V, W, S, d, cs):
+    # d: one spacing for all three directions, or (dx, dy, dz)
+    dx, dy, dz = d if isinstance(d, tuple) else (d, d, d)
     nx, ny, nz = (n - 2 for n in U.shape)
     outs = [np.zeros_like(U, order="F") for _ in range(4)]
     with scope.Domain(hip) as dom:
         hin = [dom.bind(a) for a in (U, V, W, S)]
         hout = [dom.bind(a) for a in outs]
-        st = hip.cpp_fort_shared_les_tendency(dom.ctx, 1, *hout, *hin, ftc.dp(d), ftc.dp(d), ftc.dp(d),
+        st = hip.cpp_fort_shared_les_tendency(dom.ctx, 1, *hout, *hin, ftc.dp(dx), ftc.dp(dy), ftc.dp(dz),
                                               ftc.dp(cs or 0.0), ftc.ip(nx), ftc.ip(ny), ftc.ip(nz),
                                               ftc.ip(0 if cs is None else 1))
         scope.check(hip, st)
@@ -77,6 +79,20 @@ def test_les_tendency_is_bit_exact(ftc, scope, hip, oracle, fields, cs, n):
     for name, g, w, kz in zip("UVWS", got, want, (nz, nz, nz - 1, nz)):
         sl = (slice(1, nx + 1), slice(1, ny + 1), slice(1, kz + 1))
         assert bits_equal(g[sl], w[sl]), f"{name}: " + describe_mismatch(g[sl], w[sl])
+
+
+@pytest.mark.parametrize("spacing", [(10.0, 8.0, 12.5), (4.0, 4.0, 2.5)])
+@pytest.mark.parametrize("n", [(64, 64, 64), (70, 37, 19)])
+def test_les_tendency_with_unequal_spacings_is_bit_exact(ftc, scope, hip, oracle, fields, spacing, n):
+    # Unequal spacings take the kernel that holds every per-direction coefficient; equal ones (the tests
+    # above) take the one that holds each once.
+    U, V, W, S = fields.les_fields(*n)
+    got = tendency(ftc, scope, hip, U, V, W, S, spacing, 0.17)
+    want = oracle.les_tendency(U, V, W, S, *spacing, 0.17)
+    nx, ny, nz = n
+    for name, g, w, kz in zip("UVWS", got, want, (nz, nz, nz - 1, nz)):
+        sl = (slice(1, nx + 1), slice(1, ny + 1), slice(1, kz + 1))
+        assert bits_equal(g[sl], w[sl]), f"{name} {spacing}: " + describe_mismatch(g[sl], w[sl])
 
 
 @pytest.mark.parametrize("n,steps", [((48, 40, 24), 6), ((33, 17, 9), 3), ((1, 2, 1), 2), ((512, 512, 128), 4)])
@@ -116,12 +132,14 @@ def test_les_steps_with_graph_replay_are_bit_exact_including_halos(ftc, scope, h
 @pytest.mark.parametrize("variant", [{"FTC_LES_ROWS": "2"}, {"FTC_LES_XCD_ORDER": "1"}, {"FTC_LES_UNSTAGED": "1"},
                                      {"FTC_LES_KCHUNK": "5"}, {"FTC_LES_PERSISTENT": "1"},
                                      {"FTC_LES_PERSISTENT": "1", "FTC_LES_ROWS": "2"}, {"FTC_LES_WIDTH": "32"},
-                                     {"FTC_LES_WIDTH": "128"}, {"FTC_LES_WIDTH": "32", "FTC_LES_PERSISTENT": "1"}])
+                                     {"FTC_LES_WIDTH": "128"}, {"FTC_LES_WIDTH": "32", "FTC_LES_PERSISTENT": "1"},
+                                     {"FTC_LES_ANISO": "1"}, {"FTC_LES_ANISO": "1", "FTC_LES_WIDTH": "32"}])
 @pytest.mark.parametrize("n", [(70, 37, 19), (130, 9, 33)])
 def test_selectable_step_variants_are_bit_exact(ftc, scope, hip, oracle, fields, monkeypatch, variant, n):
     # The entry reads these switches at every call: two rows per thread, the XCD-contiguous tile order,
     # the unstaged kernel, a forced k chunk, the persistent banded walk (one resident round of
-    # workgroups) and the 32 x 8 and 128 x 2 tiles. Each must give the default kernel's (the oracle's) bits.
+    # workgroups), the 32 x 8 and 128 x 2 tiles, and the general kernel at equal spacings. Each must give
+    # the default kernel's (the oracle's) bits.
     for key, value in variant.items():
         monkeypatch.setenv(key, value)
     U, V, W, S = fields.les_fields(*n)

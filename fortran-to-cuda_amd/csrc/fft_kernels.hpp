@@ -16,10 +16,10 @@
 // mixed solve) or FP64.
 //
 // The FFT is a Stockham autosort transform in LDS with radix-8 stages (one radix-4 or radix-2 stage
-// takes the rest of log2 n): each thread holds 16 complex points in registers, applies the stage's
-// twiddles and R-point DFTs there, and writes the results to their sorted places; the first stage reads
-// its operands straight from global memory and needs no twiddles. A workgroup of 256 threads takes POINTS
-// complex points (2048 or 4096): G = 2 * POINTS / n real lines of one plane. The lines of plane c are a = 0 .. A-1 and element
+// takes the rest of log2 n): each thread holds POINTS / NT complex points in registers, applies the
+// stage's twiddles and R-point DFTs there, and writes the results to their sorted places; the first stage
+// reads its operands straight from global memory and needs no twiddles. A workgroup of NT threads (256
+// or 512) takes POINTS complex points (2048 or 4096): G = 2 * POINTS / n real lines of one plane. The lines of plane c are a = 0 .. A-1 and element
 // e of line (a, c) is at in[a*ia + c*ic + e*ie]. ECONT says that a line's elements are contiguous
 // (ie == 1, the x transform): threads then walk the elements of a line; otherwise the lines are
 // contiguous (ia == 1, the y transform) and threads walk across lines, so both read and write whole
@@ -31,8 +31,7 @@
 
 namespace ftc {
 
-constexpr int kFftThreads = 256;
-constexpr int kFftMaxPoints = 4096;  // complex points per workgroup: 2048 or 4096 (8 or 16 per thread)
+constexpr int kFftMaxPoints = 4096;  // complex points per workgroup: 2048 or 4096
 
 struct HartleyPass {
     const void *in;
@@ -128,7 +127,7 @@ __device__ __forceinline__ double2 twiddle(const double2 *tw, int m, int n)
 }
 
 // One Stockham stage of radix R on the LDS lines (not the first: reads LDS, applies twiddles).
-template <int R, bool ECONT, int PT>
+template <int R, bool ECONT, int PT, int NT>
 __device__ __forceinline__ void stockham_stage(double2 *z, const double2 *tw, int n, int logn, int logl, int stride,
                                                int ns, int logns, double2 (&v)[PT])
 {
@@ -137,7 +136,7 @@ __device__ __forceinline__ void stockham_stage(double2 *z, const double2 *tw, in
 #pragma unroll
     for (int t = 0; t < PT / R; ++t) {
         int line, j;
-        butterfly_at<ECONT>(threadIdx.x + t * kFftThreads, lognr, logl, line, j);
+        butterfly_at<ECONT>(threadIdx.x + t * NT, lognr, logl, line, j);
         const double2 *zl = z + line * stride;
 #pragma unroll
         for (int r = 0; r < R; ++r) v[t * R + r] = zl[j + r * nr];
@@ -146,7 +145,7 @@ __device__ __forceinline__ void stockham_stage(double2 *z, const double2 *tw, in
 #pragma unroll
     for (int t = 0; t < PT / R; ++t) {
         int line, j;
-        butterfly_at<ECONT>(threadIdx.x + t * kFftThreads, lognr, logl, line, j);
+        butterfly_at<ECONT>(threadIdx.x + t * NT, lognr, logl, line, j);
         const int k = j & (ns - 1);
         // W^(k r) with W = e^{-2 pi i / (ns R)}: index k r n / (ns R) of the n-point table.
 #pragma unroll
@@ -162,7 +161,7 @@ __device__ __forceinline__ void stockham_stage(double2 *z, const double2 *tw, in
 
 // The first Stockham stage (ns = 1: no twiddles, results at j*R + r): operands straight from global
 // memory, two real lines (a, a+1) per complex line; lines past the plane's last read as zero.
-template <int R, class TI, bool ECONT, int PT>
+template <int R, class TI, bool ECONT, int PT, int NT>
 __device__ __forceinline__ void first_stage(const HartleyPass &h, const TI *in, double2 *z, int a0, int logl,
                                             int stride, double2 (&v)[PT])
 {
@@ -171,7 +170,7 @@ __device__ __forceinline__ void first_stage(const HartleyPass &h, const TI *in, 
 #pragma unroll
     for (int t = 0; t < PT / R; ++t) {
         int line, j;
-        butterfly_at<ECONT>(threadIdx.x + t * kFftThreads, lognr, logl, line, j);
+        butterfly_at<ECONT>(threadIdx.x + t * NT, lognr, logl, line, j);
         const int a = a0 + 2 * line;
         const TI *p0 = in + (long long)a * h.ia, *p1 = p0 + h.ia;
 #pragma unroll
@@ -188,8 +187,8 @@ __device__ __forceinline__ void first_stage(const HartleyPass &h, const TI *in, 
     }
 }
 
-template <class TI, class TO, bool ECONT, int POINTS>
-__global__ __launch_bounds__(kFftThreads) void hartley_kernel(const HartleyPass h)
+template <class TI, class TO, bool ECONT, int POINTS, int NT>
+__global__ __launch_bounds__(NT) void hartley_kernel(const HartleyPass h)
 {
     extern __shared__ double2 fft_lds[];
     const int n = h.n, logn = h.logn, G = h.G, logG = h.logG;
@@ -201,28 +200,28 @@ __global__ __launch_bounds__(kFftThreads) void hartley_kernel(const HartleyPass 
     const int a0 = blockIdx.x * G, c = blockIdx.y;
     const TI *in = static_cast<const TI *>(h.in) + (long long)c * h.ic;
     TO *out = static_cast<TO *>(h.out) + (long long)c * h.oc;
-    for (int k = tid; k < (n >> 1); k += kFftThreads) tw[k] = h.tw[k];
+    for (int k = tid; k < (n >> 1); k += NT) tw[k] = h.tw[k];
 
     // The radices: radix 8 while at least three factors of two remain, then one radix-4 or radix-2 stage.
     const int eights = logn / 3, rest = logn - 3 * eights;
-    constexpr int PT = POINTS / kFftThreads;
+    constexpr int PT = POINTS / NT;
     double2 v[PT];
     // First stage (ns = 1, no twiddles), operands from global memory.
-    if (eights) first_stage<8, TI, ECONT, PT>(h, in, z, a0, logl, stride, v);
-    else if (rest == 2) first_stage<4, TI, ECONT, PT>(h, in, z, a0, logl, stride, v);
-    else first_stage<2, TI, ECONT, PT>(h, in, z, a0, logl, stride, v);
+    if (eights) first_stage<8, TI, ECONT, PT, NT>(h, in, z, a0, logl, stride, v);
+    else if (rest == 2) first_stage<4, TI, ECONT, PT, NT>(h, in, z, a0, logl, stride, v);
+    else first_stage<2, TI, ECONT, PT, NT>(h, in, z, a0, logl, stride, v);
     __syncthreads();
     int ns = eights ? 8 : (1 << rest), logns = eights ? 3 : rest;
     for (int s = 1; s < eights; ++s) {
-        stockham_stage<8, ECONT, PT>(z, tw, n, logn, logl, stride, ns, logns, v);
+        stockham_stage<8, ECONT, PT, NT>(z, tw, n, logn, logl, stride, ns, logns, v);
         ns <<= 3;
         logns += 3;
     }
-    if (eights && rest == 2) stockham_stage<4, ECONT, PT>(z, tw, n, logn, logl, stride, ns, logns, v);
-    if (eights && rest == 1) stockham_stage<2, ECONT, PT>(z, tw, n, logn, logl, stride, ns, logns, v);
+    if (eights && rest == 2) stockham_stage<4, ECONT, PT, NT>(z, tw, n, logn, logl, stride, ns, logns, v);
+    if (eights && rest == 1) stockham_stage<2, ECONT, PT, NT>(z, tw, n, logn, logl, stride, ns, logns, v);
 
     const double sc = 0.5 * h.scale, beta = h.beta;
-    for (int idx = tid; idx < G * n; idx += kFftThreads) {
+    for (int idx = tid; idx < G * n; idx += NT) {
         int g, k;
         if constexpr (ECONT) {
             g = idx >> logn;

@@ -156,8 +156,8 @@ static int map_status(int scope_status)
     }
 }
 
-// Complex points per workgroup of a pass: 16 per thread where the lines run across threads (the y
-// transform: G = 2 * points / n lines make the 128-byte runs), 8 where a line's own elements do.
+// Complex points per workgroup of a pass: 4096 where the lines run across threads (the y transform:
+// G = 2 * points / n lines make the 128-byte runs), 2048 where a line's own elements do.
 static int hartley_points(bool econt, int n)
 {
     static const int x = std::getenv("FTC_FFT_POINTS_X") ? std::atoi(std::getenv("FTC_FFT_POINTS_X")) : 2048;
@@ -166,11 +166,21 @@ static int hartley_points(bool econt, int n)
     return std::max(p == 2048 ? 2048 : 4096, n);  // at least two real lines (n <= 4096)
 }
 
-template <class TI, class TO, int POINTS>
+// Threads per workgroup of a pass of `points` points: 256 (8 or 16 points each), or 512 for the 4096-point
+// workgroups (8 points each), whose 70 KB of LDS hold two workgroups per CU: 512 threads keep twice the
+// waves in flight. FTC_FFT_THREADS=256 selects 256 everywhere.
+static int hartley_threads(int points)
+{
+    const char *forced = std::getenv("FTC_FFT_THREADS");
+    if (forced && std::atoi(forced) == 256) return 256;
+    return points == 4096 ? 512 : 256;
+}
+
+template <class TI, class TO, int POINTS, int NT>
 static void hartley_launch(bool econt, dim3 grid, std::size_t lds, hipStream_t stream, const HartleyPass &h)
 {
-    if (econt) hipLaunchKernelGGL((hartley_kernel<TI, TO, true, POINTS>), grid, dim3(kFftThreads), lds, stream, h);
-    else hipLaunchKernelGGL((hartley_kernel<TI, TO, false, POINTS>), grid, dim3(kFftThreads), lds, stream, h);
+    if (econt) hipLaunchKernelGGL((hartley_kernel<TI, TO, true, POINTS, NT>), grid, dim3(NT), lds, stream, h);
+    else hipLaunchKernelGGL((hartley_kernel<TI, TO, false, POINTS, NT>), grid, dim3(NT), lds, stream, h);
 }
 
 template <class TI, class TO>
@@ -184,8 +194,9 @@ static int hartley_pass(const void *in, long long ie, long long ia, long long ic
     h.logG = log2_exact(h.G);
     const std::size_t lds = (std::size_t)hartley_lds_bytes(n, points);
     const dim3 grid((unsigned)((A + h.G - 1) / h.G), (unsigned)C);
-    if (points == 2048) hartley_launch<TI, TO, 2048>(econt, grid, lds, stream, h);
-    else hartley_launch<TI, TO, 4096>(econt, grid, lds, stream, h);
+    if (points == 2048) hartley_launch<TI, TO, 2048, 256>(econt, grid, lds, stream, h);
+    else if (hartley_threads(points) == 512) hartley_launch<TI, TO, 4096, 512>(econt, grid, lds, stream, h);
+    else hartley_launch<TI, TO, 4096, 256>(econt, grid, lds, stream, h);
     if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "FFT launch failed");
     return FORT_SCOPE_OK;
 }
@@ -197,10 +208,12 @@ static bool hartley_attributes()
     for (int n = 4; n <= 4096; n *= 2)
         for (int points : {2048, 4096}) bytes = std::max(bytes, hartley_lds_bytes(n, std::max(points, n)));
     bool ok = true;
-    for (const void *f : {reinterpret_cast<const void *>(&hartley_kernel<double, double, true, 2048>),
-                          reinterpret_cast<const void *>(&hartley_kernel<double, double, false, 2048>),
-                          reinterpret_cast<const void *>(&hartley_kernel<double, double, true, 4096>),
-                          reinterpret_cast<const void *>(&hartley_kernel<double, double, false, 4096>)})
+    for (const void *f : {reinterpret_cast<const void *>(&hartley_kernel<double, double, true, 2048, 256>),
+                          reinterpret_cast<const void *>(&hartley_kernel<double, double, false, 2048, 256>),
+                          reinterpret_cast<const void *>(&hartley_kernel<double, double, true, 4096, 256>),
+                          reinterpret_cast<const void *>(&hartley_kernel<double, double, false, 4096, 256>),
+                          reinterpret_cast<const void *>(&hartley_kernel<double, double, true, 4096, 512>),
+                          reinterpret_cast<const void *>(&hartley_kernel<double, double, false, 4096, 512>)})
         ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
     return ok;
 }

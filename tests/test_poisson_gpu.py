@@ -93,7 +93,7 @@ def test_unsupported_configurations_are_rejected(ftc, scope, hip):
         assert e.value.status == scope.ARGUMENT
 
 
-@pytest.mark.parametrize("n", [(256, 64, 16), (8, 16, 5), (4, 4, 3), (512, 512, 8)])
+@pytest.mark.parametrize("n", [(256, 64, 16), (8, 16, 5), (4, 4, 3), (512, 512, 8), (4096, 8, 3)])
 def test_fft_solve_matches_the_oracle_and_the_dense_solve(ftc, scope, hip, monkeypatch, n):
     # Periodic x and y of power-of-two sizes take the FFT (Hartley) transforms in the FP64 solve; with
     # FTC_POISSON_DENSE set at creation the same solver takes the dense FP64 MFMA transforms. Both are
@@ -104,6 +104,12 @@ def test_fft_solve_matches_the_oracle_and_the_dense_solve(ftc, scope, hip, monke
     want = poisson_oracle.solve(b, h)
     fft, report = run(ftc, scope, hip, b, h, False)
     assert report.relative_residual <= 1e-12
+    # The 4096-point workgroups (the y passes, and x passes of long lines) run 512 threads by default;
+    # with 256 they hold twice the points per thread. Both are the same transform, operation for operation.
+    monkeypatch.setenv("FTC_FFT_THREADS", "256")
+    fft256, _ = run(ftc, scope, hip, b, h, False)
+    monkeypatch.delenv("FTC_FFT_THREADS")
+    assert np.array_equal(fft256, fft)
     monkeypatch.setenv("FTC_POISSON_DENSE", "1")
     dense, _ = run(ftc, scope, hip, b, h, False)
     scale = np.abs(want).max()

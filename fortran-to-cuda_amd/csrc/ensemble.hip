@@ -342,6 +342,7 @@ struct ftc_ensemble {
     std::vector<LesArgs> args[2];
     GemmCall *d_calls = nullptr;  // kG1.. stage-major: calls[stage_slot * count + member]
     TileItem *d_les = nullptr, *d_tiles = nullptr;
+    TileItem *d_les_all = nullptr;  // the same items for the all-member launches, longest first per width
     GemmItem *d_gemm[6] = {};
     ColItem *d_cols = nullptr;
     double *d_tile_part = nullptr;  // one mean partial per tile item
@@ -444,7 +445,16 @@ static int build_items(ftc_ensemble &e)
     e.n_les = (int)les.size();
     e.n_tiles = (int)tiles.size();
     e.n_cols = (int)cols.size();
-    std::size_t bytes = up256(les.size() * sizeof(TileItem)) + up256(tiles.size() * sizeof(TileItem)) +
+    // The all-member launches give every item a workgroup of its own, and the dispatcher starts them in
+    // order as slots free up: so that the last ones to start are short, the items of each width are taken
+    // longest first (most planes), each member's items staying together.
+    std::vector<TileItem> les_all(les);
+    const int kc = e.les_kchunk;
+    auto planes = [&](const TileItem &t) { return std::min(kc, e.nz[t.member] - t.bz * std::min(kc, e.nz[t.member])); };
+    const auto split = les_all.begin() + (e.n_les - e.n_les32);
+    for (auto [lo, hi] : {std::make_pair(les_all.begin(), split), std::make_pair(split, les_all.end())})
+        std::stable_sort(lo, hi, [&](const TileItem &a, const TileItem &b) { return planes(a) > planes(b); });
+    std::size_t bytes = 2 * up256(les.size() * sizeof(TileItem)) + up256(tiles.size() * sizeof(TileItem)) +
                         up256(cols.size() * sizeof(ColItem)) + up256(tiles.size() * sizeof(double)) +
                         up256(e.count * sizeof(int2));
     for (int s = 0; s < 6; ++s) {
@@ -465,6 +475,7 @@ static int build_items(ftc_ensemble &e)
         return dst;
     };
     e.d_les = (TileItem *)put(les.data(), les.size() * sizeof(TileItem));
+    e.d_les_all = (TileItem *)put(les_all.data(), les_all.size() * sizeof(TileItem));
     e.d_tiles = (TileItem *)put(tiles.data(), tiles.size() * sizeof(TileItem));
     e.d_cols = (ColItem *)put(cols.data(), cols.size() * sizeof(ColItem));
     std::vector<int2> ranges(e.count);
@@ -510,12 +521,13 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
         auto launch = [&](int width, int first, int count) {
             if (!count) return;
             const dim3 grid(all ? grid_for(e, count) : count);
+            const TileItem *items = (all ? e.d_les_all : e.d_les) + first;
             if (width == 32)
                 hipLaunchKernelGGL((ens_les_kernel<true, 32>), grid, dim3(32, NTH / 32), 0, s, e.d_args[parity],
-                                   e.d_les + first, count);
+                                   items, count);
             else
                 hipLaunchKernelGGL((ens_les_kernel<true, 64>), grid, dim3(64, NTH / 64), 0, s, e.d_args[parity],
-                                   e.d_les + first, count);
+                                   items, count);
         };
         if (all) {
             launch(64, 0, e.n_les - e.n_les32);

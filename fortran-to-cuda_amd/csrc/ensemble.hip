@@ -342,7 +342,8 @@ struct ftc_ensemble {
     std::vector<LesArgs> args[2];
     GemmCall *d_calls = nullptr;  // kG1.. stage-major: calls[stage_slot * count + member]
     TileItem *d_les = nullptr, *d_tiles = nullptr;
-    TileItem *d_les_all = nullptr;  // the same items for the all-member launches, longest first per width
+    TileItem *d_les_all = nullptr;    // the same items for the all-member launches, longest first per width
+    TileItem *d_tiles_all = nullptr;  // the tiles longest first, for the launches that keep no per-tile sums
     GemmItem *d_gemm[6] = {};
     ColItem *d_cols = nullptr;
     double *d_tile_part = nullptr;  // one mean partial per tile item
@@ -454,7 +455,10 @@ static int build_items(ftc_ensemble &e)
     const auto split = les_all.begin() + (e.n_les - e.n_les32);
     for (auto [lo, hi] : {std::make_pair(les_all.begin(), split), std::make_pair(split, les_all.end())})
         std::stable_sort(lo, hi, [&](const TileItem &a, const TileItem &b) { return planes(a) > planes(b); });
-    std::size_t bytes = 2 * up256(les.size() * sizeof(TileItem)) + up256(tiles.size() * sizeof(TileItem)) +
+    std::vector<TileItem> tiles_all(tiles);
+    std::stable_sort(tiles_all.begin(), tiles_all.end(),
+                     [&](const TileItem &a, const TileItem &b) { return e.nz[a.member] > e.nz[b.member]; });
+    std::size_t bytes = 2 * up256(les.size() * sizeof(TileItem)) + 2 * up256(tiles.size() * sizeof(TileItem)) +
                         up256(cols.size() * sizeof(ColItem)) + up256(tiles.size() * sizeof(double)) +
                         up256(e.count * sizeof(int2));
     for (int s = 0; s < 6; ++s) {
@@ -477,6 +481,7 @@ static int build_items(ftc_ensemble &e)
     e.d_les = (TileItem *)put(les.data(), les.size() * sizeof(TileItem));
     e.d_les_all = (TileItem *)put(les_all.data(), les_all.size() * sizeof(TileItem));
     e.d_tiles = (TileItem *)put(tiles.data(), tiles.size() * sizeof(TileItem));
+    e.d_tiles_all = (TileItem *)put(tiles_all.data(), tiles_all.size() * sizeof(TileItem));
     e.d_cols = (ColItem *)put(cols.data(), cols.size() * sizeof(ColItem));
     std::vector<int2> ranges(e.count);
     for (int m = 0; m < e.count; ++m) ranges[m] = make_int2(e.r_tiles[m].start, e.r_tiles[m].count);
@@ -540,12 +545,12 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
     case kDiv:
         n = pick(e.r_tiles, e.n_tiles, &start);
         if (n) hipLaunchKernelGGL(ens_divergence_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
-                                  e.d_members, e.d_tiles + start, n, dst);
+                                  e.d_members, (all ? e.d_tiles_all : e.d_tiles + start), n, dst);
         break;
     case kProj:
         n = pick(e.r_tiles, e.n_tiles, &start);
         if (n) hipLaunchKernelGGL(ens_projection_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
-                                  e.d_members, e.d_args[parity], e.d_tiles + start, n, dst);
+                                  e.d_members, e.d_args[parity], (all ? e.d_tiles_all : e.d_tiles + start), n, dst);
         break;
     case kRes:
         n = pick(e.r_tiles, e.n_tiles, &start);
@@ -559,7 +564,7 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
     case kSub:
         n = pick(e.r_tiles, e.n_tiles, &start);
         if (n) hipLaunchKernelGGL(ens_subtract_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
-                                  e.d_members, e.d_tiles + start, n);
+                                  e.d_members, (all ? e.d_tiles_all : e.d_tiles + start), n);
         break;
     case kMean:
         n = pick(e.r_tiles, e.n_tiles, &start);

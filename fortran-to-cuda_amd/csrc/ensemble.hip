@@ -344,6 +344,7 @@ struct ftc_ensemble {
     TileItem *d_les = nullptr, *d_tiles = nullptr;
     TileItem *d_les_all = nullptr;    // the same items for the all-member launches, longest first per width
     TileItem *d_tiles_all = nullptr;  // the tiles longest first, for the launches that keep no per-tile sums
+    ColItem *d_cols_all = nullptr;    // the z-solve column blocks longest first
     GemmItem *d_gemm[6] = {};
     ColItem *d_cols = nullptr;
     double *d_tile_part = nullptr;  // one mean partial per tile item
@@ -455,11 +456,14 @@ static int build_items(ftc_ensemble &e)
     const auto split = les_all.begin() + (e.n_les - e.n_les32);
     for (auto [lo, hi] : {std::make_pair(les_all.begin(), split), std::make_pair(split, les_all.end())})
         std::stable_sort(lo, hi, [&](const TileItem &a, const TileItem &b) { return planes(a) > planes(b); });
+    std::vector<ColItem> cols_all(cols);
+    std::stable_sort(cols_all.begin(), cols_all.end(),
+                     [&](const ColItem &a, const ColItem &b) { return e.nz[a.member] > e.nz[b.member]; });
     std::vector<TileItem> tiles_all(tiles);
     std::stable_sort(tiles_all.begin(), tiles_all.end(),
                      [&](const TileItem &a, const TileItem &b) { return e.nz[a.member] > e.nz[b.member]; });
     std::size_t bytes = 2 * up256(les.size() * sizeof(TileItem)) + 2 * up256(tiles.size() * sizeof(TileItem)) +
-                        up256(cols.size() * sizeof(ColItem)) + up256(tiles.size() * sizeof(double)) +
+                        2 * up256(cols.size() * sizeof(ColItem)) + up256(tiles.size() * sizeof(double)) +
                         up256(e.count * sizeof(int2));
     for (int s = 0; s < 6; ++s) {
         e.n_gemm[s] = (int)gemm[s].size();
@@ -483,6 +487,7 @@ static int build_items(ftc_ensemble &e)
     e.d_tiles = (TileItem *)put(tiles.data(), tiles.size() * sizeof(TileItem));
     e.d_tiles_all = (TileItem *)put(tiles_all.data(), tiles_all.size() * sizeof(TileItem));
     e.d_cols = (ColItem *)put(cols.data(), cols.size() * sizeof(ColItem));
+    e.d_cols_all = (ColItem *)put(cols_all.data(), cols_all.size() * sizeof(ColItem));
     std::vector<int2> ranges(e.count);
     for (int m = 0; m < e.count; ++m) ranges[m] = make_int2(e.r_tiles[m].start, e.r_tiles[m].count);
     e.d_tile_ranges = (int2 *)put(ranges.data(), ranges.size() * sizeof(int2));
@@ -576,11 +581,12 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
     case kTri: {
         n = pick(e.r_cols, e.n_cols, &start);
         const std::size_t lds = 2 * (std::size_t)tridiag_chunks(e.nzmax) * kCols * sizeof(double);
+        const ColItem *cols = all ? e.d_cols_all : e.d_cols + start;
         if (n) {
             if (mx) hipLaunchKernelGGL((ens_tridiag_kernel<float, true>), dim3(all ? grid_for(e, n) : n), dim3(kCols),
-                                       lds, s, e.d_members, e.d_cols + start, n);
+                                       lds, s, e.d_members, cols, n);
             else hipLaunchKernelGGL((ens_tridiag_kernel<double, false>), dim3(all ? grid_for(e, n) : n), dim3(kCols),
-                                    lds, s, e.d_members, e.d_cols + start, n);
+                                    lds, s, e.d_members, cols, n);
         }
         break;
     }

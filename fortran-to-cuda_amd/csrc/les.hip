@@ -263,7 +263,10 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
         // One resident round: the workgroups the device holds at once (three, or two with two rows or the
         // wide tile, per CU), a multiple of eight, and no more than there are tiles (rounded up to eight).
         const unsigned tiles = grid.x * grid.y * grid.z;
-        const unsigned per_cu = rows == 2 || width == 128 ? 2u : 3u;
+        const unsigned per_cu = rows == 2 ? (unsigned)sgs3_occupancy<2, TX>()
+                                : width == 128 ? (unsigned)sgs3_occupancy<1, 128>()
+                                : width == 32  ? (unsigned)sgs3_occupancy<1, 32>()
+                                               : (unsigned)sgs3_occupancy<1, TX>();
         const unsigned resident = (unsigned)compute_units() * per_cu / kXcds * kXcds;
         grid = dim3(std::max(kXcds, std::min(resident, (tiles + kXcds - 1) / kXcds * kXcds)), 1, 1);
     }

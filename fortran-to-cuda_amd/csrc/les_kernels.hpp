@@ -397,12 +397,16 @@ struct Sgs3Tile {
     static constexpr int VRT = (VC + NTH - 1) / NTH, CRT = (CC + NTH - 1) / NTH;
 };
 
+// W is read at planes k and k+1 only while a step computes plane k (the viscosity of k+1 takes W at k+1 and
+// k, the update of k takes W at k with its ring and at k+1 in the column), so it takes two slots where U and
+// V, whose plane k+2 the viscosity of k+1 reads, take three; W is staged one plane behind them.
 template <int ROWS, int WX = TX>
 struct Sgs3Lds {
     using T = Sgs3Tile<ROWS, WX>;
-    double vel[3][3][T::VYT][T::VXT];  // slot = plane % 3
-    double sca[2][T::CYT][T::CXT];     // slot = plane & 1
-    double nu[2][T::CYT][T::CXT];      // slot = plane & 1
+    double uv[3][2][T::VYT][T::VXT];  // U, V; slot = plane % 3
+    double w[2][T::VYT][T::VXT];      // W; slot = plane & 1
+    double sca[2][T::CYT][T::CXT];    // slot = plane & 1
+    double nu[2][T::CYT][T::CXT];     // slot = plane & 1
 };
 
 // The upper z-face terms of a cell (advection and diffusion of u, v, w and s), which are the lower terms
@@ -459,8 +463,8 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
     const double ix = a.sgs[0], iy = ISO ? ix : a.sgs[1], iz = ISO ? ix : a.sgs[2];  // 1/d
     const double hx = a.sgs[6], hy = ISO ? hx : a.sgs[7], hz = ISO ? hx : a.sgs[8];  // 1/d^2
     const double cs2 = a.sgs[9];                                                    // (Cs Delta)^2
-    double rv[VR][3], rs[CR];
-    auto fetch_vel = [&](int kp) {
+    double rv[VR][2], rw[VR], rs[CR];
+    auto fetch_uv = [&](int kp) {
         if (kp < 1 || kp > nz + 2) return;
         const long long base = (long long)(kp - 1) * sk;
 #pragma unroll
@@ -468,19 +472,33 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
             if (vcell[r] >= 0) {
                 rv[r][0] = gU[voff[r] + base];
                 rv[r][1] = gV[voff[r] + base];
-                rv[r][2] = gW[voff[r] + base];
             }
     };
-    auto put_vel = [&](int kp) {
+    auto put_uv = [&](int kp) {
         if (kp < 1 || kp > nz + 2) return;
         const int slot = kp % 3;
 #pragma unroll
         for (int r = 0; r < VR; ++r)
             if (vcell[r] >= 0) {
-                (&L.vel[slot][0][0][0])[vcell[r]] = rv[r][0];
-                (&L.vel[slot][1][0][0])[vcell[r]] = rv[r][1];
-                (&L.vel[slot][2][0][0])[vcell[r]] = rv[r][2];
+                (&L.uv[slot][0][0][0])[vcell[r]] = rv[r][0];
+                (&L.uv[slot][1][0][0])[vcell[r]] = rv[r][1];
             }
+    };
+    // W planes up to nz+1: the update reads W at k+1 only on the interior faces (k <= kw = nz), and the
+    // clamped viscosity of plane nz+2 reads W at nz+1 and nz, which a plane nz+2 would overwrite.
+    auto fetch_w = [&](int kp) {
+        if (kp < 1 || kp > nz + 1) return;
+        const long long base = (long long)(kp - 1) * sk;
+#pragma unroll
+        for (int r = 0; r < VR; ++r)
+            if (vcell[r] >= 0) rw[r] = gW[voff[r] + base];
+    };
+    auto put_w = [&](int kp) {
+        if (kp < 1 || kp > nz + 1) return;
+        const int slot = kp & 1;
+#pragma unroll
+        for (int r = 0; r < VR; ++r)
+            if (vcell[r] >= 0) (&L.w[slot][0][0])[vcell[r]] = rw[r];
     };
     auto fetch_sca = [&](int kp) {
         if (kp < 1 || kp > nz + 2) return;
@@ -500,22 +518,23 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
     auto nu_plane = [&](int qk) {
         const int kk = qk < 2 ? 2 : (qk > nz + 1 ? nz + 1 : qk);
         const int s0 = kk % 3, sm = (kk + 2) % 3, sp = (kk + 1) % 3;
+        const int w0s = kk & 1, wms = (kk - 1) & 1;
         double(*nuo)[CX] = L.nu[qk & 1];
         for (int c = tid; c < CCELLS; c += NTH) {
             const int li = c % CX, lj = c / CX;
             const int vx = li + 1, vy = lj + 1;
-#define QU(s, di, dj) L.vel[s][0][vy + (dj)][vx + (di)]
-#define QV(s, di, dj) L.vel[s][1][vy + (dj)][vx + (di)]
-#define QW(s, di, dj) L.vel[s][2][vy + (dj)][vx + (di)]
+#define QU(s, di, dj) L.uv[s][0][vy + (dj)][vx + (di)]
+#define QV(s, di, dj) L.uv[s][1][vy + (dj)][vx + (di)]
+#define QW(s, di, dj) L.w[s][vy + (dj)][vx + (di)]
             double s11 = (QU(s0, 0, 0) - QU(s0, -1, 0)) * ix;
             double s22 = (QV(s0, 0, 0) - QV(s0, 0, -1)) * iy;
-            double s33 = (QW(s0, 0, 0) - QW(sm, 0, 0)) * iz;
+            double s33 = (QW(w0s, 0, 0) - QW(wms, 0, 0)) * iz;
             double dudy = ((QU(s0, 0, 1) + QU(s0, -1, 1)) - (QU(s0, 0, -1) + QU(s0, -1, -1))) * c25y;
             double dvdx = ((QV(s0, 1, 0) + QV(s0, 1, -1)) - (QV(s0, -1, 0) + QV(s0, -1, -1))) * c25x;
             double dudz = ((QU(sp, 0, 0) + QU(sp, -1, 0)) - (QU(sm, 0, 0) + QU(sm, -1, 0))) * c25z;
-            double dwdx = ((QW(s0, 1, 0) + QW(sm, 1, 0)) - (QW(s0, -1, 0) + QW(sm, -1, 0))) * c25x;
+            double dwdx = ((QW(w0s, 1, 0) + QW(wms, 1, 0)) - (QW(w0s, -1, 0) + QW(wms, -1, 0))) * c25x;
             double dvdz = ((QV(sp, 0, 0) + QV(sp, 0, -1)) - (QV(sm, 0, 0) + QV(sm, 0, -1))) * c25z;
-            double dwdy = ((QW(s0, 0, 1) + QW(sm, 0, 1)) - (QW(s0, 0, -1) + QW(sm, 0, -1))) * c25y;
+            double dwdy = ((QW(w0s, 0, 1) + QW(wms, 0, 1)) - (QW(w0s, 0, -1) + QW(wms, 0, -1))) * c25y;
 #undef QU
 #undef QV
 #undef QW
@@ -529,29 +548,36 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
 
     const double dt = a.dt;
     const double Cz = c50z;
-#define VU(s, di, dj) L.vel[s][0][ly + 2 + (dj)][tx + 2 + (di)]
-#define VV(s, di, dj) L.vel[s][1][ly + 2 + (dj)][tx + 2 + (di)]
-#define VW(s, di, dj) L.vel[s][2][ly + 2 + (dj)][tx + 2 + (di)]
+#define VU(s, di, dj) L.uv[s][0][ly + 2 + (dj)][tx + 2 + (di)]
+#define VV(s, di, dj) L.uv[s][1][ly + 2 + (dj)][tx + 2 + (di)]
+#define VW(s, di, dj) L.w[s][ly + 2 + (dj)][tx + 2 + (di)]
 #define SS(s, di, dj) L.sca[s][ly + 1 + (dj)][tx + 1 + (di)]
 #define NV(s, di, dj) L.nu[s][ly + 1 + (dj)][tx + 1 + (di)]
 
-    // Prologue: planes k0-2 .. k0 give nu(k0-1); plane k0+1 replaces k0-2 and gives nu(k0); then the upper
-    // terms of plane k0-1 (the lower terms of k0) are formed as a step forms them.
+    // Prologue: U, V at planes k0-2 .. k0 and W at k0-2, k0-1 give nu(k0-1); U, V at k0+1 and W at k0 then
+    // replace the planes k0-2 and give nu(k0); then the upper terms of plane k0-1 (the lower terms of k0)
+    // are formed as a step forms them.
     for (int kp = k0 - 2; kp <= k0; ++kp) {
-        fetch_vel(kp);
-        put_vel(kp);
+        fetch_uv(kp);
+        put_uv(kp);
+    }
+    for (int kp = k0 - 2; kp <= k0 - 1; ++kp) {
+        fetch_w(kp);
+        put_w(kp);
     }
     for (int kp = k0 - 1; kp <= k0; ++kp) {
         fetch_sca(kp);
         put_sca(kp);
     }
-    fetch_vel(k0 + 1);
+    fetch_uv(k0 + 1);
+    fetch_w(k0);
     // nu(k0-1) is clamped to plane 2 when k0 = 2 and then needs plane k0+1 (plane k0-2 does not exist).
     const bool clamped = k0 - 1 < 2;
     __syncthreads();
     if (!clamped) nu_plane(k0 - 1);
     __syncthreads();
-    put_vel(k0 + 1);
+    put_uv(k0 + 1);
+    put_w(k0);
     __syncthreads();
     if (clamped) nu_plane(k0 - 1);
     nu_plane(k0);
@@ -566,7 +592,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
         const int s0 = kv % 3, sp = (kv + 1) % 3, c0 = kv & 1, cp = (kv + 1) & 1;
         const double u0 = VU(s0, 0, 0), uzp = VU(sp, 0, 0);
         const double v0 = VV(s0, 0, 0), vzp = VV(sp, 0, 0);
-        const double w0 = VW(s0, 0, 0), wxp = VW(s0, 1, 0), wyp = VW(s0, 0, 1), wzp = VW(sp, 0, 0);
+        const double w0 = VW(c0, 0, 0), wxp = VW(c0, 1, 0), wyp = VW(c0, 0, 1), wzp = VW(cp, 0, 0);
         const double n000 = NV(c0, 0, 0), n100 = NV(c0, 1, 0), n010 = NV(c0, 0, 1), n001 = NV(cp, 0, 0);
         zc.cu = c25z * (uzp + u0) * (wxp + w0);
         zc.du = (0.25 * (((n000 + n100) + n001) + NV(cp, 1, 0))) * (uzp - u0);
@@ -584,14 +610,17 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
         zc.wyp = wyp;
         }
     }
-    fetch_vel(k0 + 2);
+    fetch_uv(k0 + 2);
+    fetch_w(k0 + 1);
     fetch_sca(k0 + 1);
     __syncthreads();
 
     for (int k = k0; k <= k1; ++k) {
-        put_vel(k + 2);
+        put_uv(k + 2);
+        put_w(k + 1);
         put_sca(k + 1);
-        fetch_vel(k + 3);
+        fetch_uv(k + 3);
+        fetch_w(k + 2);
         fetch_sca(k + 2);
         __syncthreads();
         nu_plane(k + 1);
@@ -609,8 +638,8 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
             const double uym = VU(s0, 0, -1), uzp = VU(sp, 0, 0), uzm = zc.u;
             const double v0 = VV(s0, 0, 0), vxp = VV(s0, 1, 0), vxm = VV(s0, -1, 0), vyp = VV(s0, 0, 1);
             const double vym = VV(s0, 0, -1), vzp = VV(sp, 0, 0), vzm = zc.v;
-            const double w0 = VW(s0, 0, 0), wxp = VW(s0, 1, 0), wxm = VW(s0, -1, 0), wyp = VW(s0, 0, 1);
-            const double wym = VW(s0, 0, -1), wzp = VW(sp, 0, 0), wzm = zc.w;
+            const double w0 = VW(c0, 0, 0), wxp = VW(c0, 1, 0), wxm = VW(c0, -1, 0), wyp = VW(c0, 0, 1);
+            const double wym = VW(c0, 0, -1), wzp = VW(cp, 0, 0), wzm = zc.w;
             const double n000 = NV(c0, 0, 0), n100 = NV(c0, 1, 0), nm00 = NV(c0, -1, 0);
             const double n010 = NV(c0, 0, 1), n0m0 = NV(c0, 0, -1), n001 = NV(cp, 0, 0);
             const long long p = (long long)(i - 1) + (long long)(j - 1) * sj + (long long)(k - 1) * sk;
@@ -734,8 +763,10 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
 #undef NV
 }
 
-// Workgroups per CU the staged kernel is built for: its LDS (52 KB at 64 x 4, 42 KB at 32 x 8, 74 KB at
-// 128 x 2 or with two rows per thread) leaves room for three, or two.
+// Workgroups per CU the staged kernel is built for: three where its LDS allows (38.5 KB at 32 x 8, 47 KB at
+// 64 x 4), else two (66-73 KB at 128 x 2 or with two rows per thread). Four would fit the 32 x 8 tile's LDS,
+// but its registers do not fit four: held to 128 VGPRs it spills in the k loop and took 1.62 ms against
+// 0.97 ms at three.
 template <int ROWS, int WX>
 constexpr int sgs3_occupancy()
 {

@@ -48,9 +48,10 @@ def traffic_from_profiles(name: str):
 
 def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 10.0), reps=10, check=None):
     """Pressure Poisson projection solve: the FP64 direct solve (FFT/Hartley transforms in x and y for
-    periodic power-of-two sizes, FP64 z solves) vs the mixed solve (split-bf16 transforms, FP64 residuals
-    and refinement, three solves), both resident, timed with HIP events over `reps` solves. Returns the
-    section dict and the two solutions (interiors)."""
+    periodic power-of-two sizes, FP64 z solves) vs the mixed solve (the same Hartley passes with FP32
+    workspaces, two applications, where those apply; split-bf16 transforms and three applications
+    otherwise; FP64 residuals and refinement), both resident, timed with HIP events over `reps` solves.
+    Returns the section dict and the two solutions (interiors)."""
     pressure = graft._module("pressure")
     nx, ny, nz = n
     b = fields.divergence_rhs(nx, ny, nz, h)
@@ -64,7 +65,8 @@ def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 1
     ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
     lib.ftc_event_create(ctypes.byref(ev0))
     lib.ftc_event_create(ctypes.byref(ev1))
-    for name, mixed, solves in (("fp64", False, 1), ("mixed", True, 3)):
+    pow2 = all(d >= 4 and d & (d - 1) == 0 for d in (nx, ny))
+    for name, mixed, solves in (("fp64", False, 1), ("mixed", True, 2 if pow2 else 3)):
         phi = np.zeros((nx + 4, ny + 4, nz + 4), order="F")
         hp = dom.bind(phi, lower=(-1, -1, -1))
         solver = pressure.PressureSolver(lib, dom, n, (nx * h[0], ny * h[1], nz * h[2]), mixed)
@@ -91,8 +93,9 @@ def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 1
     out["speedup_mixed_over_fp64"] = round(out["fp64"]["ms_per_solve"] / out["mixed"]["ms_per_solve"], 3)
     scale = np.abs(sols["fp64"]).max()
     out["max_rel_diff_mixed_vs_fp64"] = float(np.abs(sols["mixed"] - sols["fp64"]).max() / scale)
-    pow2 = all(d >= 4 and d & (d - 1) == 0 for d in (nx, ny))
     out["fp64"]["method"] = "fft (Hartley) x/y passes + FP64 z Thomas" if pow2 else "dense FP64 MFMA transforms"
+    out["mixed"]["method"] = ("Hartley passes with FP32 workspaces, FP64 residual" if pow2
+                              else "split-bf16 MFMA transforms, FP64 residual")
     # The FP64 FFT solve streams the field five times (x, y forward, z, y, x inverse), each a read and a
     # write of N doubles; its floor is one read of b and one write of phi.
     pass_bytes = 5 * 2 * 8 * N
@@ -109,7 +112,7 @@ def poisson_section(ftc, scope, lib, fields, n=(512, 512, 128), h=(10.0, 10.0, 1
 def projected_step_section(ftc, scope, lib, fields, n=(512, 512, 128), steps=10, dx=10.0, dt=0.2, cs=0.17):
     """A whole resident time step in ELMM's order (LES step, divergence into the solver's RHS, pressure
     solve, projection, the shift of phi by its reference-plane mean), all entries on the device in one
-    epoch, ping-ponging two field sets: the FP64 solve against the mixed one (three solves, no host
+    epoch, ping-ponging two field sets: the FP64 solve against the mixed one (two or three solves, no host
     synchronisation). The context holds each step's work until the step's flush
     (FORT_SCOPE_GRAPH_QUEUE_WORK=0): the solver's launches are captured into one node, the plane sum stays
     on the device (a queued fixed-order reduction into a one-value array that the shift reads), so each
@@ -145,8 +148,10 @@ def projected_step_section(ftc, scope, lib, fields, n=(512, 512, 128), steps=10,
         hr, hp = dom.bind(rhs, lower=(0, 0, 0), initialized=False), dom.bind(phi, lower=(-1, -1, -1))
         hpart, href = dom.bind(partial, initialized=False), dom.bind(ref, initialized=False)
         solver = pressure.PressureSolver(lib, dom, n, (nx * dx, ny * dx, nz * dx), mixed)
-        # A fixed number of solves and no residual norms: no host synchronisation inside the step.
-        solver.refinement(1e-12, 3 if mixed else 1, False)
+        # A fixed number of solves and no residual norms: no host synchronisation inside the step. The
+        # mixed solve takes two on the Hartley passes (periodic power-of-two x and y), three otherwise.
+        pow2 = all(d >= 4 and d & (d - 1) == 0 for d in (nx, ny))
+        solver.refinement(1e-12, (2 if pow2 else 3) if mixed else 1, False)
         stream = ctypes.c_void_p(dom.stream())
         ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
         lib.ftc_event_create(ctypes.byref(ev0))

@@ -40,8 +40,9 @@ struct HartleyPass {
     long long oe, oa, oc;
     int n, logn, G, logG;  // line length and real lines per workgroup (2 * POINTS / n), powers of two
     int A, C;              // lines per plane, planes
-    double scale, beta;    // out = beta * out + scale * H(in); beta == 0 reads nothing of out
+    double scale, beta;    // out = beta * out + scale * H(in) - shift; beta == 0 reads nothing of out
     const double2 *tw;     // tw[k] = (cos, -sin)(2 pi k / n), k < n/2
+    const double *shift;   // a device scalar subtracted from every output value, or none
 };
 
 // LDS bytes of a pass over lines of n points: the complex lines (one padding point each, so that the
@@ -237,7 +238,9 @@ __global__ __launch_bounds__(NT) void hartley_kernel(const HartleyPass h)
         const double hv = (g & 1) ? ((Z.y + W.y) + (Z.x - W.x)) : ((Z.x + W.x) - (Z.y - W.y));
         TO *dst = out + (long long)a * h.oa + (long long)k * h.oe;
         const double val = sc * hv;
-        store_from_double(dst, beta == 0.0 ? val : beta * load_as_double(dst) + val);
+        double o = beta == 0.0 ? val : beta * load_as_double(dst) + val;
+        if (h.shift) o = o - *h.shift;
+        store_from_double(dst, o);
     }
 }
 

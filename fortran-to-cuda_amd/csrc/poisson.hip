@@ -185,12 +185,13 @@ static void hartley_launch(bool econt, dim3 grid, std::size_t lds, hipStream_t s
 
 template <class TI, class TO>
 static int hartley_pass(const void *in, long long ie, long long ia, long long ic, void *out, long long oe, long long oa,
-                        long long oc, int n, int A, int C, double beta, const double2 *tw, hipStream_t stream)
+                        long long oc, int n, int A, int C, double beta, const double2 *tw, hipStream_t stream,
+                        const double *shift = nullptr)
 {
     const bool econt = ie == 1;
     const int points = hartley_points(econt, n);
     HartleyPass h{in, ie, ia, ic, out, oe, oa, oc, n, log2_exact(n), 2 * points / n, 0, A, C,
-                  1.0 / std::sqrt((double)n), beta, tw};
+                  1.0 / std::sqrt((double)n), beta, tw, shift};
     h.logG = log2_exact(h.G);
     const std::size_t lds = (std::size_t)hartley_lds_bytes(n, points);
     const dim3 grid((unsigned)((A + h.G - 1) / h.G), (unsigned)C);
@@ -202,20 +203,28 @@ static int hartley_pass(const void *in, long long ie, long long ia, long long ic
 }
 
 // Raises the dynamic LDS limit of the Hartley kernels to what the largest pass needs.
+template <class TI, class TO>
+static bool hartley_attributes_of(int bytes)
+{
+    bool ok = true;
+    for (const void *f : {reinterpret_cast<const void *>(&hartley_kernel<TI, TO, true, 2048, 256>),
+                          reinterpret_cast<const void *>(&hartley_kernel<TI, TO, false, 2048, 256>),
+                          reinterpret_cast<const void *>(&hartley_kernel<TI, TO, true, 4096, 256>),
+                          reinterpret_cast<const void *>(&hartley_kernel<TI, TO, false, 4096, 256>),
+                          reinterpret_cast<const void *>(&hartley_kernel<TI, TO, true, 4096, 512>),
+                          reinterpret_cast<const void *>(&hartley_kernel<TI, TO, false, 4096, 512>)})
+        ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
+    return ok;
+}
+
 static bool hartley_attributes()
 {
     int bytes = 0;
     for (int n = 4; n <= 4096; n *= 2)
         for (int points : {2048, 4096}) bytes = std::max(bytes, hartley_lds_bytes(n, std::max(points, n)));
-    bool ok = true;
-    for (const void *f : {reinterpret_cast<const void *>(&hartley_kernel<double, double, true, 2048, 256>),
-                          reinterpret_cast<const void *>(&hartley_kernel<double, double, false, 2048, 256>),
-                          reinterpret_cast<const void *>(&hartley_kernel<double, double, true, 4096, 256>),
-                          reinterpret_cast<const void *>(&hartley_kernel<double, double, false, 4096, 256>),
-                          reinterpret_cast<const void *>(&hartley_kernel<double, double, true, 4096, 512>),
-                          reinterpret_cast<const void *>(&hartley_kernel<double, double, false, 4096, 512>)})
-        ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
-    return ok;
+    // FP64 throughout (the FP64 solve); FP64 or FP32 in, FP32 between passes, FP64 out (the mixed solve).
+    return hartley_attributes_of<double, double>(bytes) && hartley_attributes_of<double, float>(bytes) &&
+           hartley_attributes_of<float, float>(bytes) && hartley_attributes_of<float, double>(bytes);
 }
 
 // One FP64 solve through the Hartley transforms: phi_out = beta * phi_out + Hx Hy T^-1 Hy Hx in, with
@@ -238,11 +247,36 @@ static int apply_inverse_hartley(poisson_solver_handle_v1 &s, const double *in, 
     return hartley_pass<double, double>(w1, 1, nx, plane, out.x, 1, out.ld1, out.ld2, nx, ny, nz, beta, s.twx, stream);
 }
 
+// One application of the mixed solve's approximate inverse through the Hartley transforms: the same five
+// passes as the FP64 solve, FP64 arithmetic inside each, the workspaces between them held in FP32. The
+// first application reads the FP64 right-hand side, a correction the FP32 residual (in_float); the last
+// pass adds into phi (beta 1 on corrections) and subtracts `shift` (phi's mean from the residual pass).
+static int apply_inverse_hartley_mixed(poisson_solver_handle_v1 &s, const void *in, long long in1, long long in2,
+                                       bool in_float, View3 out, double beta, const double *shift, hipStream_t stream)
+{
+    const int nx = s.nx, ny = s.ny, nz = s.nz;
+    const long long plane = (long long)nx * ny;
+    float *w0 = static_cast<float *>(s.w0), *w1 = static_cast<float *>(s.w1);
+    if (in_float) FTC_CHECK((hartley_pass<float, float>(in, 1, in1, in2, w0, 1, nx, plane, nx, ny, nz, 0.0, s.twx, stream)));
+    else FTC_CHECK((hartley_pass<double, float>(in, 1, in1, in2, w0, 1, nx, plane, nx, ny, nz, 0.0, s.twx, stream)));
+    FTC_CHECK((hartley_pass<float, float>(w0, nx, 1, plane, w1, nx, 1, plane, ny, nx, nz, 0.0, s.twy, stream)));
+    const unsigned blocks = (unsigned)((plane + s.cols - 1) / s.cols);
+    const std::size_t lds = 2 * (std::size_t)tridiag_chunks(nz) * s.cols * sizeof(double);
+    hipLaunchKernelGGL((tridiag_z_kernel<float, float>), dim3(blocks), dim3(s.cols), lds, stream,
+                       static_cast<const float *>(w1), TriOut<float>{w0, 0}, nx, ny, nz, s.lx, s.ly,
+                       1.0 / (s.hz * s.hz), s.cols, 0);
+    if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "tridiagonal launch failed");
+    FTC_CHECK((hartley_pass<float, float>(w0, nx, 1, plane, w1, nx, 1, plane, ny, nx, nz, 0.0, s.twy, stream)));
+    return hartley_pass<float, double>(w1, 1, nx, plane, out.x, 1, out.ld1, out.ld2, nx, ny, nz, beta, s.twx, stream,
+                                       shift);
+}
+
 // The contraction calls of one application of the approximate (mixed) or exact (FP64) inverse:
 // phi_out = beta * phi_out + Qx [T^-1 ((Qx^T in) Qy)] Qy^T.
 static int apply_inverse(poisson_solver_handle_v1 &s, const void *in, long long in1, long long in2, View3 out,
                          double beta, hipStream_t stream, bool in_split = false, const double *shift = nullptr)
 {
+    if (s.hartley && s.mixed) return apply_inverse_hartley_mixed(s, in, in1, in2, in_split, out, beta, shift, stream);
     if (s.hartley) return apply_inverse_hartley(s, static_cast<const double *>(in), in1, in2, out, beta, stream);
     const int nx = s.nx, ny = s.ny, nz = s.nz;
     const long long plane = (long long)nx * ny;
@@ -312,9 +346,17 @@ static int residual_split(poisson_solver_handle_v1 &s, View3 phiv, View3 b, doub
 {
     const dim3 grid = residual_split_grid(s);
     const long long N = (long long)s.nx * s.ny * s.nz, nb = residual_blocks(s);
-    hipLaunchKernelGGL(poisson_residual_split_kernel, grid, dim3(64, 4), 0, stream, phiv.x, phiv.ld1, phiv.ld2, b.x,
-                       b.ld1, b.ld2, reinterpret_cast<__bf16 *>(s.res), N, partial, s.phisum, s.nx, s.ny, s.nz,
-                       1.0 / (s.hx * s.hx), 1.0 / (s.hy * s.hy), 1.0 / (s.hz * s.hz), s.perx ? 1 : 0, s.pery ? 1 : 0);
+    // The Hartley passes read the residual as FP32, the split-bf16 contractions as bf16 hi and lo planes.
+    if (s.hartley)
+        hipLaunchKernelGGL(poisson_residual_split_kernel<false>, grid, dim3(64, 4), 0, stream, phiv.x, phiv.ld1,
+                           phiv.ld2, b.x, b.ld1, b.ld2, (void *)s.res, N, partial, s.phisum, s.nx, s.ny, s.nz,
+                           1.0 / (s.hx * s.hx), 1.0 / (s.hy * s.hy), 1.0 / (s.hz * s.hz), s.perx ? 1 : 0,
+                           s.pery ? 1 : 0);
+    else
+        hipLaunchKernelGGL(poisson_residual_split_kernel<true>, grid, dim3(64, 4), 0, stream, phiv.x, phiv.ld1,
+                           phiv.ld2, b.x, b.ld1, b.ld2, (void *)s.res, N, partial, s.phisum, s.nx, s.ny, s.nz,
+                           1.0 / (s.hx * s.hx), 1.0 / (s.hy * s.hy), 1.0 / (s.hz * s.hz), s.perx ? 1 : 0,
+                           s.pery ? 1 : 0);
     if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "residual launch failed");
     ReduceBox box{s.phisum, nb, 1, 1, nb, nb};
     FTC_CHECK(reduce_launch(kReduceSum, box, s.slots, s.final256, stream));
@@ -435,8 +477,7 @@ int poisson_solver_create_v1(const poisson_solver_config_v1 *c, poisson_solver_h
     s->perx = c->boundary_conditions[0] == 0;
     s->pery = c->boundary_conditions[2] == 0;
     s->mixed = (c->flags & POISSON_SOLVER_MIXED) != 0;
-    s->hartley = !s->mixed && s->perx && s->pery && hartley_size(s->nx) && hartley_size(s->ny) &&
-                 !std::getenv("FTC_POISSON_DENSE");
+    s->hartley = s->perx && s->pery && hartley_size(s->nx) && hartley_size(s->ny) && !std::getenv("FTC_POISSON_DENSE");
     s->max_solves = s->mixed ? 6 : 1;
     s->cols = 64;
     if (s->nz > kPoissonMaxNz) return perr(POISSON_SOLVER_UNSUPPORTED, "nz above 1024 exceeds the LDS of the z solve");

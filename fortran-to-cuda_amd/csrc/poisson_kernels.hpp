@@ -43,9 +43,12 @@ static __global__ void poisson_residual_kernel(const double *__restrict__ phi, l
 // phisum[block] and, when `partial` is set, the sum of r*r into partial[block] for the residual norm.
 constexpr int kResidualChunk = 16;
 
+// SPLIT: r holds bf16 hi and lo planes (lo `lo` elements after hi) for the split-bf16 contractions;
+// otherwise r holds FP32 values (the Hartley passes of the mixed solve read FP32).
+template <bool SPLIT>
 static __global__ __launch_bounds__(256) void poisson_residual_split_kernel(
     const double *__restrict__ phi, long long p1, long long p2, const double *__restrict__ b, long long b1,
-    long long b2, __bf16 *__restrict__ r, long long lo, double *__restrict__ partial, double *__restrict__ phisum,
+    long long b2, void *__restrict__ r, long long lo, double *__restrict__ partial, double *__restrict__ phisum,
     int nx, int ny, int nz, double rx, double ry, double rz, int perx, int pery)
 {
     __shared__ double sq[256], sp[256];
@@ -66,9 +69,13 @@ static __global__ __launch_bounds__(256) void poisson_residual_split_kernel(
             const double v = b[i + j * b1 + k * b2] - lap;
             const long long out = (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
             const float f = (float)v;
-            const __bf16 h = (__bf16)f;
-            r[out] = h;
-            r[out + lo] = (__bf16)(f - (float)h);
+            if constexpr (SPLIT) {
+                const __bf16 h = (__bf16)f;
+                static_cast<__bf16 *>(r)[out] = h;
+                static_cast<__bf16 *>(r)[out + lo] = (__bf16)(f - (float)h);
+            } else {
+                static_cast<float *>(r)[out] = f;
+            }
             acc2 = acc2 + v * v;
             acc = acc + c0;
             zm = c0;

@@ -50,14 +50,16 @@ def test_solve_matches_the_oracle(ftc, scope, hip, mixed, n):
         assert report.converged == 1 and report.iterations >= 2
 
 
-@pytest.mark.parametrize("n", [(64, 48, 32), (37, 29, 17)])
-def test_mixed_solve_with_a_fixed_solve_count(ftc, scope, hip, n):
-    # Three applications without residual norms (no host synchronisation), as a captured step runs them.
+@pytest.mark.parametrize("n,solves", [((64, 48, 32), 3), ((37, 29, 17), 3), ((64, 64, 32), 2), ((256, 128, 16), 2)])
+def test_mixed_solve_with_a_fixed_solve_count(ftc, scope, hip, n, solves):
+    # Applications without residual norms (no host synchronisation), as a captured step runs them: three
+    # where the transforms are split-bf16 products (other sizes), two where they are the Hartley passes
+    # with FP32 workspaces (periodic power-of-two x and y).
     h = (10.0, 10.0, 10.0)
     b = poisson_oracle.divergence_rhs(*n, h, seed=sum(n) + 1)
-    got, report = run(ftc, scope, hip, b, h, True, max_solves=3, measure=False)
+    got, report = run(ftc, scope, hip, b, h, True, max_solves=solves, measure=False)
     want = poisson_oracle.solve(b, h)
-    assert report.iterations == 3
+    assert report.iterations == solves
     assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
 
 

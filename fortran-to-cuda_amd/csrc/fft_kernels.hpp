@@ -162,24 +162,50 @@ __device__ __forceinline__ void stockham_stage(double2 *z, const double2 *tw, in
 
 // The first Stockham stage (ns = 1: no twiddles, results at j*R + r): operands straight from global
 // memory, two real lines (a, a+1) per complex line; lines past the plane's last read as zero.
+// The two real lines of a complex line side by side in memory (ia == 1, the y transform) as one vector.
+template <class TI>
+struct LinePair;
+template <>
+struct LinePair<double> {
+    using V = double2;
+};
+template <>
+struct LinePair<float> {
+    using V = float2;
+};
+
 template <int R, class TI, bool ECONT, int PT, int NT>
 __device__ __forceinline__ void first_stage(const HartleyPass &h, const TI *in, double2 *z, int a0, int logl,
                                             int stride, double2 (&v)[PT])
 {
     constexpr int LOGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
     const int lognr = h.logn - LOGR, nr = 1 << lognr;
+    // Lines across threads with both lines of every pair present (A even) and adjacent (ia == 1): each pair
+    // is one vector load, aligned because a, ie and ic are even (checked here), which covers a whole
+    // 128-byte run per load instruction instead of every other element of it.
+    const bool pairs = !ECONT && h.ia == 1 && (h.A & 1) == 0 && (h.ie & 1) == 0 && (h.ic & 1) == 0;
 #pragma unroll
     for (int t = 0; t < PT / R; ++t) {
         int line, j;
         butterfly_at<ECONT>(threadIdx.x + t * NT, lognr, logl, line, j);
         const int a = a0 + 2 * line;
         const TI *p0 = in + (long long)a * h.ia, *p1 = p0 + h.ia;
+        if (pairs) {
+            using V = typename LinePair<TI>::V;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const long long e = (long long)(j + r * nr) * h.ie;
-            const double re = a < h.A ? load_as_double(p0 + e) : 0.0;
-            const double im = a + 1 < h.A ? load_as_double(p1 + e) : 0.0;
-            v[t * R + r] = make_double2(re, im);
+            for (int r = 0; r < R; ++r) {
+                const long long e = (long long)(j + r * nr) * h.ie;
+                const V pv = a < h.A ? *reinterpret_cast<const V *>(p0 + e) : V{0, 0};
+                v[t * R + r] = make_double2((double)pv.x, (double)pv.y);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const long long e = (long long)(j + r * nr) * h.ie;
+                const double re = a < h.A ? load_as_double(p0 + e) : 0.0;
+                const double im = a + 1 < h.A ? load_as_double(p1 + e) : 0.0;
+                v[t * R + r] = make_double2(re, im);
+            }
         }
         dft<R>(&v[t * R]);
         double2 *zl = z + line * stride;

@@ -88,9 +88,12 @@ static dim3 grid_of(const LesArgs &a, int kplanes, int rows = 1, int width = TX)
 }
 
 template <bool STEP>
-static const void *staged_kernel(int rows, int width, bool iso)
+static const void *staged_kernel(int rows, int width, bool iso, bool two)
 {
     if (rows == 2) return reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 2>);
+    if (width == 32 && two)
+        return iso ? reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 32, true, true>)
+                   : reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 32, false, true>);
     if (width == 32)
         return iso ? reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 32, true>)
                    : reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 32>);
@@ -224,6 +227,9 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     const char *width_env = std::getenv("FTC_LES_WIDTH");
     int width = staged && rows == 1 && width_env ? std::atoi(width_env) : TX;
     if (width != 32 && width != 128) width = TX;
+    // FTC_LES_TWO_BARRIERS: the 32 x 8 tile with one more LDS slot per staged field and two barriers per
+    // plane (51.6 KB, still three workgroups per CU).
+    const bool two = width == 32 && std::getenv("FTC_LES_TWO_BARRIERS");
     a.kchunk = choose_kchunk(nx, ny, nz, rows, !staged ? 2 : (rows == 2 || width == 128 ? 2 : 3));
     // Tiles in launch order put each 64-column strip on one XCD, so the y neighbours that re-read a tile's
     // ghost rows share its L2. The XCD-contiguous order (FTC_LES_XCD_ORDER) also shares the x neighbours'
@@ -254,7 +260,7 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
     // Equal spacings select the kernel that holds each per-direction coefficient once (same values, same
     // arithmetic); FTC_LES_ANISO keeps the general one for comparison.
     const bool iso = *dx == *dy && *dy == *dz && !std::getenv("FTC_LES_ANISO");
-    if (staged) fn = step ? staged_kernel<true>(rows, width, iso) : staged_kernel<false>(rows, width, iso);
+    if (staged) fn = step ? staged_kernel<true>(rows, width, iso, two) : staged_kernel<false>(rows, width, iso, two);
     else if (step) fn = sgs ? kernel_of<15, true, true>() : kernel_of<15, false, true>();
     else fn = sgs ? kernel_of<15, true, false>() : kernel_of<15, false, false>();
     const uint64_t key = 0x1e5000000000000full ^ ((uint64_t)step << 8) ^ ((uint64_t)sgs << 9) ^ ((uint64_t)iso << 10);
@@ -265,7 +271,8 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
         const unsigned tiles = grid.x * grid.y * grid.z;
         const unsigned per_cu = rows == 2 ? (unsigned)sgs3_occupancy<2, TX>()
                                 : width == 128 ? (unsigned)sgs3_occupancy<1, 128>()
-                                : width == 32  ? (unsigned)sgs3_occupancy<1, 32>()
+                                : width == 32  ? (two ? (unsigned)sgs3_occupancy<1, 32, true>()
+                                                      : (unsigned)sgs3_occupancy<1, 32>())
                                                : (unsigned)sgs3_occupancy<1, TX>();
         const unsigned resident = (unsigned)compute_units() * per_cu / kXcds * kXcds;
         grid = dim3(std::max(kXcds, std::min(resident, (tiles + kXcds - 1) / kXcds * kXcds)), 1, 1);

@@ -400,13 +400,19 @@ struct Sgs3Tile {
 // W is read at planes k and k+1 only while a step computes plane k (the viscosity of k+1 takes W at k+1 and
 // k, the update of k takes W at k with its ring and at k+1 in the column), so it takes two slots where U and
 // V, whose plane k+2 the viscosity of k+1 reads, take three; W is staged one plane behind them.
-template <int ROWS, int WX = TX>
+//
+// TWO: two barriers per k-step instead of three. The third barrier keeps the next step's puts (U, V at k+3,
+// W and S at k+2) off the slots the update of k still reads; with one more slot for each of them (4, 3, 3)
+// those puts land on the planes of k-1, which no thread reads any more. The viscosity keeps two slots: the
+// next step writes it only after its first barrier, which every update of k has reached.
+template <int ROWS, int WX = TX, bool TWO = false>
 struct Sgs3Lds {
     using T = Sgs3Tile<ROWS, WX>;
-    double uv[3][2][T::VYT][T::VXT];  // U, V; slot = plane % 3
-    double w[2][T::VYT][T::VXT];      // W; slot = plane & 1
-    double sca[2][T::CYT][T::CXT];    // slot = plane & 1
-    double nu[2][T::CYT][T::CXT];     // slot = plane & 1
+    static constexpr int SU = TWO ? 4 : 3, SW = TWO ? 3 : 2, SS = TWO ? 3 : 2;
+    double uv[SU][2][T::VYT][T::VXT];  // U, V; slot = plane % SU
+    double w[SW][T::VYT][T::VXT];      // W; slot = plane % SW
+    double sca[SS][T::CYT][T::CXT];    // slot = plane % SS
+    double nu[2][T::CYT][T::CXT];      // slot = plane & 1
 };
 
 // The upper z-face terms of a cell (advection and diffusion of u, v, w and s), which are the lower terms
@@ -420,10 +426,11 @@ struct ZCarry {
 // arithmetic is the same (the same values in the same order); the kernel only holds 7 coefficients instead of
 // 20, which keeps them in scalar registers instead of spilling them. 0.25/d serves as both the advection's
 // c25 and the viscosity's quarter-difference factor (the host computes both as 0.25/d).
-template <bool STEP, int ROWS = 1, int WX = TX, bool ISO = false>
-__device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX> &L, int bx, int by, int bz)
+template <bool STEP, int ROWS = 1, int WX = TX, bool ISO = false, bool TWO = false>
+__device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX, TWO> &L, int bx, int by, int bz)
 {
     using T = Sgs3Tile<ROWS, WX>;
+    constexpr int SU = Sgs3Lds<ROWS, WX, TWO>::SU, SW = Sgs3Lds<ROWS, WX, TWO>::SW, SS = Sgs3Lds<ROWS, WX, TWO>::SS;
     constexpr int VR = T::VRT, CR = T::CRT, VCELLS = T::VC, CCELLS = T::CC;
     constexpr int VX = T::VXT, CX = T::CXT;
     const double *__restrict__ gU = a.U;
@@ -476,7 +483,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
     };
     auto put_uv = [&](int kp) {
         if (kp < 1 || kp > nz + 2) return;
-        const int slot = kp % 3;
+        const int slot = kp % SU;
 #pragma unroll
         for (int r = 0; r < VR; ++r)
             if (vcell[r] >= 0) {
@@ -495,7 +502,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
     };
     auto put_w = [&](int kp) {
         if (kp < 1 || kp > nz + 1) return;
-        const int slot = kp & 1;
+        const int slot = kp % SW;
 #pragma unroll
         for (int r = 0; r < VR; ++r)
             if (vcell[r] >= 0) (&L.w[slot][0][0])[vcell[r]] = rw[r];
@@ -509,7 +516,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
     };
     auto put_sca = [&](int kp) {
         if (kp < 1 || kp > nz + 2) return;
-        const int slot = kp & 1;
+        const int slot = kp % SS;
 #pragma unroll
         for (int r = 0; r < CR; ++r)
             if (ccell[r] >= 0) (&L.sca[slot][0][0])[ccell[r]] = rs[r];
@@ -517,8 +524,8 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
     // nu at plane qk (clamped to the interior planes: zero gradient in z) over the tile and its ring.
     auto nu_plane = [&](int qk) {
         const int kk = qk < 2 ? 2 : (qk > nz + 1 ? nz + 1 : qk);
-        const int s0 = kk % 3, sm = (kk + 2) % 3, sp = (kk + 1) % 3;
-        const int w0s = kk & 1, wms = (kk - 1) & 1;
+        const int s0 = kk % SU, sm = (kk - 1) % SU, sp = (kk + 1) % SU;
+        const int w0s = kk % SW, wms = (kk - 1) % SW;
         double(*nuo)[CX] = L.nu[qk & 1];
         for (int c = tid; c < CCELLS; c += NTH) {
             const int li = c % CX, lj = c / CX;
@@ -589,10 +596,11 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
         if (i <= nx + 1 && j <= ny + 1) {
         ZCarry &zc = zcr[rr];
         const int kv = k0 - 1;
-        const int s0 = kv % 3, sp = (kv + 1) % 3, c0 = kv & 1, cp = (kv + 1) & 1;
+        const int s0 = kv % SU, sp = (kv + 1) % SU, c0 = kv & 1, cp = (kv + 1) & 1;
+        const int ws0 = kv % SW, wsp = (kv + 1) % SW, ss0 = kv % SS, ssp = (kv + 1) % SS;
         const double u0 = VU(s0, 0, 0), uzp = VU(sp, 0, 0);
         const double v0 = VV(s0, 0, 0), vzp = VV(sp, 0, 0);
-        const double w0 = VW(c0, 0, 0), wxp = VW(c0, 1, 0), wyp = VW(c0, 0, 1), wzp = VW(cp, 0, 0);
+        const double w0 = VW(ws0, 0, 0), wxp = VW(ws0, 1, 0), wyp = VW(ws0, 0, 1), wzp = VW(wsp, 0, 0);
         const double n000 = NV(c0, 0, 0), n100 = NV(c0, 1, 0), n010 = NV(c0, 0, 1), n001 = NV(cp, 0, 0);
         zc.cu = c25z * (uzp + u0) * (wxp + w0);
         zc.du = (0.25 * (((n000 + n100) + n001) + NV(cp, 1, 0))) * (uzp - u0);
@@ -600,7 +608,7 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
         zc.dv = (0.25 * (((n000 + n010) + n001) + NV(cp, 0, 1))) * (vzp - v0);
         zc.cw = c25z * (wzp + w0) * (wzp + w0);
         zc.dw = n001 * (wzp - w0);
-        const double s0v = SS(c0, 0, 0), szp = SS(cp, 0, 0);
+        const double s0v = SS(ss0, 0, 0), szp = SS(ssp, 0, 0);
         zc.cs = Cz * w0 * (szp + s0v);
         zc.ds = (0.5 * (n000 + n001)) * (szp - s0v);
         zc.u = u0;
@@ -632,14 +640,15 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
             ZCarry &zc = zcr[rr];
             const ColumnImages ci = column_images(a, i, j);
             const PlaneImages zi = plane_images(a, k);
-            const int s0 = k % 3, sp = (k + 1) % 3;
+            const int s0 = k % SU, sp = (k + 1) % SU;
             const int c0 = k & 1, cp = (k + 1) & 1;
+            const int ws0 = k % SW, wsp = (k + 1) % SW, ss0 = k % SS, ssp = (k + 1) % SS;
             const double u0 = VU(s0, 0, 0), uxp = VU(s0, 1, 0), uxm = VU(s0, -1, 0), uyp = VU(s0, 0, 1);
             const double uym = VU(s0, 0, -1), uzp = VU(sp, 0, 0), uzm = zc.u;
             const double v0 = VV(s0, 0, 0), vxp = VV(s0, 1, 0), vxm = VV(s0, -1, 0), vyp = VV(s0, 0, 1);
             const double vym = VV(s0, 0, -1), vzp = VV(sp, 0, 0), vzm = zc.v;
-            const double w0 = VW(c0, 0, 0), wxp = VW(c0, 1, 0), wxm = VW(c0, -1, 0), wyp = VW(c0, 0, 1);
-            const double wym = VW(c0, 0, -1), wzp = VW(cp, 0, 0), wzm = zc.w;
+            const double w0 = VW(ws0, 0, 0), wxp = VW(ws0, 1, 0), wxm = VW(ws0, -1, 0), wyp = VW(ws0, 0, 1);
+            const double wym = VW(ws0, 0, -1), wzp = VW(wsp, 0, 0), wzm = zc.w;
             const double n000 = NV(c0, 0, 0), n100 = NV(c0, 1, 0), nm00 = NV(c0, -1, 0);
             const double n010 = NV(c0, 0, 1), n0m0 = NV(c0, 0, -1), n001 = NV(cp, 0, 0);
             const long long p = (long long)(i - 1) + (long long)(j - 1) * sj + (long long)(k - 1) * sk;
@@ -727,8 +736,8 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
             }
             // S: flux-form advection then the diffusion of s.
             {
-                const double s0v = SS(c0, 0, 0), sxp = SS(c0, 1, 0), sxm = SS(c0, -1, 0), syp = SS(c0, 0, 1);
-                const double sym = SS(c0, 0, -1), szp = SS(cp, 0, 0);
+                const double s0v = SS(ss0, 0, 0), sxp = SS(ss0, 1, 0), sxm = SS(ss0, -1, 0), syp = SS(ss0, 0, 1);
+                const double sym = SS(ss0, 0, -1), szp = SS(ssp, 0, 0);
                 const double Cx = c50x, Cy = c50y;
                 double x = Cx * u0 * (sxp + s0v) - Cx * uxm * (s0v + sxm);
                 double y = Cy * v0 * (syp + s0v) - Cy * vym * (s0v + sym);
@@ -754,8 +763,11 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
             zc = up;
             }
         }
-        __syncthreads();
+        if constexpr (!TWO) __syncthreads();
     }
+    // With two barriers per step the last update may still read its slots; the next tile of a persistent
+    // walk starts by writing them.
+    if constexpr (TWO) __syncthreads();
 #undef VU
 #undef VV
 #undef VW
@@ -767,17 +779,17 @@ __device__ __forceinline__ void les_sgs3_body(const LesArgs &a, Sgs3Lds<ROWS, WX
 // 64 x 4), else two (66-73 KB at 128 x 2 or with two rows per thread). Four would fit the 32 x 8 tile's LDS,
 // but its registers do not fit four: held to 128 VGPRs it spills in the k loop and took 1.62 ms against
 // 0.97 ms at three.
-template <int ROWS, int WX>
+template <int ROWS, int WX, bool TWO = false>
 constexpr int sgs3_occupancy()
 {
-    return sizeof(Sgs3Lds<ROWS, WX>) <= 160 * 1024 / 3 ? 3 : 2;
+    return sizeof(Sgs3Lds<ROWS, WX, TWO>) <= 160 * 1024 / 3 ? 3 : 2;
 }
 
-template <bool STEP, int ROWS, int WX = TX, bool ISO = false>
-__global__ __launch_bounds__(NTH, (sgs3_occupancy<ROWS, WX>())) void les_sgs3_kernel(const LesArgs a)
+template <bool STEP, int ROWS, int WX = TX, bool ISO = false, bool TWO = false>
+__global__ __launch_bounds__(NTH, (sgs3_occupancy<ROWS, WX, TWO>())) void les_sgs3_kernel(const LesArgs a)
 {
     using T = Sgs3Tile<ROWS, WX>;
-    __shared__ Sgs3Lds<ROWS, WX> L;
+    __shared__ Sgs3Lds<ROWS, WX, TWO> L;
     if (a.tile_order == 2) {
         // Persistent banded walk (xcd.hpp) over the tiles in x-fastest order, by one resident round of
         // workgroups (a multiple of eight): the band resident on one XCD re-reads its x and y neighbours'
@@ -786,14 +798,14 @@ __global__ __launch_bounds__(NTH, (sgs3_occupancy<ROWS, WX>())) void les_sgs3_ke
         const unsigned gz = (a.nz + a.kchunk - 1) / a.kchunk;
         const TileWalk walk = banded_walk(gx * gy * gz, gridDim.x, blockIdx.x);
         for (unsigned t = walk.first; t < walk.end; t += walk.step)
-            les_sgs3_body<STEP, ROWS, WX, ISO>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
+            les_sgs3_body<STEP, ROWS, WX, ISO, TWO>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
         return;
     }
     // Tiles x fastest, then y, then the k chunk: with the XCD-contiguous order the x and y neighbours
     // whose ghost rows and columns a tile re-reads run under the same L2 at about the same time.
     const unsigned gx = gridDim.x, gy = gridDim.y;
     const unsigned t = launch_tile(a.tile_order);
-    les_sgs3_body<STEP, ROWS, WX, ISO>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
+    les_sgs3_body<STEP, ROWS, WX, ISO, TWO>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
 }
 
 }  // namespace ftc

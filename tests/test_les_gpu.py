@@ -133,13 +133,16 @@ def test_les_steps_with_graph_replay_are_bit_exact_including_halos(ftc, scope, h
                                      {"FTC_LES_KCHUNK": "5"}, {"FTC_LES_PERSISTENT": "1"},
                                      {"FTC_LES_PERSISTENT": "1", "FTC_LES_ROWS": "2"}, {"FTC_LES_WIDTH": "32"},
                                      {"FTC_LES_WIDTH": "128"}, {"FTC_LES_WIDTH": "32", "FTC_LES_PERSISTENT": "1"},
-                                     {"FTC_LES_ANISO": "1"}, {"FTC_LES_ANISO": "1", "FTC_LES_WIDTH": "32"}])
+                                     {"FTC_LES_ANISO": "1"}, {"FTC_LES_ANISO": "1", "FTC_LES_WIDTH": "32"},
+                                     {"FTC_LES_WIDTH": "32", "FTC_LES_TWO_BARRIERS": "1"},
+                                     {"FTC_LES_WIDTH": "32", "FTC_LES_TWO_BARRIERS": "1", "FTC_LES_PERSISTENT": "1"},
+                                     {"FTC_LES_WIDTH": "32", "FTC_LES_TWO_BARRIERS": "1", "FTC_LES_ANISO": "1"}])
 @pytest.mark.parametrize("n", [(70, 37, 19), (130, 9, 33)])
 def test_selectable_step_variants_are_bit_exact(ftc, scope, hip, oracle, fields, monkeypatch, variant, n):
     # The entry reads these switches at every call: two rows per thread, the XCD-contiguous tile order,
     # the unstaged kernel, a forced k chunk, the persistent banded walk (one resident round of
-    # workgroups), the 32 x 8 and 128 x 2 tiles, and the general kernel at equal spacings. Each must give
-    # the default kernel's (the oracle's) bits.
+    # workgroups), the 32 x 8 and 128 x 2 tiles, the 32 x 8 tile with two barriers per plane, and the
+    # general kernel at equal spacings. Each must give the default kernel's (the oracle's) bits.
     for key, value in variant.items():
         monkeypatch.setenv(key, value)
     U, V, W, S = fields.les_fields(*n)

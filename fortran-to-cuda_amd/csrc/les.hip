@@ -17,6 +17,7 @@
 // instead of once per consuming stencil point.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <cstdlib>
 
 #include "entry.hpp"
@@ -102,6 +103,49 @@ static const void *staged_kernel(int rows, int width, bool iso, bool two)
                    : reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, 128>);
     return iso ? reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1, TX, true>)
                : reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1>);
+}
+
+// The buffer-addressed step kernels (les_step_kernel), by tile shape: 512 threads as 64 x 8 columns (the default)
+// or 32 x 16 columns, four waves per SIMD, two workgroups per CU; or 256 threads as 64 x 4 columns, three
+// workgroups per CU. The four-wave shapes load W and S after the update and read the stencil operands of each
+// component update (or of U and V together) from LDS again, which holds them within 128 registers.
+struct StepShape {
+    int threads, width, per_cu;
+    const void *iso, *general;
+};
+
+static StepShape step_shape(const char *name)
+{
+#define FTC_SHAPE(NT_, WX_, WPE_, FP_, SP_, PER_CU_)                                                        \
+    StepShape{NT_, WX_, PER_CU_, reinterpret_cast<const void *>(&les_step_kernel<NT_, WX_, true, WPE_, FP_, SP_>), \
+              reinterpret_cast<const void *>(&les_step_kernel<NT_, WX_, false, WPE_, FP_, SP_>)}
+    if (name && std::strcmp(name, "32x16") == 0) return FTC_SHAPE(512, 32, 4, 3, 2, 2);
+    if (name && std::strcmp(name, "64x4") == 0) return FTC_SHAPE(256, 64, 3, 0, 0, 3);
+    return FTC_SHAPE(512, 64, 4, 3, 1, 2);
+#undef FTC_SHAPE
+}
+
+// k chunks for the step kernel over `tiles` column tiles with `slots` workgroups resident at once: every chunk
+// but the last is a multiple of six planes (its k loop is unrolled by six from k = 2). Takes the chunk count
+// whose launch needs the fewest rounds of resident workgroups times the planes a workgroup then marches (its
+// chunk plus about three planes of prologue). Returns the chunk length and sets the number of chunks.
+static int step_chunks(long long tiles, int nz, long long slots, int &chunks)
+{
+    int best_k = nz, best_chunks = 1;
+    long long best_cost = -1;
+    for (int want = 1; want <= std::max(1, nz / 6); ++want) {
+        const int k = std::max(6, ((nz + want - 1) / want + 5) / 6 * 6);
+        const int n = (nz + k - 1) / k;
+        const long long rounds = (tiles * n + slots - 1) / slots;
+        const long long cost = rounds * (k + 3);
+        if (best_cost < 0 || cost < best_cost) best_cost = cost, best_k = k, best_chunks = n;
+    }
+    if (const char *v = std::getenv("FTC_LES_KCHUNK")) {
+        best_k = std::max(6, (std::max(1, std::atoi(v)) + 5) / 6 * 6);
+        best_chunks = (nz + best_k - 1) / best_k;
+    }
+    chunks = best_chunks;
+    return best_k;
 }
 
 static Sec3 box(std::size_t i0, std::size_t i1, std::size_t j0, std::size_t j1, std::size_t k0, std::size_t k1)
@@ -277,7 +321,24 @@ static int les_entry(bool step, fort_scope_t ctx, int mode, fort_buffer_t ou, fo
         const unsigned resident = (unsigned)compute_units() * per_cu / kXcds * kXcds;
         grid = dim3(std::max(kXcds, std::min(resident, (tiles + kXcds - 1) / kXcds * kXcds)), 1, 1);
     }
-    const dim3 block = staged ? dim3(width, NTH / width, 1) : dim3(TX, TY, 1);
+    dim3 block = staged ? dim3(width, NTH / width, 1) : dim3(TX, TY, 1);
+    // The step with Smagorinsky diffusion runs on the buffer-addressed kernel when its arrays are below 2^31
+    // bytes and no switch of the les_sgs3 kernels is set (FTC_LES_LEGACY selects those outright);
+    // FTC_LES_SHAPE picks its tile shape. Tiles go to the XCDs in contiguous runs, so a resident round of
+    // workgroups on one XCD covers a band of whole tile rows.
+    const bool legacy_switch = std::getenv("FTC_LES_LEGACY") || rows != 1 || width != TX || persistent || two ||
+                               std::getenv("FTC_LES_XCD_ORDER");
+    if (staged && step && !legacy_switch && (long long)a.sk * 8 * (nz + 2) < (1LL << 31)) {
+        const StepShape shape = step_shape(std::getenv("FTC_LES_SHAPE"));
+        const int tyt = shape.threads / shape.width;
+        const long long tiles = (long long)((nx + shape.width - 1) / shape.width) * ((ny + tyt - 1) / tyt);
+        int chunks = 1;
+        a.kchunk = step_chunks(tiles, nz, (long long)compute_units() * shape.per_cu, chunks);
+        a.tile_order = 1;
+        grid = dim3((nx + shape.width - 1) / shape.width, (ny + tyt - 1) / tyt, chunks);
+        block = dim3(shape.threads, 1, 1);
+        fn = iso ? shape.iso : shape.general;
+    }
     return enqueue_kernel(ctx, key, fn, grid, block, 0, &a, sizeof a, b);
 }
 

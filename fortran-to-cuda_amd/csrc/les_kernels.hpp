@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "xcd.hpp"
 
 #pragma clang fp contract(off)
@@ -806,6 +808,493 @@ __global__ __launch_bounds__(NTH, (sgs3_occupancy<ROWS, WX, TWO>())) void les_sg
     const unsigned gx = gridDim.x, gy = gridDim.y;
     const unsigned t = launch_tile(a.tile_order);
     les_sgs3_body<STEP, ROWS, WX, ISO, TWO>(a, L, t % gx, (t / gx) % gy, t / (gx * gy));
+}
+
+// ------------------------------------------------------------------------------------------------ buffer-addressed step
+// The same step with every global access made through buffer resources: a uniform per-plane byte offset in a
+// scalar register and a k-invariant 32-bit byte offset per thread, so no 64-bit address is rebuilt per load
+// or store. The block is one-dimensional (NT threads forming WX x NT/WX columns) and the launch bounds ask
+// for WPE waves per SIMD. Arrays must be below 2^31 bytes (checked by the entry). The arithmetic, its order
+// and the values read are those of les_sgs3_body.
+typedef unsigned int ftc_u32x2 __attribute__((ext_vector_type(2)));
+template <int N>
+using IC = std::integral_constant<int, N>;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const double *p, int bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(p), 0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ double buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// Nontemporal store (the outputs are not read again by the launch).
+__device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, int voff, int soff, double v)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ftc_u32x2, v), r, voff, soff, 2);
+}
+
+template <int NT, int WX>
+struct StepTile {
+    static constexpr int TYT = NT / WX;
+    static constexpr int VXT = WX + 4, VYT = TYT + 4, CXT = WX + 2, CYT = TYT + 2;
+    static constexpr int VC = VXT * VYT, CC = CXT * CYT;
+    static constexpr int VR = (VC + NT - 1) / NT, CR = (CC + NT - 1) / NT;
+};
+
+template <int NT, int WX>
+struct StepLds {
+    using T = StepTile<NT, WX>;
+    double uv[3][2][T::VYT][T::VXT];  // U, V; slot = plane % 3
+    double w[2][T::VYT][T::VXT];      // W; slot = plane & 1
+    double sca[2][T::CYT][T::CXT];    // slot = plane & 1
+    double nu[2][T::CYT][T::CXT];     // slot = plane & 1
+};
+
+// FP: where the global loads of the next planes are issued in a k-step: 0 before the viscosity (they are then
+// in flight through the whole step), 1 between the U/V and the W/S updates, 2 after the update, 3 U and V before
+// the viscosity and W and S after the update.
+// The stencil operands of the update of plane k (slots s0, sp of U/V, c0, cp of W, S and nu), read from LDS.
+#define LES_STEP_OPERANDS                                                                                    \
+    const double u0 = VU(s0, 0, 0), uxp = VU(s0, 1, 0), uxm = VU(s0, -1, 0), uyp = VU(s0, 0, 1);             \
+    const double uym = VU(s0, 0, -1), uzp = VU(sp, 0, 0), uzm = zc.u;                                        \
+    const double v0 = VV(s0, 0, 0), vxp = VV(s0, 1, 0), vxm = VV(s0, -1, 0), vyp = VV(s0, 0, 1);             \
+    const double vym = VV(s0, 0, -1), vzp = VV(sp, 0, 0), vzm = zc.v;                                        \
+    const double w0 = VW(c0, 0, 0), wxp = VW(c0, 1, 0), wxm = VW(c0, -1, 0), wyp = VW(c0, 0, 1);             \
+    const double wym = VW(c0, 0, -1), wzp = VW(cp, 0, 0), wzm = zc.w;                                        \
+    const double n000 = NV(c0, 0, 0), n100 = NV(c0, 1, 0), nm00 = NV(c0, -1, 0);                             \
+    const double n010 = NV(c0, 0, 1), n0m0 = NV(c0, 0, -1), n001 = NV(cp, 0, 0);                             \
+    (void)u0, (void)uxp, (void)uxm, (void)uyp, (void)uym, (void)uzp, (void)uzm, (void)v0, (void)vxp, (void)vxm, \
+        (void)vyp, (void)vym, (void)vzp, (void)vzm, (void)w0, (void)wxp, (void)wxm, (void)wyp, (void)wym,   \
+        (void)wzp, (void)wzm, (void)n000, (void)n100, (void)nm00, (void)n010, (void)n0m0, (void)n001
+// SPLIT: the compiler may keep no operand across the component updates, so each group reads its own from LDS
+// again; that holds fewer registers at a time (more LDS reads). SPLIT 1: U | V | W | S | carry; SPLIT 2:
+// U and V | W | S and carry. Block b is 0 (U), 1 (V), 2 (W), 3 (S) or 4 (the carried values).
+#define LES_STEP_FENCE(b)                                                                       \
+    if constexpr ((SPLIT == 1 && (b) > 0) || (SPLIT == 2 && ((b) == 2 || (b) == 3)))          \
+    asm volatile("" ::: "memory")
+
+template <int NT, int WX, bool ISO, int FP, int SPLIT>
+__device__ __forceinline__ void les_step_body(const LesArgs &a, StepLds<NT, WX> &L, int bx, int by, int k0, int k1)
+{
+    using T = StepTile<NT, WX>;
+    constexpr int VR = T::VR, CR = T::CR, VX = T::VXT, CX = T::CXT, VC = T::VC, CC = T::CC;
+    const int tid = threadIdx.x;
+    const int tx = tid % WX, ty = tid / WX;
+    const int i0 = 2 + bx * WX, j0 = 2 + by * T::TYT;
+    const int i = i0 + tx, j = j0 + ty;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int sj = (int)a.sj;
+    const int skb = (int)(a.sk * 8);
+    const int bytes = (int)(a.sk * 8 * (nz + 2));
+    const __amdgpu_buffer_rsrc_t rU = field_rsrc(a.U, bytes), rV = field_rsrc(a.V, bytes);
+    const __amdgpu_buffer_rsrc_t rW = field_rsrc(a.W, bytes), rS = field_rsrc(a.S, bytes);
+
+    // Byte offsets within a plane of the staged cells this thread loads (periodic images of the ring), and the
+    // velocity-tile index of each viscosity ring cell it computes.
+    int voff[VR], coff[CR], nvi[CR];
+#pragma unroll
+    for (int r = 0; r < VR; ++r) {
+        const int c = tid + r * NT;
+        const int cc = c < VC ? c : 0;
+        const int ii = periodic_image(i0 - 2 + cc % VX, nx), jj = periodic_image(j0 - 2 + cc / VX, ny);
+        voff[r] = ((ii - 1) + (jj - 1) * sj) * 8;
+    }
+#pragma unroll
+    for (int r = 0; r < CR; ++r) {
+        const int c = tid + r * NT;
+        const int cc = c < CC ? c : 0;
+        const int li = cc % CX, lj = cc / CX;
+        const int ii = periodic_image(i0 - 1 + li, nx), jj = periodic_image(j0 - 1 + lj, ny);
+        coff[r] = ((ii - 1) + (jj - 1) * sj) * 8;
+        nvi[r] = (lj + 1) * VX + li + 1;
+    }
+    const double c25x = a.c25[0], c25y = ISO ? c25x : a.c25[1], c25z = ISO ? c25x : a.c25[2];
+    const double c50x = a.c50[0], c50y = ISO ? c50x : a.c50[1], c50z = ISO ? c50x : a.c50[2];
+    const double c125x = a.c125[0], c125y = ISO ? c125x : a.c125[1], c125z = ISO ? c125x : a.c125[2];
+    const double ix = a.sgs[0], iy = ISO ? ix : a.sgs[1], iz = ISO ? ix : a.sgs[2];  // 1/d
+    const double hx = a.sgs[6], hy = ISO ? hx : a.sgs[7], hz = ISO ? hx : a.sgs[8];  // 1/d^2
+    const double cs2 = a.sgs[9];                                                    // (Cs Delta)^2
+    double rv[VR][2], rw[VR], rs[CR];
+#define VLIVE(r) ((r) < VR - 1 || tid + (r) * NT < VC)
+#define CLIVE(r) ((r) < CR - 1 || tid + (r) * NT < CC)
+    auto fetch_uv = [&](int kp) {
+        if (kp < 1 || kp > nz + 2) return;
+        const int so = (kp - 1) * skb;
+#pragma unroll
+        for (int r = 0; r < VR; ++r)
+            if (VLIVE(r)) {
+                rv[r][0] = buffer_load(rU, voff[r], so);
+                rv[r][1] = buffer_load(rV, voff[r], so);
+            }
+    };
+    auto put_uv = [&](int kp, int slot) {
+        if (kp < 1 || kp > nz + 2) return;
+#pragma unroll
+        for (int r = 0; r < VR; ++r)
+            if (VLIVE(r)) {
+                (&L.uv[slot][0][0][0])[tid + r * NT] = rv[r][0];
+                (&L.uv[slot][1][0][0])[tid + r * NT] = rv[r][1];
+            }
+    };
+    // W planes up to nz+1 (see les_sgs3_body).
+    auto fetch_w = [&](int kp) {
+        if (kp < 1 || kp > nz + 1) return;
+        const int so = (kp - 1) * skb;
+#pragma unroll
+        for (int r = 0; r < VR; ++r)
+            if (VLIVE(r)) rw[r] = buffer_load(rW, voff[r], so);
+    };
+    auto put_w = [&](int kp, int slot) {
+        if (kp < 1 || kp > nz + 1) return;
+#pragma unroll
+        for (int r = 0; r < VR; ++r)
+            if (VLIVE(r)) (&L.w[slot][0][0])[tid + r * NT] = rw[r];
+    };
+    auto fetch_sca = [&](int kp) {
+        if (kp < 1 || kp > nz + 2) return;
+        const int so = (kp - 1) * skb;
+#pragma unroll
+        for (int r = 0; r < CR; ++r)
+            if (CLIVE(r)) rs[r] = buffer_load(rS, coff[r], so);
+    };
+    auto put_sca = [&](int kp, int slot) {
+        if (kp < 1 || kp > nz + 2) return;
+#pragma unroll
+        for (int r = 0; r < CR; ++r)
+            if (CLIVE(r)) (&L.sca[slot][0][0])[tid + r * NT] = rs[r];
+    };
+    auto fetch_all = [&](int k) {
+        fetch_uv(k + 3);
+        fetch_w(k + 2);
+        fetch_sca(k + 2);
+    };
+    // nu of one plane over the tile and its ring, from the U/V slots of that plane (s0) and the planes below
+    // (sm) and above (sp), the W slots of the plane (w0s) and the one below (wms), into nu slot `oslot`.
+    auto nu_slots = [&](int s0, int sm, int sp, int w0s, int wms, int oslot) __attribute__((always_inline)) {
+        double *nuo = &L.nu[oslot][0][0];
+#pragma unroll
+        for (int r = 0; r < CR; ++r) {
+            if (!CLIVE(r)) continue;
+            const int vi = nvi[r];
+#define QU(s, di, dj) (&L.uv[s][0][0][0])[vi + (dj) * VX + (di)]
+#define QV(s, di, dj) (&L.uv[s][1][0][0])[vi + (dj) * VX + (di)]
+#define QW(s, di, dj) (&L.w[s][0][0])[vi + (dj) * VX + (di)]
+            double s11 = (QU(s0, 0, 0) - QU(s0, -1, 0)) * ix;
+            double s22 = (QV(s0, 0, 0) - QV(s0, 0, -1)) * iy;
+            double s33 = (QW(w0s, 0, 0) - QW(wms, 0, 0)) * iz;
+            double dudy = ((QU(s0, 0, 1) + QU(s0, -1, 1)) - (QU(s0, 0, -1) + QU(s0, -1, -1))) * c25y;
+            double dvdx = ((QV(s0, 1, 0) + QV(s0, 1, -1)) - (QV(s0, -1, 0) + QV(s0, -1, -1))) * c25x;
+            double dudz = ((QU(sp, 0, 0) + QU(sp, -1, 0)) - (QU(sm, 0, 0) + QU(sm, -1, 0))) * c25z;
+            double dwdx = ((QW(w0s, 1, 0) + QW(wms, 1, 0)) - (QW(w0s, -1, 0) + QW(wms, -1, 0))) * c25x;
+            double dvdz = ((QV(sp, 0, 0) + QV(sp, 0, -1)) - (QV(sm, 0, 0) + QV(sm, 0, -1))) * c25z;
+            double dwdy = ((QW(w0s, 0, 1) + QW(wms, 0, 1)) - (QW(w0s, 0, -1) + QW(wms, 0, -1))) * c25y;
+#undef QU
+#undef QV
+#undef QW
+            double s12 = 0.5 * (dudy + dvdx);
+            double s13 = 0.5 * (dudz + dwdx);
+            double s23 = 0.5 * (dvdz + dwdy);
+            double ss = 2.0 * ((s11 * s11 + s22 * s22) + s33 * s33) + 4.0 * ((s12 * s12 + s13 * s13) + s23 * s23);
+            nuo[tid + r * NT] = cs2 * sqrt(ss);
+        }
+    };
+    // nu at plane qk, clamped to the interior planes (zero gradient in z).
+    auto nu_plane = [&](int qk) {
+        const int kk = qk < 2 ? 2 : (qk > nz + 1 ? nz + 1 : qk);
+        nu_slots(kk % 3, (kk - 1) % 3, (kk + 1) % 3, kk & 1, (kk - 1) & 1, qk & 1);
+    };
+
+    const double dt = a.dt;
+    const double Cz = c50z;
+    const int vt = (ty + 2) * VX + tx + 2, ct = (ty + 1) * CX + tx + 1;
+#define VU(s, di, dj) (&L.uv[s][0][0][0])[vt + (dj) * VX + (di)]
+#define VV(s, di, dj) (&L.uv[s][1][0][0])[vt + (dj) * VX + (di)]
+#define VW(s, di, dj) (&L.w[s][0][0])[vt + (dj) * VX + (di)]
+#define SS(s, di, dj) (&L.sca[s][0][0])[ct + (dj) * CX + (di)]
+#define NV(s, di, dj) (&L.nu[s][0][0])[ct + (dj) * CX + (di)]
+
+    // Prologue as in les_sgs3_body.
+    for (int kp = k0 - 2; kp <= k0; ++kp) {
+        fetch_uv(kp);
+        put_uv(kp, kp % 3);
+    }
+    for (int kp = k0 - 2; kp <= k0 - 1; ++kp) {
+        fetch_w(kp);
+        put_w(kp, kp & 1);
+    }
+    for (int kp = k0 - 1; kp <= k0; ++kp) {
+        fetch_sca(kp);
+        put_sca(kp, kp & 1);
+    }
+    fetch_uv(k0 + 1);
+    fetch_w(k0);
+    const bool clamped = k0 - 1 < 2;
+    __syncthreads();
+    if (!clamped) nu_plane(k0 - 1);
+    __syncthreads();
+    put_uv(k0 + 1, (k0 + 1) % 3);
+    put_w(k0, k0 & 1);
+    __syncthreads();
+    if (clamped) nu_plane(k0 - 1);
+    nu_plane(k0);
+    __syncthreads();
+    const bool inside = i <= nx + 1 && j <= ny + 1;
+    ZCarry zc = {};
+    if (inside) {
+        const int kv = k0 - 1;
+        const int s0 = kv % 3, sp = (kv + 1) % 3, c0 = kv & 1, cp = (kv + 1) & 1;
+        const double u0 = VU(s0, 0, 0), uzp = VU(sp, 0, 0);
+        const double v0 = VV(s0, 0, 0), vzp = VV(sp, 0, 0);
+        const double w0 = VW(c0, 0, 0), wxp = VW(c0, 1, 0), wyp = VW(c0, 0, 1), wzp = VW(cp, 0, 0);
+        const double n000 = NV(c0, 0, 0), n100 = NV(c0, 1, 0), n010 = NV(c0, 0, 1), n001 = NV(cp, 0, 0);
+        zc.cu = c25z * (uzp + u0) * (wxp + w0);
+        zc.du = (0.25 * (((n000 + n100) + n001) + NV(cp, 1, 0))) * (uzp - u0);
+        zc.cv = c25z * (vzp + v0) * (wyp + w0);
+        zc.dv = (0.25 * (((n000 + n010) + n001) + NV(cp, 0, 1))) * (vzp - v0);
+        zc.cw = c25z * (wzp + w0) * (wzp + w0);
+        zc.dw = n001 * (wzp - w0);
+        const double s0v = SS(c0, 0, 0), szp = SS(cp, 0, 0);
+        zc.cs = Cz * w0 * (szp + s0v);
+        zc.ds = (0.5 * (n000 + n001)) * (szp - s0v);
+        zc.u = u0;
+        zc.v = v0;
+        zc.w = w0;
+        zc.wxp = wxp;
+        zc.wyp = wyp;
+    }
+    fetch_uv(k0 + 2);
+    fetch_w(k0 + 1);
+    fetch_sca(k0 + 1);
+    __syncthreads();
+
+    const __amdgpu_buffer_rsrc_t rOU = field_rsrc(a.OU, bytes), rOV = field_rsrc(a.OV, bytes);
+    const __amdgpu_buffer_rsrc_t rOW = field_rsrc(a.OW, bytes), rOS = field_rsrc(a.OS, bytes);
+    const int vo = ((i - 1) + (j - 1) * sj) * 8;
+    const bool edge = i == 2 || i == nx + 1 || j == 2 || j == ny + 1;
+    // One k-step for a plane k with k % 6 == PH: every LDS slot index is a constant, so each LDS access is a
+    // fixed offset from the thread's base address.
+    auto step = [&](int k, auto phase) __attribute__((always_inline)) {
+        constexpr int PH = decltype(phase)::value;
+        put_uv(k + 2, (PH + 2) % 3);
+        put_w(k + 1, (PH + 1) & 1);
+        put_sca(k + 1, (PH + 1) & 1);
+        if constexpr (FP == 0) fetch_all(k);
+        if constexpr (FP == 3) fetch_uv(k + 3);
+        __syncthreads();
+        if (k + 1 <= nz + 1) {
+            nu_slots((PH + 1) % 3, PH % 3, (PH + 2) % 3, (PH + 1) & 1, PH & 1, (PH + 1) & 1);
+        } else {
+            // Plane nz+2 takes the viscosity of plane nz+1 (zero gradient), computed by the previous step.
+#pragma unroll
+            for (int r = 0; r < CR; ++r)
+                if (CLIVE(r)) (&L.nu[(PH + 1) & 1][0][0])[tid + r * NT] = (&L.nu[PH & 1][0][0])[tid + r * NT];
+        }
+        __syncthreads();
+        const int so = (k - 1) * skb;
+        const bool zlo = k == 2, zhi = k == nz + 1;
+        // Writes val at the cell and at every ghost whose image it is, as put_with_images: the x and y images
+        // and their combinations, each with the z images unless zmode is 1 (negated for zmode 0).
+        auto put = [&](__amdgpu_buffer_rsrc_t F, double val, int zmode) {
+            const double zval = zmode == 0 ? -val : val;
+            const bool zi = zmode != 1;
+            buffer_store(F, vo, so, val);
+            if (zi && zlo) buffer_store(F, vo, so - skb, zval);
+            if (zi && zhi) buffer_store(F, vo, so + skb, zval);
+            if (edge) {
+                // The image offsets are formed from the plane's offset, so they stay inside the k loop instead
+                // of holding registers through every step for the few boundary columns that need them.
+                const int base = vo + so;
+                auto with_z = [&](int q) {
+                    buffer_store(F, q, 0, val);
+                    if (zi && zlo) buffer_store(F, q - skb, 0, zval);
+                    if (zi && zhi) buffer_store(F, q + skb, 0, zval);
+                };
+                const bool hxa = i == 2, hxb = i == nx + 1, hya = j == 2, hyb = j == ny + 1;
+                const int oxa = nx * 8, oya = ny * sj * 8;
+                if (hxa) with_z(base + oxa);
+                if (hxb) with_z(base - oxa);
+                if (hya) with_z(base + oya);
+                if (hyb) with_z(base - oya);
+                if (hxa && hya) with_z(base + oxa + oya);
+                if (hxa && hyb) with_z(base + oxa - oya);
+                if (hxb && hya) with_z(base - oxa + oya);
+                if (hxb && hyb) with_z(base - oxa - oya);
+            }
+        };
+        if (inside) {
+            constexpr int s0 = PH % 3, sp = (PH + 1) % 3;
+            constexpr int c0 = PH & 1, cp = (PH + 1) & 1;
+            ZCarry up;
+            // U: CDU (cdu.f90:48-107) then the diffusion of u.
+            {
+                LES_STEP_FENCE(0);
+                LES_STEP_OPERANDS;
+                const double vxpym = VV(s0, 1, -1), wxpzm = zc.wxp;
+                double x = c25x * (uxp + u0) * (uxp + u0) - c25x * (u0 + uxm) * (u0 + uxm);
+                double y = c25y * (uyp + u0) * (vxp + v0) - c25y * (u0 + uym) * (vxpym + vym);
+                up.cu = c25z * (uzp + u0) * (wxp + w0);
+                double z = up.cu - zc.cu;
+                double t = -((x + y) + z);
+                double Vadv = ((v0 + vxp) + vym) + vxpym;
+                double Wadv = ((w0 + wxp) + wzm) + wxpzm;
+                t = t - ((c50x * (uxp - uxm) * u0 + c125y * (uyp - uym) * Vadv) + c125z * (uzp - uzm) * Wadv);
+                t = t * 0.5;
+                double fxp = n100 * (uxp - u0);
+                double fxm = n000 * (u0 - uxm);
+                double exy0 = 0.25 * (((n000 + n100) + n010) + NV(c0, 1, 1));
+                double exym = 0.25 * (((n0m0 + NV(c0, 1, -1)) + n000) + n100);
+                double fyp = exy0 * (uyp - u0);
+                double fym = exym * (u0 - uym);
+                double exz0 = 0.25 * (((n000 + n100) + n001) + NV(cp, 1, 0));
+                up.du = exz0 * (uzp - u0);
+                double fzp = up.du;
+                double fzm = zc.du;
+                t = t + (((fxp - fxm) * hx + (fyp - fym) * hy) + (fzp - fzm) * hz);
+                put(rOU, u0 + dt * t, 0);
+            }
+            // V: CDV (cvd.f90:44-98) then the diffusion of v.
+            {
+                LES_STEP_FENCE(1);
+                LES_STEP_OPERANDS;
+                const double uxmyp = VU(s0, -1, 1), wypzm = zc.wyp;
+                double y = c25y * (vyp + v0) * (vyp + v0) - c25y * (v0 + vym) * (v0 + vym);
+                double x = c25x * (vxp + v0) * (uyp + u0) - c25x * (v0 + vxm) * (uxmyp + uxm);
+                up.cv = c25z * (vzp + v0) * (wyp + w0);
+                double z = up.cv - zc.cv;
+                double t = -((y + x) + z);
+                double Uadv = ((u0 + uyp) + uxm) + uxmyp;
+                double Wadv = ((w0 + wyp) + wzm) + wypzm;
+                t = t - ((c125x * (vxp - vxm) * Uadv + c50y * (vyp - vym) * v0) + c125z * (vzp - vzm) * Wadv);
+                t = t * 0.5;
+                double exy0 = 0.25 * (((n000 + n100) + n010) + NV(c0, 1, 1));
+                double exyxm = 0.25 * (((nm00 + n000) + NV(c0, -1, 1)) + n010);
+                double fxp = exy0 * (vxp - v0);
+                double fxm = exyxm * (v0 - vxm);
+                double fyp = n010 * (vyp - v0);
+                double fym = n000 * (v0 - vym);
+                double eyz0 = 0.25 * (((n000 + n010) + n001) + NV(cp, 0, 1));
+                up.dv = eyz0 * (vzp - v0);
+                double fzp = up.dv;
+                double fzm = zc.dv;
+                t = t + (((fxp - fxm) * hx + (fyp - fym) * hy) + (fzp - fzm) * hz);
+                put(rOV, v0 + dt * t, 0);
+            }
+            if constexpr (FP == 1) fetch_all(k);
+            // W on the interior faces: CDW (cdw.f90:44-98) then the diffusion of w.
+            up.cw = zc.cw;
+            up.dw = zc.dw;
+            if (k <= a.kw) {
+                LES_STEP_FENCE(2);
+                LES_STEP_OPERANDS;
+                const double uxmzp = VU(sp, -1, 0), vymzp = VV(sp, 0, -1);
+                up.cw = c25z * (wzp + w0) * (wzp + w0);
+                double z = up.cw - zc.cw;
+                double y = c25y * (wyp + w0) * (vzp + v0) - c25y * (w0 + wym) * (vym + vymzp);
+                double x = c25x * (wxp + w0) * (uzp + u0) - c25x * (w0 + wxm) * (uxmzp + uxm);
+                double t = -((z + y) + x);
+                double Uadv = ((u0 + uzp) + uxm) + uxmzp;
+                double Vadv = ((v0 + vzp) + vym) + vymzp;
+                t = t - ((c125x * (wxp - wxm) * Uadv + c125y * (wyp - wym) * Vadv) + c50z * (wzp - wzm) * w0);
+                t = t * 0.5;
+                double exz0 = 0.25 * (((n000 + n100) + n001) + NV(cp, 1, 0));
+                double exzxm = 0.25 * (((nm00 + n000) + NV(cp, -1, 0)) + n001);
+                double fxp = exz0 * (wxp - w0);
+                double fxm = exzxm * (w0 - wxm);
+                double eyz0 = 0.25 * (((n000 + n010) + n001) + NV(cp, 0, 1));
+                double eyzym = 0.25 * (((n0m0 + n000) + NV(cp, 0, -1)) + n001);
+                double fyp = eyz0 * (wyp - w0);
+                double fym = eyzym * (w0 - wym);
+                up.dw = n001 * (wzp - w0);
+                double fzp = up.dw;
+                double fzm = zc.dw;
+                t = t + (((fxp - fxm) * hx + (fyp - fym) * hy) + (fzp - fzm) * hz);
+                put(rOW, w0 + dt * t, 1);
+            }
+            // S: flux-form advection then the diffusion of s.
+            {
+                LES_STEP_FENCE(3);
+                LES_STEP_OPERANDS;
+                const double s0v = SS(c0, 0, 0), sxp = SS(c0, 1, 0), sxm = SS(c0, -1, 0), syp = SS(c0, 0, 1);
+                const double sym = SS(c0, 0, -1), szp = SS(cp, 0, 0);
+                const double Cx = c50x, Cy = c50y;
+                double x = Cx * u0 * (sxp + s0v) - Cx * uxm * (s0v + sxm);
+                double y = Cy * v0 * (syp + s0v) - Cy * vym * (s0v + sym);
+                up.cs = Cz * w0 * (szp + s0v);
+                double z = up.cs - zc.cs;
+                double t = -((x + y) + z);
+                double fxp = (0.5 * (n000 + n100)) * (sxp - s0v);
+                double fxm = (0.5 * (nm00 + n000)) * (s0v - sxm);
+                double fyp = (0.5 * (n000 + n010)) * (syp - s0v);
+                double fym = (0.5 * (n0m0 + n000)) * (s0v - sym);
+                up.ds = (0.5 * (n000 + n001)) * (szp - s0v);
+                double fzp = up.ds;
+                double fzm = zc.ds;
+                t = t + (((fxp - fxm) * hx + (fyp - fym) * hy) + (fzp - fzm) * hz);
+                put(rOS, s0v + dt * t, 2);
+            }
+            {
+                LES_STEP_FENCE(4);
+                LES_STEP_OPERANDS;
+                up.u = u0;
+                up.v = v0;
+                up.w = w0;
+                up.wxp = wxp;
+                up.wyp = wyp;
+            }
+            zc = up;
+        } else if constexpr (FP == 1) {
+            fetch_all(k);
+        }
+        if constexpr (FP == 2) fetch_all(k);
+        if constexpr (FP == 3) {
+            fetch_w(k + 2);
+            fetch_sca(k + 2);
+        }
+        __syncthreads();
+    };
+    // k0 % 6 == 2: every chunk but the last is a multiple of six planes (the entry chooses them so).
+    for (int k = k0; k <= k1; k += 6) {
+        step(k, IC<2>{});
+        if (k + 1 > k1) break;
+        step(k + 1, IC<3>{});
+        if (k + 2 > k1) break;
+        step(k + 2, IC<4>{});
+        if (k + 3 > k1) break;
+        step(k + 3, IC<5>{});
+        if (k + 4 > k1) break;
+        step(k + 4, IC<0>{});
+        if (k + 5 > k1) break;
+        step(k + 5, IC<1>{});
+    }
+#undef VU
+#undef VV
+#undef VW
+#undef SS
+#undef NV
+#undef VLIVE
+#undef CLIVE
+#undef LES_STEP_OPERANDS
+#undef LES_STEP_FENCE
+}
+
+// One tile of NT / WX rows by WX columns per workgroup, k chunks of a.kchunk planes, tiles x fastest, then y,
+// then the chunk, in the order a.tile_order names (launch order or XCD-contiguous).
+template <int NT, int WX, bool ISO, int WPE, int FP, int SPLIT>
+__global__ __attribute__((amdgpu_flat_work_group_size(NT, NT), amdgpu_waves_per_eu(WPE, WPE))) void
+les_step_kernel(const LesArgs a)
+{
+    __shared__ StepLds<NT, WX> L;
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned t = launch_tile(a.tile_order);
+    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int k0 = 2 + bz * a.kchunk;
+    if (k0 > a.nz + 1) return;  // uniform over the workgroup
+    // The last chunk takes the planes that remain.
+    const int k1 = bz == (int)gridDim.z - 1 ? a.nz + 1 : min(k0 + a.kchunk - 1, a.nz + 1);
+    les_step_body<NT, WX, ISO, FP, SPLIT>(a, L, bx, by, k0, k1);
 }
 
 }  // namespace ftc

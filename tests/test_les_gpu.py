@@ -152,3 +152,55 @@ def test_selectable_step_variants_are_bit_exact(ftc, scope, hip, oracle, fields,
     for name, g, w, kz in zip("UVWS", got, want, (nz, nz, nz - 1, nz)):
         sl = (slice(1, nx + 1), slice(1, ny + 1), slice(1, kz + 1))
         assert bits_equal(g[sl], w[sl]), f"{name} {variant}: " + describe_mismatch(g[sl], w[sl])
+
+
+def les_steps(ftc, scope, hip, fields, n, steps, spacing=(10.0, 10.0, 10.0), dt=0.2, cs=0.17):
+    """`steps` LES steps through the step entry from the seeded fields; returns the device's final fields."""
+    nx, ny, nz = n
+    U, V, W, S = fields.les_fields(nx, ny, nz, *spacing)
+    a = [U.copy(order="F"), V.copy(order="F"), W.copy(order="F"), S.copy(order="F")]
+    b = [np.zeros_like(U, order="F") for _ in range(4)]
+    with scope.Domain(hip) as dom:
+        ha = [dom.bind(x) for x in a]
+        hb = [dom.bind(x, initialized=True) for x in b]
+        for step in range(steps):
+            src, dst = (ha, hb) if step % 2 == 0 else (hb, ha)
+            st = hip.cpp_fort_shared_les_step(dom.ctx, 1, *dst, *src, *(ftc.dp(d) for d in spacing), ftc.dp(dt),
+                                              ftc.dp(cs), ftc.ip(nx), ftc.ip(ny), ftc.ip(nz), ftc.ip(1))
+            scope.check(hip, st)
+            dom.flush()
+        final = hb if steps % 2 == 1 else ha
+        dom.observe(*final)
+    return (b if steps % 2 == 1 else a), (U, V, W, S)
+
+
+@pytest.mark.parametrize("variant", [{}, {"FTC_LES_SHAPE": "32x16"}, {"FTC_LES_SHAPE": "64x4"},
+                                     {"FTC_LES_KCHUNK": "6"}, {"FTC_LES_SHAPE": "32x16", "FTC_LES_KCHUNK": "12"},
+                                     {"FTC_LES_SHAPE": "64x4", "FTC_LES_KCHUNK": "18"}, {"FTC_LES_LEGACY": "1"},
+                                     {"FTC_LES_ANISO": "1"}, {"FTC_LES_ANISO": "1", "FTC_LES_SHAPE": "32x16"}])
+@pytest.mark.parametrize("n", [(70, 37, 19), (130, 9, 33), (5, 3, 2), (64, 16, 13)])
+def test_step_kernel_shapes_and_chunks_are_bit_exact_including_halos(ftc, scope, hip, oracle, fields, monkeypatch,
+                                                                     variant, n):
+    # The step entry's buffer-addressed kernel in each tile shape (64 x 8, 32 x 16, 64 x 4 columns), with k
+    # chunks forced short (the last chunk taking the rest), in its general-coefficient form, and the les_sgs3
+    # kernel it replaced: every one must give the oracle's bits, ghost cells included, over three steps.
+    for key, value in variant.items():
+        monkeypatch.setenv(key, value)
+    print(f"LES step kernel {variant} {n}", flush=True)
+    got, start = les_steps(ftc, scope, hip, fields, n, 3)
+    want = list(start)
+    for _ in range(3):
+        want = oracle.les_step(*want, 10.0, 10.0, 10.0, 0.2, 0.17)
+    for name, g, w in zip("UVWS", got, want):
+        assert bits_equal(g, w), f"{name} {variant}: " + describe_mismatch(g, w)
+
+
+@pytest.mark.parametrize("spacing", [(10.0, 8.0, 12.5), (4.0, 4.0, 2.5)])
+@pytest.mark.parametrize("n", [(70, 37, 19), (33, 17, 9)])
+def test_steps_with_unequal_spacings_are_bit_exact_including_halos(ftc, scope, hip, oracle, fields, spacing, n):
+    got, start = les_steps(ftc, scope, hip, fields, n, 2, spacing)
+    want = list(start)
+    for _ in range(2):
+        want = oracle.les_step(*want, *spacing, 0.2, 0.17)
+    for name, g, w in zip("UVWS", got, want):
+        assert bits_equal(g, w), f"{name} {spacing}: " + describe_mismatch(g, w)

@@ -811,11 +811,16 @@ __global__ __launch_bounds__(NTH, (sgs3_occupancy<ROWS, WX, TWO>())) void les_sg
 }
 
 // ------------------------------------------------------------------------------------------------ buffer-addressed step
-// The same step with every global access made through buffer resources: a uniform per-plane byte offset in a
-// scalar register and a k-invariant 32-bit byte offset per thread, so no 64-bit address is rebuilt per load
-// or store. The block is one-dimensional (NT threads forming WX x NT/WX columns) and the launch bounds ask
-// for WPE waves per SIMD. Arrays must be below 2^31 bytes (checked by the entry). The arithmetic, its order
-// and the values read are those of les_sgs3_body.
+// The step kernel the step entry launches. Same staging, same values and the same arithmetic in the same order as
+// les_sgs3_body (three U/V planes, two W, scalar and viscosity planes in LDS, the z-face terms carried in
+// registers), built for four waves per SIMD:
+// - every global access goes through a buffer resource: a uniform per-plane byte offset in a scalar register and
+//   a k-invariant 32-bit byte offset per thread, so no 64-bit address is rebuilt per load or store; ghost images
+//   are offsets from the cell's, formed inside the k loop where a boundary column needs them;
+// - the k loop is unrolled by six, so every LDS slot (plane % 3, plane % 2) is a constant and each LDS access a
+//   fixed offset from the thread's base address;
+// - the block is one-dimensional: NT threads forming WX x NT/WX columns.
+// Arrays must be below 2^31 bytes (checked by the entry).
 typedef unsigned int ftc_u32x2 __attribute__((ext_vector_type(2)));
 template <int N>
 using IC = std::integral_constant<int, N>;
@@ -853,9 +858,6 @@ struct StepLds {
     double nu[2][T::CYT][T::CXT];     // slot = plane & 1
 };
 
-// FP: where the global loads of the next planes are issued in a k-step: 0 before the viscosity (they are then
-// in flight through the whole step), 1 between the U/V and the W/S updates, 2 after the update, 3 U and V before
-// the viscosity and W and S after the update.
 // The stencil operands of the update of plane k (slots s0, sp of U/V, c0, cp of W, S and nu), read from LDS.
 #define LES_STEP_OPERANDS                                                                                    \
     const double u0 = VU(s0, 0, 0), uxp = VU(s0, 1, 0), uxm = VU(s0, -1, 0), uyp = VU(s0, 0, 1);             \
@@ -876,6 +878,9 @@ struct StepLds {
     if constexpr ((SPLIT == 1 && (b) > 0) || (SPLIT == 2 && ((b) == 2 || (b) == 3)))          \
     asm volatile("" ::: "memory")
 
+// One column tile over planes k0..k1 (k0 % 6 == 2). FP: where the global loads of the next planes are issued in
+// a k-step: 0 all before the viscosity (in flight through the whole step), 3 U and V before the viscosity and W
+// and S after the update (their registers are then free during it). SPLIT: see LES_STEP_FENCE.
 template <int NT, int WX, bool ISO, int FP, int SPLIT>
 __device__ __forceinline__ void les_step_body(const LesArgs &a, StepLds<NT, WX> &L, int bx, int by, int k0, int k1)
 {
@@ -1182,7 +1187,6 @@ __device__ __forceinline__ void les_step_body(const LesArgs &a, StepLds<NT, WX> 
                 t = t + (((fxp - fxm) * hx + (fyp - fym) * hy) + (fzp - fzm) * hz);
                 put(rOV, v0 + dt * t, 0);
             }
-            if constexpr (FP == 1) fetch_all(k);
             // W on the interior faces: CDW (cdw.f90:44-98) then the diffusion of w.
             up.cw = zc.cw;
             up.dw = zc.dw;
@@ -1245,10 +1249,7 @@ __device__ __forceinline__ void les_step_body(const LesArgs &a, StepLds<NT, WX> 
                 up.wyp = wyp;
             }
             zc = up;
-        } else if constexpr (FP == 1) {
-            fetch_all(k);
         }
-        if constexpr (FP == 2) fetch_all(k);
         if constexpr (FP == 3) {
             fetch_w(k + 2);
             fetch_sca(k + 2);

@@ -838,7 +838,7 @@ __device__ __forceinline__ double buffer_load(__amdgpu_buffer_rsrc_t r, int voff
 // Nontemporal store (the outputs are not read again by the launch).
 __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, int voff, int soff, double v)
 {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ftc_u32x2, v), r, voff, soff, 2);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ftc_u32x2, v), r, voff, soff, 0);
 }
 
 template <int NT, int WX>

@@ -239,9 +239,25 @@ static int apply_inverse_hartley(poisson_solver_handle_v1 &s, const double *in, 
     FTC_CHECK((hartley_pass<double, double>(w0, nx, 1, plane, w1, nx, 1, plane, ny, nx, nz, 0.0, s.twy, stream)));
     const unsigned blocks = (unsigned)((plane + s.cols - 1) / s.cols);
     const std::size_t lds = 2 * (std::size_t)tridiag_chunks(nz) * s.cols * sizeof(double);
-    hipLaunchKernelGGL((tridiag_z_kernel<double, double>), dim3(blocks), dim3(s.cols), lds, stream,
-                       static_cast<const double *>(w1), TriOut<double>{w0, 0}, nx, ny, nz, s.lx, s.ly,
-                       1.0 / (s.hz * s.hz), s.cols, 0);
+    const double rz = 1.0 / (s.hz * s.hz);
+    // Columns of 64, 96 or 128 levels are solved held in registers (the input read once); others, or with
+    // FTC_TRIDIAG_CHUNKED, by the chunked sweep that re-reads each chunk for the back substitution.
+    const bool resident = (nz == 64 || nz == 96 || nz == 128) && plane * nz * 8 < (1LL << 31) &&
+                          !std::getenv("FTC_TRIDIAG_CHUNKED");
+    const unsigned columns = (unsigned)((plane + 63) / 64);
+    if (resident && nz == 128)
+        hipLaunchKernelGGL(tridiag_z_resident_kernel<128>, dim3(columns), dim3(64), 0, stream,
+                           static_cast<const double *>(w1), w0, nx, ny, s.lx, s.ly, rz);
+    else if (resident && nz == 96)
+        hipLaunchKernelGGL(tridiag_z_resident_kernel<96>, dim3(columns), dim3(64), 0, stream,
+                           static_cast<const double *>(w1), w0, nx, ny, s.lx, s.ly, rz);
+    else if (resident)
+        hipLaunchKernelGGL(tridiag_z_resident_kernel<64>, dim3(columns), dim3(64), 0, stream,
+                           static_cast<const double *>(w1), w0, nx, ny, s.lx, s.ly, rz);
+    else
+        hipLaunchKernelGGL((tridiag_z_kernel<double, double>), dim3(blocks), dim3(s.cols), lds, stream,
+                           static_cast<const double *>(w1), TriOut<double>{w0, 0}, nx, ny, nz, s.lx, s.ly, rz, s.cols,
+                           0);
     if (hipGetLastError() != hipSuccess) return fort_scope_report_error(FORT_SCOPE_EXECUTION, "tridiagonal launch failed");
     FTC_CHECK((hartley_pass<double, double>(w0, nx, 1, plane, w1, nx, 1, plane, ny, nx, nz, 0.0, s.twy, stream)));
     return hartley_pass<double, double>(w1, 1, nx, plane, out.x, 1, out.ld1, out.ld2, nx, ny, nz, beta, s.twx, stream);

@@ -345,4 +345,80 @@ __global__ __launch_bounds__(64, 3) void tridiag_z_kernel(const T *__restrict__ 
     tridiag_column_prefetch<T, TOUT>(in, out, plane, nz, lam, rz, lds, c, cols, col);
 }
 
+// The FP64 z solve with the whole column held in registers, so its input is read once: one wave per 64 columns
+// and one wave per SIMD (up to 512 registers). The forward sweep keeps d' in place of the input and the c' value
+// entering each chunk; the back substitution recomputes a chunk's c' from its checkpoint (c' depends on the
+// eigenvalue and k only) with the same operations, so every value is the one tridiag_column computes, in the
+// same order. Columns of exactly NZ points (a constant, so no level needs a bound check); arrays below 2^31
+// bytes (the caller checks both), addressed through buffer resources so that the level offsets stay scalar. The
+// null mode (lambda == 0) takes tridiag_null_mode.
+template <int NZ>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(1, 1))) void
+tridiag_z_resident_kernel(const double *__restrict__ in, double *__restrict__ out, int nx, int ny,
+                          const double *__restrict__ lx, const double *__restrict__ ly, double rz)
+{
+    static_assert(NZ % kTriChunk == 0, "whole chunks");
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const long long col = (long long)blockIdx.x * 64 + threadIdx.x;
+    const long long plane = (long long)nx * ny;
+    if (col >= plane) return;
+    const double lam = lx[col % nx] + ly[col / nx];
+    if (lam == 0.0) {
+        tridiag_null_mode<double, double>(in + col, TriOut<double>{out + col, 0}, plane, NZ, rz);
+        return;
+    }
+    const int bytes = (int)(plane * NZ * 8), pb = (int)(plane * 8), vo = (int)(col * 8);
+    const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(in), 0, bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(out, 0, bytes, 0x00020000);
+    constexpr int CH = NZ / kTriChunk;
+    double d[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        d[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rin, vo, k * pb, 0));
+    auto diagonal = [&](int k) {
+        double diag = -2.0 * rz + lam;
+        if (k == 0) diag = diag + rz;
+        if (k == NZ - 1) diag = diag + rz;
+        return diag;
+    };
+    double entry[CH];
+    double c = 0.0, dd = 0.0;
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        entry[j] = c;
+#pragma unroll
+        for (int i = 0; i < kTriChunk; ++i) {
+            const int k = j * kTriChunk + i;
+            const double m = 1.0 / (diagonal(k) - rz * c);
+            c = rz * m;
+            dd = (d[k] - rz * dd) * m;
+            d[k] = dd;
+        }
+    }
+    double psi = 0.0;
+#pragma unroll
+    for (int j = CH - 1; j >= 0; --j) {
+        // The chunk's level offsets come from an opaque base, so they are not the load offsets kept live in
+        // scalar registers from the first loop to this one.
+        int base = j * kTriChunk * pb;
+        asm volatile("" : "+s"(base));
+        double cl[kTriChunk];
+        double cc = entry[j];
+        // Opaque too: the compiler would otherwise reuse the forward sweep's c' values and keep all of them live.
+        asm volatile("" : "+v"(cc));
+#pragma unroll
+        for (int i = 0; i < kTriChunk; ++i) {
+            const double m = 1.0 / (diagonal(j * kTriChunk + i) - rz * cc);
+            cc = rz * m;
+            cl[i] = cc;
+        }
+#pragma unroll
+        for (int i = kTriChunk - 1; i >= 0; --i) {
+            const int k = j * kTriChunk + i;
+            psi = k == NZ - 1 ? d[k] : d[k] - cl[i] * psi;
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, psi), rout, vo, base + i * pb, 0);
+        }
+    }
+}
+
 }  // namespace ftc

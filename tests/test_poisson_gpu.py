@@ -117,3 +117,17 @@ def test_fft_solve_matches_the_oracle_and_the_dense_solve(ftc, scope, hip, monke
     scale = np.abs(want).max()
     assert np.abs(fft - want).max() <= 1e-11 * scale
     assert np.abs(fft - dense).max() <= 1e-11 * scale
+
+
+@pytest.mark.parametrize("n", [(64, 32, 64), (32, 64, 96), (128, 64, 128), (64, 64, 48)])
+def test_resident_z_solve_equals_the_chunked_sweep(ftc, scope, hip, monkeypatch, n):
+    # The FP64 Hartley solve takes the z solve that holds whole columns in registers for 64, 96 or 128 levels
+    # (48 levels keep the chunked sweep on both sides); it must give the chunked sweep's bits.
+    h = (10.0, 10.0, 10.0)
+    b = poisson_oracle.divergence_rhs(*n, h, seed=sum(n) + 7)
+    resident, _ = run(ftc, scope, hip, b, h, False, max_solves=1, measure=False)
+    monkeypatch.setenv("FTC_TRIDIAG_CHUNKED", "1")
+    chunked, _ = run(ftc, scope, hip, b, h, False, max_solves=1, measure=False)
+    assert np.array_equal(resident.view(np.uint64), chunked.view(np.uint64)), np.abs(resident - chunked).max()
+    want = poisson_oracle.solve(b, h)
+    assert np.abs(resident - want).max() / np.abs(want).max() <= 1e-11

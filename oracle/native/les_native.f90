@@ -231,17 +231,18 @@ contains
   end subroutine halo_fill
 end module les_native
 
-! les_native NX NY NZ STEPS REPEATS INPUT [OUTPUT]
+! les_native NX NY NZ STEPS REPEATS INPUT [OUTPUT [FINAL]]
 ! Reads U, V, W, S (each (NX+2)(NY+2)(NZ+2) REAL64 values, Fortran order, one after another) from INPUT,
 ! dx = dy = dz = 10 m, dt = 0.2 s, Cs = 0.17. Runs one warm-up step (written to OUTPUT, if given, for the
 ! bit comparison), then REPEATS repetitions of STEPS steps, each timed with system_clock around its step
-! loop, and prints the threads and every repetition's ms per step, then the median.
+! loop, and prints the threads and every repetition's ms per step, then the median. FINAL, if given,
+! receives the fields after all 1 + REPEATS*STEPS steps.
 program les_native_bench
   use les_native
   use omp_lib, only: omp_get_max_threads
   implicit none
   integer :: nx, ny, nz, steps, repeats, r, n, unit, idx
-  character(len=512) :: arg, input, output
+  character(len=512) :: arg, input, output, final
   real(rk), allocatable, target :: f(:,:,:,:,:)
   real(rk), allocatable :: nu(:,:,:), times(:)
   real(rk) :: q(10), swap
@@ -255,7 +256,9 @@ program les_native_bench
   call get_command_argument(5, arg); read(arg, *) repeats
   call get_command_argument(6, input)
   output = ''
+  final = ''
   if (command_argument_count() >= 7) call get_command_argument(7, output)
+  if (command_argument_count() >= 8) call get_command_argument(8, final)
 
   ! f(:,:,:,field,copy): fields U, V, W, S of the current and the next state
   allocate(f(nx+2, ny+2, nz+2, 4, 2), nu(nx+2, ny+2, nz+2), times(max(repeats, 1)))
@@ -304,4 +307,9 @@ program les_native_bench
     times(idx + 1) = swap
   end do
   if (repeats > 0) write(*, '(a,f0.4)') 'median_ms_per_step ', times((repeats + 1) / 2)
+  if (len_trim(final) > 0) then
+    open(newunit=unit, file=trim(final), access='stream', form='unformatted', status='replace')
+    write(unit) f(:,:,:,:,cur)
+    close(unit)
+  end if
 end program les_native_bench

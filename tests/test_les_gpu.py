@@ -204,3 +204,29 @@ def test_steps_with_unequal_spacings_are_bit_exact_including_halos(ftc, scope, h
         want = oracle.les_step(*want, *spacing, 0.2, 0.17)
     for name, g, w in zip("UVWS", got, want):
         assert bits_equal(g, w), f"{name} {spacing}: " + describe_mismatch(g, w)
+
+
+def test_hundred_resident_steps_at_full_size_equal_the_native_build(ftc, scope, hip, fields, tmp_path):
+    # 512 x 512 x 128 fields resident for 100 steps, each a graph replay, against the native Fortran build of
+    # the same step (oracle/_build/les_native, itself pinned to the C restatement) run for 100 steps from the
+    # same fields: every value, ghost cells included, bit for bit.
+    import os
+    import subprocess
+    from pathlib import Path
+
+    native = Path(__file__).resolve().parents[1] / "oracle" / "_build" / "les_native"
+    assert native.exists(), "oracle/_build/les_native is not built (make -C oracle)"
+    nx, ny, nz, steps = 512, 512, 128, 100
+    print(f"LES {steps} steps at {nx}x{ny}x{nz} against the native build", flush=True)
+    got, start = les_steps(ftc, scope, hip, fields, (nx, ny, nz), steps)
+    np.concatenate([a.ravel(order="F") for a in start]).tofile(tmp_path / "in.bin")
+    threads = str(min(16, len(os.sched_getaffinity(0))))
+    run = subprocess.run([str(native), str(nx), str(ny), str(nz), str(steps - 1), "1", str(tmp_path / "in.bin"),
+                          str(tmp_path / "first.bin"), str(tmp_path / "final.bin")], capture_output=True, text=True,
+                         timeout=240, env=dict(os.environ, OMP_NUM_THREADS=threads))
+    assert run.returncode == 0, run.stderr
+    raw = np.fromfile(tmp_path / "final.bin", dtype=np.float64)
+    size = (nx + 2) * (ny + 2) * (nz + 2)
+    for f, (name, g) in enumerate(zip("UVWS", got)):
+        want = raw[f * size:(f + 1) * size].reshape((nx + 2, ny + 2, nz + 2), order="F")
+        assert bits_equal(g, want), f"{name}: " + describe_mismatch(g, want)

@@ -359,6 +359,8 @@ def main() -> int:
                    help="only time a plain 2 GiB device copy (HBM ceiling and byte-counter calibration)")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the Poisson (FP64 vs mixed) and chain-contraction sections")
+    p.add_argument("--poisson-only", action="store_true",
+                   help="only time the FP64 and mixed pressure solves at 512x512x128 and print their section")
     p.add_argument("--projected-step-only", action="store_true",
                    help="only time the whole projected step (LES step, divergence, FP64 and mixed solve, "
                         "projection) and print its section")
@@ -376,6 +378,9 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
 
+    if args.poisson_only:
+        print(json.dumps({"poisson": poisson_section(ftc, scope, lib, fields, reps=5)[0]}))
+        return 0
     if args.projected_step_only:
         print(json.dumps({"projected_step": projected_step_section(ftc, scope, lib, fields, steps=3)}))
         return 0

@@ -359,6 +359,8 @@ def main() -> int:
                    help="only time a plain 2 GiB device copy (HBM ceiling and byte-counter calibration)")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the Poisson (FP64 vs mixed) and chain-contraction sections")
+    p.add_argument("--chain-only", action="store_true",
+                   help="only time D = A*B*C at N = 4096 (FP64 MFMA and split-bf16) and print its section")
     p.add_argument("--poisson-only", action="store_true",
                    help="only time the FP64 and mixed pressure solves at 512x512x128 and print their section")
     p.add_argument("--projected-step-only", action="store_true",
@@ -378,6 +380,14 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
 
+    if args.chain_only:
+        section, results, inputs = chain_section(ftc, scope, lib)
+        A, B, C = inputs
+        ref = (A @ B) @ C
+        for name, D in results.items():
+            section[name]["max_rel_error_vs_fp64_cpu"] = float(np.abs(D - ref).max() / np.abs(ref).max())
+        print(json.dumps({"chain_contraction": section}))
+        return 0
     if args.poisson_only:
         print(json.dumps({"poisson": poisson_section(ftc, scope, lib, fields, reps=5)[0]}))
         return 0

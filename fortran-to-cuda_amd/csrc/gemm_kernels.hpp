@@ -146,7 +146,22 @@ __device__ __forceinline__ void gemm_f64_db_tile(const GemmCall &g, F64DbLds<BM,
 #pragma unroll
         for (int y = 0; y < TN; ++y) acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
     double ra[NA], rb[NB];
+    // A tile whose rows, columns and K-slices all lie inside the operands loads without bounds checks.
+    const bool whole = m0 + BM <= g.M && n0 + BN <= g.N && g.K % BK == 0;
     auto load = [&](int k0) {
+        if (whole) {
+#pragma unroll
+            for (int e = 0; e < NA; ++e) {
+                const int idx = t + e * kGemmThreads, m = idx % BM, k = idx / BM;
+                ra[e] = A[(long long)(m0 + m) + (long long)(k0 + k) * g.lda];
+            }
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                const int idx = t + e * kGemmThreads, k = idx % BK, n = idx / BK;
+                rb[e] = B[(long long)(k0 + k) + (long long)(n0 + n) * g.ldb];
+            }
+            return;
+        }
 #pragma unroll
         for (int e = 0; e < NA; ++e) {
             const int idx = t + e * kGemmThreads, m = idx % BM, k = idx / BM;
@@ -172,19 +187,25 @@ __device__ __forceinline__ void gemm_f64_db_tile(const GemmCall &g, F64DbLds<BM,
             L.Bs[stage][idx % BK][idx / BK] = rb[e];
         }
     };
+    // The operand fragments of MFMA step kk+4 are read from LDS while the products of step kk run.
     auto multiply = [&](int stage) {
+        double af[2][TM], bfr[2][TN];
+        auto fragments = [&](int buf, int kk) {
+#pragma unroll
+            for (int x = 0; x < TM; ++x) af[buf][x] = L.As[stage][kk + lane / 16][wm * WTM + x * 16 + lane % 16];
+#pragma unroll
+            for (int y = 0; y < TN; ++y) bfr[buf][y] = L.Bs[stage][kk + lane / 16][wn * WTN + y * 16 + lane % 16];
+        };
+        fragments(0, 0);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
-            double af[TM], bfr[TN];
-#pragma unroll
-            for (int x = 0; x < TM; ++x) af[x] = L.As[stage][kk + lane / 16][wm * WTM + x * 16 + lane % 16];
-#pragma unroll
-            for (int y = 0; y < TN; ++y) bfr[y] = L.Bs[stage][kk + lane / 16][wn * WTN + y * 16 + lane % 16];
+            const int cur = (kk / 4) & 1;
+            if (kk + 4 < BK) fragments(cur ^ 1, kk + 4);
 #pragma unroll
             for (int x = 0; x < TM; ++x)
 #pragma unroll
                 for (int y = 0; y < TN; ++y)
-                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bfr[y], af[x], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bfr[cur][y], af[cur][x], acc[x][y], 0, 0, 0);
         }
     };
     load(0);

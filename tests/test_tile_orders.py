@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 import pytest
 
-GRIDS = [(1, 1, 1), (7, 5, 3), (13, 1, 1), (3, 17, 2), (8, 128, 3), (64, 64, 4), (9, 9, 9)]
+GRIDS = [(1, 1, 1), (7, 5, 3), (13, 1, 1), (3, 17, 2), (8, 128, 3), (64, 64, 4), (9, 9, 9), (32, 32, 1), (17, 8, 2)]
 
 
 def visits(hip, scheme, grid, workgroups=0):

@@ -64,19 +64,35 @@ struct ColItem {
 constexpr int kCols = 64;
 
 // ------------------------------------------------------------------------------------------------ kernels
-// One persistent launch per tile width (32 x 8 or 64 x 4 columns) over the LES tiles of the members
-// that take that width. args is __restrict__: no store of the step writes a member's argument block, so
-// its fields can stay in registers across the body's stores.
-template <bool STEP, int WX>
-__global__ __launch_bounds__(NTH, 3) void ens_les_kernel(const LesArgs *__restrict__ args,
-                                                         const TileItem *__restrict__ items, int n)
+// One launch per tile width over the LES tiles of the members that take that width, one workgroup per item
+// (the dispatcher starts them in item order as slots free up): the buffer-addressed step body of the
+// single-domain step kernel, 512 threads as 64 x 8 or 32 x 16 columns at four waves per SIMD (two workgroups
+// per CU). An item marches the planes of its k chunk; every chunk but a member's last is a multiple of six
+// planes and starts at a plane k = 2 (mod 6), as the body's unrolled k loop requires. args is __restrict__:
+// no store of the step writes a member's argument block.
+constexpr int kStepThreads = 512;
+
+template <int WX>
+constexpr int step_split() { return WX == 64 ? 1 : 2; }
+
+template <int WX, bool ISO>
+__global__ __attribute__((amdgpu_flat_work_group_size(kStepThreads, kStepThreads), amdgpu_waves_per_eu(4, 4))) void
+ens_les_kernel(const LesArgs *__restrict__ args, const TileItem *__restrict__ items)
 {
-    __shared__ Sgs3Lds<1, WX> L;
-    for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        const TileItem t = items[it];
-        les_sgs3_body<STEP, 1, WX>(args[t.member], L, t.bx, t.by, t.bz);
-        __syncthreads();
-    }
+    __shared__ StepLds<kStepThreads, WX> L;
+    const TileItem t = items[blockIdx.x];
+    const LesArgs &a = args[t.member];
+    const int k0 = 2 + t.bz * a.kchunk;
+    const int k1 = min(k0 + a.kchunk - 1, a.nz + 1);
+    les_step_body<kStepThreads, WX, ISO, 3, step_split<WX>()>(a, L, t.bx, t.by, k0, k1);
+}
+
+// The k chunk of a member of nz planes: the tuned length rounded up to a multiple of six, or the whole
+// column.
+static int les_chunk(int kchunk, int nz)
+{
+    const int k = (std::max(kchunk, 1) + 5) / 6 * 6;
+    return std::min(k, nz);
 }
 
 // The LES tile width of a member under a tile policy: 0 every member 64 wide, 1 every member 32 wide,
@@ -328,7 +344,8 @@ struct ftc_ensemble {
     double dx = 0, dy = 0, dz = 0, dt = 0, cs = 0;
     std::vector<int> nx, ny, nz;
     // tuning
-    int les_kchunk = 16, gemm_tile = 1, blocks_per_cu = 4, cus = 256;
+    int les_kchunk = 24, gemm_tile = 1, blocks_per_cu = 4, cus = 256;
+    bool les_iso = true;         // equal spacings: the LES kernels hold one coefficient per kind
     int les_policy = 0;          // LES tile widths: see les_width
     std::vector<int> les_w;      // the width each member takes
     int n_les32 = 0;             // LES items of 32-wide members, after the n_les - n_les32 of 64-wide ones
@@ -412,8 +429,8 @@ static int build_items(ftc_ensemble &e)
         for (int m = 0; m < e.count; ++m) {
             if (les_width(e.les_policy, e.nx[m]) != width) continue;
             e.les_w[m] = width;
-            const int nx = e.nx[m], ny = e.ny[m], nz = e.nz[m], rows = NTH / width;
-            const int kchunk = std::min(e.les_kchunk, nz);
+            const int nx = e.nx[m], ny = e.ny[m], nz = e.nz[m], rows = kStepThreads / width;
+            const int kchunk = les_chunk(e.les_kchunk, nz);
             Range r{(int)les.size(), 0};
             for (int bz = 0; bz * kchunk < nz; ++bz)
                 for (int by = 0; by < (ny + rows - 1) / rows; ++by)
@@ -451,8 +468,10 @@ static int build_items(ftc_ensemble &e)
     // order as slots free up: so that the last ones to start are short, the items of each width are taken
     // longest first (most planes), each member's items staying together.
     std::vector<TileItem> les_all(les);
-    const int kc = e.les_kchunk;
-    auto planes = [&](const TileItem &t) { return std::min(kc, e.nz[t.member] - t.bz * std::min(kc, e.nz[t.member])); };
+    auto planes = [&](const TileItem &t) {
+        const int kc = les_chunk(e.les_kchunk, e.nz[t.member]);
+        return std::min(kc, e.nz[t.member] - t.bz * kc);
+    };
     const auto split = les_all.begin() + (e.n_les - e.n_les32);
     for (auto [lo, hi] : {std::make_pair(les_all.begin(), split), std::make_pair(split, les_all.end())})
         std::stable_sort(lo, hi, [&](const TileItem &a, const TileItem &b) { return planes(a) > planes(b); });
@@ -496,7 +515,7 @@ static int build_items(ftc_ensemble &e)
     for (int s = 0; s < 6; ++s) e.d_gemm[s] = (GemmItem *)put(gemm[s].data(), gemm[s].size() * sizeof(GemmItem));
     // LES arguments carry the k chunk.
     for (int par = 0; par < 2; ++par) {
-        for (int m = 0; m < e.count; ++m) e.args[par][m].kchunk = std::min(e.les_kchunk, e.nz[m]);
+        for (int m = 0; m < e.count; ++m) e.args[par][m].kchunk = les_chunk(e.les_kchunk, e.nz[m]);
         if (hipMemcpy(e.d_args[par], e.args[par].data(), e.count * sizeof(LesArgs), hipMemcpyHostToDevice) != hipSuccess)
             return eerr(FORT_SCOPE_RESOURCE, "argument upload failed");
     }
@@ -530,14 +549,15 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
         // One launch per width: the 64-wide members' tiles, then the 32-wide members'.
         auto launch = [&](int width, int first, int count) {
             if (!count) return;
-            const dim3 grid(all ? grid_for(e, count) : count);
+            const dim3 grid(count);
             const TileItem *items = (all ? e.d_les_all : e.d_les) + first;
-            if (width == 32)
-                hipLaunchKernelGGL((ens_les_kernel<true, 32>), grid, dim3(32, NTH / 32), 0, s, e.d_args[parity],
-                                   items, count);
-            else
-                hipLaunchKernelGGL((ens_les_kernel<true, 64>), grid, dim3(64, NTH / 64), 0, s, e.d_args[parity],
-                                   items, count);
+            const void *fn = width == 32 ? (e.les_iso ? reinterpret_cast<const void *>(&ens_les_kernel<32, true>)
+                                                      : reinterpret_cast<const void *>(&ens_les_kernel<32, false>))
+                                         : (e.les_iso ? reinterpret_cast<const void *>(&ens_les_kernel<64, true>)
+                                                      : reinterpret_cast<const void *>(&ens_les_kernel<64, false>));
+            const LesArgs *args = e.d_args[parity];
+            void *kargs[] = {&args, &items};
+            (void)hipLaunchKernel(fn, grid, dim3(kStepThreads), kargs, 0, s);
         };
         if (all) {
             launch(64, 0, e.n_les - e.n_les32);
@@ -728,6 +748,7 @@ int ftc_ensemble_create_v1(int count, const int32_t *sizes, double dx, double dy
     e->mode = mode;
     e->count = count;
     e->dx = dx, e->dy = dy, e->dz = dz, e->dt = dt, e->cs = cs;
+    e->les_iso = dx == dy && dy == dz;
     for (int m = 0; m < count; ++m) {
         const int a = sizes[3 * m], b = sizes[3 * m + 1], c = sizes[3 * m + 2];
         if (a < 2 || b < 2 || c < 2 || a > 4096 || b > 4096 || c > 150)
@@ -977,7 +998,7 @@ int ftc_ensemble_autotune_v1(ftc_ensemble *e, const char *cache_path)
     float best = 1e30f;
     int best_k = e->les_kchunk, best_p = e->les_policy;
     for (int policy : {0, 2, 1})
-        for (int k : {8, 16, 32, 64, 160}) {
+        for (int k : {12, 24, 48, 96}) {
             e->les_policy = policy;
             e->les_kchunk = k;
             FTC_CHECK(build_items(*e));

@@ -17,8 +17,8 @@ SIZES = [(40, 36, 12), (33, 47, 9), (64, 32, 16), (8, 6, 4)]
 D, DT, CS = (10.0, 10.0, 10.0), 0.2, 0.17
 
 
-def member_fields(fields, n, seed):
-    return fields.les_fields(*n, *D, seed=seed)
+def member_fields(fields, n, seed, d=D):
+    return fields.les_fields(*n, *d, seed=seed)
 
 
 def test_projected_oracle_step_is_divergence_free(oracle, fields):
@@ -31,8 +31,8 @@ def test_projected_oracle_step_is_divergence_free(oracle, fields):
 
 
 def run(hip, ens_mod, fields, mode, steps, tune=None, solves=None, sizes=SIZES, seed=100, projection=True,
-        policy=None):
-    e = ens_mod.Ensemble(hip, sizes, D, DT, CS, mode)
+        policy=None, d=D):
+    e = ens_mod.Ensemble(hip, sizes, d, DT, CS, mode)
     try:
         if solves:
             e.solves(solves)
@@ -43,7 +43,7 @@ def run(hip, ens_mod, fields, mode, steps, tune=None, solves=None, sizes=SIZES, 
         if tune is not None:
             e.autotune(tune)
         for m, n in enumerate(sizes):
-            e.upload(m, *member_fields(fields, n, seed + m))
+            e.upload(m, *member_fields(fields, n, seed + m, d))
         e.step(steps)
         e.synchronize()
         out = [e.download(m) + ([e.pressure(m)] if projection else []) for m in range(len(sizes))]
@@ -168,3 +168,21 @@ def test_seeded_64_members_projected_step_matches_the_oracle(hip, ens_mod, field
             worst = max(worst, err)
             assert err <= (phi_bound if name == "P" else bound), (m, n, name, err)
     print(f"worst field error {worst:.2e}")
+
+
+# Unequal spacings take the LES kernels that hold a coefficient per direction; members of 26 to 40 planes span
+# two k chunks of the default length (24 planes), and both tile widths are used.
+@pytest.mark.gpu
+@pytest.mark.parametrize("policy", [0, 1])
+def test_les_stage_with_unequal_spacings_is_bit_exact_against_the_oracle(hip, ens_mod, oracle, fields, policy):
+    sizes, d = [(40, 36, 30), (33, 47, 26), (70, 20, 40), (8, 6, 5)], (10.0, 8.0, 6.0)
+    print(f"ensemble LES stage, spacings {d}, tile policy {policy}", flush=True)
+    got, info = run(hip, ens_mod, fields, ens_mod.FP64_BATCHED, 2, sizes=sizes, seed=700, projection=False,
+                    policy=policy, d=d)
+    assert info.les_policy == policy
+    for m, n in enumerate(sizes):
+        state = list(member_fields(fields, n, 700 + m, d))
+        for _ in range(2):
+            state = list(oracle.les_step(*state, *d, DT, CS))
+        for name, g, w in zip("UVWS", got[m], state):
+            assert bits_equal(g, w), (m, n, name, describe_mismatch(g, w))

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <functional>
@@ -291,6 +292,209 @@ __global__ __launch_bounds__(kGemmThreads) void ens_gemm_f64_kernel(const GemmCa
     }
 }
 
+// The FP64 solve's transforms apply a small dense matrix (n <= kXformMax) along x or along y of a member's
+// field. Side 0 (x): Y = alpha Q X + beta Y, X seen as n x (ny nz) with x fastest (column c = j + ny k is
+// contiguous), Q column-major n x n. Side 1 (y): Y = alpha X R + beta Y, X seen as (nx nz) x n, row r =
+// i + nx k at i + nx ny k, column m at stride nx; Q holds R transposed, so that an MFMA operand fragment is
+// sixteen consecutive elements in both sides.
+struct XformCall {
+    const double *Q;
+    const double *X;
+    double *Y;
+    int n, nx, ny, nz;
+    double alpha, beta;
+};
+constexpr int kXformMax = 128;
+
+// The transform of one block of 64 columns (x) or 64 rows (y) of one call per workgroup of 256 threads (four
+// waves). The block of X is staged in LDS, the transform axis padded with zeros to a multiple of 16. Wave w
+// takes the block's columns (rows) 16 w .. 16 w + 15 and every transform tile (n / 16 rounded up, a template
+// argument so that the loop holds no branch and the MFMAs accumulate in place), and K runs in ascending steps
+// of four, one v_mfma_f64_16x16x4f64 per step and tile. The operand roles and the order
+// of the MFMAs over K are those of gemm_f64_tile on the same call, so the results are the same bits; the
+// output index of a lane's first operand is the store's contiguous one. The Q fragments come from memory
+// (the matrix is read by every block of the member, so from L2) one K slice ahead of their MFMAs.
+constexpr int kXformBlock = 64;
+constexpr int kQPad = 15;  // zero rows after each FP64 basis matrix: K runs to the next multiple of 16
+
+template <int SIDE>
+struct Xform {
+    static constexpr int BB = kXformBlock;
+    // The staged block as [block index][transform index] on both sides, so that a wave's K steps are 32 bytes
+    // apart in LDS (all within one DS offset range of its base address).
+    static constexpr int LD = kXformMax + 1;
+    static constexpr int LDS = BB * LD;  // doubles of one staged block
+    static constexpr int kOut = 0x7fff0000;  // a byte offset past every buffer below: such loads return zero
+    static constexpr int MG = 256 / BB;      // y side: BB rows per pass, MG values of m at once
+    static constexpr int NS = SIDE == 0 ? (BB / 4) * (kXformMax / 64) : kXformMax / MG;
+
+    // The block in two halves, each with every load first (a buffer load past the block's bytes returns zero,
+    // so the padding needs no branch), then its LDS stores.
+    static __device__ __forceinline__ void stage(const XformCall &g, int blk, double *Xs, int t)
+    {
+        const int n = g.n, kp = (n + 15) & ~15, b0 = blk * BB;
+        const int lane = t & 63, wave = t >> 6;
+        const int plane = g.nx * g.ny;
+        const int total = SIDE == 0 ? g.ny * g.nz : g.nx * g.nz;
+        constexpr int H = NS / 2;
+        double v[H];
+        if (SIDE == 0) {
+            // Column c = wave + 4 s, rows k = lane + 64 h: the half is h.
+            const int cb = min(BB, total - b0);
+            const __amdgpu_buffer_rsrc_t rX = field_rsrc(g.X + (long long)b0 * n, cb * n * 8);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = lane + 64 * h;
+#pragma unroll
+                for (int s = 0; s < BB / 4; ++s) v[s] = buffer_load(rX, k < n ? ((wave + 4 * s) * n + k) * 8 : kOut, 0);
+#pragma unroll
+                for (int s = 0; s < BB / 4; ++s)
+                    if (k < kp) Xs[(wave + 4 * s) * LD + k] = v[s];
+            }
+        } else {
+            // Row rl, m = mg + MG s: the half is s / H.
+            const __amdgpu_buffer_rsrc_t rX = field_rsrc(g.X, plane * g.nz * 8);
+            const int rl = t % BB, mg = t / BB;
+            const int r = b0 + rl;
+            const int base = r < total ? (r % g.nx) + (r / g.nx) * plane : -1;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int s = 0; s < H; ++s) {
+                    const int m = mg + MG * (s + H * h);
+                    v[s] = buffer_load(rX, base >= 0 && m < n ? (base + m * g.nx) * 8 : kOut, 0);
+                }
+#pragma unroll
+                for (int s = 0; s < H; ++s) {
+                    const int m = mg + MG * (s + H * h);
+                    if (m < kp) Xs[rl * LD + m] = v[s];
+                }
+            }
+        }
+    }
+
+    // The MFMAs over the staged block and the stores of the output block. Lane (q16, q4) of transform tile x
+    // reads Q(16 x + q16, kk + q4) at the lane's byte offset (16 x + q16 + q4 n) 8 plus the step's scalar
+    // offset kk n 8; rows past n read zero (their lane offset is out of range), and K past n reads the kQPad
+    // zero rows that follow the matrix and meets zero rows of the staged block.
+    // KS: K slices of 16 (n / 16 rounded up, also the number of transform tiles, all of them this wave's).
+    template <int KS>
+    static __device__ __forceinline__ void multiply(const XformCall &g, int blk, const double *Xs, int lane, int wave)
+    {
+        constexpr int XT = KS;
+        const int n = g.n, b0 = blk * BB;
+        const int plane = g.nx * g.ny;
+        const int total = SIDE == 0 ? g.ny * g.nz : g.nx * g.nz;
+        const int q16 = lane % 16, q4 = lane / 16;
+        const __amdgpu_buffer_rsrc_t rQ = field_rsrc(g.Q, n * (n + kQPad) * 8);
+        int qoff[XT];
+#pragma unroll
+        for (int x = 0; x < XT; ++x) {
+            const int row = 16 * x + q16;
+            qoff[x] = row < n ? (row + q4 * n) * 8 : kOut;
+        }
+        const int bcol = 16 * wave + q16;  // this lane's index within the block
+        const double *xp = Xs + bcol * LD + q4;
+        constexpr int XS = 4;  // LDS step of one K step
+        f64x4 acc[XT];
+#pragma unroll
+        for (int x = 0; x < XT; ++x) acc[x] = f64x4{0.0, 0.0, 0.0, 0.0};
+        // K runs over KS slices of 16 (the tiled contraction's slices, zeros past n), each slice's Q fragments
+        // loaded while the previous slice multiplies; the trip counts are constants, so the loop unrolls into
+        // straight code with exact waits on the loads.
+        const int slice_bytes = 16 * n * 8;
+        double qa[4][XT], qb[4][XT];
+        auto load = [&](double (&q)[4][XT], int sl) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int x = 0; x < XT; ++x) q[u][x] = buffer_load(rQ, qoff[x], sl * slice_bytes + u * 4 * n * 8);
+        };
+        auto run = [&](const double (&q)[4][XT], int sl) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double xf = xp[(4 * sl + u) * XS];
+#pragma unroll
+                for (int x = 0; x < XT; ++x)
+                    acc[x] = SIDE == 0 ? __builtin_amdgcn_mfma_f64_16x16x4f64(xf, q[u][x], acc[x], 0, 0, 0)
+                                       : __builtin_amdgcn_mfma_f64_16x16x4f64(q[u][x], xf, acc[x], 0, 0, 0);
+            }
+        };
+        load(qa, 0);
+#pragma unroll
+        for (int sl = 0; sl < KS; sl += 2) {
+            if (sl + 1 < KS) load(qb, sl + 1);
+            run(qa, sl);
+            if (sl + 1 < KS) {
+                if (sl + 2 < KS) load(qa, sl + 2);
+                run(qb, sl + 1);
+            }
+        }
+        // Lane l, element e of tile x: side 0 holds Y(transform index 16 x + q16, block index 16 w + q4 + 4e);
+        // side 1 holds Y(block index 16 w + q16, transform index 16 x + q4 + 4e).
+        long long rowbase = 0;
+        bool rok = true;
+        if (SIDE == 1) {
+            const int r = b0 + bcol;
+            rok = r < total;
+            rowbase = rok ? (long long)(r % g.nx) + (long long)(r / g.nx) * plane : 0;
+        }
+#pragma unroll
+        for (int x = 0; x < XT; ++x) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                long long idx;
+                bool ok;
+                if (SIDE == 0) {
+                    const int m = 16 * x + q16;
+                    const int c = b0 + 16 * wave + q4 + 4 * e;
+                    ok = m < n && c < total;
+                    idx = m + (long long)c * n;
+                } else {
+                    const int jj = 16 * x + q4 + 4 * e;
+                    ok = rok && jj < n;
+                    idx = rowbase + (long long)jj * g.nx;
+                }
+                if (ok) {
+                    double v = g.alpha * acc[x][e];
+                    if (g.beta != 0.0) v = v + g.beta * g.Y[idx];
+                    g.Y[idx] = v;
+                }
+            }
+        }
+    }
+
+    static __device__ __forceinline__ void compute(const XformCall &g, int blk, const double *Xs, int t)
+    {
+        const int lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
+        switch ((g.n + 15) / 16) {
+        case 1: multiply<1>(g, blk, Xs, lane, wave); break;
+        case 2: multiply<2>(g, blk, Xs, lane, wave); break;
+        case 3: multiply<3>(g, blk, Xs, lane, wave); break;
+        case 4: multiply<4>(g, blk, Xs, lane, wave); break;
+        case 5: multiply<5>(g, blk, Xs, lane, wave); break;
+        case 6: multiply<6>(g, blk, Xs, lane, wave); break;
+        case 7: multiply<7>(g, blk, Xs, lane, wave); break;
+        default: multiply<8>(g, blk, Xs, lane, wave); break;
+        }
+    }
+};
+
+// One block per workgroup: stage, then multiply. The LDS block allows two workgroups per CU; declaring that
+// occupancy holds a wave to 256 registers, where the MFMAs accumulate in place.
+template <int SIDE>
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2, 2))) void
+ens_xform_kernel(const XformCall *__restrict__ calls, const GemmItem *__restrict__ items)
+{
+    using X = Xform<SIDE>;
+    __shared__ double Xs[X::LDS];
+    const GemmItem it = items[blockIdx.x];
+    const XformCall g = calls[it.call];
+    X::stage(g, it.mt, Xs, threadIdx.x);
+    __syncthreads();
+    X::compute(g, it.mt, Xs, threadIdx.x);
+}
+
 template <class TA, class TB, class TO, int BM, int BN, bool TOUT>
 __global__ __launch_bounds__(kGemmThreads) void ens_gemm_bf16_kernel(const GemmCall *calls, const GemmItem *items, int n)
 {
@@ -306,24 +510,44 @@ __global__ __launch_bounds__(kGemmThreads) void ens_gemm_bf16_kernel(const GemmC
 static std::size_t up256(std::size_t x) { return (x + 255) & ~std::size_t(255); }
 
 // The basis of the periodic 1D second difference (same construction as the pressure provider).
-static void periodic_basis(int n, double h, std::vector<double> &Q, std::vector<double> &lam)
+// The periodic basis of an axis of n points as a table of 4n values: cosines r2 cos(2 pi j / n), sines
+// r2 sin(2 pi j / n), the constant r1, and the alternating r1 (j = 0), -r1 (j = n / 2), with r1 = 1/sqrt(n) and
+// r2 = sqrt(2/n). Mode 0 is the constant, modes 2f - 1 and 2f (0 < 2f < n) the cosine and sine of frequency
+// f, and mode n - 1 of an even n the alternating vector: B(i, mode) = table[section n + (f i mod n)].
+static void basis_table(int n, std::vector<double> &table)
+{
+    const double pi = 3.14159265358979323846;
+    const double r2 = std::sqrt(2.0 / n), r1 = 1.0 / std::sqrt((double)n);
+    table.assign((std::size_t)4 * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        table[j] = r2 * std::cos(2.0 * pi * j / n);
+        table[n + j] = r2 * std::sin(2.0 * pi * j / n);
+        table[2 * n + j] = r1;
+        table[3 * n + j] = j == 0 ? r1 : -r1;
+    }
+}
+
+// The basis matrix Q(i, mode) (column-major, n x n) from the table, and the eigenvalues of the periodic
+// second difference over spacing h.
+static void periodic_basis(int n, double h, const std::vector<double> &table, std::vector<double> &Q,
+                           std::vector<double> &lam)
 {
     Q.assign((std::size_t)n * n, 0.0);
     lam.assign(n, 0.0);
     const double pi = 3.14159265358979323846;
-    const double r2 = std::sqrt(2.0 / n), r1 = 1.0 / std::sqrt((double)n);
-    for (int i = 0; i < n; ++i) Q[i] = r1;
+    for (int i = 0; i < n; ++i) Q[i] = table[2 * n];
     for (int f = 1; 2 * f < n; ++f) {
         const double l = (2.0 * std::cos(2.0 * pi * f / n) - 2.0) / (h * h);
         for (int i = 0; i < n; ++i) {
-            Q[(std::size_t)(2 * f - 1) * n + i] = r2 * std::cos(2.0 * pi * f * i / n);
-            Q[(std::size_t)(2 * f) * n + i] = r2 * std::sin(2.0 * pi * f * i / n);
+            const int j = (int)((long long)f * i % n);
+            Q[(std::size_t)(2 * f - 1) * n + i] = table[j];
+            Q[(std::size_t)(2 * f) * n + i] = table[n + j];
         }
         lam[2 * f - 1] = l;
         lam[2 * f] = l;
     }
     if (n % 2 == 0) {
-        for (int i = 0; i < n; ++i) Q[(std::size_t)(n - 1) * n + i] = (i % 2 ? -r1 : r1);
+        for (int i = 0; i < n; ++i) Q[(std::size_t)(n - 1) * n + i] = table[3 * n + (int)((long long)(n / 2) * i % n)];
         lam[n - 1] = -4.0 / (h * h);
     }
 }
@@ -358,6 +582,8 @@ struct ftc_ensemble {
     LesArgs *d_args[2] = {nullptr, nullptr};
     std::vector<LesArgs> args[2];
     GemmCall *d_calls = nullptr;  // kG1.. stage-major: calls[stage_slot * count + member]
+    XformCall *d_xcalls = nullptr;  // the same calls as axis transforms (FP64 mode), same indexing
+    bool xform[6] = {};             // transform slot runs on ens_xform_kernel (FP64, every member's n small)
     TileItem *d_les = nullptr, *d_tiles = nullptr;
     TileItem *d_les_all = nullptr;    // the same items for the all-member launches, longest first per width
     TileItem *d_tiles_all = nullptr;  // the tiles longest first, for the launches that keep no per-tile sums
@@ -409,6 +635,31 @@ static void member_calls(const ftc_ensemble &e, const EnsMember &m, const double
     out[5] = GemmCall{Qx, m.w1, m.phi, d, d, d, nx, ny, nx, nz, nx, nx, nx, 0, plane, plane, 1.0, 1.0};
 }
 
+// The FP64 mode's transform calls as axis transforms: slots G1, G4, G1c and G4c along x, G2 and G3 along y.
+// G1 and G1c multiply by Qx^T, G4 and G4c by Qx (from the left); G2 by Qy and G3 by Qy^T (from the right, so
+// their Q is Qy^T and Qy).
+static void member_xforms(const EnsMember &m, const double *Qx, const double *QxT, const double *Qy,
+                          const double *QyT, XformCall out[6])
+{
+    const double *rhs = m.rhs, *res = m.res;
+    double *w0 = static_cast<double *>(m.w0), *w1 = static_cast<double *>(m.w1);
+    auto x = [&](const double *Q, const double *X, double *Y, double beta) {
+        return XformCall{Q, X, Y, m.nx, m.nx, m.ny, m.nz, 1.0, beta};
+    };
+    auto y = [&](const double *Qt, const double *X, double *Y) {
+        return XformCall{Qt, X, Y, m.ny, m.nx, m.ny, m.nz, 1.0, 0.0};
+    };
+    out[0] = x(QxT, rhs, w0, 0.0);
+    out[1] = y(QyT, w0, w1);
+    out[2] = y(Qy, w0, w1);
+    out[3] = x(Qx, w1, m.phi, 0.0);
+    out[4] = x(QxT, res, w0, 0.0);
+    out[5] = x(Qx, w1, m.phi, 1.0);
+}
+
+// Transform slot s runs along y (G2, G3) or along x (the others).
+static bool slot_along_y(int s) { return s == 1 || s == 2; }
+
 static int tile_bm(int tile) { return tile == 0 ? 64 : 32; }
 
 // Builds (or rebuilds, after tuning) every work list and uploads it.
@@ -448,10 +699,16 @@ static int build_items(ftc_ensemble &e)
         e.r_tiles.push_back(rt);
         for (int s = 0; s < 6; ++s) {
             Range rg{(int)gemm[s].size(), 0};
-            const int mtiles = (nx + bm - 1) / bm, ntiles = (ny + bm - 1) / bm;
-            for (int b = 0; b < nz; ++b)
-                for (int nt = 0; nt < ntiles; ++nt)
-                    for (int mt = 0; mt < mtiles; ++mt) gemm[s].push_back({s * e.count + m, mt, nt, b});
+            if (e.xform[s]) {
+                // Axis transform: blocks of 64 columns (x) or 64 rows (y).
+                const long long extent = slot_along_y(s) ? (long long)nx * nz : (long long)ny * nz;
+                for (int b = 0; (long long)b * kXformBlock < extent; ++b) gemm[s].push_back({s * e.count + m, b, 0, 0});
+            } else {
+                const int mtiles = (nx + bm - 1) / bm, ntiles = (ny + bm - 1) / bm;
+                for (int b = 0; b < nz; ++b)
+                    for (int nt = 0; nt < ntiles; ++nt)
+                        for (int mt = 0; mt < mtiles; ++mt) gemm[s].push_back({s * e.count + m, mt, nt, b});
+            }
             rg.count = (int)gemm[s].size() - rg.start;
             e.r_gemm[s].push_back(rg);
         }
@@ -618,7 +875,10 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
         const dim3 grid(all ? grid_for(e, n) : n), block(kGemmThreads);
         const GemmItem *items = e.d_gemm[slot] + start;
         const bool big = e.gemm_tile == 0;
-        if (!mx) {
+        if (e.xform[slot]) {
+            if (slot_along_y(slot)) hipLaunchKernelGGL(ens_xform_kernel<1>, dim3(n), block, 0, s, e.d_xcalls, items);
+            else hipLaunchKernelGGL(ens_xform_kernel<0>, dim3(n), block, 0, s, e.d_xcalls, items);
+        } else if (!mx) {
             if (big) hipLaunchKernelGGL((ens_gemm_f64_kernel<64, 64>), grid, block, 0, s, e.d_calls, items, n);
             else hipLaunchKernelGGL((ens_gemm_f64_kernel<32, 32>), grid, block, 0, s, e.d_calls, items, n);
         } else if (slot == 0 || slot == 4) {
@@ -772,10 +1032,12 @@ int ftc_ensemble_create_v1(int count, const int32_t *sizes, double dx, double dy
         const std::size_t N = (std::size_t)e->nx[m] * e->ny[m] * e->nz[m];
         const std::size_t X = e->nx[m], Y = e->ny[m];
         bytes += 8 * up256(P * 8) + 3 * up256(N * 8) + 2 * up256(N * wbytes);
-        bytes += 2 * up256(X * X * 8) + 2 * up256(Y * Y * 8) + 2 * up256(X * X * 4) + 2 * up256(Y * Y * 4);
+        bytes += 2 * up256(X * (X + kQPad) * 8) + 2 * up256(Y * (Y + kQPad) * 8) + 2 * up256(X * X * 4) +
+                 2 * up256(Y * Y * 4);
         bytes += up256(X * 8) + up256(Y * 8) + up256(257 * 8);
     }
-    bytes += up256(count * sizeof(EnsMember)) + 2 * up256(count * sizeof(LesArgs)) + up256(6 * count * sizeof(GemmCall));
+    bytes += up256(count * sizeof(EnsMember)) + 2 * up256(count * sizeof(LesArgs)) + up256(6 * count * sizeof(GemmCall)) +
+             up256(6 * count * sizeof(XformCall));
     e->arena_bytes = bytes;
     if (hipMalloc(&e->arena, bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -791,6 +1053,15 @@ int ftc_ensemble_create_v1(int count, const int32_t *sizes, double dx, double dy
         return r;
     };
     std::vector<GemmCall> calls(6 * count);
+    std::vector<XformCall> xcalls(6 * count);
+    // The FP64 mode applies its transforms along each axis with ens_xform_kernel when every member's extent
+    // along that axis fits its LDS block; otherwise (or under FTC_ENS_TILED_TRANSFORMS) as contractions over
+    // tiles, which give the same bits.
+    const int max_nx = *std::max_element(e->nx.begin(), e->nx.end());
+    const int max_ny = *std::max_element(e->ny.begin(), e->ny.end());
+    const bool tiled = std::getenv("FTC_ENS_TILED_TRANSFORMS") != nullptr;
+    for (int s = 0; s < 6; ++s)
+        e->xform[s] = !mx && !tiled && (slot_along_y(s) ? max_ny : max_nx) <= kXformMax;
     e->args[0].resize(count);
     e->args[1].resize(count);
     for (int m = 0; m < count; ++m) {
@@ -808,8 +1079,11 @@ int ftc_ensemble_create_v1(int count, const int32_t *sizes, double dx, double dy
         mm.w0 = carve(N * wbytes);
         mm.w1 = carve(N * wbytes);
         const int X = mm.nx, Y = mm.ny;
-        double *Qx = (double *)carve((std::size_t)X * X * 8), *QxT = (double *)carve((std::size_t)X * X * 8);
-        double *Qy = (double *)carve((std::size_t)Y * Y * 8), *QyT = (double *)carve((std::size_t)Y * Y * 8);
+        // kQPad spare rows of zeros after each FP64 matrix: ens_xform_kernel reads K up to the next multiple of
+        // 16 past n with the step in the scalar offset, which the buffer's range check does not cover.
+        const std::size_t qxb = (std::size_t)X * (X + kQPad) * 8, qyb = (std::size_t)Y * (Y + kQPad) * 8;
+        double *Qx = (double *)carve(qxb), *QxT = (double *)carve(qxb);
+        double *Qy = (double *)carve(qyb), *QyT = (double *)carve(qyb);
         float *Qxf = (float *)carve((std::size_t)X * X * 4), *QxTf = (float *)carve((std::size_t)X * X * 4);
         float *Qyf = (float *)carve((std::size_t)Y * Y * 4), *QyTf = (float *)carve((std::size_t)Y * Y * 4);
         mm.lx = (double *)carve((std::size_t)X * 8);
@@ -818,9 +1092,11 @@ int ftc_ensemble_create_v1(int count, const int32_t *sizes, double dx, double dy
         mm.gx = dt / dx, mm.gy = dt / dy, mm.gz = dt / dz;
         mm.rdx = 1.0 / dx, mm.rdy = 1.0 / dy, mm.rdz = 1.0 / dz, mm.rdt = 1.0 / dt;
         mm.rx = 1.0 / (dx * dx), mm.ry = 1.0 / (dy * dy), mm.rz = 1.0 / (dz * dz);
-        std::vector<double> qx, lx, qy, ly;
-        periodic_basis(X, dx, qx, lx);
-        periodic_basis(Y, dy, qy, ly);
+        std::vector<double> qx, lx, qy, ly, tabx, taby;
+        basis_table(X, tabx);
+        basis_table(Y, taby);
+        periodic_basis(X, dx, tabx, qx, lx);
+        periodic_basis(Y, dy, taby, qy, ly);
         std::vector<double> qxt(qx.size()), qyt(qy.size());
         for (int a = 0; a < X; ++a)
             for (int b = 0; b < X; ++b) qxt[(std::size_t)b * X + a] = qx[(std::size_t)a * X + b];
@@ -842,6 +1118,9 @@ int ftc_ensemble_create_v1(int count, const int32_t *sizes, double dx, double dy
         GemmCall c6[6];
         member_calls(*e, mm, Qx, QxT, Qy, QyT, Qxf, QxTf, Qyf, QyTf, c6);
         for (int s = 0; s < 6; ++s) calls[s * count + m] = c6[s];
+        XformCall x6[6];
+        member_xforms(mm, Qx, QxT, Qy, QyT, x6);
+        for (int s = 0; s < 6; ++s) xcalls[s * count + m] = x6[s];
         for (int par = 0; par < 2; ++par) {
             LesArgs a{};
             a.U = mm.f[par][0], a.V = mm.f[par][1], a.W = mm.f[par][2], a.S = mm.f[par][3];
@@ -861,8 +1140,10 @@ int ftc_ensemble_create_v1(int count, const int32_t *sizes, double dx, double dy
     e->d_args[0] = (LesArgs *)carve(count * sizeof(LesArgs));
     e->d_args[1] = (LesArgs *)carve(count * sizeof(LesArgs));
     e->d_calls = (GemmCall *)carve(6 * count * sizeof(GemmCall));
+    e->d_xcalls = (XformCall *)carve(6 * count * sizeof(XformCall));
     if (hipMemcpy(e->d_members, e->members.data(), count * sizeof(EnsMember), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(e->d_calls, calls.data(), calls.size() * sizeof(GemmCall), hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(e->d_calls, calls.data(), calls.size() * sizeof(GemmCall), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(e->d_xcalls, xcalls.data(), xcalls.size() * sizeof(XformCall), hipMemcpyHostToDevice) != hipSuccess)
         return eerr(FORT_SCOPE_RESOURCE, "description upload failed");
     FTC_CHECK(build_items(*e));
     if (mode != FTC_ENSEMBLE_FP64_PER_MEMBER) FTC_CHECK(capture(*e));

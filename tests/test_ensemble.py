@@ -85,6 +85,35 @@ def test_batched_fp64_equals_per_member_fp64_bit_for_bit(hip, ens_mod, fields):
             assert bits_equal(g, w)
 
 
+# The FP64 transforms run as axis transforms (one LDS block of 64 columns or rows per workgroup) when every
+# member's extent along the axis is at most 128, else as tiled contractions; both keep the operand roles and
+# the order of the MFMAs over K, so they give the same bits. The second size list puts one member above 128
+# in x (x transforms tiled, y transforms along the axis), the third one above 128 in y.
+@pytest.mark.gpu
+@pytest.mark.parametrize("sizes", [SIZES, [(150, 40, 8), (40, 36, 12), (33, 100, 6)],
+                                   [(40, 140, 7), (64, 32, 10), (128, 17, 5)]])
+def test_axis_transforms_equal_the_tiled_contractions_bit_for_bit(hip, ens_mod, fields, sizes, monkeypatch):
+    print(f"ensemble transforms, sizes {sizes}", flush=True)
+    ref, _ = run(hip, ens_mod, fields, ens_mod.FP64_BATCHED, 2, sizes=sizes)
+    monkeypatch.setenv("FTC_ENS_TILED_TRANSFORMS", "1")
+    tiled, _ = run(hip, ens_mod, fields, ens_mod.FP64_BATCHED, 2, sizes=sizes)
+    for m in range(len(sizes)):
+        for name, g, w in zip("UVWSP", ref[m], tiled[m]):
+            assert bits_equal(g, w), (m, name, describe_mismatch(g, w))
+
+
+@pytest.mark.gpu
+def test_members_wider_than_an_axis_block_match_the_oracle(hip, ens_mod, oracle, fields):
+    sizes = [(150, 40, 8), (40, 36, 12), (33, 100, 6)]
+    got, _ = run(hip, ens_mod, fields, ens_mod.FP64_BATCHED, 1, sizes=sizes)
+    for m, n in enumerate(sizes):
+        *state, phi = oracle.projected_step(*member_fields(fields, n, 100 + m), *D, DT, CS)
+        for name, g, w in zip("UVWS", got[m][:4], state):
+            scale = np.abs(w).max()
+            assert np.abs(g - w).max() <= 1e-12 * scale, (m, name, np.abs(g - w).max() / scale)
+        assert np.abs(got[m][4] - phi).max() <= 1e-11 * np.abs(phi).max()
+
+
 @pytest.mark.gpu
 def test_mixed_batched_is_within_1e10_of_fp64(hip, ens_mod, fields):
     ref, _ = run(hip, ens_mod, fields, ens_mod.FP64_PER_MEMBER, 2)

@@ -356,7 +356,8 @@ def main() -> int:
     p.add_argument("--cpu-steps", type=int, default=2,
                    help="steps per timed repeat of the native CPU baseline (0: skip)")
     p.add_argument("--copy-probe", action="store_true",
-                   help="only time a plain 2 GiB device copy (HBM ceiling and byte-counter calibration)")
+                   help="only time a plain 2 GiB device copy (HBM ceiling and byte-counter calibration), "
+                        "the other HBM rates and the FP64 matrix-core ceiling")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the Poisson (FP64 vs mixed) and chain-contraction sections")
     p.add_argument("--chain-only", action="store_true",
@@ -404,6 +405,14 @@ def main() -> int:
             scope.check(lib, lib.ftc_bandwidth_probe(kind, n, 10, ctypes.byref(ms)))
             out[name] = {"bytes_per_launch": factor * n, "ms_per_launch": round(ms.value, 4),
                          "GB_per_s": round(factor * n / (ms.value * 1e-3) / 1e9, 1)}
+        # The FP64 matrix-core ceiling: four-wave workgroups, 1, 2 and 4 per CU (waves per SIMD), eight
+        # independent 16x16x4 FP64 MFMAs per step.
+        flops = ctypes.c_double()
+        for wps in (1, 2, 4):
+            scope.check(lib, lib.ftc_mfma_probe(wps, 4000, 3, ctypes.byref(ms), ctypes.byref(flops)))
+            out[f"mfma_f64_{wps}_waves_per_simd"] = {
+                "ms_per_launch": round(ms.value, 4),
+                "TFLOP_per_s": round(flops.value / (ms.value * 1e-3) / 1e12, 1)}
         print(json.dumps(out), flush=True)
         return 0
 

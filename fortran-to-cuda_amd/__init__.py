@@ -59,6 +59,7 @@ HELPER_SIGNATURES = {
     "ftc_device_count": (c_int, []),
     "ftc_copy_probe": (c_int, [ctypes.c_size_t, c_int, POINTER(c_float)]),
     "ftc_bandwidth_probe": (c_int, [c_int, ctypes.c_size_t, c_int, POINTER(c_float)]),
+    "ftc_mfma_probe": (c_int, [c_int, c_int, c_int, POINTER(c_float), POINTER(ctypes.c_double)]),
     "ftc_tile_visit_probe_v1": (c_int, [c_int, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                                         POINTER(ctypes.c_uint)]),
     "ftc_gemm_device": (c_int, [POINTER(GemmDesc), c_int, c_int, c_void_p]),

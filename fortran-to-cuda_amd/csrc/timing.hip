@@ -99,6 +99,24 @@ __global__ void ftc_tile_visit_kernel(int scheme, unsigned total, unsigned *coun
     }
 }
 
+// The FP64 MFMA issue rate: each wave runs `iters` steps of eight independent v_mfma_f64_16x16x4f64
+// accumulations with no memory traffic in the loop; one double per lane is stored at the end (never read).
+__global__ __launch_bounds__(256) void ftc_mfma_f64_kernel(double *__restrict__ out, int iters, double a0)
+{
+    typedef double f64x4 __attribute__((ext_vector_type(4)));
+    f64x4 acc[8];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) acc[x] = f64x4{0.0, 0.0, 0.0, 0.0};
+    const double a = a0 + threadIdx.x, b = 0.5 * a0 + threadIdx.x;
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+        for (int x = 0; x < 8; ++x) acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[x], 0, 0, 0);
+    double sum = 0.0;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) sum += (acc[x][0] + acc[x][1]) + (acc[x][2] + acc[x][3]);
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
 extern "C" {
 
 const void *ftc_iota_kernel(void) { return reinterpret_cast<const void *>(&ftc_iota); }
@@ -181,6 +199,47 @@ int ftc_bandwidth_probe(int kind, size_t n, int reps, float *ms_per_launch)
     if (s) (void)hipStreamDestroy(s);
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
+    return status;
+}
+
+// The FP64 matrix-core ceiling of this part: waves_per_simd workgroups of four waves per CU, each wave `iters`
+// steps of eight independent 16x16x4 FP64 MFMAs (2048 flops each), reps timed launches after one warm-up.
+// Writes the milliseconds and the flops of one launch.
+int ftc_mfma_probe(int waves_per_simd, int iters, int reps, float *ms_per_launch, double *flops_per_launch)
+{
+    if (!ms_per_launch || !flops_per_launch || waves_per_simd <= 0 || waves_per_simd > 8 || iters <= 0 || reps <= 0)
+        return FORT_SCOPE_ARGUMENT;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return FORT_SCOPE_RESOURCE;
+    const unsigned blocks = (unsigned)cus * (unsigned)waves_per_simd;
+    *flops_per_launch = (double)blocks * 4 * iters * 8 * 2048;
+    double *out = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int status = FORT_SCOPE_RESOURCE;
+    if (hipMalloc(&out, (size_t)blocks * 256 * sizeof(double)) == hipSuccess &&
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+        hipEventCreate(&e1) == hipSuccess) {
+        hipLaunchKernelGGL(ftc_mfma_f64_kernel, dim3(blocks), dim3(256), 0, s, out, iters, 1.0);
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(ftc_mfma_f64_kernel, dim3(blocks), dim3(256), 0, s, out, iters, 1.0);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) == hipSuccess) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            *ms_per_launch = ms / reps;
+            status = FORT_SCOPE_OK;
+        } else {
+            status = FORT_SCOPE_EXECUTION;
+        }
+    }
+    (void)hipGetLastError();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (out) (void)hipFree(out);
     return status;
 }
 

@@ -71,6 +71,10 @@ int ftc_copy_probe(size_t n, int reps, float *ms_per_launch);
 /* The HBM ceilings of this part: kind 0 copy (the probe above), 1 copy with nontemporal loads and stores,
  * 2 read only, 3 write only, over n doubles (n even). No reference counterpart. */
 int ftc_bandwidth_probe(int kind, size_t n, int reps, float *ms_per_launch);
+/* The FP64 matrix-core ceiling of this part: waves_per_simd (1..8) four-wave workgroups per CU, each wave
+ * iters steps of eight independent 16x16x4 FP64 MFMAs (2048 flops each, no memory traffic), launched reps
+ * times after a warm-up; writes the milliseconds and the flops of one launch. No reference counterpart. */
+int ftc_mfma_probe(int waves_per_simd, int iters, int reps, float *ms_per_launch, double *flops_per_launch);
 /* Visit counts of the tile orders the kernels use (csrc/xcd.hpp), for a check that each covers every tile
  * exactly once (as compiler/tests/test_cuda_coordinate_quotient.py counts the visits of the reference's
  * grid-stride decode): scheme 0 the launch order and 1 the XCD-contiguous order of a gx x gy x gz grid, 2

@@ -101,7 +101,11 @@ __global__ void ftc_tile_visit_kernel(int scheme, unsigned total, unsigned *coun
 
 // The FP64 MFMA issue rate: each wave runs `iters` steps of eight independent v_mfma_f64_16x16x4f64
 // accumulations with no memory traffic in the loop; one double per lane is stored at the end (never read).
-__global__ __launch_bounds__(256) void ftc_mfma_f64_kernel(double *__restrict__ out, int iters, double a0)
+// Declaring four waves per SIMD holds a wave to 128 registers, so the accumulators stay where the MFMAs
+// write them (with the larger budget the compiler keeps them in VGPRs and copies them through AGPRs around
+// every step).
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(4, 4))) void
+ftc_mfma_f64_kernel(double *__restrict__ out, int iters, double a0)
 {
     typedef double f64x4 __attribute__((ext_vector_type(4)));
     f64x4 acc[8];

@@ -106,37 +106,55 @@ static int les_width(int policy, int nx)
     return idle32 < idle64 ? 32 : 64;
 }
 
-// rhs = div(U*) / dt on every cell of the member (U* = buffer set `src`).
-__global__ __launch_bounds__(256) void ens_divergence_kernel(const EnsMember *ms, const TileItem *items, int n, int src)
+// rhs = div(U*) / dt on every cell of the member (U* = buffer set `src`). An item is a block of kFlat
+// consecutive columns of a member's plane (x fastest), one thread per column, so every lane of a block but
+// the member's last holds a column whatever the member's width.
+constexpr int kFlat = 256;
+
+__global__ __launch_bounds__(kFlat) void ens_divergence_kernel(const EnsMember *ms, const ColItem *items, int n, int src)
 {
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        const TileItem t = items[it];
+        const ColItem t = items[it];
         const EnsMember m = ms[t.member];
-        const int a = 1 + t.bx * 64 + threadIdx.x, b = 1 + t.by * 4 + threadIdx.y;
-        if (a > m.nx || b > m.ny) continue;
+        const int col = t.block * kFlat + threadIdx.x;
+        if (col >= m.nx * m.ny) continue;
+        const int a = 1 + col % m.nx, b = 1 + col / m.nx;
         const double *U = src ? m.f[1][0] : m.f[0][0], *V = src ? m.f[1][1] : m.f[0][1];
         const double *W = src ? m.f[1][2] : m.f[0][2];
-#pragma unroll 4
-        for (int c = 1; c <= m.nz; ++c) {
-            const long long p = a + b * m.sj + c * m.sk;
-            const long long q = (a - 1) + (long long)m.nx * ((b - 1) + (long long)m.ny * (c - 1));
-            m.rhs[q] = (((U[p] - U[p - 1]) * m.rdx + (V[p] - V[p - m.sj]) * m.rdy) + (W[p] - W[p - m.sk]) * m.rdz) * m.rdt;
+        // Planes in batches of four, every load of a batch before its stores (rhs is not read here).
+        constexpr int kB = 4;
+        const long long plane = (long long)m.nx * m.ny;
+        for (int c0 = 1; c0 <= m.nz; c0 += kB) {
+            double u0[kB], u1[kB], v0[kB], v1[kB], w0[kB], w1[kB];
+#pragma unroll
+            for (int d = 0; d < kB; ++d) {
+                const long long p = a + b * m.sj + min(c0 + d, m.nz) * m.sk;
+                u0[d] = U[p], u1[d] = U[p - 1], v0[d] = V[p], v1[d] = V[p - m.sj], w0[d] = W[p], w1[d] = W[p - m.sk];
+            }
+#pragma unroll
+            for (int d = 0; d < kB; ++d) {
+                const int c = c0 + d;
+                if (c > m.nz) break;
+                const long long q = (a - 1) + (long long)m.nx * (b - 1) + plane * (c - 1);
+                m.rhs[q] = (((u0[d] - u1[d]) * m.rdx + (v0[d] - v1[d]) * m.rdy) + (w0[d] - w1[d]) * m.rdz) * m.rdt;
+            }
         }
     }
 }
 
 // U -= dt grad(phi) on the interior faces of buffer set `dst`, in place, with the halo images (the same
 // stores and images as put_with_images; the column's image offsets and phi's periodic neighbours are
-// formed once per column).
-__global__ __launch_bounds__(256) void ens_projection_kernel(const EnsMember *ms, const LesArgs *args,
-                                                             const TileItem *items, int n, int dst)
+// formed once per column). Items as for the divergence.
+__global__ __launch_bounds__(kFlat) void ens_projection_kernel(const EnsMember *ms, const LesArgs *args,
+                                                               const ColItem *items, int n, int dst)
 {
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        const TileItem t = items[it];
+        const ColItem t = items[it];
         const EnsMember m = ms[t.member];
         const LesArgs g = args[t.member];
-        const int a = 1 + t.bx * 64 + threadIdx.x, b = 1 + t.by * 4 + threadIdx.y;
-        if (a > m.nx || b > m.ny) continue;
+        const int col = t.block * kFlat + threadIdx.x;
+        if (col >= m.nx * m.ny) continue;
+        const int a = 1 + col % m.nx, b = 1 + col / m.nx;
         double *U = dst ? m.f[1][0] : m.f[0][0], *V = dst ? m.f[1][1] : m.f[0][1];
         double *W = dst ? m.f[1][2] : m.f[0][2];
         const long long plane = (long long)m.nx * m.ny;
@@ -144,15 +162,35 @@ __global__ __launch_bounds__(256) void ens_projection_kernel(const EnsMember *ms
         const long long q0 = (a - 1) + (long long)m.nx * (b - 1);
         const long long qx0 = (a == m.nx ? 0 : a) + (long long)m.nx * (b - 1);  // (a % nx), row b-1
         const long long qy0 = (a - 1) + (long long)m.nx * (b == m.ny ? 0 : b);  // column a-1, row (b % ny)
-#pragma unroll 4
-        for (int c = 1; c <= m.nz; ++c) {
-            const long long p = a + b * m.sj + c * m.sk;
-            const long long o = plane * (c - 1);
-            const double f = m.phi[q0 + o];
-            const PlaneImages zi = plane_images(g, c + 1);
-            put_column(U, p, U[p] - m.gx * (m.phi[qx0 + o] - f), ci, zi, 0);
-            put_column(V, p, V[p] - m.gy * (m.phi[qy0 + o] - f), ci, zi, 0);
-            if (c < m.nz) put_column(W, p, W[p] - m.gz * (m.phi[q0 + o + plane] - f), ci, zi, 1);
+        // Planes in batches of four (eight measured slower): every load of a batch before its stores, so that
+        // they are in flight together (the stores write the batch's own cells and ghost images, which no load
+        // of it reads).
+        constexpr int kB = 4;
+        for (int c0 = 1; c0 <= m.nz; c0 += kB) {
+            double u[kB], v[kB], w[kB], f[kB], fx[kB], fy[kB], fz[kB];
+#pragma unroll
+            for (int d = 0; d < kB; ++d) {
+                const int c = min(c0 + d, m.nz);
+                const long long p = a + b * m.sj + c * m.sk;
+                const long long o = plane * (c - 1);
+                u[d] = U[p];
+                v[d] = V[p];
+                w[d] = W[p];
+                f[d] = m.phi[q0 + o];
+                fx[d] = m.phi[qx0 + o];
+                fy[d] = m.phi[qy0 + o];
+                fz[d] = m.phi[q0 + o + (c < m.nz ? plane : 0)];
+            }
+#pragma unroll
+            for (int d = 0; d < kB; ++d) {
+                const int c = c0 + d;
+                if (c > m.nz) break;
+                const long long p = a + b * m.sj + c * m.sk;
+                const PlaneImages zi = plane_images(g, c + 1);
+                put_column(U, p, u[d] - m.gx * (fx[d] - f[d]), ci, zi, 0);
+                put_column(V, p, v[d] - m.gy * (fy[d] - f[d]), ci, zi, 0);
+                if (c < m.nz) put_column(W, p, w[d] - m.gz * (fz[d] - f[d]), ci, zi, 1);
+            }
         }
     }
 }
@@ -592,8 +630,9 @@ struct ftc_ensemble {
     ColItem *d_cols = nullptr;
     double *d_tile_part = nullptr;  // one mean partial per tile item
     int2 *d_tile_ranges = nullptr;  // (first tile item, count) per member
-    std::vector<Range> r_les, r_tiles, r_gemm[6], r_cols;
-    int n_les = 0, n_tiles = 0, n_gemm[6] = {}, n_cols = 0;
+    ColItem *d_flat = nullptr, *d_flat_all = nullptr;  // blocks of kFlat plane columns; all: longest first
+    std::vector<Range> r_les, r_tiles, r_gemm[6], r_cols, r_flat;
+    int n_les = 0, n_tiles = 0, n_gemm[6] = {}, n_cols = 0, n_flat = 0;
     void *items_arena = nullptr;
     int nzmax = 0;
     int parity = 0;
@@ -667,10 +706,11 @@ static int build_items(ftc_ensemble &e)
 {
     std::vector<TileItem> les, tiles;
     std::vector<GemmItem> gemm[6];
-    std::vector<ColItem> cols;
+    std::vector<ColItem> cols, flat;
     e.n_les32 = 0;
     e.r_tiles.clear();
     e.r_cols.clear();
+    e.r_flat.clear();
     for (auto &r : e.r_gemm) r.clear();
     const int bm = tile_bm(e.gemm_tile);
     // LES tiles: the members of width 64 first, then those of width 32, each member's tiles contiguous.
@@ -717,10 +757,15 @@ static int build_items(ftc_ensemble &e)
         for (int b = 0; b * (long long)kCols < plane; ++b) cols.push_back({m, b});
         rc.count = (int)cols.size() - rc.start;
         e.r_cols.push_back(rc);
+        Range rf{(int)flat.size(), 0};
+        for (int b = 0; b * (long long)kFlat < plane; ++b) flat.push_back({m, b});
+        rf.count = (int)flat.size() - rf.start;
+        e.r_flat.push_back(rf);
     }
     e.n_les = (int)les.size();
     e.n_tiles = (int)tiles.size();
     e.n_cols = (int)cols.size();
+    e.n_flat = (int)flat.size();
     // The all-member launches give every item a workgroup of its own, and the dispatcher starts them in
     // order as slots free up: so that the last ones to start are short, the items of each width are taken
     // longest first (most planes), each member's items staying together.
@@ -735,12 +780,15 @@ static int build_items(ftc_ensemble &e)
     std::vector<ColItem> cols_all(cols);
     std::stable_sort(cols_all.begin(), cols_all.end(),
                      [&](const ColItem &a, const ColItem &b) { return e.nz[a.member] > e.nz[b.member]; });
+    std::vector<ColItem> flat_all(flat);
+    std::stable_sort(flat_all.begin(), flat_all.end(),
+                     [&](const ColItem &a, const ColItem &b) { return e.nz[a.member] > e.nz[b.member]; });
     std::vector<TileItem> tiles_all(tiles);
     std::stable_sort(tiles_all.begin(), tiles_all.end(),
                      [&](const TileItem &a, const TileItem &b) { return e.nz[a.member] > e.nz[b.member]; });
     std::size_t bytes = 2 * up256(les.size() * sizeof(TileItem)) + 2 * up256(tiles.size() * sizeof(TileItem)) +
-                        2 * up256(cols.size() * sizeof(ColItem)) + up256(tiles.size() * sizeof(double)) +
-                        up256(e.count * sizeof(int2));
+                        2 * up256(cols.size() * sizeof(ColItem)) + 2 * up256(flat.size() * sizeof(ColItem)) +
+                        up256(tiles.size() * sizeof(double)) + up256(e.count * sizeof(int2));
     for (int s = 0; s < 6; ++s) {
         e.n_gemm[s] = (int)gemm[s].size();
         bytes += up256(gemm[s].size() * sizeof(GemmItem));
@@ -764,6 +812,8 @@ static int build_items(ftc_ensemble &e)
     e.d_tiles_all = (TileItem *)put(tiles_all.data(), tiles_all.size() * sizeof(TileItem));
     e.d_cols = (ColItem *)put(cols.data(), cols.size() * sizeof(ColItem));
     e.d_cols_all = (ColItem *)put(cols_all.data(), cols_all.size() * sizeof(ColItem));
+    e.d_flat = (ColItem *)put(flat.data(), flat.size() * sizeof(ColItem));
+    e.d_flat_all = (ColItem *)put(flat_all.data(), flat_all.size() * sizeof(ColItem));
     std::vector<int2> ranges(e.count);
     for (int m = 0; m < e.count; ++m) ranges[m] = make_int2(e.r_tiles[m].start, e.r_tiles[m].count);
     e.d_tile_ranges = (int2 *)put(ranges.data(), ranges.size() * sizeof(int2));
@@ -825,14 +875,14 @@ static void launch_stage(ftc_ensemble &e, int stage, int member, int parity)
         break;
     }
     case kDiv:
-        n = pick(e.r_tiles, e.n_tiles, &start);
-        if (n) hipLaunchKernelGGL(ens_divergence_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
-                                  e.d_members, (all ? e.d_tiles_all : e.d_tiles + start), n, dst);
+        n = pick(e.r_flat, e.n_flat, &start);
+        if (n) hipLaunchKernelGGL(ens_divergence_kernel, dim3(all ? grid_for(e, n) : n), dim3(kFlat), 0, s,
+                                  e.d_members, (all ? e.d_flat_all : e.d_flat + start), n, dst);
         break;
     case kProj:
-        n = pick(e.r_tiles, e.n_tiles, &start);
-        if (n) hipLaunchKernelGGL(ens_projection_kernel, dim3(all ? grid_for(e, n) : n), dim3(64, 4), 0, s,
-                                  e.d_members, e.d_args[parity], (all ? e.d_tiles_all : e.d_tiles + start), n, dst);
+        n = pick(e.r_flat, e.n_flat, &start);
+        if (n) hipLaunchKernelGGL(ens_projection_kernel, dim3(all ? grid_for(e, n) : n), dim3(kFlat), 0, s,
+                                  e.d_members, e.d_args[parity], (all ? e.d_flat_all : e.d_flat + start), n, dst);
         break;
     case kRes:
         n = pick(e.r_tiles, e.n_tiles, &start);

@@ -364,6 +364,9 @@ def main() -> int:
                    help="only time D = A*B*C at N = 4096 (FP64 MFMA and split-bf16) and print its section")
     p.add_argument("--poisson-only", action="store_true",
                    help="only time the FP64 and mixed pressure solves at 512x512x128 and print their section")
+    p.add_argument("--ensemble-only", action="store_true",
+                   help="only time the 64-member ensemble (per member, batched FP64, batched mixed) and print "
+                        "its section")
     p.add_argument("--projected-step-only", action="store_true",
                    help="only time the whole projected step (LES step, divergence, FP64 and mixed solve, "
                         "projection) and print its section")
@@ -391,6 +394,9 @@ def main() -> int:
         return 0
     if args.poisson_only:
         print(json.dumps({"poisson": poisson_section(ftc, scope, lib, fields, reps=5)[0]}))
+        return 0
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_section(lib, fields)}))
         return 0
     if args.projected_step_only:
         print(json.dumps({"projected_step": projected_step_section(ftc, scope, lib, fields, steps=3)}))

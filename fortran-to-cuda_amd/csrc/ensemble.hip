@@ -320,7 +320,8 @@ __global__ __launch_bounds__(kCols, 3) void ens_tridiag_kernel(const EnsMember *
 }
 
 template <int BM, int BN>
-__global__ __launch_bounds__(kGemmThreads) void ens_gemm_f64_kernel(const GemmCall *calls, const GemmItem *items, int n)
+__global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(gemm_small_waves(BM)))) void
+ens_gemm_f64_kernel(const GemmCall *calls, const GemmItem *items, int n)
 {
     __shared__ F64Lds<BM, BN> L;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
@@ -534,7 +535,8 @@ ens_xform_kernel(const XformCall *__restrict__ calls, const GemmItem *__restrict
 }
 
 template <class TA, class TB, class TO, int BM, int BN, bool TOUT>
-__global__ __launch_bounds__(kGemmThreads) void ens_gemm_bf16_kernel(const GemmCall *calls, const GemmItem *items, int n)
+__global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(gemm_small_waves(BM)))) void
+ens_gemm_bf16_kernel(const GemmCall *calls, const GemmItem *items, int n)
 {
     __shared__ Bf16TnLds<BM, BN, 32> L;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {

@@ -252,8 +252,13 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_f64_db_kernel(const GemmCal
     gemm_f64_db_tile<BM, BN, WM, WN, BK>(g, L, mt, nt, b);
 }
 
+// Tiles below 128 x 128 declare four waves per SIMD: with a larger register budget the compiler keeps their
+// accumulators in VGPRs and copies them through AGPRs around every group of MFMAs.
+constexpr int gemm_small_waves(int bm) { return bm >= 128 ? 1 : 4; }
+
 template <int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(kGemmThreads) void gemm_f64_kernel(const GemmCall g)
+__global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(gemm_small_waves(BM)))) void
+gemm_f64_kernel(const GemmCall g)
 {
     __shared__ F64Lds<BM, BN> L;
     int mt, nt;
@@ -364,7 +369,8 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const GemmCall &g, Bf16Lds<BM, 
 }
 
 template <class TA, class TB, class TO, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(kGemmThreads) void gemm_bf16x3_kernel(const GemmCall g)
+__global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(gemm_small_waves(BM)))) void
+gemm_bf16x3_kernel(const GemmCall g)
 {
     __shared__ Bf16Lds<BM, BN> L;
     int mt, nt;
@@ -715,7 +721,8 @@ __device__ __forceinline__ void gemm_bf16x3_tn_tile(const GemmCall &g, Bf16TnLds
 }
 
 template <class TA, class TB, class TO, int BM, int BN, bool TOUT, int BK = 32>
-__global__ __launch_bounds__(kGemmThreads) void gemm_bf16x3_tn_kernel(const GemmCall g)
+__global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(gemm_small_waves(BM)))) void
+gemm_bf16x3_tn_kernel(const GemmCall g)
 {
     __shared__ Bf16TnLds<BM, BN, BK> L;
     int mt, nt;

@@ -1098,6 +1098,11 @@ __device__ __forceinline__ void les_step_body(const LesArgs &a, StepLds<NT, WX> 
                 if (CLIVE(r)) (&L.nu[(PH + 1) & 1][0][0])[tid + r * NT] = (&L.nu[PH & 1][0][0])[tid + r * NT];
         }
         __syncthreads();
+        // The update runs at a raised wave priority: the SIMD then issues the update's arithmetic ahead of the
+        // other resident workgroup's viscosity and staging, so the two workgroups drift apart and one's loads
+        // overlap the other's arithmetic (0.80-0.82 -> 0.76-0.77 ms at 512 x 512 x 128, the same bits).
+        // Raised during the viscosity as well, it gains nothing.
+        __builtin_amdgcn_s_setprio(1);
         const int so = (k - 1) * skb;
         const bool zlo = k == 2, zhi = k == nz + 1;
         // Writes val at the cell and at every ghost whose image it is, as put_with_images: the x and y images
@@ -1250,6 +1255,7 @@ __device__ __forceinline__ void les_step_body(const LesArgs &a, StepLds<NT, WX> 
             }
             zc = up;
         }
+        __builtin_amdgcn_s_setprio(0);
         if constexpr (FP == 3) {
             fetch_w(k + 2);
             fetch_sca(k + 2);

@@ -475,7 +475,7 @@ def main() -> int:
     bytes_alg = 8 * (4 * P + 4 * I)
     achieved = bytes_alg / (gpu_ms_per_step * 1e-3) / 1e9
     # The step's kernel: the 64 x 8 buffer-addressed kernel, its equal-spacing variant when dx = dy = dz (as here).
-    traffic = traffic_from_profiles(f"ftc::les_step_kernel<512, 64, {'true' if dx == dy == dz else 'false'}, 4, 3, 1>")
+    traffic = traffic_from_profiles(f"ftc::les_step_kernel<512, 32, {'true' if dx == dy == dz else 'false'}, 4, 3, 2>")
 
     extras = {}
     if not args.no_extras:

@@ -105,10 +105,12 @@ static const void *staged_kernel(int rows, int width, bool iso, bool two)
                : reinterpret_cast<const void *>(&les_sgs3_kernel<STEP, 1>);
 }
 
-// The buffer-addressed step kernels (les_step_kernel), by tile shape: 512 threads as 64 x 8 columns (the default)
-// or 32 x 16 columns, four waves per SIMD, two workgroups per CU; or 256 threads as 64 x 4 columns, three
-// workgroups per CU. The four-wave shapes load W and S after the update and read the stencil operands of each
-// component update (or of U and V together) from LDS again, which holds them within 128 registers.
+// The buffer-addressed step kernels (les_step_kernel), by tile shape: 512 threads as 32 x 16 columns (the
+// default) or 64 x 8 columns, four waves per SIMD, two workgroups per CU; or 256 threads as 64 x 4 columns,
+// three workgroups per CU. The four-wave shapes load W and S after the update and read the stencil operands
+// of each group of component updates (U and V | W | S: 32 x 16; each component: 64 x 8) from LDS again,
+// which holds them within 128 registers. With the update at a raised wave priority the 32 x 16 tile is the
+// faster (0.751 against 0.778-0.780 ms on one box; the same bits).
 struct StepShape {
     int threads, width, per_cu;
     const void *iso, *general;
@@ -119,9 +121,9 @@ static StepShape step_shape(const char *name)
 #define FTC_SHAPE(NT_, WX_, WPE_, FP_, SP_, PER_CU_)                                                        \
     StepShape{NT_, WX_, PER_CU_, reinterpret_cast<const void *>(&les_step_kernel<NT_, WX_, true, WPE_, FP_, SP_>), \
               reinterpret_cast<const void *>(&les_step_kernel<NT_, WX_, false, WPE_, FP_, SP_>)}
-    if (name && std::strcmp(name, "32x16") == 0) return FTC_SHAPE(512, 32, 4, 3, 2, 2);
+    if (name && std::strcmp(name, "64x8") == 0) return FTC_SHAPE(512, 64, 4, 3, 1, 2);
     if (name && std::strcmp(name, "64x4") == 0) return FTC_SHAPE(256, 64, 3, 0, 0, 3);
-    return FTC_SHAPE(512, 64, 4, 3, 1, 2);
+    return FTC_SHAPE(512, 32, 4, 3, 2, 2);
 #undef FTC_SHAPE
 }
 

@@ -174,10 +174,10 @@ def les_steps(ftc, scope, hip, fields, n, steps, spacing=(10.0, 10.0, 10.0), dt=
     return (b if steps % 2 == 1 else a), (U, V, W, S)
 
 
-@pytest.mark.parametrize("variant", [{}, {"FTC_LES_SHAPE": "32x16"}, {"FTC_LES_SHAPE": "64x4"},
-                                     {"FTC_LES_KCHUNK": "6"}, {"FTC_LES_SHAPE": "32x16", "FTC_LES_KCHUNK": "12"},
+@pytest.mark.parametrize("variant", [{}, {"FTC_LES_SHAPE": "64x8"}, {"FTC_LES_SHAPE": "64x4"},
+                                     {"FTC_LES_KCHUNK": "6"}, {"FTC_LES_SHAPE": "64x8", "FTC_LES_KCHUNK": "12"},
                                      {"FTC_LES_SHAPE": "64x4", "FTC_LES_KCHUNK": "18"}, {"FTC_LES_LEGACY": "1"},
-                                     {"FTC_LES_ANISO": "1"}, {"FTC_LES_ANISO": "1", "FTC_LES_SHAPE": "32x16"}])
+                                     {"FTC_LES_ANISO": "1"}, {"FTC_LES_ANISO": "1", "FTC_LES_SHAPE": "64x8"}])
 @pytest.mark.parametrize("n", [(70, 37, 19), (130, 9, 33), (5, 3, 2), (64, 16, 13)])
 def test_step_kernel_shapes_and_chunks_are_bit_exact_including_halos(ftc, scope, hip, oracle, fields, monkeypatch,
                                                                      variant, n):

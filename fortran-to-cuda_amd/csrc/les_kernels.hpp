@@ -835,16 +835,22 @@ __device__ __forceinline__ double buffer_load(__amdgpu_buffer_rsrc_t r, int voff
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
-// Nontemporal store (the outputs are not read again by the launch).
+// Plain store, no cache hint: the row ends that neighbouring tiles both write merge in L2.
 __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, int voff, int soff, double v)
 {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ftc_u32x2, v), r, voff, soff, 0);
 }
 
+// The scalar and viscosity planes (one-cell ring, CWT = WX + 2 cells wide) are laid out with the velocity
+// planes' row pitch, CXT = VXT = WX + 4: the ring cell at linear index c then reads the velocity tile at
+// c + VXT + 1, so consecutive lanes read consecutive doubles across a row end as well. With the narrower
+// pitch the index jumped by two doubles at every row end, and each 16-lane group that crossed one took a
+// two-way bank conflict on every velocity read. The two extra columns hold duplicates of the last ring
+// column and are never read.
 template <int NT, int WX>
 struct StepTile {
     static constexpr int TYT = NT / WX;
-    static constexpr int VXT = WX + 4, VYT = TYT + 4, CXT = WX + 2, CYT = TYT + 2;
+    static constexpr int VXT = WX + 4, VYT = TYT + 4, CXT = WX + 4, CWT = WX + 2, CYT = TYT + 2;
     static constexpr int VC = VXT * VYT, CC = CXT * CYT;
     static constexpr int VR = (VC + NT - 1) / NT, CR = (CC + NT - 1) / NT;
 };
@@ -911,7 +917,7 @@ __device__ __forceinline__ void les_step_body(const LesArgs &a, StepLds<NT, WX> 
     for (int r = 0; r < CR; ++r) {
         const int c = tid + r * NT;
         const int cc = c < CC ? c : 0;
-        const int li = cc % CX, lj = cc / CX;
+        const int li = min(cc % CX, T::CWT - 1), lj = cc / CX;  // a padding column repeats the last ring column
         const int ii = periodic_image(i0 - 1 + li, nx), jj = periodic_image(j0 - 1 + lj, ny);
         coff[r] = ((ii - 1) + (jj - 1) * sj) * 8;
         nvi[r] = (lj + 1) * VX + li + 1;
